@@ -1,0 +1,769 @@
+// hdg_api.cpp -- host runtime behind the C ABI of include/hdg.h.
+//
+// Owns device memory, builds the skeleton dof map and the symbolic skeleton
+// pattern, and enqueues the gfx950 kernels on the context stream.  Mirrors the
+// operator flow of src/HybridAffineFEOperators.jl:10-150.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/hdg.h"
+#include "hdg_internal.h"
+
+
+
+using namespace hdg;
+
+struct hdg_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  int64_t* d_count = nullptr;  // scratch counter for singular-cell counts
+  int32_t* d_perm = nullptr;   // field permutation of the map-level API
+  int perm_cap = 0;
+};
+
+namespace {
+
+// RAII device buffer
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  hipError_t alloc(size_t b) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = b;
+    if (b == 0) return hipSuccess;
+    return hipMalloc(&p, b);
+  }
+  template <class T>
+  T* as() const {
+    return reinterpret_cast<T*>(p);
+  }
+};
+
+int fail(hdg_ctx* ctx, int code, const std::string& msg) {
+  if (ctx) ctx->err = msg;
+  return code;
+}
+
+int hipfail(hdg_ctx* ctx, hipError_t e, const char* what) {
+  return fail(ctx, HDG_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIP_TRY(ctx, expr, what)            \
+  do {                                      \
+    hipError_t _e = (expr);                 \
+    if (_e != hipSuccess) return hipfail(ctx, _e, what); \
+  } while (0)
+
+template <class T>
+hipError_t upload(DevBuf& b, const std::vector<T>& v, hipStream_t st) {
+  hipError_t e = b.alloc(v.size() * sizeof(T));
+  if (e != hipSuccess || v.empty()) return e;
+  e = hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st);
+  if (e != hipSuccess) return e;
+  return hipStreamSynchronize(st);
+}
+
+}  // namespace
+
+struct hdg_mesh {
+  hdg_ctx* ctx = nullptr;
+  hdg_problem prob{};
+  TablesHost th;
+  Tables tdev{};
+  int D = 2;
+  int64_t ncells = 0, nfacets = 0, nfree = 0, ndir = 0, nnz = 0, nff = 0;
+  // host copies used for set-up / Dirichlet interpolation
+  std::vector<double> x0, jac;
+  std::vector<int32_t> facet_cell;  // [nfacets][2]
+  std::vector<int8_t> facet_lpos;   // [nfacets][2]
+  std::vector<int8_t> facet_dir;
+  // device
+  DevBuf tables, d_x0, d_jac, d_sign, d_coef, d_cell_dofs, d_lam_dir;
+  DevBuf d_S, d_g, d_W, d_info;
+  DevBuf d_fptr, d_pptr, d_pfacet, d_pcell, d_plpos, d_gcell, d_glpos;
+  DevBuf d_colptr, d_rowval, d_nzval, d_rhs;
+  const double* src_qp = nullptr;
+  bool condensed = false;
+
+  MeshDev meshdev() const {
+    MeshDev m;
+    m.ncells = ncells;
+    m.x0 = d_x0.as<double>();
+    m.jac = d_jac.as<double>();
+    m.sign = d_sign.as<int8_t>();
+    m.coef = d_coef.as<double>();
+    m.src = src_qp;
+    m.source = prob.source;
+    m.tau = prob.tau;
+    m.young = prob.young;
+    return m;
+  }
+  PairTable pairs() const {
+    PairTable P;
+    P.fptr = d_fptr.as<int64_t>();
+    P.pptr = d_pptr.as<int32_t>();
+    P.pfacet = d_pfacet.as<int32_t>();
+    P.pcell = d_pcell.as<int32_t>();
+    P.plpos = d_plpos.as<uint8_t>();
+    P.gcell = d_gcell.as<int32_t>();
+    P.glpos = d_glpos.as<uint8_t>();
+    P.nff = nff;
+    P.ntf = th.ntf;
+    P.nb = th.nb;
+    return P;
+  }
+};
+
+extern "C" {
+
+int hdg_ctx_create(int device, hdg_ctx** out) {
+  if (!out) return HDG_ERR_ARG;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return HDG_ERR_NO_DEVICE;
+  if (device < 0 || device >= ndev) return HDG_ERR_ARG;
+  std::unique_ptr<hdg_ctx> c(new hdg_ctx());
+  c->device = device;
+  if (hipSetDevice(device) != hipSuccess) return HDG_ERR_HIP;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return HDG_ERR_HIP;
+  if (hipMalloc(&c->d_count, sizeof(int64_t)) != hipSuccess) return HDG_ERR_ALLOC;
+  *out = c.release();
+  return HDG_OK;
+}
+
+int hdg_ctx_destroy(hdg_ctx* ctx) {
+  if (!ctx) return HDG_ERR_ARG;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  if (ctx->d_count) (void)hipFree(ctx->d_count);
+  if (ctx->d_perm) (void)hipFree(ctx->d_perm);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return HDG_OK;
+}
+
+const char* hdg_last_error(const hdg_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+void* hdg_ctx_stream(hdg_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+int hdg_ctx_synchronize(hdg_ctx* ctx) {
+  if (!ctx) return HDG_ERR_ARG;
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+  return HDG_OK;
+}
+
+int hdg_memcpy_d2h(hdg_ctx* ctx, void* dst, const void* src, int64_t bytes) {
+  if (!ctx || bytes < 0) return HDG_ERR_ARG;
+  if (bytes == 0) return HDG_OK;
+  HIP_TRY(ctx, hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyDeviceToHost, ctx->stream), "d2h");
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream), "d2h sync");
+  return HDG_OK;
+}
+
+int hdg_memcpy_h2d(hdg_ctx* ctx, void* dst, const void* src, int64_t bytes) {
+  if (!ctx || bytes < 0) return HDG_ERR_ARG;
+  if (bytes == 0) return HDG_OK;
+  HIP_TRY(ctx, hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyHostToDevice, ctx->stream), "h2d");
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream), "h2d sync");
+  return HDG_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Field partition -> permutation (condensed index -> natural index), with the
+// checks of _check_preconditions (src/StaticCondensationMap.jl:16-34).
+int make_perm(hdg_ctx* ctx, const hdg_fields* f, std::vector<int32_t>& perm, int& ni, int& nb) {
+  if (!f || f->nfields <= 0 || !f->field_sizes) return fail(ctx, HDG_ERR_ARG, "fields: empty");
+  if (f->n_interior < 0 || f->n_boundary < 0 ||
+      f->n_interior + f->n_boundary != f->nfields)
+    return fail(ctx, HDG_ERR_ARG, "fields: interior and boundary fields must partition all fields");
+  std::vector<int> off(f->nfields + 1, 0);
+  for (int i = 0; i < f->nfields; ++i) {
+    if (f->field_sizes[i] < 0) return fail(ctx, HDG_ERR_ARG, "fields: negative block size");
+    off[i + 1] = off[i] + f->field_sizes[i];
+  }
+  std::vector<char> touched(f->nfields, 0);
+  perm.clear();
+  ni = nb = 0;
+  for (int pass = 0; pass < 2; ++pass) {
+    const int cnt = pass == 0 ? f->n_interior : f->n_boundary;
+    const int32_t* lst = pass == 0 ? f->interior_fields : f->boundary_fields;
+    for (int k = 0; k < cnt; ++k) {
+      const int fld = lst[k];
+      if (fld < 0 || fld >= f->nfields) return fail(ctx, HDG_ERR_ARG, "fields: index out of range");
+      if (touched[fld]) return fail(ctx, HDG_ERR_ARG, "fields: repeated field");
+      touched[fld] = 1;
+      for (int i = off[fld]; i < off[fld + 1]; ++i) perm.push_back(i);
+      (pass == 0 ? ni : nb) += f->field_sizes[fld];
+    }
+  }
+  return HDG_OK;
+}
+
+int upload_perm(hdg_ctx* ctx, const std::vector<int32_t>& perm) {
+  if ((int)perm.size() > ctx->perm_cap) {
+    if (ctx->d_perm) (void)hipFree(ctx->d_perm);
+    ctx->d_perm = nullptr;
+    if (hipMalloc(&ctx->d_perm, perm.size() * sizeof(int32_t)) != hipSuccess)
+      return fail(ctx, HDG_ERR_ALLOC, "alloc perm");
+    ctx->perm_cap = (int)perm.size();
+  }
+  HIP_TRY(ctx, hipMemcpyAsync(ctx->d_perm, perm.data(), perm.size() * sizeof(int32_t),
+                              hipMemcpyHostToDevice, ctx->stream),
+          "upload perm");
+  return HDG_OK;
+}
+
+int count_singular(hdg_ctx* ctx, const int32_t* info, int64_t n, int64_t* out) {
+  HIP_TRY(ctx, launch_count_nonzero(info, n, ctx->d_count, ctx->stream), "count kernel");
+  int64_t h = 0;
+  HIP_TRY(ctx, hipMemcpyAsync(&h, ctx->d_count, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream),
+          "count copy");
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream), "count sync");
+  *out = h;
+  return HDG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int hdg_static_condensation(hdg_ctx* ctx, const hdg_fields* fields, int64_t ncells,
+                            const double* A, const double* b, double* S, double* g,
+                            int32_t* info) {
+  if (!ctx) return HDG_ERR_ARG;
+  if (ncells < 0) return fail(ctx, HDG_ERR_ARG, "ncells < 0");
+  std::vector<int32_t> perm;
+  int ni, nb;
+  int rc = make_perm(ctx, fields, perm, ni, nb);
+  if (rc) return rc;
+  if (ncells == 0) return HDG_OK;
+  if (!A || !b || !S || !g || !info) return fail(ctx, HDG_ERR_ARG, "null buffer");
+  if (ni == 0) return fail(ctx, HDG_ERR_ARG, "no interior dofs");
+  if (!condense_supported(ni + nb))
+    return fail(ctx, HDG_ERR_UNSUPPORTED, "local system size not supported: " + std::to_string(ni + nb));
+  HIP_TRY(ctx, hipSetDevice(ctx->device), "hipSetDevice");
+  if ((rc = upload_perm(ctx, perm))) return rc;
+  DenseIn in{A, b, ctx->d_perm, ni + nb, ni, nb};
+  CondenseOut out{S, g, nullptr, nullptr, info};
+  HIP_TRY(ctx, launch_condense_dense(in, ncells, out, ctx->stream), "condense kernel");
+  int64_t bad = 0;
+  if ((rc = count_singular(ctx, info, ncells, &bad))) return rc;
+  if (bad) return fail(ctx, HDG_SINGULAR, std::to_string(bad) + " cell(s) with singular A_ii");
+  return HDG_OK;
+}
+
+int hdg_backward_static_condensation(hdg_ctx* ctx, const hdg_fields* fields, int64_t ncells,
+                                     const double* A, const double* b, const double* x,
+                                     double* xi, int32_t* info) {
+  if (!ctx) return HDG_ERR_ARG;
+  if (ncells < 0) return fail(ctx, HDG_ERR_ARG, "ncells < 0");
+  std::vector<int32_t> perm;
+  int ni, nb;
+  int rc = make_perm(ctx, fields, perm, ni, nb);
+  if (rc) return rc;
+  if (ncells == 0) return HDG_OK;
+  if (!A || !b || !x || !xi || !info) return fail(ctx, HDG_ERR_ARG, "null buffer");
+  if (ni == 0) return fail(ctx, HDG_ERR_ARG, "no interior dofs");
+  if (!condense_supported(ni))
+    return fail(ctx, HDG_ERR_UNSUPPORTED, "interior size not supported: " + std::to_string(ni));
+  HIP_TRY(ctx, hipSetDevice(ctx->device), "hipSetDevice");
+  if ((rc = upload_perm(ctx, perm))) return rc;
+  DenseIn in{A, b, ctx->d_perm, ni + nb, ni, nb};
+  HIP_TRY(ctx, launch_backward_dense(in, x, ncells, xi, info, ctx->stream), "backward kernel");
+  int64_t bad = 0;
+  if ((rc = count_singular(ctx, info, ncells, &bad))) return rc;
+  if (bad) return fail(ctx, HDG_SINGULAR, std::to_string(bad) + " cell(s) with singular A_ii");
+  return HDG_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Operator-level API
+// ---------------------------------------------------------------------------
+int hdg_mesh_create(hdg_ctx* ctx, const hdg_problem* prob, const hdg_mesh_desc* desc,
+                    hdg_mesh** out) {
+  if (!ctx) return HDG_ERR_ARG;
+  if (!prob || !desc || !out) return fail(ctx, HDG_ERR_ARG, "null argument");
+  *out = nullptr;
+  const int D = prob->dim;
+  if (D != 2 && D != 3) return fail(ctx, HDG_ERR_ARG, "dim must be 2 or 3");
+  if (prob->physics != HDG_DARCY && prob->physics != HDG_ELASTICITY)
+    return fail(ctx, HDG_ERR_ARG, "unknown physics");
+  if (prob->k_flux < 0 || prob->k_pot < 0 || prob->k_trace < 0 || prob->k_flux > 6 ||
+      prob->k_pot > 7 || prob->k_trace > 6 || prob->quad_degree < 0 || prob->quad_degree > 20)
+    return fail(ctx, HDG_ERR_ARG, "orders out of range");
+  const int64_t nc = desc->ncells, nf = desc->nfacets;
+  if (nc < 0 || nf < 0 || (nc > 0 && (!desc->x0 || !desc->jac || !desc->cell_facets || !desc->coef)) ||
+      (nf > 0 && !desc->facet_dirichlet))
+    return fail(ctx, HDG_ERR_ARG, "mesh descriptor incomplete");
+  if (nc > INT32_MAX || nf > INT32_MAX) return fail(ctx, HDG_ERR_ARG, "mesh too large for int32 ids");
+  HIP_TRY(ctx, hipSetDevice(ctx->device), "hipSetDevice");
+
+  std::unique_ptr<hdg_mesh> m(new hdg_mesh());
+  m->ctx = ctx;
+  m->prob = *prob;
+  m->D = D;
+  m->ncells = nc;
+  m->nfacets = nf;
+  std::string err;
+  if (build_tables(prob->physics, D, prob->space, prob->k_flux, prob->k_pot, prob->k_trace,
+                   prob->quad_degree, m->th, err))
+    return fail(ctx, HDG_ERR_ARG, err);
+  TablesHost& th = m->th;
+  if (th.nqc > 128) return fail(ctx, HDG_ERR_UNSUPPORTED, "too many cell quadrature points");
+  if (!condense_supported(th.n))
+    return fail(ctx, HDG_ERR_UNSUPPORTED, "local system size not supported: " + std::to_string(th.n));
+  const int nF = 2 * D, ntf = th.ntf, nb = th.nb;
+
+  // ---- facet -> cells (ascending cell order) and owner signs ----
+  m->facet_cell.assign((size_t)nf * 2, -1);
+  m->facet_lpos.assign((size_t)nf * 2, -1);
+  std::vector<int8_t> sign((size_t)nc * nF, 1);
+  for (int64_t c = 0; c < nc; ++c)
+    for (int lf = 0; lf < nF; ++lf) {
+      const int64_t f = desc->cell_facets[c * nF + lf];
+      if (f < 0 || f >= nf) return fail(ctx, HDG_ERR_ARG, "cell_facets out of range");
+      if (m->facet_cell[2 * f] < 0) {
+        m->facet_cell[2 * f] = (int32_t)c;
+        m->facet_lpos[2 * f] = (int8_t)lf;
+        sign[c * nF + lf] = 1;  // first cell around the facet owns it
+      } else if (m->facet_cell[2 * f + 1] < 0 && m->facet_cell[2 * f] != c) {
+        m->facet_cell[2 * f + 1] = (int32_t)c;
+        m->facet_lpos[2 * f + 1] = (int8_t)lf;
+        sign[c * nF + lf] = -1;
+      } else {
+        return fail(ctx, HDG_ERR_ARG, "facet shared by more than two cells (non-conforming)");
+      }
+    }
+  m->facet_dir.assign(desc->facet_dirichlet, desc->facet_dirichlet + nf);
+
+  // ---- trace dof ids: free facets consecutive in facet order, Dirichlet
+  //      facets numbered -1,-2,... (Gridap L2 facet space) ----
+  std::vector<int64_t> first((size_t)nf);
+  std::vector<int32_t> ffid((size_t)nf, -1);
+  int64_t fr = 0, dr = 0, nff = 0;
+  for (int64_t f = 0; f < nf; ++f) {
+    if (m->facet_dir[f]) {
+      first[f] = -(dr + 1);
+      dr += ntf;
+    } else {
+      first[f] = fr;
+      fr += ntf;
+      ffid[f] = (int32_t)nff++;
+    }
+  }
+  if (fr > INT32_MAX || dr > INT32_MAX) return fail(ctx, HDG_ERR_ARG, "too many trace dofs");
+  m->nfree = fr;
+  m->ndir = dr;
+  m->nff = nff;
+  std::vector<int32_t> cell_dofs((size_t)nc * nb);
+  for (int64_t c = 0; c < nc; ++c)
+    for (int lf = 0; lf < nF; ++lf) {
+      const int64_t f = desc->cell_facets[c * nF + lf];
+      for (int k = 0; k < ntf; ++k)
+        cell_dofs[(c * nF + lf) * ntf + k] = (int32_t)(first[f] >= 0 ? first[f] + k : first[f] - k);
+    }
+
+  // ---- facet-pair structure of the skeleton matrix ----
+  std::vector<int32_t> pptr((size_t)nff + 1, 0), pfacet, pcell;
+  std::vector<uint8_t> plpos;
+  std::vector<int64_t> fptr((size_t)nff + 1, 0);
+  std::vector<int32_t> gcell((size_t)nff * 2, -1);
+  std::vector<uint8_t> glpos((size_t)nff * 2, 0);
+  pfacet.reserve((size_t)nff * (2 * nF - 1));
+  pcell.reserve((size_t)nff * (2 * nF - 1) * 2);
+  plpos.reserve((size_t)nff * (2 * nF - 1) * 2);
+  struct Cand {
+    int32_t F;
+    int32_t cell;
+    uint8_t lp;
+  };
+  std::vector<Cand> cand;
+  for (int64_t f = 0; f < nf; ++f) {
+    const int32_t G = ffid[f];
+    if (G < 0) continue;
+    cand.clear();
+    for (int k = 0; k < 2; ++k) {
+      const int32_t c = m->facet_cell[2 * f + k];
+      if (c < 0) continue;
+      gcell[2 * G + k] = c;
+      const int lg = m->facet_lpos[2 * f + k];
+      glpos[2 * G + k] = (uint8_t)lg;
+      for (int lf = 0; lf < nF; ++lf) {
+        const int32_t F = ffid[desc->cell_facets[(int64_t)c * nF + lf]];
+        if (F < 0) continue;
+        cand.push_back({F, c, (uint8_t)(lg | (lf << 4))});
+      }
+    }
+    std::stable_sort(cand.begin(), cand.end(), [](const Cand& x, const Cand& y) {
+      return x.F != y.F ? x.F < y.F : x.cell < y.cell;
+    });
+    int32_t cnt = 0;
+    for (size_t i = 0; i < cand.size();) {
+      size_t j = i;
+      pfacet.push_back(cand[i].F);
+      int32_t c2[2] = {-1, -1};
+      uint8_t l2[2] = {0, 0};
+      int k = 0;
+      while (j < cand.size() && cand[j].F == cand[i].F) {
+        if (k < 2) {
+          c2[k] = cand[j].cell;
+          l2[k] = cand[j].lp;
+        }
+        ++k;
+        ++j;
+      }
+      if (k > 2) return fail(ctx, HDG_ERR_ARG, "facet pair with more than two cells");
+      pcell.push_back(c2[0]);
+      pcell.push_back(c2[1]);
+      plpos.push_back(l2[0]);
+      plpos.push_back(l2[1]);
+      ++cnt;
+      i = j;
+    }
+    pptr[G + 1] = pptr[G] + cnt;
+    fptr[G + 1] = fptr[G] + (int64_t)cnt * ntf * ntf;
+  }
+  m->nnz = fptr[nff];
+  std::vector<int64_t> colptr((size_t)m->nfree + 1, 0);
+  for (int64_t G = 0; G < nff; ++G) {
+    const int64_t np = pptr[G + 1] - pptr[G];
+    for (int k = 0; k < ntf; ++k) colptr[G * ntf + k] = fptr[G] + (int64_t)k * np * ntf;
+  }
+  colptr[m->nfree] = m->nnz;
+
+  // ---- uploads ----
+  hipStream_t st = ctx->stream;
+  {
+    // tables: one allocation, doubles first then bytes
+    std::vector<const std::vector<double>*> dv = {&th.Mpp, &th.Gpq, &th.Gqp, &th.Fqp, &th.Fqq,
+                                                   &th.Fpm, &th.Fqm, &th.Fmm, &th.Bp, &th.Bq,
+                                                   &th.xq};
+    size_t nd = 0;
+    for (auto* v : dv) nd += v->size();
+    size_t bytes = nd * sizeof(double) + th.pmask.size() + th.qmask.size() + 16;
+    std::vector<char> host(bytes, 0);
+    HIP_TRY(ctx, m->tables.alloc(bytes), "alloc tables");
+    size_t off = 0;
+    std::vector<size_t> offs;
+    for (auto* v : dv) {
+      offs.push_back(off);
+      if (!v->empty()) std::memcpy(host.data() + off, v->data(), v->size() * sizeof(double));
+      off += v->size() * sizeof(double);
+    }
+    size_t off_pm = off;
+    std::memcpy(host.data() + off, th.pmask.data(), th.pmask.size());
+    off += th.pmask.size();
+    size_t off_qm = off;
+    std::memcpy(host.data() + off, th.qmask.data(), th.qmask.size());
+    HIP_TRY(ctx, hipMemcpyAsync(m->tables.p, host.data(), bytes, hipMemcpyHostToDevice, st),
+            "upload tables");
+    HIP_TRY(ctx, hipStreamSynchronize(st), "sync");
+    char* base = m->tables.as<char>();
+    Tables& T = m->tdev;
+    T.phys = th.phys;
+    T.D = th.D;
+    T.nF = th.nF;
+    T.np = th.np;
+    T.nq = th.nq;
+    T.nm = th.nm;
+    T.ncp = th.ncp;
+    T.ncq = th.ncq;
+    T.ncm = th.ncm;
+    T.na = th.na;
+    T.nbf = th.nbf;
+    T.ntf = th.ntf;
+    T.ni = th.ni;
+    T.nb = th.nb;
+    T.n = th.n;
+    T.nqc = th.nqc;
+    const double** ptrs[] = {&T.Mpp, &T.Gpq, &T.Gqp, &T.Fqp, &T.Fqq, &T.Fpm,
+                             &T.Fqm, &T.Fmm, &T.Bp,  &T.Bq,  &T.xq};
+    for (size_t i = 0; i < dv.size(); ++i) *ptrs[i] = reinterpret_cast<const double*>(base + offs[i]);
+    T.pmask = reinterpret_cast<const uint8_t*>(base + off_pm);
+    T.qmask = reinterpret_cast<const uint8_t*>(base + off_qm);
+  }
+  m->x0.assign(desc->x0, desc->x0 + nc * D);
+  m->jac.assign(desc->jac, desc->jac + nc * D * D);
+  for (int64_t c = 0; c < nc; ++c) {
+    double det = D == 2 ? m->jac[c * 4] * m->jac[c * 4 + 3] - m->jac[c * 4 + 2] * m->jac[c * 4 + 1]
+                        : 1.0;
+    if (D == 3) {
+      const double* J = &m->jac[c * 9];
+      det = J[0] * (J[4] * J[8] - J[5] * J[7]) - J[3] * (J[1] * J[8] - J[2] * J[7]) +
+            J[6] * (J[1] * J[5] - J[2] * J[4]);
+    }
+    if (!(std::fabs(det) > 0.0)) return fail(ctx, HDG_ERR_ARG, "degenerate cell " + std::to_string(c));
+  }
+  std::vector<double> coef(desc->coef, desc->coef + nc);
+  HIP_TRY(ctx, upload(m->d_x0, m->x0, st), "upload x0");
+  HIP_TRY(ctx, upload(m->d_jac, m->jac, st), "upload jac");
+  HIP_TRY(ctx, upload(m->d_sign, sign, st), "upload sign");
+  HIP_TRY(ctx, upload(m->d_coef, coef, st), "upload coef");
+  HIP_TRY(ctx, upload(m->d_cell_dofs, cell_dofs, st), "upload dofs");
+  HIP_TRY(ctx, upload(m->d_fptr, fptr, st), "upload fptr");
+  HIP_TRY(ctx, upload(m->d_pptr, pptr, st), "upload pptr");
+  HIP_TRY(ctx, upload(m->d_pfacet, pfacet, st), "upload pfacet");
+  HIP_TRY(ctx, upload(m->d_pcell, pcell, st), "upload pcell");
+  HIP_TRY(ctx, upload(m->d_plpos, plpos, st), "upload plpos");
+  HIP_TRY(ctx, upload(m->d_gcell, gcell, st), "upload gcell");
+  HIP_TRY(ctx, upload(m->d_glpos, glpos, st), "upload glpos");
+  HIP_TRY(ctx, upload(m->d_colptr, colptr, st), "upload colptr");
+  std::vector<double> zeros((size_t)std::max<int64_t>(m->ndir, 1), 0.0);
+  HIP_TRY(ctx, upload(m->d_lam_dir, zeros, st), "upload lam_dir");
+  // resident outputs
+  HIP_TRY(ctx, m->d_S.alloc((size_t)nc * nb * nb * sizeof(double)), "alloc S");
+  HIP_TRY(ctx, m->d_g.alloc((size_t)nc * nb * sizeof(double)), "alloc g");
+  HIP_TRY(ctx, m->d_W.alloc((size_t)nc * (nb + 1) * th.ni * sizeof(double)), "alloc W");
+  HIP_TRY(ctx, m->d_info.alloc((size_t)std::max<int64_t>(nc, 1) * sizeof(int32_t)), "alloc info");
+  HIP_TRY(ctx, m->d_rowval.alloc((size_t)std::max<int64_t>(m->nnz, 1) * sizeof(int32_t)), "alloc rowval");
+  HIP_TRY(ctx, m->d_nzval.alloc((size_t)std::max<int64_t>(m->nnz, 1) * sizeof(double)), "alloc nzval");
+  HIP_TRY(ctx, m->d_rhs.alloc((size_t)std::max<int64_t>(m->nfree, 1) * sizeof(double)), "alloc rhs");
+  HIP_TRY(ctx, launch_symbolic(m->pairs(), m->d_rowval.as<int32_t>(), st), "symbolic kernel");
+  HIP_TRY(ctx, hipStreamSynchronize(st), "symbolic sync");
+  *out = m.release();
+  return HDG_OK;
+}
+
+int hdg_mesh_create_cartesian(hdg_ctx* ctx, const hdg_problem* prob, const int64_t* cells,
+                              const double* domain, const double* coef, hdg_mesh** out) {
+  if (!ctx) return HDG_ERR_ARG;
+  if (!prob || !cells || !domain || !coef || !out) return fail(ctx, HDG_ERR_ARG, "null argument");
+  const int D = prob->dim;
+  if (D != 2 && D != 3) return fail(ctx, HDG_ERR_ARG, "dim must be 2 or 3");
+  int64_t n[3] = {1, 1, 1}, nc = 1;
+  for (int a = 0; a < D; ++a) {
+    if (cells[a] <= 0) return fail(ctx, HDG_ERR_ARG, "cells must be positive");
+    n[a] = cells[a];
+    nc *= n[a];
+  }
+  const int nF = 2 * D;
+  std::vector<int32_t> cf((size_t)nc * nF);
+  std::vector<int8_t> bnd;
+  bnd.reserve((size_t)nc * 2);
+  const int64_t stride[3] = {1, n[0], n[0] * n[1]};
+  int64_t next = 0;
+  // Gridap CartesianDiscreteModel: cells lexicographic (x fastest), facets
+  // numbered at first encounter over (cell, local facet)
+  for (int64_t c = 0; c < nc; ++c) {
+    int64_t idx[3] = {0, 0, 0}, t = c;
+    for (int a = 0; a < D; ++a) {
+      idx[a] = t % n[a];
+      t /= n[a];
+    }
+    for (int lf = 0; lf < nF; ++lf) {
+      const int dir = D - 1 - lf / 2, side = lf & 1;
+      if (side == 0 && idx[dir] > 0) {
+        cf[c * nF + lf] = cf[(c - stride[dir]) * nF + (lf ^ 1)];
+      } else {
+        cf[c * nF + lf] = (int32_t)next++;
+        bnd.push_back((side == 0 && idx[dir] == 0) || (side == 1 && idx[dir] == n[dir] - 1) ? 1 : 0);
+      }
+    }
+  }
+  std::vector<double> x0((size_t)nc * D), jac((size_t)nc * D * D, 0.0);
+  double h[3];
+  for (int a = 0; a < D; ++a) h[a] = (domain[2 * a + 1] - domain[2 * a]) / n[a];
+  for (int64_t c = 0; c < nc; ++c) {
+    int64_t t = c;
+    for (int a = 0; a < D; ++a) {
+      x0[c * D + a] = domain[2 * a] + (t % n[a]) * h[a];
+      t /= n[a];
+      jac[c * D * D + a * (D + 1)] = h[a];
+    }
+  }
+  hdg_mesh_desc desc;
+  desc.ncells = nc;
+  desc.nfacets = next;
+  desc.x0 = x0.data();
+  desc.jac = jac.data();
+  desc.cell_facets = cf.data();
+  desc.facet_dirichlet = bnd.data();
+  desc.coef = coef;
+  return hdg_mesh_create(ctx, prob, &desc, out);
+}
+
+int hdg_mesh_destroy(hdg_mesh* mesh) {
+  if (!mesh) return HDG_ERR_ARG;
+  (void)hipSetDevice(mesh->ctx->device);
+  (void)hipStreamSynchronize(mesh->ctx->stream);
+  delete mesh;
+  return HDG_OK;
+}
+
+int hdg_mesh_get_info(const hdg_mesh* m, hdg_mesh_info* info) {
+  if (!m || !info) return HDG_ERR_ARG;
+  info->ni = m->th.ni;
+  info->nb = m->th.nb;
+  info->n_flux = m->th.na;
+  info->n_pot = m->th.nbf;
+  info->n_trace_facet = m->th.ntf;
+  info->nq_cell = m->th.nqc;
+  info->ncells = m->ncells;
+  info->nfacets = m->nfacets;
+  info->nfree = m->nfree;
+  info->ndir = m->ndir;
+  info->nnz = m->nnz;
+  info->nfree_total = m->ncells * m->th.ni + m->nfree;
+  return HDG_OK;
+}
+
+int hdg_mesh_set_dirichlet(hdg_mesh* m, const double* lam_dir) {
+  if (!m) return HDG_ERR_ARG;
+  if (m->ndir > 0 && !lam_dir) return fail(m->ctx, HDG_ERR_ARG, "null Dirichlet values");
+  if (m->ndir == 0) return HDG_OK;
+  HIP_TRY(m->ctx, hipSetDevice(m->ctx->device), "hipSetDevice");
+  HIP_TRY(m->ctx, hipMemcpyAsync(m->d_lam_dir.p, lam_dir, m->ndir * sizeof(double),
+                                 hipMemcpyHostToDevice, m->ctx->stream),
+          "upload Dirichlet");
+  HIP_TRY(m->ctx, hipStreamSynchronize(m->ctx->stream), "sync");
+  m->condensed = false;
+  return HDG_OK;
+}
+
+}  // extern "C"
+
+namespace {
+void mms_trace(int src, int D, const double* x, double E, double nu, double* u) {
+  for (int a = 0; a < 3; ++a) u[a] = 0.0;
+  if (src == HDG_SOURCE_DARCY_KAT) {
+    u[0] = -3.14;
+  } else if (src == HDG_SOURCE_DARCY_LINEAR) {
+    for (int a = 0; a < D; ++a) u[0] += x[a];
+  } else if (src == HDG_SOURCE_DARCY_SINPROD) {
+    double p = 1.0;
+    for (int a = 0; a < D; ++a) p *= std::sin(M_PI * x[a]);
+    u[0] = p;
+  } else if (src == HDG_SOURCE_ELAS_LINEAR) {
+    for (int i = 0; i < D; ++i)
+      for (int j = 0; j < D; ++j) u[i] += (1.0 + i + 2.0 * j) / (D + 1.0) * x[j];
+  } else if (src == HDG_SOURCE_ELAS_SIN) {
+    for (int i = 0; i < D; ++i) u[i] = std::sin(M_PI * (x[i] + 0.5 * x[(i + 1) % D]));
+  }
+  (void)E;
+  (void)nu;
+}
+}  // namespace
+
+extern "C" {
+
+int hdg_mesh_set_dirichlet_mms(hdg_mesh* m) {
+  if (!m) return HDG_ERR_ARG;
+  const int D = m->D, nm = m->th.nm, ncm = m->th.ncm;
+  std::vector<double> vals((size_t)std::max<int64_t>(m->ndir, 1), 0.0);
+  int64_t k = 0;
+  for (int64_t f = 0; f < m->nfacets; ++f) {
+    if (!m->facet_dir[f]) continue;
+    const int64_t c = m->facet_cell[2 * f];
+    const int lf = m->facet_lpos[2 * f];
+    const int fix = D - 1 - lf / 2;
+    for (int i = 0; i < nm; ++i) {
+      double xi[3] = {0, 0, 0}, x[3] = {0, 0, 0};
+      int j = 0;
+      for (int a = 0; a < D; ++a)
+        xi[a] = (a == fix) ? double(lf & 1) : m->th.trace_nodes[(size_t)i * (D - 1) + j++];
+      for (int a = 0; a < D; ++a) {
+        x[a] = m->x0[c * D + a];
+        for (int e = 0; e < D; ++e) x[a] += m->jac[c * D * D + a + D * e] * xi[e];
+      }
+      double u[3];
+      mms_trace(m->prob.source, D, x, m->prob.young, 0.0, u);
+      for (int comp = 0; comp < ncm; ++comp) vals[k + comp * nm + i] = u[comp];
+    }
+    k += (int64_t)ncm * nm;
+  }
+  return hdg_mesh_set_dirichlet(m, vals.data());
+}
+
+int hdg_mesh_set_source(hdg_mesh* m, const double* src_qp) {
+  if (!m) return HDG_ERR_ARG;
+  m->src_qp = src_qp;
+  m->condensed = false;
+  return HDG_OK;
+}
+
+int hdg_condense_assemble(hdg_mesh* m, int precision, int64_t* n_singular) {
+  if (!m) return HDG_ERR_ARG;
+  hdg_ctx* ctx = m->ctx;
+  if (precision != HDG_FP64 && precision != HDG_MIXED)
+    return fail(ctx, HDG_ERR_ARG, "unknown precision");
+  if (precision == HDG_MIXED) return fail(ctx, HDG_ERR_UNSUPPORTED, "mixed precision not built yet");
+  if (m->prob.source == HDG_SOURCE_QP && !m->src_qp)
+    return fail(ctx, HDG_ERR_STATE, "source values at quadrature points not set");
+  HIP_TRY(ctx, hipSetDevice(ctx->device), "hipSetDevice");
+  hipStream_t st = ctx->stream;
+  CondenseOut out{m->d_S.as<double>(), m->d_g.as<double>(), m->d_W.as<double>(), nullptr,
+                  m->d_info.as<int32_t>()};
+  HIP_TRY(ctx, launch_condense_fused(m->tdev, m->meshdev(), out, st), "condense kernel");
+  HIP_TRY(ctx, launch_assemble(m->pairs(), m->d_S.as<double>(), m->d_g.as<double>(),
+                               m->d_cell_dofs.as<int32_t>(), m->d_lam_dir.as<double>(),
+                               m->d_nzval.as<double>(), m->d_rhs.as<double>(), st),
+          "assemble kernels");
+  int64_t bad = 0;
+  int rc = count_singular(ctx, m->d_info.as<int32_t>(), m->ncells, &bad);
+  if (rc) return rc;
+  if (n_singular) *n_singular = bad;
+  m->condensed = true;
+  if (bad) return fail(ctx, HDG_SINGULAR, std::to_string(bad) + " cell(s) with singular A_ii");
+  return HDG_OK;
+}
+
+int hdg_skeleton_system(hdg_mesh* m, const int64_t** colptr, const int32_t** rowval,
+                        const double** nzval, const double** rhs) {
+  if (!m) return HDG_ERR_ARG;
+  if (colptr) *colptr = m->d_colptr.as<int64_t>();
+  if (rowval) *rowval = m->d_rowval.as<int32_t>();
+  if (nzval) *nzval = m->d_nzval.as<double>();
+  if (rhs) *rhs = m->d_rhs.as<double>();
+  return m->condensed ? HDG_OK : fail(m->ctx, HDG_ERR_STATE, "skeleton system not assembled yet");
+}
+
+int hdg_cell_blocks(hdg_mesh* m, const double** S, const double** g, const int32_t** cell_dofs) {
+  if (!m) return HDG_ERR_ARG;
+  if (S) *S = m->d_S.as<double>();
+  if (g) *g = m->d_g.as<double>();
+  if (cell_dofs) *cell_dofs = m->d_cell_dofs.as<int32_t>();
+  return HDG_OK;
+}
+
+int hdg_recover(hdg_mesh* m, const double* lam_free, double* free_values) {
+  if (!m) return HDG_ERR_ARG;
+  hdg_ctx* ctx = m->ctx;
+  if (!m->condensed) return fail(ctx, HDG_ERR_STATE, "recover called before condense");
+  if ((m->nfree > 0 && !lam_free) || !free_values) return fail(ctx, HDG_ERR_ARG, "null buffer");
+  HIP_TRY(ctx, hipSetDevice(ctx->device), "hipSetDevice");
+  HIP_TRY(ctx, launch_recover(m->d_W.as<double>(), m->d_cell_dofs.as<int32_t>(), lam_free,
+                              m->d_lam_dir.as<double>(), free_values, m->ncells, m->th.ni,
+                              m->th.nb, m->th.na, m->nfree, ctx->stream),
+          "recover kernel");
+  return HDG_OK;
+}
+
+int hdg_cell_systems(hdg_mesh* m, int64_t first, int64_t count, double* A, double* b) {
+  if (!m) return HDG_ERR_ARG;
+  hdg_ctx* ctx = m->ctx;
+  if (first < 0 || count < 0 || first + count > m->ncells || (count > 0 && (!A || !b)))
+    return fail(ctx, HDG_ERR_ARG, "cell range");
+  if (m->prob.source == HDG_SOURCE_QP && !m->src_qp)
+    return fail(ctx, HDG_ERR_STATE, "source values at quadrature points not set");
+  HIP_TRY(ctx, hipSetDevice(ctx->device), "hipSetDevice");
+  HIP_TRY(ctx, launch_cell_systems(m->tdev, m->meshdev(), first, count, A, b, ctx->stream),
+          "cell systems kernel");
+  return HDG_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,406 @@
+// hdg_fe.cpp -- reference finite-element tables for the fused cell kernels.
+//
+// The reference integrates every weak-form term per cell with Gauss rules of
+// degree 2(order+1) (test/DarcyHDGTests.jl:57-61) through Gridap's lazy
+// arrays.  For affine cells that quadrature is exact for the polynomial
+// products of the bilinear forms, so it is done once here on the reference
+// cell; the GPU kernels apply the per-cell affine pull-back (|det J|, J^-1,
+// facet normals and measures, kappa, owner sign).  The right-hand side keeps
+// its per-cell quadrature (tables Bp/Bq hold basis values times weights).
+//
+// Bases: Lagrangian, equispaced nodes, nodes ordered face by face as Gridap's
+// n-cube polytopes enumerate them (vertices, edges, faces, interior; within a
+// dimension by free-direction set, fixed coordinates first-direction
+// fastest).  space Q: tensor products of 1D Lagrange polynomials; space P:
+// total degree <= k with the lattice nodes of that order (nodal basis from a
+// Vandermonde solve in centred monomials).
+#include <cmath>
+#include <cstring>
+
+#include "hdg_internal.h"
+
+namespace hdg {
+namespace {
+
+// Gauss-Legendre nodes/weights on [0,1]: Newton on P_n from the asymptotic
+// guess, three-term recurrence for P_n and P_n'.
+void gauss_legendre01(int n, std::vector<double>& x, std::vector<double>& w) {
+  x.assign(n, 0.0);
+  w.assign(n, 0.0);
+  for (int i = 0; i < n; ++i) {
+    double t = std::cos(M_PI * (4.0 * i + 3.0) / (4.0 * n + 2.0));
+    double pn = 0.0, dpn = 1.0;
+    for (int it = 0; it < 100; ++it) {
+      double pm1 = 1.0, p = t;
+      if (n == 0) p = 1.0;
+      for (int k = 2; k <= n; ++k) {
+        double pk = ((2 * k - 1) * t * p - (k - 1) * pm1) / k;
+        pm1 = p;
+        p = pk;
+      }
+      pn = p;
+      dpn = n * (pm1 - t * p) / (1.0 - t * t);
+      double step = pn / dpn;
+      t -= step;
+      if (std::fabs(step) <= 1e-16 * std::fabs(t) + 1e-300) break;
+    }
+    {
+      double pm1 = 1.0, p = t;
+      for (int k = 2; k <= n; ++k) {
+        double pk = ((2 * k - 1) * t * p - (k - 1) * pm1) / k;
+        pm1 = p;
+        p = pk;
+      }
+      dpn = n * (pm1 - t * p) / (1.0 - t * t);
+    }
+    // t_i descending in i: map so that x is ascending
+    x[i] = 0.5 * (1.0 - t);
+    w[i] = 1.0 / ((1.0 - t * t) * dpn * dpn);
+  }
+}
+
+struct Basis {
+  int d = 0, k = 0;
+  bool pspace = false;
+  int n = 0;
+  std::vector<int> lat;       // [n][3] lattice index of each node (nodes = lat/k)
+  std::vector<double> coef;   // P space: [n][n] monomial coefficients (row: function)
+  std::vector<int> mono;      // P space: [n][3] monomial exponents
+
+  double node(int i, int a) const { return k == 0 ? 0.5 : double(lat[3 * i + a]) / k; }
+
+  // value and reference gradient of function i at x
+  void eval(const double* x, double* val, double* grad) const {
+    if (k == 0) {
+      for (int i = 0; i < n; ++i) {
+        val[i] = 1.0;
+        if (grad)
+          for (int a = 0; a < 3; ++a) grad[3 * i + a] = 0.0;
+      }
+      return;
+    }
+    if (!pspace) {
+      // tensor Lagrange: prod_a l_{lat_a}(x_a)
+      double L[3][8], dL[3][8];
+      for (int a = 0; a < d; ++a) lagrange1d(x[a], L[a], dL[a]);
+      for (int i = 0; i < n; ++i) {
+        double v = 1.0;
+        for (int a = 0; a < d; ++a) v *= L[a][lat[3 * i + a]];
+        val[i] = v;
+        if (grad)
+          for (int g = 0; g < 3; ++g) {
+            if (g >= d) {
+              grad[3 * i + g] = 0.0;
+              continue;
+            }
+            double dv = 1.0;
+            for (int a = 0; a < d; ++a) dv *= (a == g) ? dL[a][lat[3 * i + a]] : L[a][lat[3 * i + a]];
+            grad[3 * i + g] = dv;
+          }
+      }
+      return;
+    }
+    // P space: phi_i = sum_m coef[i][m] prod_a (x_a - 1/2)^{e_a}
+    std::vector<double> mv(n), md(3 * n);
+    for (int m = 0; m < n; ++m) {
+      double v = 1.0;
+      for (int a = 0; a < d; ++a) v *= ipow(x[a] - 0.5, mono[3 * m + a]);
+      mv[m] = v;
+      for (int g = 0; g < 3; ++g) {
+        if (g >= d || mono[3 * m + g] == 0) {
+          md[3 * m + g] = 0.0;
+          continue;
+        }
+        double dv = 1.0;
+        for (int a = 0; a < d; ++a) {
+          int e = mono[3 * m + a];
+          dv *= (a == g) ? e * ipow(x[a] - 0.5, e - 1) : ipow(x[a] - 0.5, e);
+        }
+        md[3 * m + g] = dv;
+      }
+    }
+    for (int i = 0; i < n; ++i) {
+      double v = 0.0, g0 = 0.0, g1 = 0.0, g2 = 0.0;
+      for (int m = 0; m < n; ++m) {
+        double c = coef[(size_t)i * n + m];
+        v += c * mv[m];
+        g0 += c * md[3 * m];
+        g1 += c * md[3 * m + 1];
+        g2 += c * md[3 * m + 2];
+      }
+      val[i] = v;
+      if (grad) {
+        grad[3 * i] = g0;
+        grad[3 * i + 1] = g1;
+        grad[3 * i + 2] = g2;
+      }
+    }
+  }
+
+  static double ipow(double b, int e) {
+    double r = 1.0;
+    for (int i = 0; i < e; ++i) r *= b;
+    return r;
+  }
+
+  // 1D Lagrange basis on nodes j/k, j = 0..k
+  void lagrange1d(double x, double* L, double* dL) const {
+    for (int j = 0; j <= k; ++j) {
+      double xj = double(j) / k, v = 1.0, dv = 0.0;
+      for (int m = 0; m <= k; ++m) {
+        if (m == j) continue;
+        double xm = double(m) / k, den = xj - xm;
+        dv = dv * (x - xm) / den + v / den;
+        v *= (x - xm) / den;
+      }
+      L[j] = v;
+      dL[j] = dv;
+    }
+  }
+};
+
+// Nodes of the d-cube in Gridap's face order (see file header).
+void cube_lattice(int d, int k, bool pspace, std::vector<int>& lat) {
+  lat.clear();
+  if (k == 0 || d == 0) {
+    lat.assign(3, 0);
+    return;
+  }
+  for (int fd = 0; fd <= d; ++fd) {
+    // subsets of size fd of {0..d-1}, lexicographic
+    std::vector<std::vector<int>> subsets;
+    std::vector<int> comb(fd);
+    for (int i = 0; i < fd; ++i) comb[i] = i;
+    while (true) {
+      subsets.push_back(comb);
+      int i = fd - 1;
+      while (i >= 0 && comb[i] == d - fd + i) --i;
+      if (i < 0) break;
+      ++comb[i];
+      for (int j = i + 1; j < fd; ++j) comb[j] = comb[j - 1] + 1;
+    }
+    for (const auto& fr : subsets) {
+      bool isfree[3] = {false, false, false};
+      for (int a : fr) isfree[a] = true;
+      std::vector<int> fixed;
+      for (int a = 0; a < d; ++a)
+        if (!isfree[a]) fixed.push_back(a);
+      int nfix = (int)fixed.size();
+      int nown = 1;
+      for (int i = 0; i < fd; ++i) nown *= (k - 1);
+      for (int fv = 0; fv < (1 << nfix); ++fv)
+        for (int o = 0; o < nown; ++o) {
+          int a[3] = {0, 0, 0}, t = o;
+          for (int i = 0; i < fd; ++i) {
+            a[fr[i]] = 1 + t % (k - 1);
+            t /= (k - 1);
+          }
+          for (int i = 0; i < nfix; ++i) a[fixed[i]] = ((fv >> i) & 1) ? k : 0;
+          if (pspace && a[0] + a[1] + a[2] > k) continue;
+          lat.push_back(a[0]);
+          lat.push_back(a[1]);
+          lat.push_back(a[2]);
+        }
+    }
+  }
+}
+
+bool solve_dense(int n, std::vector<double>& A, std::vector<double>& B, int nrhs) {
+  // row-major A[n][n], B[n][nrhs]; Gaussian elimination with partial pivoting
+  for (int j = 0; j < n; ++j) {
+    int p = j;
+    for (int i = j + 1; i < n; ++i)
+      if (std::fabs(A[(size_t)i * n + j]) > std::fabs(A[(size_t)p * n + j])) p = i;
+    if (A[(size_t)p * n + j] == 0.0) return false;
+    if (p != j) {
+      for (int c = 0; c < n; ++c) std::swap(A[(size_t)j * n + c], A[(size_t)p * n + c]);
+      for (int c = 0; c < nrhs; ++c) std::swap(B[(size_t)j * nrhs + c], B[(size_t)p * nrhs + c]);
+    }
+    for (int i = j + 1; i < n; ++i) {
+      double l = A[(size_t)i * n + j] / A[(size_t)j * n + j];
+      if (l == 0.0) continue;
+      for (int c = j; c < n; ++c) A[(size_t)i * n + c] -= l * A[(size_t)j * n + c];
+      for (int c = 0; c < nrhs; ++c) B[(size_t)i * nrhs + c] -= l * B[(size_t)j * nrhs + c];
+    }
+  }
+  for (int j = n - 1; j >= 0; --j)
+    for (int c = 0; c < nrhs; ++c) {
+      double s = B[(size_t)j * nrhs + c];
+      for (int m = j + 1; m < n; ++m) s -= A[(size_t)j * n + m] * B[(size_t)m * nrhs + c];
+      B[(size_t)j * nrhs + c] = s / A[(size_t)j * n + j];
+    }
+  return true;
+}
+
+bool make_basis(int d, int k, bool pspace, Basis& b) {
+  b.d = d;
+  b.k = k;
+  b.pspace = pspace && k > 0;
+  cube_lattice(d, k, pspace, b.lat);
+  b.n = (int)b.lat.size() / 3;
+  if (!b.pspace) return true;
+  // monomials of total degree <= k
+  b.mono.clear();
+  for (int e2 = 0; e2 <= (d > 2 ? k : 0); ++e2)
+    for (int e1 = 0; e1 <= (d > 1 ? k : 0); ++e1)
+      for (int e0 = 0; e0 <= k; ++e0)
+        if (e0 + e1 + e2 <= k) {
+          b.mono.push_back(e0);
+          b.mono.push_back(e1);
+          b.mono.push_back(e2);
+        }
+  int n = b.n;
+  if ((int)b.mono.size() / 3 != n) return false;
+  // V[node][mono]; coefficients: V^T C^T = I  -> solve V X = I, coef = X^T
+  std::vector<double> V((size_t)n * n), X((size_t)n * n, 0.0);
+  for (int i = 0; i < n; ++i) {
+    for (int m = 0; m < n; ++m) {
+      double v = 1.0;
+      for (int a = 0; a < d; ++a) v *= Basis::ipow(b.node(i, a) - 0.5, b.mono[3 * m + a]);
+      V[(size_t)i * n + m] = v;
+    }
+    X[(size_t)i * n + i] = 1.0;
+  }
+  if (!solve_dense(n, V, X, n)) return false;
+  // X = V^{-1}: phi_i(x) = sum_m X[m][i] mono_m(x)
+  b.coef.assign((size_t)n * n, 0.0);
+  for (int i = 0; i < n; ++i)
+    for (int m = 0; m < n; ++m) b.coef[(size_t)i * n + m] = X[(size_t)m * n + i];
+  return true;
+}
+
+// facet f of the reference D-cube: fixes direction D-1-f/2 at f%2
+void facet_point(int D, int f, const double* s, double* xi) {
+  int fix = D - 1 - f / 2, j = 0;
+  for (int a = 0; a < D; ++a) xi[a] = (a == fix) ? double(f % 2) : s[j++];
+  for (int a = D; a < 3; ++a) xi[a] = 0.0;
+}
+
+}  // namespace
+
+int build_tables(int phys, int D, int space, int kp, int kq, int km, int quad_degree,
+                 TablesHost& t, std::string& err) {
+  if (D != 2 && D != 3) {
+    err = "dimension must be 2 or 3";
+    return -1;
+  }
+  bool ps = space == 1;
+  Basis bp, bq, bm;
+  if (!make_basis(D, kp, ps, bp) || !make_basis(D, kq, ps, bq) || !make_basis(D - 1, km, ps, bm)) {
+    err = "could not build reference bases";
+    return -1;
+  }
+  t = TablesHost();
+  t.phys = phys;
+  t.D = D;
+  t.nF = 2 * D;
+  t.np = bp.n;
+  t.nq = bq.n;
+  t.nm = bm.n;
+  if (phys == kDarcy) {
+    t.ncp = D;
+    t.ncq = 1;
+    t.ncm = 1;
+  } else {
+    t.ncp = D * (D + 1) / 2;
+    t.ncq = D;
+    t.ncm = D;
+  }
+  t.na = t.ncp * t.np;
+  t.nbf = t.ncq * t.nq;
+  t.ntf = t.ncm * t.nm;
+  t.ni = t.na + t.nbf;
+  t.nb = t.nF * t.ntf;
+  t.n = t.ni + t.nb;
+
+  int m1 = (quad_degree + 2) / 2;  // ceil((degree+1)/2) points per direction
+  std::vector<double> x1, w1;
+  gauss_legendre01(m1, x1, w1);
+  int nqc = 1, nqf = 1;
+  for (int a = 0; a < D; ++a) nqc *= m1;
+  for (int a = 0; a < D - 1; ++a) nqf *= m1;
+  t.nqc = nqc;
+
+  const int np = t.np, nq = t.nq, nm = t.nm;
+  t.Mpp.assign((size_t)np * np, 0.0);
+  t.Gpq.assign((size_t)D * np * nq, 0.0);
+  t.Gqp.assign((size_t)D * nq * np, 0.0);
+  t.Bp.assign((size_t)nqc * np, 0.0);
+  t.Bq.assign((size_t)nqc * nq, 0.0);
+  t.xq.assign((size_t)nqc * 3, 0.0);
+  std::vector<double> vp(np), gp(3 * np), vq(nq), gq(3 * nq), vm(nm);
+  for (int q = 0; q < nqc; ++q) {
+    double xi[3] = {0, 0, 0}, w = 1.0;
+    int r = q;
+    for (int a = 0; a < D; ++a) {
+      xi[a] = x1[r % m1];
+      w *= w1[r % m1];
+      r /= m1;
+    }
+    for (int a = 0; a < 3; ++a) t.xq[3 * q + a] = xi[a];
+    bp.eval(xi, vp.data(), gp.data());
+    bq.eval(xi, vq.data(), gq.data());
+    for (int i = 0; i < np; ++i) {
+      t.Bp[(size_t)q * np + i] = vp[i] * w;
+      for (int j = 0; j < np; ++j) t.Mpp[(size_t)i * np + j] += vp[i] * vp[j] * w;
+      for (int e = 0; e < D; ++e)
+        for (int j = 0; j < nq; ++j) t.Gpq[((size_t)e * np + i) * nq + j] += gp[3 * i + e] * vq[j] * w;
+    }
+    for (int i = 0; i < nq; ++i) {
+      t.Bq[(size_t)q * nq + i] = vq[i] * w;
+      for (int e = 0; e < D; ++e)
+        for (int j = 0; j < np; ++j) t.Gqp[((size_t)e * nq + i) * np + j] += gq[3 * i + e] * vp[j] * w;
+    }
+  }
+  const int nF = t.nF;
+  t.Fqp.assign((size_t)nF * nq * np, 0.0);
+  t.Fqq.assign((size_t)nF * nq * nq, 0.0);
+  t.Fpm.assign((size_t)nF * np * nm, 0.0);
+  t.Fqm.assign((size_t)nF * nq * nm, 0.0);
+  t.Fmm.assign((size_t)nm * nm, 0.0);
+  t.pmask.assign(np, 0);
+  t.qmask.assign(nq, 0);
+  std::vector<double> pp2((size_t)nF * np, 0.0), qq2((size_t)nF * nq, 0.0);
+  for (int f = 0; f < nF; ++f)
+    for (int q = 0; q < nqf; ++q) {
+      double s[3] = {0, 0, 0}, w = 1.0;
+      int r = q;
+      for (int a = 0; a < D - 1; ++a) {
+        s[a] = x1[r % m1];
+        w *= w1[r % m1];
+        r /= m1;
+      }
+      double xi[3];
+      facet_point(D, f, s, xi);
+      bp.eval(xi, vp.data(), nullptr);
+      bq.eval(xi, vq.data(), nullptr);
+      bm.eval(s, vm.data(), nullptr);
+      for (int i = 0; i < nq; ++i) {
+        qq2[(size_t)f * nq + i] += vq[i] * vq[i] * w;
+        for (int j = 0; j < np; ++j) t.Fqp[((size_t)f * nq + i) * np + j] += vq[i] * vp[j] * w;
+        for (int j = 0; j < nq; ++j) t.Fqq[((size_t)f * nq + i) * nq + j] += vq[i] * vq[j] * w;
+        for (int j = 0; j < nm; ++j) t.Fqm[((size_t)f * nq + i) * nm + j] += vq[i] * vm[j] * w;
+      }
+      for (int i = 0; i < np; ++i) {
+        pp2[(size_t)f * np + i] += vp[i] * vp[i] * w;
+        for (int j = 0; j < nm; ++j) t.Fpm[((size_t)f * np + i) * nm + j] += vp[i] * vm[j] * w;
+      }
+      if (f == 0)
+        for (int i = 0; i < nm; ++i)
+          for (int j = 0; j < nm; ++j) t.Fmm[(size_t)i * nm + j] += vm[i] * vm[j] * w;
+    }
+  // support masks: bit f set when the trace of the function on facet f is not
+  // identically zero (its facet L2 norm is positive)
+  for (int f = 0; f < nF; ++f) {
+    for (int i = 0; i < np; ++i)
+      if (pp2[(size_t)f * np + i] > 1e-300) t.pmask[i] |= (uint8_t)(1u << f);
+    for (int i = 0; i < nq; ++i)
+      if (qq2[(size_t)f * nq + i] > 1e-300) t.qmask[i] |= (uint8_t)(1u << f);
+  }
+  t.trace_nodes.assign((size_t)nm * (D - 1 > 0 ? D - 1 : 1), 0.0);
+  for (int i = 0; i < nm; ++i)
+    for (int a = 0; a < D - 1; ++a) t.trace_nodes[(size_t)i * (D - 1) + a] = bm.node(i, a);
+  return 0;
+}
+
+}  // namespace hdg

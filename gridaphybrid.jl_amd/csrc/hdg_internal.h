@@ -1,0 +1,137 @@
+// hdg_internal.h -- structures shared by the host runtime (hdg_api.cpp,
+// hdg_fe.cpp) and the gfx950 kernels (hdg_kernels.hip).
+#pragma once
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+namespace hdg {
+
+constexpr int kMaxD = 3;
+constexpr int kMaxFacets = 6;
+
+enum Physics { kDarcy = 0, kElasticity = 1 };
+
+// Reference tables of one (physics, D, space, orders) class.  Scalar bases:
+//   p = flux (Darcy) / stress (elasticity) scalar basis, np functions
+//   q = potential (Darcy) / displacement (elasticity) scalar basis, nq
+//   m = trace scalar basis on the reference facet, nm
+// Exact Gauss integrals on the reference cell [0,1]^D / reference facets;
+// every matrix is stored row-major [row][col].
+struct TablesHost {
+  int phys = 0, D = 2, nF = 4, np = 0, nq = 0, nm = 0;
+  int ncp = 0, ncq = 0, ncm = 0;  // field components (Darcy: D,1,1; elasticity: nsym,D,D)
+  int na = 0, nbf = 0, ntf = 0, ni = 0, nb = 0, n = 0;
+  int nqc = 0;                    // cell quadrature points
+  std::vector<double> Mpp;        // [np][np]
+  std::vector<double> Gpq;        // [D][np][nq]  int d_e p_i q_j
+  std::vector<double> Gqp;        // [D][nq][np]  int d_e q_i p_j
+  std::vector<double> Fqp;        // [nF][nq][np] int_F q_i p_j
+  std::vector<double> Fqq;        // [nF][nq][nq]
+  std::vector<double> Fpm;        // [nF][np][nm]
+  std::vector<double> Fqm;        // [nF][nq][nm]
+  std::vector<double> Fmm;        // [nm][nm]
+  std::vector<uint8_t> pmask;     // [np] bit F: p_i has a nonzero trace on facet F
+  std::vector<uint8_t> qmask;     // [nq]
+  std::vector<double> Bp;         // [nqc][np] p_i(xi_q) * w_q
+  std::vector<double> Bq;         // [nqc][nq]
+  std::vector<double> xq;         // [nqc][3]  reference quadrature points
+  std::vector<double> trace_nodes;  // [nm][D-1] trace nodes on the reference facet
+};
+
+// Device view of the tables (pointers into one device allocation).
+struct Tables {
+  int phys, D, nF, np, nq, nm;
+  int ncp, ncq, ncm;
+  int na, nbf, ntf, ni, nb, n;
+  int nqc;
+  const double* Mpp;
+  const double* Gpq;
+  const double* Gqp;
+  const double* Fqp;
+  const double* Fqq;
+  const double* Fpm;
+  const double* Fqm;
+  const double* Fmm;
+  const uint8_t* pmask;
+  const uint8_t* qmask;
+  const double* Bp;
+  const double* Bq;
+  const double* xq;
+};
+
+// Device view of a mesh for the fused element generator.
+struct MeshDev {
+  int64_t ncells;
+  const double* x0;        // [ncells][D]
+  const double* jac;       // [ncells][D*D] column-major
+  const int8_t* sign;      // [ncells][2D]  n . n_o
+  const double* coef;      // [ncells] kappa or nu
+  const double* src;       // [ncells][nqc][ncomp] or null
+  int source;              // hdg_source
+  double tau;
+  double young;
+};
+
+// Outputs of the condensation kernels.
+struct CondenseOut {
+  double* S;        // [ncells][nb][nb] column-major
+  double* g;        // [ncells][nb]
+  double* W;        // [ncells][nb+1][ni] column-major: A_ii^-1 [A_ib b_i]  (may be null)
+  float* W32;       // mixed path alternative (unused when null)
+  int32_t* info;    // [ncells]
+};
+
+// Dense cell systems in HBM for the map-level API.
+struct DenseIn {
+  const double* A;       // [ncells][n][n] column-major, natural field order
+  const double* b;       // [ncells][n]
+  const int32_t* perm;   // [n] condensed index -> natural index (device)
+  int n;
+  int ni, nb;
+};
+
+
+// Facet-pair structure of the skeleton matrix (see the assembly kernels).
+struct PairTable {
+  const int64_t* fptr;     // [nfree_facets+1] first nnz of facet G's columns
+  const int32_t* pptr;     // [nfree_facets+1] first pair of facet G
+  const int32_t* pfacet;   // [npairs] free facet id of F
+  const int32_t* pcell;    // [npairs][2] contributing cells (-1 if absent)
+  const uint8_t* plpos;    // [npairs][2] (lpos_G | lpos_F << 4)
+  const int32_t* gcell;    // [nfree_facets][2] cells around G (-1 if absent)
+  const uint8_t* glpos;    // [nfree_facets][2] local position of G in them
+  int64_t nff;             // free facets
+  int ntf, nb;
+};
+
+}  // namespace hdg
+
+#include <hip/hip_runtime_api.h>
+namespace hdg {
+// Kernel launchers (hdg_kernels.hip)
+bool condense_supported(int n);
+hipError_t launch_condense_fused(const Tables& T, const MeshDev& M, const CondenseOut& out,
+                                 hipStream_t st);
+hipError_t launch_condense_dense(const DenseIn& in, int64_t ncells, const CondenseOut& out,
+                                 hipStream_t st);
+hipError_t launch_backward_dense(const DenseIn& in, const double* x, int64_t ncells, double* xi,
+                                 int32_t* info, hipStream_t st);
+hipError_t launch_recover(const double* W, const int32_t* cell_dofs, const double* lam_free,
+                          const double* lam_dir, double* out, int64_t ncells, int ni, int nb,
+                          int na, int64_t nfree, hipStream_t st);
+hipError_t launch_symbolic(const PairTable& P, int32_t* rowval, hipStream_t st);
+hipError_t launch_assemble(const PairTable& P, const double* S, const double* g,
+                           const int32_t* cell_dofs, const double* lam_dir, double* nzval,
+                           double* rhs, hipStream_t st);
+hipError_t launch_cell_systems(const Tables& T, const MeshDev& M, int64_t first, int64_t count,
+                               double* A, double* b, hipStream_t st);
+hipError_t launch_count_nonzero(const int32_t* info, int64_t n, int64_t* count_dev,
+                                hipStream_t st);
+
+// Host helpers (hdg_fe.cpp)
+int build_tables(int phys, int D, int space, int kp, int kq, int km, int quad_degree,
+                 TablesHost& t, std::string& err);
+
+}  // namespace hdg

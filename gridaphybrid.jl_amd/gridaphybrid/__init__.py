@@ -1,0 +1,425 @@
+"""GridapHybrid's hybridization API on MI355X.
+
+Host-side mirror of the reference interface for the per-cell hybrid path:
+
+* ``StaticCondensationMap(interior_fields, boundary_fields)`` and
+  ``BackwardStaticCondensationMap(...)`` -- the reference's Gridap maps
+  (src/StaticCondensationMap.jl, src/BackwardStaticCondensationMap.jl),
+  evaluated for a whole batch of cells in one GPU launch;
+* ``HybridAffineFEOperator(weakform, trial, test, bulk_fields, skeleton_fields)``
+  and ``solve(op)`` (src/HybridAffineFEOperators.jl:10-100): fused cell
+  integration + condensation + deterministic skeleton assembly on the GPU,
+  the skeleton solve on the host (sparse direct), recovery on the GPU.
+
+Field indices are 1-based, as in the reference's Julia calls
+(``StaticCondensationMap([1,2],[3])``).  All numerics run in libhdg (HIP,
+gfx950); nothing here computes on the CPU except the host sparse skeleton
+solve, which the reference also keeps on the host.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from ._lib import (DARCY, ELASTICITY, FP64, MIXED, SPACE_P, SPACE_Q, HDGError,  # noqa: F401
+                   SingularError, check, lib)
+
+__all__ = [
+    "Context", "StaticCondensationMap", "BackwardStaticCondensationMap",
+    "CartesianDiscreteModel", "HDGSpaces", "DarcyHDG", "ElasticityHDG",
+    "HybridAffineFEOperator", "solve", "HDGError", "SingularError", "FP64", "MIXED",
+]
+
+_torch = None
+
+
+def _t():
+    global _torch
+    if _torch is None:
+        import torch
+        _torch = torch
+    return _torch
+
+
+# ----------------------------------------------------------------------------
+# Context
+# ----------------------------------------------------------------------------
+class Context:
+    """One libhdg context (device + stream).  Torch work that feeds or reads
+    the library should run inside ``with ctx.stream():`` so it is ordered
+    with the library's kernels."""
+
+    _default = None
+
+    def __init__(self, device=0):
+        torch = _t()
+        if not torch.cuda.is_available():
+            raise HDGError(_lib.HDG_ERR_NO_DEVICE, "no HIP device visible")
+        h = C.c_void_p()
+        check(lib().hdg_ctx_create(int(device), C.byref(h)), None, "hdg_ctx_create")
+        self.h = h
+        self.device = int(device)
+        self._stream = torch.cuda.ExternalStream(lib().hdg_ctx_stream(h),
+                                                 device=torch.device("cuda", self.device))
+
+    @classmethod
+    def default(cls):
+        if cls._default is None:
+            cls._default = Context(0)
+        return cls._default
+
+    def stream(self):
+        return _t().cuda.stream(self._stream)
+
+    @property
+    def torch_stream(self):
+        return self._stream
+
+    def synchronize(self):
+        check(lib().hdg_ctx_synchronize(self.h), self.h, "synchronize")
+
+    def close(self):
+        if self.h:
+            lib().hdg_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _dev(x, dtype):
+    """Contiguous device tensor of the given dtype."""
+    torch = _t()
+    return torch.as_tensor(x, dtype=dtype, device="cuda").contiguous()
+
+
+def _colmajor(A):
+    """(ncells, n, m) logical -> device buffer laid out [cell][col][row]."""
+    return A.transpose(1, 2).contiguous()
+
+
+def _fields(field_sizes, interior_fields, boundary_fields):
+    fs = (C.c_int32 * len(field_sizes))(*[int(v) for v in field_sizes])
+    it = (C.c_int32 * max(len(interior_fields), 1))(*[int(v) - 1 for v in interior_fields])
+    bd = (C.c_int32 * max(len(boundary_fields), 1))(*[int(v) - 1 for v in boundary_fields])
+    f = _lib.Fields(len(field_sizes), fs, len(interior_fields), it, len(boundary_fields), bd)
+    return f, (fs, it, bd)
+
+
+def _check_preconditions(interior_fields, boundary_fields):
+    """src/StaticCondensationMap.jl:16-34 (1-based fields)."""
+    nf = len(interior_fields) + len(boundary_fields)
+    seen = set()
+    for f in list(interior_fields) + list(boundary_fields):
+        if not (1 <= f <= nf) or f in seen:
+            return False
+        seen.add(f)
+    return len(seen) == nf
+
+
+# ----------------------------------------------------------------------------
+# Map-level API
+# ----------------------------------------------------------------------------
+class StaticCondensationMap:
+    """Batched StaticCondensationMap (src/StaticCondensationMap.jl:3-196).
+
+    ``evaluate(A, b, field_sizes)``: A (ncells, n, n) and b (ncells, n) float64
+    with the fields in their natural order; returns (S, g) with
+    S = A22 - A21 A11^-1 A12 and g = b2 - A21 A11^-1 b1 per cell."""
+
+    def __init__(self, interior_fields, boundary_fields, ctx=None):
+        if not _check_preconditions(interior_fields, boundary_fields):
+            raise HDGError(_lib.HDG_ERR_ARG, "interior/boundary fields must partition 1:nfields")
+        self.interior_fields = list(interior_fields)
+        self.boundary_fields = list(boundary_fields)
+        self.ctx = ctx
+
+    def _sizes(self, field_sizes):
+        ni = sum(field_sizes[f - 1] for f in self.interior_fields)
+        nb = sum(field_sizes[f - 1] for f in self.boundary_fields)
+        return ni, nb
+
+    def evaluate(self, A, b, field_sizes, return_info=False):
+        torch = _t()
+        ctx = self.ctx or Context.default()
+        ni, nb = self._sizes(field_sizes)
+        with ctx.stream():
+            Ad = _colmajor(_dev(A, torch.float64))
+            bd = _dev(b, torch.float64)
+            nc = Ad.shape[0]
+            S = torch.empty((nc, nb, nb), dtype=torch.float64, device="cuda")
+            g = torch.empty((nc, nb), dtype=torch.float64, device="cuda")
+            info = torch.empty((nc,), dtype=torch.int32, device="cuda")
+        f, keep = _fields(field_sizes, self.interior_fields, self.boundary_fields)
+        rc = lib().hdg_static_condensation(ctx.h, C.byref(f), nc, Ad.data_ptr(), bd.data_ptr(),
+                                           S.data_ptr(), g.data_ptr(), info.data_ptr())
+        ctx.synchronize()
+        if rc != _lib.HDG_SINGULAR or not return_info:
+            check(rc, ctx.h, "StaticCondensationMap")
+        S = S.transpose(1, 2)  # stored column-major per cell
+        return (S, g, info) if return_info else (S, g)
+
+
+class BackwardStaticCondensationMap:
+    """Batched BackwardStaticCondensationMap
+    (src/BackwardStaticCondensationMap.jl:6-151).  ``evaluate(A, b, x,
+    field_sizes)`` returns the per-field blocks [interior fields..., x] as the
+    reference's ReblockInteriorDofsMap does."""
+
+    def __init__(self, interior_fields, boundary_fields, ctx=None):
+        self.sc = StaticCondensationMap(interior_fields, boundary_fields, ctx)
+
+    def evaluate(self, A, b, x, field_sizes):
+        torch = _t()
+        ctx = self.sc.ctx or Context.default()
+        ni, nb = self.sc._sizes(field_sizes)
+        with ctx.stream():
+            Ad = _colmajor(_dev(A, torch.float64))
+            bd = _dev(b, torch.float64)
+            xd = _dev(x, torch.float64)
+            nc = Ad.shape[0]
+            xi = torch.empty((nc, ni), dtype=torch.float64, device="cuda")
+            info = torch.empty((nc,), dtype=torch.int32, device="cuda")
+        f, keep = _fields(field_sizes, self.sc.interior_fields, self.sc.boundary_fields)
+        rc = lib().hdg_backward_static_condensation(ctx.h, C.byref(f), nc, Ad.data_ptr(),
+                                                    bd.data_ptr(), xd.data_ptr(), xi.data_ptr(),
+                                                    info.data_ptr())
+        ctx.synchronize()
+        check(rc, ctx.h, "BackwardStaticCondensationMap")
+        out, off = [], 0
+        for fld in self.sc.interior_fields:
+            out.append(xi[:, off:off + field_sizes[fld - 1]])
+            off += field_sizes[fld - 1]
+        out.append(xd)
+        return out
+
+
+# ----------------------------------------------------------------------------
+# Operator-level API
+# ----------------------------------------------------------------------------
+@dataclass
+class CartesianDiscreteModel:
+    """CartesianDiscreteModel(domain, partition): cells lexicographic, x
+    fastest; Dirichlet trace on every boundary facet."""
+    domain: tuple
+    partition: tuple
+
+    @property
+    def D(self):
+        return len(self.partition)
+
+    @property
+    def num_cells(self):
+        return int(np.prod(self.partition))
+
+
+@dataclass
+class HDGSpaces:
+    """The trial/test MultiFieldFESpace [flux, potential, trace] (Darcy) or
+    [stress, displacement, trace] (elasticity) on a model."""
+    model: CartesianDiscreteModel
+    k_flux: int
+    k_pot: int
+    k_trace: int
+    space: int = SPACE_Q
+
+    @staticmethod
+    def darcy_baseline(model, order):
+        """Q_k flux, Q_k potential, Q_k trace."""
+        return HDGSpaces(model, order, order, order, SPACE_Q)
+
+    @staticmethod
+    def darcy_reference_test(model):
+        """test/DarcyHDGTests.jl:34-37: P1 flux, P0 potential, P1 trace."""
+        return HDGSpaces(model, 1, 0, 1, SPACE_P)
+
+    @staticmethod
+    def elasticity(model, order):
+        """test/LinearElasticityHDGTests.jl:88-94: P_k stress, P_{k+1} u, P_k trace."""
+        return HDGSpaces(model, order, order + 1, order, SPACE_P)
+
+
+@dataclass
+class DarcyHDG:
+    """Darcy HDG weak form of test/DarcyHDGTests.jl:125-135, with kappa^-1 on
+    the mass term.  source: a built-in manufactured solution id
+    (_lib.SOURCE_*) or per-cell source values at quadrature points."""
+    tau: float = 1.0
+    kappa: object = 1.0
+    source: int = _lib.SOURCE_DARCY_LINEAR
+    source_qp: object = None
+    quad_degree: int = None
+    physics: int = DARCY
+
+
+@dataclass
+class ElasticityHDG:
+    """Elasticity HDG weak form of test/LinearElasticityHDGTests.jl:140-150
+    (no bulk-to-skeleton projection).  nu may vary per cell."""
+    tau: float = 1.0
+    nu: object = 0.3
+    E: float = 1.0
+    source: int = _lib.SOURCE_ELAS_SIN
+    source_qp: object = None
+    quad_degree: int = None
+    physics: int = ELASTICITY
+
+
+class HybridAffineFEOperator:
+    """HybridAffineFEOperator(weakform, trial, test, bulk_fields,
+    skeleton_fields) -- src/HybridAffineFEOperators.jl:10-50.
+
+    Construction runs the fused cell integration + StaticCondensationMap for
+    every cell, the Dirichlet lifting and the deterministic skeleton assembly
+    on the GPU; the condensed system and the resident local solves stay on the
+    device until ``solve``."""
+
+    def __init__(self, weakform, trial, test, bulk_fields, skeleton_fields, precision=FP64,
+                 ctx=None):
+        if list(bulk_fields) != [1, 2] or list(skeleton_fields) != [3]:
+            raise HDGError(_lib.HDG_ERR_UNSUPPORTED,
+                           "hybrid operators here hybridize fields [1,2] onto trace field [3]")
+        if trial is not test and trial != test:
+            raise HDGError(_lib.HDG_ERR_ARG, "trial and test spaces must match")
+        self.weakform, self.trial, self.test = weakform, trial, test
+        self.bulk_fields, self.skeleton_fields = list(bulk_fields), list(skeleton_fields)
+        self.precision = precision
+        self.ctx = ctx or Context.default()
+        self.mesh = _create_mesh(self.ctx, weakform, trial)
+        self.info = _lib.MeshInfo()
+        check(lib().hdg_mesh_get_info(self.mesh, C.byref(self.info)), self.ctx.h, "info")
+        self._src_keep = None
+        if getattr(weakform, "source_qp", None) is not None:
+            torch = _t()
+            with self.ctx.stream():
+                self._src_keep = _dev(weakform.source_qp, torch.float64)
+            check(lib().hdg_mesh_set_source(self.mesh, self._src_keep.data_ptr()), self.ctx.h,
+                  "set_source")
+        check(lib().hdg_mesh_set_dirichlet_mms(self.mesh), self.ctx.h, "dirichlet")
+        self.assemble()
+
+    def assemble(self):
+        nbad = C.c_int64(0)
+        rc = lib().hdg_condense_assemble(self.mesh, int(self.precision), C.byref(nbad))
+        check(rc, self.ctx.h, "HybridAffineFEOperator")
+
+    def skeleton_system(self):
+        """Host scipy CSC matrix and rhs of the skeleton system."""
+        import scipy.sparse as sp
+        torch = _t()
+        cp, rv, nz, rh = C.c_void_p(), C.c_void_p(), C.c_void_p(), C.c_void_p()
+        check(lib().hdg_skeleton_system(self.mesh, C.byref(cp), C.byref(rv), C.byref(nz),
+                                        C.byref(rh)), self.ctx.h, "skeleton_system")
+        nfree, nnz = self.info.nfree, self.info.nnz
+        colptr = np.empty(nfree + 1, np.int64)
+        rowval = np.empty(nnz, np.int32)
+        nzval = np.empty(nnz, np.float64)
+        rhs = np.empty(nfree, np.float64)
+        L = lib()
+        for host, dev in ((colptr, cp), (rowval, rv), (nzval, nz), (rhs, rh)):
+            check(L.hdg_memcpy_d2h(self.ctx.h, host.ctypes.data, dev, host.nbytes), self.ctx.h,
+                  "d2h")
+        _ = torch
+        return sp.csc_matrix((nzval, rowval, colptr), shape=(nfree, nfree)), rhs
+
+    def cell_blocks(self):
+        """(S, g, cell_dofs) device tensors: S (ncells, nb, nb)."""
+        torch = _t()
+        S, g, d = C.c_void_p(), C.c_void_p(), C.c_void_p()
+        check(lib().hdg_cell_blocks(self.mesh, C.byref(S), C.byref(g), C.byref(d)), self.ctx.h,
+              "cell_blocks")
+        nc, nb = self.info.ncells, self.info.nb
+        Sh = np.empty((nc, nb, nb)), np.empty((nc, nb)), np.empty((nc, nb), np.int32)
+        for host, dev in zip(Sh, (S, g, d)):
+            check(lib().hdg_memcpy_d2h(self.ctx.h, host.ctypes.data, dev, host.nbytes),
+                  self.ctx.h, "d2h")
+        _ = torch
+        return Sh[0].transpose(0, 2, 1), Sh[1], Sh[2]
+
+    def recover(self, lam_free):
+        """BackwardStaticCondensationMap over all cells + scatter
+        (src/HybridAffineFEOperators.jl:102-150): free values [q; u; lambda]."""
+        torch = _t()
+        with self.ctx.stream():
+            lam = _dev(lam_free, torch.float64)
+            out = torch.empty((self.info.nfree_total,), dtype=torch.float64, device="cuda")
+        check(lib().hdg_recover(self.mesh, lam.data_ptr() if lam.numel() else None,
+                                out.data_ptr()), self.ctx.h, "recover")
+        self.ctx.synchronize()
+        return out
+
+    def close(self):
+        if getattr(self, "mesh", None):
+            lib().hdg_mesh_destroy(self.mesh)
+            self.mesh = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+@dataclass
+class HybridSolution:
+    """The MultiFieldFEFunction's free values and its field blocks."""
+    free_values: np.ndarray
+    ncells: int
+    n_flux: int
+    n_pot: int
+    lam: np.ndarray
+    extra: dict = field(default_factory=dict)
+
+    @property
+    def flux(self):
+        return self.free_values[: self.ncells * self.n_flux].reshape(self.ncells, self.n_flux)
+
+    @property
+    def potential(self):
+        a = self.ncells * self.n_flux
+        return self.free_values[a: a + self.ncells * self.n_pot].reshape(self.ncells, self.n_pot)
+
+
+def solve(op: HybridAffineFEOperator):
+    """solve(op) (src/HybridAffineFEOperators.jl:67-100): host sparse direct
+    solve of the skeleton system, then GPU recovery."""
+    import scipy.sparse.linalg as spla
+    K, rhs = op.skeleton_system()
+    lam = spla.spsolve(K.tocsc(), rhs) if op.info.nfree else np.zeros(0)
+    out = op.recover(lam).cpu().numpy()
+    return HybridSolution(out, op.info.ncells, op.info.n_flux, op.info.n_pot, lam)
+
+
+def _problem(weakform, spaces):
+    k = max(spaces.k_flux, spaces.k_pot, spaces.k_trace)
+    qd = weakform.quad_degree if weakform.quad_degree is not None else 2 * (spaces.k_flux + 1)
+    if weakform.physics == DARCY:
+        young = 1.0
+    else:
+        young = weakform.E
+    _ = k
+    return _lib.Problem(int(weakform.physics), spaces.model.D, int(spaces.space),
+                        int(spaces.k_flux), int(spaces.k_pot), int(spaces.k_trace), int(qd),
+                        float(weakform.tau), float(young),
+                        _lib.SOURCE_QP if weakform.source_qp is not None else int(weakform.source))
+
+
+def _create_mesh(ctx, weakform, spaces):
+    model = spaces.model
+    D = model.D
+    nc = model.num_cells
+    coef = weakform.kappa if weakform.physics == DARCY else weakform.nu
+    coef = np.ascontiguousarray(np.broadcast_to(np.asarray(coef, np.float64), (nc,)))
+    prob = _problem(weakform, spaces)
+    cells = (C.c_int64 * 3)(*(list(model.partition) + [1] * (3 - D)))
+    dom = (C.c_double * 6)(*(list(model.domain) + [0.0] * (6 - 2 * D)))
+    h = C.c_void_p()
+    check(lib().hdg_mesh_create_cartesian(ctx.h, C.byref(prob), cells, dom, coef.ctypes.data,
+                                          C.byref(h)), ctx.h, "mesh_create")
+    return h

@@ -1,0 +1,214 @@
+/*
+ * hdg.h -- C ABI of libhdg, the MI355X (gfx950) engine for GridapHybrid's
+ * per-cell hybrid pipeline: cell+facet integration of the HDG local systems,
+ * static condensation, deterministic skeleton assembly and backward recovery.
+ *
+ * Conventions
+ *   - plain pointers and sizes; "device" pointers are HIP device memory owned
+ *     by the caller, "host" pointers are ordinary host memory;
+ *   - per-cell matrices are column-major and cell-major ([cell][col][row]),
+ *     the layout of Julia's Matrix{Float64};
+ *   - ids are 0-based; trace dof ids keep Gridap's convention of negative ids
+ *     (-1,-2,...) for Dirichlet dofs;
+ *   - every function returns an hdg_status; hdg_last_error() describes the
+ *     last failure of a context;
+ *   - a context is not thread-safe (one per host thread, as Gridap caches).
+ */
+#ifndef HDG_H
+#define HDG_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  HDG_OK = 0,
+  HDG_ERR_ARG = -1,         /* invalid argument / precondition (Gridap @check) */
+  HDG_ERR_HIP = -2,         /* HIP runtime error */
+  HDG_ERR_ALLOC = -3,       /* device allocation failed */
+  HDG_ERR_UNSUPPORTED = -4, /* size/physics combination not built */
+  HDG_ERR_STATE = -5,       /* call order violated (e.g. recover before condense) */
+  HDG_ERR_NO_DEVICE = -6,   /* no HIP device / library not usable */
+  HDG_SINGULAR = 1          /* a local A_ii had a zero pivot (getrf info > 0) */
+} hdg_status;
+
+typedef enum { HDG_FP64 = 0, HDG_MIXED = 1 } hdg_precision;
+typedef enum { HDG_DARCY = 0, HDG_ELASTICITY = 1 } hdg_physics;
+typedef enum { HDG_SPACE_Q = 0, HDG_SPACE_P = 1 } hdg_space;
+
+/* Built-in sources (manufactured solutions).  HDG_SOURCE_QP means the source
+ * values at cell quadrature points are supplied with hdg_mesh_set_source. */
+typedef enum {
+  HDG_SOURCE_ZERO = 0,
+  HDG_SOURCE_DARCY_KAT = 1,     /* flux (1+x_a), potential -3.14 (test/DarcyHDGTests.jl:9-14) */
+  HDG_SOURCE_DARCY_LINEAR = 2,  /* potential sum x_a, flux -kappa grad */
+  HDG_SOURCE_DARCY_SINPROD = 3, /* potential prod sin(pi x_a), flux -kappa grad */
+  HDG_SOURCE_ELAS_LINEAR = 11,  /* linear displacement (exact in the spaces) */
+  HDG_SOURCE_ELAS_SIN = 12,     /* smooth trigonometric displacement */
+  HDG_SOURCE_QP = 100
+} hdg_source;
+
+typedef struct hdg_ctx hdg_ctx;
+typedef struct hdg_mesh hdg_mesh;
+
+/* ------------------------------------------------------------------------ */
+/* Context                                                                  */
+/* ------------------------------------------------------------------------ */
+/* Creates a context on HIP device `device` with its own (default-priority)
+ * stream.  Replaces nothing in the reference: Gridap's per-lazy-array caches
+ * (src/StaticCondensationMap.jl:41-70) are the analogue. */
+int hdg_ctx_create(int device, hdg_ctx** out);
+int hdg_ctx_destroy(hdg_ctx* ctx);
+const char* hdg_last_error(const hdg_ctx* ctx);
+/* The stream all work of the context is enqueued on (a hipStream_t). */
+void* hdg_ctx_stream(hdg_ctx* ctx);
+int hdg_ctx_synchronize(hdg_ctx* ctx);
+/* Copies between host and device on the context stream (synchronous). */
+int hdg_memcpy_d2h(hdg_ctx* ctx, void* host_dst, const void* dev_src, int64_t bytes);
+int hdg_memcpy_h2d(hdg_ctx* ctx, void* dev_dst, const void* host_src, int64_t bytes);
+
+/* ------------------------------------------------------------------------ */
+/* Map-level API: batched StaticCondensationMap / BackwardStaticCondensationMap */
+/* ------------------------------------------------------------------------ */
+/* Field partition shared by the two maps.  field_sizes[nfields] are the
+ * block sizes of the cell system (brs == bcs, src/StaticCondensationMap.jl:52);
+ * interior_fields / boundary_fields are 0-based and must partition
+ * 0..nfields-1 (_check_preconditions, src/StaticCondensationMap.jl:16-34). */
+typedef struct {
+  int nfields;
+  const int32_t* field_sizes;
+  int n_interior;
+  const int32_t* interior_fields;
+  int n_boundary;
+  const int32_t* boundary_fields;
+} hdg_fields;
+
+/* StaticCondensationMap.evaluate! over ncells cells
+ * (replaces src/StaticCondensationMap.jl:152-196 applied cell by cell by
+ * lazy_map, src/HybridAffineFEOperators.jl:333-341).
+ *   A    device [ncells][n][n] column-major, n = sum(field_sizes), fields in
+ *        their natural order (the map extracts interior/boundary blocks);
+ *   b    device [ncells][n];
+ *   S    device [ncells][nb][nb] column-major  (A22 - A21 A11^-1 A12);
+ *   g    device [ncells][nb]                   (b2 - A21 A11^-1 b1);
+ *   info device [ncells] int32 (LAPACK getrf info, 0 = ok).
+ * Returns HDG_SINGULAR if any cell had a zero pivot (the reference throws at
+ * the @check info==0 of :180). */
+int hdg_static_condensation(hdg_ctx* ctx, const hdg_fields* fields, int64_t ncells,
+                            const double* A, const double* b, double* S, double* g,
+                            int32_t* info);
+
+/* BackwardStaticCondensationMap.evaluate! over ncells cells
+ * (replaces src/BackwardStaticCondensationMap.jl:61-102):
+ *   x    device [ncells][nb] boundary (skeleton) values of each cell;
+ *   xi   device [ncells][ni] recovered interior values, interior fields
+ *        concatenated in the order of interior_fields (ReblockInteriorDofsMap
+ *        splits them, :133-151). */
+int hdg_backward_static_condensation(hdg_ctx* ctx, const hdg_fields* fields, int64_t ncells,
+                                     const double* A, const double* b, const double* x,
+                                     double* xi, int32_t* info);
+
+/* ------------------------------------------------------------------------ */
+/* Operator-level API: HybridAffineFEOperator + solve                        */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+  int physics;      /* hdg_physics */
+  int dim;          /* 2 or 3 */
+  int space;        /* hdg_space of the cell and trace spaces */
+  int k_flux;       /* Darcy flux order / elasticity stress order */
+  int k_pot;        /* Darcy potential order / elasticity displacement order */
+  int k_trace;      /* trace order */
+  int quad_degree;  /* Gauss degree of cell and facet rules (2(k+1) in the reference) */
+  double tau;       /* HDG stabilisation */
+  double young;     /* elasticity E */
+  int source;       /* hdg_source */
+} hdg_problem;
+
+/* Mesh of affine cells (parallelograms / parallelepipeds, which includes every
+ * CartesianDiscreteModel).  All arrays are host memory, copied at creation.
+ *   x0[ncells*dim]           cell origin (first vertex);
+ *   jac[ncells*dim*dim]      column-major Jacobian of the affine cell map;
+ *   cell_facets[ncells*2dim] facet ids in Gridap's local facet order;
+ *   facet_dirichlet[nfacets] 1 if the facet carries Dirichlet trace dofs;
+ *   coef[ncells]             kappa (Darcy) or Poisson ratio (elasticity).
+ * Owner signs n.n_o follow from cell_facets: the first (lowest id) cell
+ * around a facet owns it (src/Skeleton.jl:135-140). */
+typedef struct {
+  int64_t ncells;
+  int64_t nfacets;
+  const double* x0;
+  const double* jac;
+  const int32_t* cell_facets;
+  const int8_t* facet_dirichlet;
+  const double* coef;
+} hdg_mesh_desc;
+
+typedef struct {
+  int ni, nb;            /* local interior / skeleton sizes */
+  int n_flux, n_pot;     /* interior field sizes */
+  int n_trace_facet;     /* trace dofs per facet */
+  int nq_cell;           /* cell quadrature points */
+  int64_t ncells, nfacets;
+  int64_t nfree;         /* free trace dofs (skeleton system size) */
+  int64_t ndir;          /* Dirichlet trace dofs */
+  int64_t nnz;           /* nonzeros of the skeleton matrix */
+  int64_t nfree_total;   /* length of the recovered free-value vector */
+} hdg_mesh_info;
+
+/* Uploads the mesh, builds the trace dof map (Gridap numbering,
+ * src/HybridAffineFEOperators.jl:388-439) and the symbolic skeleton pattern
+ * (CSC, rows sorted per column), and keeps them resident. */
+int hdg_mesh_create(hdg_ctx* ctx, const hdg_problem* prob, const hdg_mesh_desc* desc,
+                    hdg_mesh** out);
+/* Convenience: CartesianDiscreteModel(domain, cells) topology and geometry
+ * built in the library (domain = [x0,x1,y0,y1(,z0,z1)]); Dirichlet on every
+ * boundary facet (tags 5:8 in 2D, all boundary faces in 3D). */
+int hdg_mesh_create_cartesian(hdg_ctx* ctx, const hdg_problem* prob, const int64_t* cells,
+                              const double* domain, const double* coef, hdg_mesh** out);
+int hdg_mesh_destroy(hdg_mesh* mesh);
+int hdg_mesh_get_info(const hdg_mesh* mesh, hdg_mesh_info* info);
+
+/* Dirichlet trace values, host [ndir] (Gridap's dirichlet_values of the trial
+ * skeleton space).  hdg_mesh_set_dirichlet_mms interpolates the built-in
+ * manufactured trace at the facet nodes instead. */
+int hdg_mesh_set_dirichlet(hdg_mesh* mesh, const double* lam_dir);
+int hdg_mesh_set_dirichlet_mms(hdg_mesh* mesh);
+/* Source values at cell quadrature points, device [ncells][nq_cell][ncomp]
+ * (Darcy: f_1..f_D, g; elasticity: f_1..f_D); used when source == HDG_SOURCE_QP.
+ * The pointer is borrowed until the next set or mesh destruction. */
+int hdg_mesh_set_source(hdg_mesh* mesh, const double* src_qp);
+
+/* HybridAffineFEOperator construction (src/HybridAffineFEOperators.jl:10-50):
+ * fused cell integration + StaticCondensationMap for every cell, Dirichlet
+ * lifting and deterministic assembly of the skeleton CSC matrix and rhs.
+ * Keeps W = A_ii^-1 [A_ib b_i] resident for recovery.  *n_singular (host,
+ * may be NULL) receives the number of cells with a zero pivot. */
+int hdg_condense_assemble(hdg_mesh* mesh, int precision, int64_t* n_singular);
+
+/* Device pointers of the resident skeleton system (valid until the next
+ * hdg_condense_assemble or destruction): colptr int64 [nfree+1], rowval int32
+ * [nnz], nzval double [nnz], rhs double [nfree]. */
+int hdg_skeleton_system(hdg_mesh* mesh, const int64_t** colptr, const int32_t** rowval,
+                        const double** nzval, const double** rhs);
+/* Device pointers of the per-cell condensed blocks S [ncells][nb][nb], g
+ * [ncells][nb] and trace dof ids [ncells][nb]. */
+int hdg_cell_blocks(hdg_mesh* mesh, const double** S, const double** g,
+                    const int32_t** cell_dofs);
+
+/* solve!'s recovery (src/HybridAffineFEOperators.jl:89-150 with
+ * BackwardStaticCondensationMap): lam_free device [nfree] (skeleton
+ * solution), free_values device [nfree_total] = [q of all cells; u of all
+ * cells; lam_free], Gridap's consecutive multi-field numbering. */
+int hdg_recover(hdg_mesh* mesh, const double* lam_free, double* free_values);
+
+/* Per-cell element systems (A, b) as the fused kernel generates them,
+ * device [ncells][n][n] / [ncells][n] for cells [first, first+count): a
+ * debugging/parity entry point; not on the hot path. */
+int hdg_cell_systems(hdg_mesh* mesh, int64_t first, int64_t count, double* A, double* b);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HDG_H */

@@ -1,0 +1,207 @@
+"""GPU parity: libhdg (gfx950) against the CPU oracle on the same seeded inputs.
+
+Tolerances: the FP64 path must match the oracle to relative 1e-12 (max-norm
+error over max-norm of the oracle value); integer outputs (dof maps, CSC
+pattern) must match bit for bit.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-12
+
+
+def rel(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    d = np.max(np.abs(a - b)) if a.size else 0.0
+    s = np.max(np.abs(b)) if b.size else 1.0
+    return d / (s if s > 0 else 1.0)
+
+
+def random_block_system(rng, sizes, nc, untouched=True):
+    """Cell systems with the block pattern of test/StaticCondensationMapTests.jl:6-23
+    (interior-interior off-diagonal blocks untouched = zero), made well conditioned."""
+    n = sum(sizes)
+    A = rng.standard_normal((nc, n, n))
+    if untouched and len(sizes) == 3:
+        o = np.cumsum([0] + list(sizes))
+        A[:, o[1]:o[2], o[1]:o[3]] = 0.0
+        A[:, o[2]:o[3], o[1]:o[3]] = 0.0
+        A[:, o[1]:o[2], o[1]:o[2]] = rng.standard_normal((nc, sizes[1], sizes[1]))
+    A += n * np.eye(n)[None]
+    b = rng.standard_normal((nc, n))
+    return A, b
+
+
+@pytest.mark.parametrize("sizes", [(3, 3, 3), (8, 4, 8), (18, 9, 12), (24, 8, 24), (81, 27, 54)])
+def test_static_condensation_map(ctx, sizes):
+    import gridaphybrid as gh
+    rng = np.random.default_rng(1234 + sum(sizes))
+    nc = 37
+    A, b = random_block_system(rng, sizes, nc)
+    ni = sizes[0] + sizes[1]
+    nb = sizes[2]
+    print(f"static condensation sizes={sizes} ncells={nc}", flush=True)
+    S, g = gh.StaticCondensationMap([1, 2], [3], ctx).evaluate(A, b, list(sizes))
+    So, go, info = O.condense(A, b, ni, nb)
+    assert not info.any()
+    assert rel(S.cpu().numpy(), So) < RTOL
+    assert rel(g.cpu().numpy(), go) < RTOL
+    # backward map on the same cells
+    x = rng.standard_normal((nc, nb))
+    q, u, xx = gh.BackwardStaticCondensationMap([1, 2], [3], ctx).evaluate(A, b, x, list(sizes))
+    xio, info = O.recover(A, b, x, ni, nb)
+    xi = np.concatenate([q.cpu().numpy(), u.cpu().numpy()], axis=1)
+    assert rel(xi, xio) < RTOL
+    assert np.array_equal(xx.cpu().numpy(), x)
+
+
+def test_static_condensation_field_order(ctx):
+    """interior/boundary fields need not be contiguous: condense field 2 out."""
+    import gridaphybrid as gh
+    rng = np.random.default_rng(7)
+    sizes = [5, 4, 6]
+    A, b = random_block_system(rng, sizes, 9, untouched=False)
+    S, g = gh.StaticCondensationMap([2], [1, 3], ctx).evaluate(A, b, sizes)
+    perm = np.r_[5:9, 0:5, 9:15]
+    Ap = A[:, perm][:, :, perm]
+    So, go, info = O.condense(Ap, b[:, perm], 4, 11)
+    assert rel(S.cpu().numpy(), So) < RTOL and rel(g.cpu().numpy(), go) < RTOL
+
+
+def test_singular_cell_reports_info(ctx):
+    """A cell whose interior block is all zero fails alone (getrf info = 1)."""
+    import gridaphybrid as gh
+    rng = np.random.default_rng(3)
+    A, b = random_block_system(rng, (3, 3, 3), 4)
+    A[2, :6, :6] = 0.0
+    S, g, info = gh.StaticCondensationMap([1, 2], [3], ctx).evaluate(A, b, [3, 3, 3],
+                                                                    return_info=True)
+    info = info.cpu().numpy()
+    assert info[2] == 1 and info[[0, 1, 3]].sum() == 0
+    So, go, _ = O.condense(A[[0, 1, 3]], b[[0, 1, 3]], 6, 3)
+    assert rel(S.cpu().numpy()[[0, 1, 3]], So) < RTOL
+    with pytest.raises(gh.SingularError):
+        gh.StaticCondensationMap([1, 2], [3], ctx).evaluate(A, b, [3, 3, 3])
+
+
+def _device_cell_systems(op, first, count):
+    import torch
+    n = op.info.ni + op.info.nb
+    A = torch.empty((count, n, n), dtype=torch.float64, device="cuda")
+    b = torch.empty((count, n), dtype=torch.float64, device="cuda")
+    from gridaphybrid import _lib
+    _lib.check(_lib.lib().hdg_cell_systems(op.mesh, first, count, A.data_ptr(), b.data_ptr()),
+               op.ctx.h, "cell_systems")
+    op.ctx.synchronize()
+    return A.transpose(1, 2).cpu().numpy(), b.cpu().numpy()
+
+
+def _operator(ctx, D, k, n, kappa=None, source=None, spaces=None, mms=O.MMS_LINEAR, tau=1.0):
+    import gridaphybrid as gh
+    model = gh.CartesianDiscreteModel(tuple([0.0, 1.0] * D), tuple([n] * D))
+    X = spaces(model) if spaces else gh.HDGSpaces.darcy_baseline(model, k)
+    wf = gh.DarcyHDG(tau=tau, kappa=1.0 if kappa is None else kappa,
+                     source=mms if source is None else source)
+    return gh.HybridAffineFEOperator(wf, X, X, [1, 2], [3], ctx=ctx)
+
+
+CASES = [
+    # (D, k, n, mms, lognormal kappa)
+    (2, 1, 8, O.MMS_LINEAR, False),     # cfg1
+    (2, 2, 16, O.MMS_SINPROD, True),    # cfg2 shape, smaller mesh
+    (2, 3, 6, O.MMS_SINPROD, True),
+    (3, 1, 3, O.MMS_LINEAR, True),
+    (3, 2, 3, O.MMS_SINPROD, True),     # cfg3 shape, smaller mesh
+]
+
+
+@pytest.mark.parametrize("D,k,n,mms,logn", CASES)
+def test_operator_parity(ctx, D, k, n, mms, logn):
+    print(f"operator parity D={D} k={k} n={n} mms={mms} lognormal={logn}", flush=True)
+    import gridaphybrid as gh
+    nc = n ** D
+    kappa = O.lognormal_kappa(nc, seed=20261202) if logn else np.ones(nc)
+    prob = O.Problem.darcy(D, k, mms=mms)
+    ref = O.hdg_solve(prob, tuple([n] * D), coef=kappa)
+    op = _operator(ctx, D, k, n, kappa=kappa, mms=mms)
+    assert op.info.nfree == ref.nfree and op.info.nnz == len(ref.rowval)
+    # element systems (fused generator) vs per-cell quadrature
+    A, b = _device_cell_systems(op, 0, nc)
+    assert rel(A, ref.A) < RTOL and rel(b, ref.b) < RTOL
+    # condensed blocks and dof ids
+    S, g, cd = op.cell_blocks()
+    assert np.array_equal(cd, ref.cell_dofs)
+    assert rel(S, ref.S) < RTOL and rel(g, ref.g) < RTOL
+    # skeleton CSC: pattern bit-exact, values to tolerance
+    K, rhs = op.skeleton_system()
+    assert np.array_equal(K.indptr, ref.colptr) and np.array_equal(K.indices, ref.rowval)
+    assert rel(K.data, ref.nzval) < RTOL and rel(rhs, ref.rhs) < RTOL
+    # recovery from the oracle's skeleton solution
+    fv = op.recover(ref.lam).cpu().numpy()
+    assert rel(fv, ref.free_values) < RTOL
+    # full solve through the public API
+    sol = gh.solve(op)
+    assert rel(sol.free_values, ref.free_values) < 1e-10
+    op.close()
+
+
+def test_reference_kat_on_gpu(ctx):
+    """test/DarcyHDGTests.jl:170-174 on the GPU path: P1/P0/P1 on 2x2 quads,
+    ||u - u_h|| < 1e-12 (DarcyHDGTests.jl:142)."""
+    import gridaphybrid as gh
+    model = gh.CartesianDiscreteModel((0.0, 1.0, 0.0, 1.0), (2, 2))
+    X = gh.HDGSpaces.darcy_reference_test(model)
+    op = gh.HybridAffineFEOperator(gh.DarcyHDG(tau=1.0, source=O.MMS_KAT), X, X, [1, 2], [3],
+                                   ctx=ctx)
+    sol = gh.solve(op)
+    prob = O.Problem.darcy_reference_kat(2)
+    mesh = O.cartesian(2, (2, 2))
+    xi = np.concatenate([sol.flux, sol.potential], axis=1)
+    err = O.l2_errors(prob, mesh, np.ones(4), xi)
+    assert err[0] < 1e-12 and err[1] < 1e-12
+
+
+def test_condense_deterministic(ctx):
+    """Two assemblies of the same operator are bitwise identical (no atomics)."""
+    kappa = O.lognormal_kappa(24 * 24, seed=5)
+    op = _operator(ctx, 2, 2, 24, kappa=kappa, mms=O.MMS_SINPROD)
+    K1, r1 = op.skeleton_system()
+    op.assemble()
+    K2, r2 = op.skeleton_system()
+    assert np.array_equal(K1.data, K2.data) and np.array_equal(r1, r2)
+    op.close()
+
+
+@pytest.mark.parametrize("D,k,n", [(2, 2, 512), (3, 2, 32)])
+def test_full_size_consistency(ctx, D, k, n):
+    """At benchmark sizes: with a manufactured solution that lies in the
+    discrete spaces (linear potential), recovery from the exact trace must
+    reproduce the exact flux and potential (consistency of every cell's
+    condense + recover), whatever the per-cell kappa."""
+    print(f"full-size consistency D={D} k={k} n={n}", flush=True)
+    import gridaphybrid as gh
+    nc = n ** D
+    kappa = O.lognormal_kappa(nc, seed=11)
+    op = _operator(ctx, D, k, n, kappa=kappa, mms=O.MMS_LINEAR)
+    # exact trace: the linear potential at the free trace nodes, in free-dof
+    # order (free facets consecutive in facet order)
+    prob = O.Problem.darcy(D, k, mms=O.MMS_LINEAR)
+    mesh = O.cartesian(D, tuple([n] * D))
+    cd, nfree, ndir, fd = O.trace_dofs(mesh, prob.sizes()["n_tf"])
+    inv = 1 - fd
+    lam_all = O.dirichlet_values(prob, mesh, inv, int(inv.sum()) * prob.sizes()["n_tf"])
+    assert len(lam_all) == op.info.nfree
+    fv = op.recover(lam_all).cpu().numpy()
+    nf = op.info.n_flux
+    xi = np.concatenate([fv[:nc * nf].reshape(nc, nf),
+                         fv[nc * nf:nc * op.info.ni].reshape(nc, -1)], axis=1)
+    err = O.l2_errors(prob, mesh, kappa, xi)
+    assert err[0] < 1e-9 * np.sqrt(np.mean(kappa ** 2)) + 1e-9 and err[1] < 1e-10
+    op.close()
