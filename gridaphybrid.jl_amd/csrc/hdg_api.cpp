@@ -91,7 +91,7 @@ struct hdg_mesh {
   std::vector<int8_t> facet_dir;
   // device
   DevBuf tables, d_x0, d_jac, d_sign, d_coef, d_cell_dofs, d_lam_dir;
-  DevBuf d_S, d_g, d_W, d_info;
+  DevBuf d_S, d_g, d_U, d_Y, d_info, d_alpha, d_srcq;
   DevBuf d_fptr, d_pptr, d_pfacet, d_pcell, d_plpos, d_gcell, d_glpos;
   DevBuf d_colptr, d_rowval, d_nzval, d_rhs;
   const double* src_qp = nullptr;
@@ -105,6 +105,8 @@ struct hdg_mesh {
     m.sign = d_sign.as<int8_t>();
     m.coef = d_coef.as<double>();
     m.src = src_qp;
+    m.alpha = d_alpha.as<double>();
+    m.srcq = d_srcq.as<double>();
     m.source = prob.source;
     m.tau = prob.tau;
     m.young = prob.young;
@@ -259,7 +261,10 @@ int hdg_static_condensation(hdg_ctx* ctx, const hdg_fields* fields, int64_t ncel
   HIP_TRY(ctx, hipSetDevice(ctx->device), "hipSetDevice");
   if ((rc = upload_perm(ctx, perm))) return rc;
   DenseIn in{A, b, ctx->d_perm, ni + nb, ni, nb};
-  CondenseOut out{S, g, nullptr, nullptr, info};
+  CondenseOut out;
+  out.S = S;
+  out.g = g;
+  out.info = info;
   HIP_TRY(ctx, launch_condense_dense(in, ncells, out, ctx->stream), "condense kernel");
   int64_t bad = 0;
   if ((rc = count_singular(ctx, info, ncells, &bad))) return rc;
@@ -455,7 +460,9 @@ int hdg_mesh_create(hdg_ctx* ctx, const hdg_problem* prob, const hdg_mesh_desc* 
                                                    &th.xq};
     size_t nd = 0;
     for (auto* v : dv) nd += v->size();
-    size_t bytes = nd * sizeof(double) + th.pmask.size() + th.qmask.size() + 16;
+    size_t bytes = nd * sizeof(double) + th.prog_val.size() * sizeof(double) +
+                   th.prog_off.size() * sizeof(int32_t) + th.pmask.size() + th.qmask.size() +
+                   th.prog_id.size() + 16;
     std::vector<char> host(bytes, 0);
     HIP_TRY(ctx, m->tables.alloc(bytes), "alloc tables");
     size_t off = 0;
@@ -465,11 +472,20 @@ int hdg_mesh_create(hdg_ctx* ctx, const hdg_problem* prob, const hdg_mesh_desc* 
       if (!v->empty()) std::memcpy(host.data() + off, v->data(), v->size() * sizeof(double));
       off += v->size() * sizeof(double);
     }
+    size_t off_pv = off;
+    std::memcpy(host.data() + off, th.prog_val.data(), th.prog_val.size() * sizeof(double));
+    off += th.prog_val.size() * sizeof(double);
+    size_t off_po = off;
+    std::memcpy(host.data() + off, th.prog_off.data(), th.prog_off.size() * sizeof(int32_t));
+    off += th.prog_off.size() * sizeof(int32_t);
     size_t off_pm = off;
     std::memcpy(host.data() + off, th.pmask.data(), th.pmask.size());
     off += th.pmask.size();
     size_t off_qm = off;
     std::memcpy(host.data() + off, th.qmask.data(), th.qmask.size());
+    off += th.qmask.size();
+    size_t off_pi = off;
+    std::memcpy(host.data() + off, th.prog_id.data(), th.prog_id.size());
     HIP_TRY(ctx, hipMemcpyAsync(m->tables.p, host.data(), bytes, hipMemcpyHostToDevice, st),
             "upload tables");
     HIP_TRY(ctx, hipStreamSynchronize(st), "sync");
@@ -496,6 +512,10 @@ int hdg_mesh_create(hdg_ctx* ctx, const hdg_problem* prob, const hdg_mesh_desc* 
     for (size_t i = 0; i < dv.size(); ++i) *ptrs[i] = reinterpret_cast<const double*>(base + offs[i]);
     T.pmask = reinterpret_cast<const uint8_t*>(base + off_pm);
     T.qmask = reinterpret_cast<const uint8_t*>(base + off_qm);
+    T.nalpha = th.nalpha;
+    T.prog_val = reinterpret_cast<const double*>(base + off_pv);
+    T.prog_off = reinterpret_cast<const int32_t*>(base + off_po);
+    T.prog_id = reinterpret_cast<const uint8_t*>(base + off_pi);
   }
   m->x0.assign(desc->x0, desc->x0 + nc * D);
   m->jac.assign(desc->jac, desc->jac + nc * D * D);
@@ -528,7 +548,13 @@ int hdg_mesh_create(hdg_ctx* ctx, const hdg_problem* prob, const hdg_mesh_desc* 
   // resident outputs
   HIP_TRY(ctx, m->d_S.alloc((size_t)nc * nb * nb * sizeof(double)), "alloc S");
   HIP_TRY(ctx, m->d_g.alloc((size_t)nc * nb * sizeof(double)), "alloc g");
-  HIP_TRY(ctx, m->d_W.alloc((size_t)nc * (nb + 1) * th.ni * sizeof(double)), "alloc W");
+  HIP_TRY(ctx, m->d_U.alloc((size_t)nc * ((size_t)th.ni * (th.ni + 1) / 2) * sizeof(double)),
+          "alloc U");
+  HIP_TRY(ctx, m->d_Y.alloc((size_t)nc * (nb + 1) * th.ni * sizeof(double)), "alloc Y");
+  HIP_TRY(ctx, m->d_alpha.alloc((size_t)std::max<int64_t>(nc, 1) * 72 * sizeof(double)),
+          "alloc alpha");
+  HIP_TRY(ctx, m->d_srcq.alloc((size_t)std::max<int64_t>(nc, 1) * th.nqc * 4 * sizeof(double)),
+          "alloc srcq");
   HIP_TRY(ctx, m->d_info.alloc((size_t)std::max<int64_t>(nc, 1) * sizeof(int32_t)), "alloc info");
   HIP_TRY(ctx, m->d_rowval.alloc((size_t)std::max<int64_t>(m->nnz, 1) * sizeof(int32_t)), "alloc rowval");
   HIP_TRY(ctx, m->d_nzval.alloc((size_t)std::max<int64_t>(m->nnz, 1) * sizeof(double)), "alloc nzval");
@@ -657,17 +683,15 @@ void mms_trace(int src, int D, const double* x, double E, double nu, double* u) 
   (void)E;
   (void)nu;
 }
-}  // namespace
 
-extern "C" {
-
-int hdg_mesh_set_dirichlet_mms(hdg_mesh* m) {
-  if (!m) return HDG_ERR_ARG;
+// Built-in manufactured trace at the nodes of the facets with
+// facet_dir == want, facet by facet, component-major within a facet.
+std::vector<double> trace_values(const hdg_mesh* m, int8_t want, int64_t count) {
   const int D = m->D, nm = m->th.nm, ncm = m->th.ncm;
-  std::vector<double> vals((size_t)std::max<int64_t>(m->ndir, 1), 0.0);
+  std::vector<double> vals((size_t)std::max<int64_t>(count, 1), 0.0);
   int64_t k = 0;
   for (int64_t f = 0; f < m->nfacets; ++f) {
-    if (!m->facet_dir[f]) continue;
+    if ((m->facet_dir[f] != 0) != (want != 0)) continue;
     const int64_t c = m->facet_cell[2 * f];
     const int lf = m->facet_lpos[2 * f];
     const int fix = D - 1 - lf / 2;
@@ -686,7 +710,24 @@ int hdg_mesh_set_dirichlet_mms(hdg_mesh* m) {
     }
     k += (int64_t)ncm * nm;
   }
+  return vals;
+}
+}  // namespace
+
+extern "C" {
+
+int hdg_mesh_set_dirichlet_mms(hdg_mesh* m) {
+  if (!m) return HDG_ERR_ARG;
+  std::vector<double> vals = trace_values(m, 1, m->ndir);
   return hdg_mesh_set_dirichlet(m, vals.data());
+}
+
+int hdg_mesh_interpolate_trace(hdg_mesh* m, double* lam_free) {
+  if (!m) return HDG_ERR_ARG;
+  if (m->nfree > 0 && !lam_free) return fail(m->ctx, HDG_ERR_ARG, "null output");
+  std::vector<double> vals = trace_values(m, 0, m->nfree);
+  if (m->nfree > 0) std::memcpy(lam_free, vals.data(), (size_t)m->nfree * sizeof(double));
+  return HDG_OK;
 }
 
 int hdg_mesh_set_source(hdg_mesh* m, const double* src_qp) {
@@ -696,7 +737,7 @@ int hdg_mesh_set_source(hdg_mesh* m, const double* src_qp) {
   return HDG_OK;
 }
 
-int hdg_condense_assemble(hdg_mesh* m, int precision, int64_t* n_singular) {
+int hdg_condense(hdg_mesh* m, int precision) {
   if (!m) return HDG_ERR_ARG;
   hdg_ctx* ctx = m->ctx;
   if (precision != HDG_FP64 && precision != HDG_MIXED)
@@ -705,20 +746,43 @@ int hdg_condense_assemble(hdg_mesh* m, int precision, int64_t* n_singular) {
   if (m->prob.source == HDG_SOURCE_QP && !m->src_qp)
     return fail(ctx, HDG_ERR_STATE, "source values at quadrature points not set");
   HIP_TRY(ctx, hipSetDevice(ctx->device), "hipSetDevice");
-  hipStream_t st = ctx->stream;
-  CondenseOut out{m->d_S.as<double>(), m->d_g.as<double>(), m->d_W.as<double>(), nullptr,
-                  m->d_info.as<int32_t>()};
-  HIP_TRY(ctx, launch_condense_fused(m->tdev, m->meshdev(), out, st), "condense kernel");
+  CondenseOut out;
+  out.S = m->d_S.as<double>();
+  out.g = m->d_g.as<double>();
+  out.U = m->d_U.as<double>();
+  out.Y = m->d_Y.as<double>();
+  out.info = m->d_info.as<int32_t>();
+  HIP_TRY(ctx, launch_condense_fused(m->tdev, m->meshdev(), out, ctx->stream), "condense kernel");
+  m->condensed = true;
+  return HDG_OK;
+}
+
+int hdg_assemble(hdg_mesh* m) {
+  if (!m) return HDG_ERR_ARG;
+  hdg_ctx* ctx = m->ctx;
+  if (!m->condensed) return fail(ctx, HDG_ERR_STATE, "assemble called before condense");
+  HIP_TRY(ctx, hipSetDevice(ctx->device), "hipSetDevice");
   HIP_TRY(ctx, launch_assemble(m->pairs(), m->d_S.as<double>(), m->d_g.as<double>(),
                                m->d_cell_dofs.as<int32_t>(), m->d_lam_dir.as<double>(),
-                               m->d_nzval.as<double>(), m->d_rhs.as<double>(), st),
+                               m->d_nzval.as<double>(), m->d_rhs.as<double>(), ctx->stream),
           "assemble kernels");
-  int64_t bad = 0;
-  int rc = count_singular(ctx, m->d_info.as<int32_t>(), m->ncells, &bad);
+  return HDG_OK;
+}
+
+int hdg_mesh_singular_count(hdg_mesh* m, int64_t* n_singular) {
+  if (!m || !n_singular) return HDG_ERR_ARG;
+  HIP_TRY(m->ctx, hipSetDevice(m->ctx->device), "hipSetDevice");
+  return count_singular(m->ctx, m->d_info.as<int32_t>(), m->ncells, n_singular);
+}
+
+int hdg_condense_assemble(hdg_mesh* m, int precision, int64_t* n_singular) {
+  int rc = hdg_condense(m, precision);
   if (rc) return rc;
+  if ((rc = hdg_assemble(m))) return rc;
+  int64_t bad = 0;
+  if ((rc = hdg_mesh_singular_count(m, &bad))) return rc;
   if (n_singular) *n_singular = bad;
-  m->condensed = true;
-  if (bad) return fail(ctx, HDG_SINGULAR, std::to_string(bad) + " cell(s) with singular A_ii");
+  if (bad) return fail(m->ctx, HDG_SINGULAR, std::to_string(bad) + " cell(s) with singular A_ii");
   return HDG_OK;
 }
 
@@ -746,7 +810,8 @@ int hdg_recover(hdg_mesh* m, const double* lam_free, double* free_values) {
   if (!m->condensed) return fail(ctx, HDG_ERR_STATE, "recover called before condense");
   if ((m->nfree > 0 && !lam_free) || !free_values) return fail(ctx, HDG_ERR_ARG, "null buffer");
   HIP_TRY(ctx, hipSetDevice(ctx->device), "hipSetDevice");
-  HIP_TRY(ctx, launch_recover(m->d_W.as<double>(), m->d_cell_dofs.as<int32_t>(), lam_free,
+  HIP_TRY(ctx, launch_recover(m->d_U.as<double>(), m->d_Y.as<double>(),
+                              m->d_cell_dofs.as<int32_t>(), lam_free,
                               m->d_lam_dir.as<double>(), free_values, m->ncells, m->th.ni,
                               m->th.nb, m->th.na, m->nfree, ctx->stream),
           "recover kernel");

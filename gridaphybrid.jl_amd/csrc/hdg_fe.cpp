@@ -20,6 +20,7 @@
 #include "hdg_internal.h"
 
 namespace hdg {
+void build_program(TablesHost& t);
 namespace {
 
 // Gauss-Legendre nodes/weights on [0,1]: Newton on P_n from the asymptotic
@@ -400,7 +401,178 @@ int build_tables(int phys, int D, int space, int kp, int kq, int km, int quad_de
   t.trace_nodes.assign((size_t)nm * (D - 1 > 0 ? D - 1 : 1), 0.0);
   for (int i = 0; i < nm; ++i)
     for (int a = 0; a < D - 1; ++a) t.trace_nodes[(size_t)i * (D - 1) + a] = bm.node(i, a);
+  build_program(t);
   return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Entry programs.  Every entry (row, col) of the condensed cell matrix
+// [A_ii A_ib; A_bi A_bb] is a short linear combination of per-cell
+// coefficients alpha[id] (affine pull-back factors, kappa, owner signs) with
+// reference-integral values; the program lists, entry by entry in
+// column-major order, the (id, value) terms with a nonzero value.
+// ---------------------------------------------------------------------------
+namespace {
+struct Prog {
+  std::vector<int32_t>& off;
+  std::vector<uint8_t>& id;
+  std::vector<double>& val;
+  void add(int cid, double v) {
+    if (v != 0.0) {
+      id.push_back((uint8_t)cid);
+      val.push_back(v);
+    }
+  }
+};
+inline void sym_ij(int D, int a, int& i, int& j) {
+  int t = 0;
+  for (int x = 0; x < D; ++x)
+    for (int y = x; y < D; ++y) {
+      if (t == a) {
+        i = x;
+        j = y;
+        return;
+      }
+      ++t;
+    }
+  i = j = 0;
+}
+}  // namespace
+
+void build_program(TablesHost& t) {
+  const int D = t.D, np = t.np, nq = t.nq, nm = t.nm, na = t.na, ni = t.ni, n = t.n;
+  const int nF = t.nF;
+  t.prog_off.assign(1, 0);
+  t.prog_id.clear();
+  t.prog_val.clear();
+  Prog P{t.prog_off, t.prog_id, t.prog_val};
+  auto Mpp = [&](int i, int j) { return t.Mpp[(size_t)i * np + j]; };
+  auto Gpq = [&](int e, int i, int j) { return t.Gpq[((size_t)e * np + i) * nq + j]; };
+  auto Gqp = [&](int e, int i, int j) { return t.Gqp[((size_t)e * nq + i) * np + j]; };
+  auto Fqp = [&](int F, int i, int j) { return t.Fqp[((size_t)F * nq + i) * np + j]; };
+  auto Fqq = [&](int F, int i, int j) { return t.Fqq[((size_t)F * nq + i) * nq + j]; };
+  auto Fpm = [&](int F, int i, int m) { return t.Fpm[((size_t)F * np + i) * nm + m]; };
+  auto Fqm = [&](int F, int i, int m) { return t.Fqm[((size_t)F * nq + i) * nm + m]; };
+  auto Fmm = [&](int m, int k) { return t.Fmm[(size_t)m * nm + k]; };
+  auto pm = [&](int i, int F) { return (t.pmask[i] >> F) & 1; };
+  auto qm = [&](int i, int F) { return (t.qmask[i] >> F) & 1; };
+  if (t.phys == kDarcy) {
+    // alpha: [0] detJ/kappa; [1 + e*D + a] -detJ Ji(e,a); [kAn + 3F + a] aF nF_a;
+    //        [kTs + F] tau sF aF
+    const int kJ = 1, kAn = 10, kTs = 28;
+    t.nalpha = 34;
+    for (int col = 0; col < n; ++col)
+      for (int row = 0; row < n; ++row) {
+        if (row < na) {
+          const int a = row / np, i = row % np;
+          if (col < na) {
+            const int b = col / np, j = col % np;
+            if (a == b) P.add(0, Mpp(i, j));
+          } else if (col < ni) {
+            const int j = col - na;
+            for (int e = 0; e < D; ++e) P.add(kJ + e * D + a, Gpq(e, i, j));
+          } else {
+            const int F = (col - ni) / nm, m = (col - ni) % nm;
+            if (pm(i, F)) P.add(kAn + 3 * F + a, Fpm(F, i, m));
+          }
+        } else if (row < ni) {
+          const int i = row - na;
+          if (col < na) {
+            const int b = col / np, j = col % np;
+            for (int e = 0; e < D; ++e) P.add(kJ + e * D + b, Gqp(e, i, j));
+            for (int F = 0; F < nF; ++F)
+              if (qm(i, F) && pm(j, F)) P.add(kAn + 3 * F + b, Fqp(F, i, j));
+          } else if (col < ni) {
+            const int j = col - na;
+            for (int F = 0; F < nF; ++F)
+              if (qm(i, F) && qm(j, F)) P.add(kTs + F, Fqq(F, i, j));
+          } else {
+            const int F = (col - ni) / nm, m = (col - ni) % nm;
+            if (qm(i, F)) P.add(kTs + F, -Fqm(F, i, m));
+          }
+        } else {
+          const int F = (row - ni) / nm, m = (row - ni) % nm;
+          if (col < na) {
+            const int b = col / np, j = col % np;
+            if (pm(j, F)) P.add(kAn + 3 * F + b, Fpm(F, j, m));
+          } else if (col < ni) {
+            const int j = col - na;
+            if (qm(j, F)) P.add(kTs + F, Fqm(F, j, m));
+          } else {
+            const int G = (col - ni) / nm, m2 = (col - ni) % nm;
+            if (G == F) P.add(kTs + F, -Fmm(m, m2));
+          }
+        }
+        t.prog_off.push_back((int32_t)t.prog_id.size());
+      }
+  } else {
+    // alpha: [kK + a*6 + b] K_ab detJ; [kJ + e*D + d] detJ Ji(e,d);
+    //        [kAn + 3F + d] aF nF_d; [kT + F] tau aF
+    const int kK = 0, kJ = 36, kAn = 45, kT = 63;
+    t.nalpha = 69;
+    const int ntf = t.ntf;
+    for (int col = 0; col < n; ++col)
+      for (int row = 0; row < n; ++row) {
+        if (row < na) {
+          const int a = row / np, i = row % np;
+          int ia, ja;
+          sym_ij(D, a, ia, ja);
+          if (col < na) {
+            const int b = col / np, j = col % np;
+            P.add(kK + a * 6 + b, Mpp(i, j));
+          } else if (col < ni) {
+            const int c = (col - na) / nq, j = (col - na) % nq;
+            if (c == ja)
+              for (int e = 0; e < D; ++e) P.add(kJ + e * D + ia, Gpq(e, i, j));
+            if (c == ia && ia != ja)
+              for (int e = 0; e < D; ++e) P.add(kJ + e * D + ja, Gpq(e, i, j));
+          } else {
+            const int F = (col - ni) / ntf, r = (col - ni) % ntf, c = r / nm, m = r % nm;
+            if (pm(i, F)) {
+              if (c == ia) P.add(kAn + 3 * F + ja, -Fpm(F, i, m));
+              if (c == ja && ia != ja) P.add(kAn + 3 * F + ia, -Fpm(F, i, m));
+            }
+          }
+        } else if (row < ni) {
+          const int c = (row - na) / nq, j = (row - na) % nq;
+          if (col < na) {
+            const int a = col / np, i = col % np;
+            int ia, ja;
+            sym_ij(D, a, ia, ja);
+            if (c == ja)
+              for (int e = 0; e < D; ++e) P.add(kJ + e * D + ia, -Gpq(e, i, j));
+            if (c == ia && ia != ja)
+              for (int e = 0; e < D; ++e) P.add(kJ + e * D + ja, -Gpq(e, i, j));
+          } else if (col < ni) {
+            const int c2 = (col - na) / nq, j2 = (col - na) % nq;
+            if (c2 == c)
+              for (int F = 0; F < nF; ++F)
+                if (qm(j, F) && qm(j2, F)) P.add(kT + F, Fqq(F, j, j2));
+          } else {
+            const int F = (col - ni) / ntf, r = (col - ni) % ntf, c2 = r / nm, m = r % nm;
+            if (c2 == c && qm(j, F)) P.add(kT + F, -Fqm(F, j, m));
+          }
+        } else {
+          const int F = (row - ni) / ntf, rr = (row - ni) % ntf, c = rr / nm, m = rr % nm;
+          if (col < na) {
+            const int a = col / np, i = col % np;
+            int ia, ja;
+            sym_ij(D, a, ia, ja);
+            if (pm(i, F)) {
+              if (c == ia) P.add(kAn + 3 * F + ja, Fpm(F, i, m));
+              if (c == ja && ia != ja) P.add(kAn + 3 * F + ia, Fpm(F, i, m));
+            }
+          } else if (col < ni) {
+            const int c2 = (col - na) / nq, j = (col - na) % nq;
+            if (c2 == c && qm(j, F)) P.add(kT + F, -Fqm(F, j, m));
+          } else {
+            const int G = (col - ni) / ntf, r2 = (col - ni) % ntf, c2 = r2 / nm, m2 = r2 % nm;
+            if (G == F && c2 == c) P.add(kT + F, Fmm(m, m2));
+          }
+        }
+        t.prog_off.push_back((int32_t)t.prog_id.size());
+      }
+  }
 }
 
 }  // namespace hdg

@@ -38,6 +38,11 @@ struct TablesHost {
   std::vector<double> Bq;         // [nqc][nq]
   std::vector<double> xq;         // [nqc][3]  reference quadrature points
   std::vector<double> trace_nodes;  // [nm][D-1] trace nodes on the reference facet
+  // entry program (column-major entries of the n x n matrix)
+  int nalpha = 0;
+  std::vector<int32_t> prog_off;  // [n*n+1]
+  std::vector<uint8_t> prog_id;   // [nterms] coefficient index
+  std::vector<double> prog_val;   // [nterms] reference-integral value
 };
 
 // Device view of the tables (pointers into one device allocation).
@@ -59,6 +64,10 @@ struct Tables {
   const double* Bp;
   const double* Bq;
   const double* xq;
+  int nalpha;
+  const int32_t* prog_off;
+  const uint8_t* prog_id;
+  const double* prog_val;
 };
 
 // Device view of a mesh for the fused element generator.
@@ -68,19 +77,22 @@ struct MeshDev {
   const double* jac;       // [ncells][D*D] column-major
   const int8_t* sign;      // [ncells][2D]  n . n_o
   const double* coef;      // [ncells] kappa or nu
-  const double* src;       // [ncells][nqc][ncomp] or null
+  const double* src;       // [ncells][nqc][ncomp] or null (user source values)
+  double* alpha;           // [ncells][72] per-cell entry-program coefficients (scratch)
+  double* srcq;            // [ncells][nqc][4] |det J| * source at quadrature points (scratch)
   int source;              // hdg_source
   double tau;
   double young;
 };
 
-// Outputs of the condensation kernels.
+// Outputs of the condensation kernels (any pointer may be null).
 struct CondenseOut {
-  double* S;        // [ncells][nb][nb] column-major
-  double* g;        // [ncells][nb]
-  double* W;        // [ncells][nb+1][ni] column-major: A_ii^-1 [A_ib b_i]  (may be null)
-  float* W32;       // mixed path alternative (unused when null)
-  int32_t* info;    // [ncells]
+  double* S = nullptr;     // [ncells][nb][nb] column-major
+  double* g = nullptr;     // [ncells][nb]
+  double* U = nullptr;     // [ncells][ni(ni+1)/2] packed upper factor, LAPACK row order
+  double* Y = nullptr;     // [ncells][nb+1][ni] L^-1 P [A_ib b_i], LAPACK row order
+  double* X = nullptr;     // [ncells][ni] back-substituted rhs (backward map)
+  int32_t* info = nullptr; // [ncells]
 };
 
 // Dense cell systems in HBM for the map-level API.
@@ -118,9 +130,9 @@ hipError_t launch_condense_dense(const DenseIn& in, int64_t ncells, const Conden
                                  hipStream_t st);
 hipError_t launch_backward_dense(const DenseIn& in, const double* x, int64_t ncells, double* xi,
                                  int32_t* info, hipStream_t st);
-hipError_t launch_recover(const double* W, const int32_t* cell_dofs, const double* lam_free,
-                          const double* lam_dir, double* out, int64_t ncells, int ni, int nb,
-                          int na, int64_t nfree, hipStream_t st);
+hipError_t launch_recover(const double* U, const double* Y, const int32_t* cell_dofs,
+                          const double* lam_free, const double* lam_dir, double* out,
+                          int64_t ncells, int ni, int nb, int na, int64_t nfree, hipStream_t st);
 hipError_t launch_symbolic(const PairTable& P, int32_t* rowval, hipStream_t st);
 hipError_t launch_assemble(const PairTable& P, const double* S, const double* g,
                            const int32_t* cell_dofs, const double* lam_dir, double* nzval,

@@ -151,11 +151,6 @@ __device__ inline void elas_source(int src, int D, const double* x, double E, do
 // --------------------------------------------------------------------------
 constexpr int kMaxQP = 128;
 
-struct FusedShared {
-  CellGeo geo;
-  double src[kMaxQP][4];  // source values at cell quadrature points
-};
-
 __device__ inline void sym_pair(int D, int a, int& i, int& j) {
   // upper triangle, row-major: 2D (11,12,22), 3D (11,12,13,22,23,33);
   // packed as 2-bit (i, j) pairs to stay out of scratch memory
@@ -164,6 +159,101 @@ __device__ inline void sym_pair(int D, int a, int& i, int& j) {
   j = (code >> (4 * a + 2)) & 3u;
 }
 
+constexpr int kMaxAlpha = 72;
+
+// Per-cell preparation (runs at the start of every condensation): the
+// affine pull-back coefficients alpha of the entry program (layout fixed by
+// build_program in hdg_fe.cpp) and the source values at the cell quadrature
+// points, pre-multiplied by |det J|.  One thread per cell.
+template <int PHYS>
+__global__ void __launch_bounds__(128) prep_kernel(Tables T, MeshDev M) {
+  const int D = T.D;
+  for (int64_t cell = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; cell < M.ncells;
+       cell += (int64_t)gridDim.x * blockDim.x) {
+    CellGeo g;
+    cell_geometry(M, D, cell, g);
+    double* al = M.alpha + cell * kMaxAlpha;
+    if (PHYS == kDarcy) {
+      al[0] = g.detJ / g.coef;
+      for (int t = 1; t < 10; ++t) {
+        const int e = (t - 1) / D, a = (t - 1) % D;
+        al[t] = e < D ? -g.detJ * g.Ji[e + D * a] : 0.0;
+      }
+      for (int t = 10; t < 28; ++t) {
+        const int F = (t - 10) / 3, a = (t - 10) % 3;
+        al[t] = (F < 2 * D && a < D) ? g.aF[F] * g.nF[F][a] : 0.0;
+      }
+      for (int t = 28; t < kMaxAlpha; ++t) {
+        const int F = t - 28;
+        al[t] = F < 2 * D ? M.tau * g.sF[F] * g.aF[F] : 0.0;
+      }
+    } else {
+      const int nsym = D * (D + 1) / 2;
+      const double nu = g.coef, c1 = (1.0 + nu) / M.young, c2 = nu / M.young;
+      for (int t = 0; t < 36; ++t) {
+        const int a = t / 6, b = t % 6;
+        double v = 0.0;
+        if (a < nsym && b < nsym) {
+          int ia, ja, ib, jb;
+          sym_pair(D, a, ia, ja);
+          sym_pair(D, b, ib, jb);
+          const double dd = (a == b) ? (ia == ja ? 1.0 : 2.0) : 0.0;
+          const double tra = ia == ja ? 1.0 : 0.0, trb = ib == jb ? 1.0 : 0.0;
+          v = (c1 * dd - c2 * tra * trb) * g.detJ;
+        }
+        al[t] = v;
+      }
+      for (int t = 36; t < 45; ++t) {
+        const int e = (t - 36) / D, d = (t - 36) % D;
+        al[t] = e < D ? g.detJ * g.Ji[e + D * d] : 0.0;
+      }
+      for (int t = 45; t < 63; ++t) {
+        const int F = (t - 45) / 3, d = (t - 45) % 3;
+        al[t] = (F < 2 * D && d < D) ? g.aF[F] * g.nF[F][d] : 0.0;
+      }
+      for (int t = 63; t < kMaxAlpha; ++t) {
+        const int F = t - 63;
+        al[t] = F < 2 * D ? M.tau * g.aF[F] : 0.0;
+      }
+    }
+    const int ncomp = PHYS == kDarcy ? D + 1 : D;
+    double* sq = M.srcq + cell * (int64_t)T.nqc * 4;
+    for (int q = 0; q < T.nqc; ++q) {
+      double v[4] = {0.0, 0.0, 0.0, 0.0};
+      if (M.source == 100) {
+        const double* p = M.src + (cell * T.nqc + q) * ncomp;
+        for (int c = 0; c < 4; ++c)
+          if (c < ncomp) v[c] = p[c];
+      } else if (M.source != 0) {
+        double x[3];
+        for (int a = 0; a < 3; ++a) {
+          double t = g.x0[a];
+          for (int e = 0; e < 3; ++e)
+            if (a < D && e < D) t += g.J[a + D * e] * T.xq[3 * q + e];
+          x[a] = t;
+        }
+        double f[3], gs = 0.0;
+        if (PHYS == kDarcy) {
+          darcy_source(M.source, D, x, g.coef, f, &gs);
+          for (int c = 0; c < 3; ++c)
+            if (c < D) v[c] = f[c];
+          v[D] = gs;
+        } else {
+          elas_source(M.source, D, x, M.young, g.coef, f);
+          for (int c = 0; c < 3; ++c) v[c] = f[c];
+        }
+      }
+      for (int c = 0; c < 4; ++c) sq[q * 4 + c] = v[c] * g.detJ;
+    }
+  }
+}
+
+struct FusedShared {
+  double alpha[kMaxAlpha];  // per-cell coefficients of the entry program
+  double src[kMaxQP][4];    // |det J| * source values at cell quadrature points
+};
+
+template <int PHYS>
 struct FusedGen {
   Tables T;
   MeshDev M;
@@ -171,225 +261,43 @@ struct FusedGen {
   using Shared = FusedShared;
 
   __device__ void begin(int64_t cell, Shared& sh) const {
-    const int D = T.D;
-    if (threadIdx.x == 0) {
-      cell_geometry(M, D, cell, sh.geo);
-      if (T.phys == kElasticity) {
-        double nu = sh.geo.coef, c1 = (1.0 + nu) / M.young, c2 = nu / M.young;
-        int nsym = D * (D + 1) / 2;
-        for (int a = 0; a < nsym; ++a)
-          for (int b = 0; b < nsym; ++b) {
-            int ia, ja, ib, jb;
-            sym_pair(D, a, ia, ja);
-            sym_pair(D, b, ib, jb);
-            double dd = (a == b) ? (ia == ja ? 1.0 : 2.0) : 0.0;
-            double tra = ia == ja ? 1.0 : 0.0, trb = ib == jb ? 1.0 : 0.0;
-            sh.geo.K[a][b] = c1 * dd - c2 * tra * trb;
-          }
-      }
-    }
-    __syncthreads();
-    // source values at the cell quadrature points
-    const int ncomp = T.phys == kDarcy ? D + 1 : D;
-    for (int q = threadIdx.x; q < T.nqc; q += blockDim.x) {
-      double v0 = 0.0, v1 = 0.0, v2 = 0.0, v3 = 0.0;
-      if (M.source == 100) {
-        const double* p = M.src + (cell * T.nqc + q) * ncomp;
-        v0 = p[0];
-        v1 = p[1];
-        if (ncomp > 2) v2 = p[2];
-        if (ncomp > 3) v3 = p[3];
-      } else if (M.source != 0) {
-        double x[3];
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-          double t = sh.geo.x0[a];
-#pragma unroll
-          for (int e = 0; e < 3; ++e)
-            if (a < D && e < D) t += sh.geo.J[a + D * e] * T.xq[3 * q + e];
-          x[a] = t;
-        }
-        double f[3], gs = 0.0;
-        if (T.phys == kDarcy)
-          darcy_source(M.source, D, x, sh.geo.coef, f, &gs);
-        else
-          elas_source(M.source, D, x, M.young, sh.geo.coef, f);
-        v0 = f[0];
-        v1 = f[1];
-        v2 = f[2];
-        if (T.phys == kDarcy) {
-          // potential source g sits right after the D flux components
-          if (D == 2) v2 = gs; else v3 = gs;
-        }
-      }
-      sh.src[q][0] = v0;
-      sh.src[q][1] = v1;
-      sh.src[q][2] = v2;
-      sh.src[q][3] = v3;
-    }
+    for (int t = threadIdx.x; t < kMaxAlpha; t += blockDim.x)
+      sh.alpha[t] = M.alpha[cell * kMaxAlpha + t];
+    for (int t = threadIdx.x; t < T.nqc * 4; t += blockDim.x)
+      sh.src[t >> 2][t & 3] = M.srcq[cell * (int64_t)T.nqc * 4 + t];
   }
 
   __device__ double entry(const Shared& sh, int row, int col) const {
-    return T.phys == kDarcy ? darcy(sh, row, col) : elas(sh, row, col);
+    const int n = T.n;
+    if (col < n) {
+      const int e = col * n + row;
+      const int k0 = T.prog_off[e], k1 = T.prog_off[e + 1];
+      double acc = 0.0;
+      for (int k = k0; k < k1; ++k) acc = fma(sh.alpha[T.prog_id[k]], T.prog_val[k], acc);
+      return acc;
+    }
+    return rhs(sh, row);
   }
 
-  // Darcy HDG, test/DarcyHDGTests.jl:125-135 (kappa^-1 on the mass term)
-  __device__ double darcy(const Shared& sh, int row, int col) const {
-    const CellGeo& g = sh.geo;
-    const int D = T.D, np = T.np, nq = T.nq, nm = T.nm, na = T.na, ni = T.ni, n = T.n;
-    const double tau = M.tau;
-    if (row < na) {  // flux test v (component a, function i)
-      const int a = row / np, i = row - a * np;
-      if (col < na) {
-        const int b = col / np, j = col - b * np;
-        return a == b ? T.Mpp[i * np + j] * (g.detJ / g.coef) : 0.0;
+  // right-hand side: per-cell quadrature of the source against the test basis
+  __device__ double rhs(const Shared& sh, int row) const {
+    const int D = T.D, np = T.np, nq = T.nq, na = T.na, ni = T.ni;
+    if (row >= ni) return 0.0;
+    double s = 0.0;
+    if (PHYS == kDarcy) {
+      if (row < na) {  // v . f
+        const int a = row / np, i = row - a * np;
+        for (int q = 0; q < T.nqc; ++q) s += T.Bp[q * np + i] * sh.src[q][a];
+      } else {  // w g
+        const int i = row - na;
+        for (int q = 0; q < T.nqc; ++q) s += T.Bq[q * nq + i] * sh.src[q][D];
       }
-      if (col < ni) {  // -(div v) p
-        const int j = col - na;
-        double s = 0.0;
-        for (int e = 0; e < D; ++e) s += g.Ji[e + D * a] * T.Gpq[(e * np + i) * nq + j];
-        return -g.detJ * s;
-      }
-      if (col < n) {  // (v.n) lambda
-        const int F = (col - ni) / nm, m = col - ni - F * nm;
-        if (!((T.pmask[i] >> F) & 1)) return 0.0;
-        return g.aF[F] * g.nF[F][a] * T.Fpm[(F * np + i) * nm + m];
-      }
-      double s = 0.0;  // rhs: v . f
-      for (int q = 0; q < T.nqc; ++q) s += T.Bp[q * np + i] * sh.src[q][a];
-      return s * g.detJ;
+      return s;
     }
-    if (row < ni) {  // potential test w
-      const int i = row - na;
-      if (col < na) {  // -grad(w).q + w (q.n)
-        const int b = col / np, j = col - b * np;
-        double s = 0.0;
-        for (int e = 0; e < D; ++e) s += g.Ji[e + D * b] * T.Gqp[(e * nq + i) * np + j];
-        double r = -g.detJ * s;
-        unsigned fm = T.qmask[i] & T.pmask[j];
-        for (int F = 0; F < 2 * D; ++F)
-          if ((fm >> F) & 1) r += g.aF[F] * g.nF[F][b] * T.Fqp[(F * nq + i) * np + j];
-        return r;
-      }
-      if (col < ni) {  // tau w p (n.n_o)
-        const int j = col - na;
-        unsigned fm = T.qmask[i] & T.qmask[j];
-        double r = 0.0;
-        for (int F = 0; F < 2 * D; ++F)
-          if ((fm >> F) & 1) r += tau * g.sF[F] * g.aF[F] * T.Fqq[(F * nq + i) * nq + j];
-        return r;
-      }
-      if (col < n) {  // -tau w lambda (n.n_o)
-        const int F = (col - ni) / nm, m = col - ni - F * nm;
-        if (!((T.qmask[i] >> F) & 1)) return 0.0;
-        return -tau * g.sF[F] * g.aF[F] * T.Fqm[(F * nq + i) * nm + m];
-      }
-      double s = 0.0;  // rhs: w g
-      for (int q = 0; q < T.nqc; ++q) s += T.Bq[q * nq + i] * sh.src[q][D];
-      return s * g.detJ;
-    }
-    // trace test mu on facet F
-    const int F = (row - ni) / nm, m = row - ni - F * nm;
-    if (col < na) {  // mu (q.n)
-      const int b = col / np, j = col - b * np;
-      if (!((T.pmask[j] >> F) & 1)) return 0.0;
-      return g.aF[F] * g.nF[F][b] * T.Fpm[(F * np + j) * nm + m];
-    }
-    if (col < ni) {  // tau mu p (n.n_o)
-      const int j = col - na;
-      if (!((T.qmask[j] >> F) & 1)) return 0.0;
-      return tau * g.sF[F] * g.aF[F] * T.Fqm[(F * nq + j) * nm + m];
-    }
-    if (col < n) {  // -tau mu lambda (n.n_o)
-      const int G = (col - ni) / nm, m2 = col - ni - G * nm;
-      if (G != F) return 0.0;
-      return -tau * g.sF[F] * g.aF[F] * T.Fmm[m * nm + m2];
-    }
-    return 0.0;
-  }
-
-  // Elasticity HDG, test/LinearElasticityHDGTests.jl:140-150 (no projection)
-  __device__ double elas(const Shared& sh, int row, int col) const {
-    const CellGeo& g = sh.geo;
-    const int D = T.D, np = T.np, nq = T.nq, nm = T.nm, na = T.na, ni = T.ni, n = T.n;
-    const int ntf = T.ntf;
-    const double tau = M.tau;
-    if (row < na) {  // stress test v = p_i E_a
-      const int a = row / np, i = row - a * np;
-      int ia, ja;
-      sym_pair(D, a, ia, ja);
-      if (col < na) {
-        const int b = col / np, j = col - b * np;
-        return g.K[a][b] * T.Mpp[i * np + j] * g.detJ;
-      }
-      if (col < ni) {  // (div v) . u
-        const int c = (col - na) / nq, j = col - na - c * nq;
-        double s = 0.0;
-        if (c == ja)
-          for (int e = 0; e < D; ++e) s += g.Ji[e + D * ia] * T.Gpq[(e * np + i) * nq + j];
-        if (c == ia && ia != ja)
-          for (int e = 0; e < D; ++e) s += g.Ji[e + D * ja] * T.Gpq[(e * np + i) * nq + j];
-        return g.detJ * s;
-      }
-      if (col < n) {  // -(v.n) . uhat
-        const int F = (col - ni) / ntf, r = col - ni - F * ntf, c = r / nm, m = r - c * nm;
-        if (!((T.pmask[i] >> F) & 1)) return 0.0;
-        double vn = (c == ia ? g.nF[F][ja] : 0.0) + ((c == ja && ia != ja) ? g.nF[F][ia] : 0.0);
-        return -g.aF[F] * vn * T.Fpm[(F * np + i) * nm + m];
-      }
-      return 0.0;
-    }
-    if (row < ni) {  // displacement test omega = q_j e_c
-      const int c = (row - na) / nq, j = row - na - c * nq;
-      if (col < na) {  // -omega . div(sigma)
-        const int a = col / np, i = col - a * np;
-        int ia, ja;
-        sym_pair(D, a, ia, ja);
-        double s = 0.0;
-        if (c == ja)
-          for (int e = 0; e < D; ++e) s += g.Ji[e + D * ia] * T.Gpq[(e * np + i) * nq + j];
-        if (c == ia && ia != ja)
-          for (int e = 0; e < D; ++e) s += g.Ji[e + D * ja] * T.Gpq[(e * np + i) * nq + j];
-        return -g.detJ * s;
-      }
-      if (col < ni) {  // tau omega . u
-        const int c2 = (col - na) / nq, j2 = col - na - c2 * nq;
-        if (c2 != c) return 0.0;
-        unsigned fm = T.qmask[j] & T.qmask[j2];
-        double r = 0.0;
-        for (int F = 0; F < 2 * D; ++F)
-          if ((fm >> F) & 1) r += tau * g.aF[F] * T.Fqq[(F * nq + j) * nq + j2];
-        return r;
-      }
-      if (col < n) {  // -tau omega . uhat
-        const int F = (col - ni) / ntf, r = col - ni - F * ntf, c2 = r / nm, m = r - c2 * nm;
-        if (c2 != c || !((T.qmask[j] >> F) & 1)) return 0.0;
-        return -tau * g.aF[F] * T.Fqm[(F * nq + j) * nm + m];
-      }
-      double s = 0.0;  // rhs: -omega . f
-      for (int q = 0; q < T.nqc; ++q) s += T.Bq[q * nq + j] * sh.src[q][c];
-      return -s * g.detJ;
-    }
-    const int F = (row - ni) / ntf, rr = row - ni - F * ntf, c = rr / nm, m = rr - c * nm;
-    if (col < na) {  // mu . (sigma n)
-      const int a = col / np, i = col - a * np;
-      int ia, ja;
-      sym_pair(D, a, ia, ja);
-      if (!((T.pmask[i] >> F) & 1)) return 0.0;
-      double vn = (c == ia ? g.nF[F][ja] : 0.0) + ((c == ja && ia != ja) ? g.nF[F][ia] : 0.0);
-      return g.aF[F] * vn * T.Fpm[(F * np + i) * nm + m];
-    }
-    if (col < ni) {  // -tau mu . u
-      const int c2 = (col - na) / nq, j = col - na - c2 * nq;
-      if (c2 != c || !((T.qmask[j] >> F) & 1)) return 0.0;
-      return -tau * g.aF[F] * T.Fqm[(F * nq + j) * nm + m];
-    }
-    if (col < n) {  // tau mu . uhat
-      const int G = (col - ni) / ntf, r2 = col - ni - G * ntf, c2 = r2 / nm, m2 = r2 - c2 * nm;
-      if (G != F || c2 != c) return 0.0;
-      return tau * g.aF[F] * T.Fmm[m * nm + m2];
-    }
-    return 0.0;
+    if (row < na) return 0.0;
+    const int c = (row - na) / nq, j = row - na - c * nq;  // -omega . f
+    for (int q = 0; q < T.nqc; ++q) s += T.Bq[q * nq + j] * sh.src[q][c];
+    return -s;
   }
 };
 
@@ -433,18 +341,27 @@ struct GenCall {
     return gen.entry(sh, row, col, cell);
   }
 };
-template <>
-struct GenCall<FusedGen> {
-  __device__ static double entry(const FusedGen& gen, const FusedShared& sh, int row, int col,
-                                 int64_t) {
+template <int PHYS>
+struct GenCall<FusedGen<PHYS>> {
+  __device__ static double entry(const FusedGen<PHYS>& gen, const FusedShared& sh, int row,
+                                 int col, int64_t) {
     return gen.entry(sh, row, col);
   }
 };
 
 // --------------------------------------------------------------------------
 // Augmented elimination kernel
+//
+// Outputs (CondenseOut):
+//   S, g  boundary rows after the ni elimination steps;
+//   U     packed upper factor of A_ii in LAPACK row order: column c holds rows
+//         0..c at U[c(c+1)/2 + k];
+//   Y     L^-1 P [A_ib b_i] in LAPACK row order, [nb+1][ni] column-major;
+//   X     (backward map only) the back-substituted right-hand side [ni].
+// The hot loops are branch-free: inactive rows get a zero multiplier, the
+// pivot row index is made wave-uniform, and every update is a plain FMA.
 // --------------------------------------------------------------------------
-template <int PR, int PC, int R, int CC, class Gen>
+template <int PR, int PC, int R, int CC, class Gen, bool BACKSUB>
 __global__ void __launch_bounds__(PR * PC)
     condense_kernel(Gen gen, CondenseOut out, int64_t ncells, int ni, int nb) {
   constexpr int NT = PR * PC;
@@ -452,8 +369,8 @@ __global__ void __launch_bounds__(PR * PC)
   constexpr int NCMAX = PC * CC;
   static_assert(PR <= 64 && (64 % PR) == 0, "thread-grid columns must sit inside one wavefront");
   static_assert(R <= 32, "row mask is 32 bits");
-  const int n = ni + nb;  // rows
-  const int ncol = n + 1; // columns incl. right-hand side
+  const int n = ni + nb;   // rows
+  const int ncol = n + 1;  // columns incl. right-hand side
   const int tid = threadIdx.x;
   const int tr = tid % PR;
   const int tc = tid / PR;
@@ -470,7 +387,7 @@ __global__ void __launch_bounds__(PR * PC)
 
   for (int64_t cell = blockIdx.x; cell < ncells; cell += gridDim.x) {
     gen.begin(cell, s_gen);
-    for (int i = tid; i < ni; i += NT) {
+    for (int i = tid; i < NMAX; i += NT) {
       s_pos[i] = i;
       s_at[i] = i;
     }
@@ -509,180 +426,185 @@ __global__ void __launch_bounds__(PR * PC)
     }
 
     // ---------------- forward elimination over the ni interior pivots ----
+    // One barrier per step: the owners of column j (one half-wave group)
+    // pick the pivot, publish it with the multipliers and update the LAPACK
+    // position bookkeeping; after the barrier every thread group fetches the
+    // pivot row from its own lane p % PR by shuffle and updates its tile.
+#pragma unroll 1
     for (int j = 0; j < ni; ++j) {
       const int buf = j & 1;
       const int jc = j % PC, jl = j / PC;
       if (tc == jc) {
-        double bav = -1.0, bval = 0.0;
-        int bpos = INT_MAX, brow = -1;
+        double colv[R];
 #pragma unroll
-        for (int c = 0; c < CC; ++c) {
+        for (int r = 0; r < R; ++r) colv[r] = 0.0;
+#pragma unroll
+        for (int c = 0; c < CC; ++c)
           if (c == jl) {
 #pragma unroll
-            for (int r = 0; r < R; ++r) {
-              const int row = tr + PR * r;
-              if (row < ni && ((act >> r) & 1u)) {
-                const double v = a[r][c];
-                const double av = fabs(v);
-                const int p = s_pos[row];
-                if (av > bav || (av == bav && p < bpos)) {
-                  bav = av;
-                  bval = v;
-                  bpos = p;
-                  brow = row;
-                }
-              }
-            }
+            for (int r = 0; r < R; ++r) colv[r] = a[r][c];
           }
-        }
-#pragma unroll
-        for (int off = 1; off < PR; off <<= 1) {
-          const double oav = __shfl_xor(bav, off, PR);
-          const double oval = __shfl_xor(bval, off, PR);
-          const int opos = __shfl_xor(bpos, off, PR);
-          const int orow = __shfl_xor(brow, off, PR);
-          if (oav > bav || (oav == bav && opos < bpos)) {
-            bav = oav;
-            bval = oval;
-            bpos = opos;
-            brow = orow;
-          }
-        }
-        const double rinv = bval != 0.0 ? 1.0 / bval : 0.0;
-        if (tr == 0) {
-          s_piv[buf] = brow;
-          s_diag[j] = bval;
-          if (bval == 0.0 && s_info == 0) s_info = j + 1;
-        }
-#pragma unroll
-        for (int c = 0; c < CC; ++c) {
-          if (c == jl) {
-#pragma unroll
-            for (int r = 0; r < R; ++r) {
-              const int row = tr + PR * r;
-              if (((act >> r) & 1u) && row != brow) s_lcol[buf][row] = a[r][c] * rinv;
-            }
-          }
-        }
-      }
-      __syncthreads();
-      const int p = s_piv[buf];
-      if (tr == p % PR) {
-        const int pl = p / PR;
+        double bav = -1.0;
+        int bpos = INT_MAX;
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-          if (r == pl) {
+          const int row = tr + PR * r;
+          const bool cand = row < ni && ((act >> r) & 1u);
+          const double av = cand ? fabs(colv[r]) : -1.0;
+          const int pr = s_pos[row];
+          const bool better = av > bav || (av == bav && cand && pr < bpos);
+          bav = better ? av : bav;
+          bpos = better ? pr : bpos;
+        }
+        double m = bav;
 #pragma unroll
-            for (int c = 0; c < CC; ++c) {
-              const int col = tc + PC * c;
-              if (col < ncol) s_prow[buf][col] = a[r][c];
-            }
-            act &= ~(1u << r);
-          }
+        for (int off = 1; off < PR; off <<= 1) m = fmax(m, __shfl_xor(m, off, PR));
+        int key = bav == m ? bpos : INT_MAX;
+#pragma unroll
+        for (int off = 1; off < PR; off <<= 1) key = min(key, __shfl_xor(key, off, PR));
+        const int p = s_at[key];  // pivot row (smallest position among the maxima)
+        const int pl = p / PR;
+        double mine = 0.0;
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+          if (r == pl) mine = colv[r];
+        const double pv = __shfl(mine, p % PR, PR);
+        const double rinv = pv != 0.0 ? 1.0 / pv : 0.0;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const int row = tr + PR * r;
+          const bool use = ((act >> r) & 1u) && row != p;
+          s_lcol[buf][row] = use ? colv[r] * rinv : 0.0;
+        }
+        if (tr == 0) {
+          s_piv[buf] = p;
+          s_diag[j] = pv;
+          if (pv == 0.0 && s_info == 0) s_info = j + 1;
+          // LAPACK interchange of positions j and pos(p) = key
+          const int q = s_at[j];
+          s_at[key] = q;
+          s_pos[q] = key;
+          s_at[j] = p;
+          s_pos[p] = j;
         }
       }
-      if (tid == 0) {
-        // LAPACK row interchange bookkeeping: rows at positions j and pos[p] swap
-        const int pp = s_pos[p];
-        const int q = s_at[j];
-        s_at[pp] = q;
-        s_pos[q] = pp;
-        s_at[j] = p;
-        s_pos[p] = j;
+      __syncthreads();
+      const int p = __builtin_amdgcn_readfirstlane(s_piv[buf]);
+      const int pl = p / PR, plane = p % PR;
+      if (tr == plane) {
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+          if (r == pl) {  // wave-uniform
+#pragma unroll
+            for (int c = 0; c < CC; ++c) s_prow[buf][tc + PC * c] = a[r][c];
+          }
+        act &= ~(1u << pl);
       }
       __syncthreads();
+      double l[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const double lv = s_lcol[buf][tr + PR * r];
+        l[r] = ((act >> r) & 1u) ? lv : 0.0;
+      }
+      double u[CC];
+#pragma unroll
+      for (int c = 0; c < CC; ++c) u[c] = s_prow[buf][tc + PC * c];
 #pragma unroll
       for (int c = 0; c < CC; ++c) {
-        if (c < jl) continue;  // whole local column already eliminated
-        const int col = tc + PC * c;
-        if (col <= j || col >= ncol) continue;
-        const double u = s_prow[buf][col];
+        if (c < jl) continue;  // wave-uniform: local column fully eliminated
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-          if ((act >> r) & 1u) {
-            const int row = tr + PR * r;
-            a[r][c] = fma(-s_lcol[buf][row], u, a[r][c]);
-          }
-        }
+        for (int r = 0; r < R; ++r) a[r][c] = fma(-l[r], u[c], a[r][c]);
       }
     }
-
     __syncthreads();  // last step's pivot row / multipliers fully consumed
-
-    // ---------------- back substitution on the interior right-hand sides -
-    // (act now flags exactly the boundary rows; only needed when W is kept)
     const int info = s_info;
-    if (out.W != nullptr) {
+
+    // ---------------- back substitution (backward map only) ---------------
+    if (BACKSUB) {
+#pragma unroll 1
       for (int k = ni - 1; k >= 0; --k) {
         const int buf = k & 1;
-        const int p = s_at[k];
+        const int p = __builtin_amdgcn_readfirstlane(s_at[k]);
+        const int pl = p / PR;
         const double d = s_diag[k];
         if (tr == p % PR) {
-          const int pl = p / PR;
 #pragma unroll
-          for (int r = 0; r < R; ++r) {
+          for (int r = 0; r < R; ++r)
             if (r == pl) {
 #pragma unroll
               for (int c = 0; c < CC; ++c) {
                 const int col = tc + PC * c;
-                if (col >= ni && col < ncol) {
-                  const double x = d != 0.0 ? a[r][c] / d : 0.0;
-                  a[r][c] = x;
-                  s_prow[buf][col] = x;
-                }
+                const double x = d != 0.0 ? a[r][c] / d : 0.0;
+                a[r][c] = col >= ni ? x : a[r][c];
+                s_prow[buf][col] = x;
               }
             }
-          }
         }
         const int kc = k % PC, kl = k / PC;
         if (tc == kc) {
 #pragma unroll
-          for (int c = 0; c < CC; ++c) {
+          for (int c = 0; c < CC; ++c)
             if (c == kl) {
 #pragma unroll
               for (int r = 0; r < R; ++r) {
                 const int row = tr + PR * r;
-                if (row < ni && s_pos[row] < k) s_lcol[buf][row] = a[r][c];
+                const bool use = row < ni && s_pos[row] < k;
+                s_lcol[buf][row] = use ? a[r][c] : 0.0;
               }
             }
-          }
         }
         __syncthreads();
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-          const int row = tr + PR * r;
-          if (row < ni && s_pos[row] < k) {
-            const double l = s_lcol[buf][row];
+          const double lr = s_lcol[buf][tr + PR * r];
 #pragma unroll
-            for (int c = 0; c < CC; ++c) {
-              const int col = tc + PC * c;
-              if (col >= ni && col < ncol) a[r][c] = fma(-l, s_prow[buf][col], a[r][c]);
-            }
+          for (int c = 0; c < CC; ++c) {
+            const int col = tc + PC * c;
+            const double u = col >= ni ? s_prow[buf][col] : 0.0;
+            a[r][c] = fma(-lr, u, a[r][c]);
           }
         }
       }
+      __syncthreads();
     }
 
-    // ---------------- outputs ----------------
+    // ---------------- outputs, one slab of PC columns at a time ----------
+    // (registers -> LDS with constant indices, then coalesced global stores)
+#pragma unroll 1
+    for (int c = 0; c < CC; ++c) {
+      if (PC * c + PC <= ni && out.U == nullptr) continue;  // nothing to store
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-      const int row = tr + PR * r;
-      if (row >= n) continue;
+      for (int c2 = 0; c2 < CC; ++c2)
+        if (c2 == c) {
 #pragma unroll
-      for (int c = 0; c < CC; ++c) {
-        const int col = tc + PC * c;
-        if (col < ni || col >= ncol) continue;
-        const double v = a[r][c];
+          for (int r = 0; r < R; ++r) s_slab[tc][tr + PR * r] = a[r][c2];
+        }
+      __syncthreads();
+      for (int t = tid; t < n * PC; t += NT) {
+        const int cc = t / n, row = t - cc * n;
+        const int col = PC * c + cc;
+        if (col >= ncol) continue;
+        const double v = s_slab[cc][row];
         if (row >= ni) {
+          if (col < ni) continue;
           if (col < n) {
             if (out.S) out.S[(cell * nb + (col - ni)) * nb + (row - ni)] = v;
-          } else {
-            if (out.g) out.g[cell * nb + (row - ni)] = v;
+          } else if (out.g) {
+            out.g[cell * nb + (row - ni)] = v;
           }
-        } else if (out.W) {
-          out.W[(cell * (nb + 1) + (col - ni)) * ni + s_pos[row]] = v;
+        } else {
+          const int k = s_pos[row];
+          if (col < ni) {
+            if (out.U && col >= k)
+              out.U[cell * ((int64_t)ni * (ni + 1) / 2) + col * (col + 1) / 2 + k] = v;
+          } else {
+            if (out.Y) out.Y[(cell * (nb + 1) + (col - ni)) * ni + k] = v;
+            if (out.X && col == n) out.X[cell * ni + k] = v;
+          }
         }
       }
+      __syncthreads();
     }
     if (tid == 0 && out.info) out.info[cell] = info;
     __syncthreads();  // LDS reuse by the next cell
@@ -690,37 +612,200 @@ __global__ void __launch_bounds__(PR * PC)
 }
 
 // --------------------------------------------------------------------------
-// Recovery: x_i = w - W lambda_cell, scattered into the free-value vector
-// [q of all cells; u of all cells; lambda_free].
+// Wave-per-cell elimination for small systems (n <= 64 rows, n+1 <= NCOL
+// columns): lane i owns row i of [A | b] in registers.  The column loop is
+// unrolled so every register index is a compile-time constant; the pivot row
+// is broadcast with readlane (its values land in SGPRs and feed the FMAs as
+// scalar operands); the LAPACK position of each row lives in a register.
+// No LDS traffic and no barriers in the elimination.
 // --------------------------------------------------------------------------
-__global__ void recover_kernel(const double* __restrict__ W, const int32_t* __restrict__ cell_dofs,
-                               const double* __restrict__ lam_free,
-                               const double* __restrict__ lam_dir, double* __restrict__ out,
-                               int64_t ncells, int ni, int nb, int na, int cells_per_block) {
-  extern __shared__ double s_lam[];  // [cells_per_block][nb]
-  const int64_t c0 = (int64_t)blockIdx.x * cells_per_block;
-  for (int t = threadIdx.x; t < cells_per_block * nb; t += blockDim.x) {
-    const int lc = t / nb, j = t - lc * nb;
-    const int64_t cell = c0 + lc;
-    double v = 0.0;
-    if (cell < ncells) {
-      const int32_t d = cell_dofs[cell * nb + j];
-      v = d >= 0 ? lam_free[d] : lam_dir[-(int64_t)d - 1];
+__device__ inline double wave_readlane_f64(double v, int lane) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+  return __hiloint2double(hi, lo);
+}
+
+template <int NI, int NB, class Gen, bool BACKSUB>
+__global__ void __launch_bounds__(64)
+    condense_wave_kernel(Gen gen, CondenseOut out, int64_t ncells) {
+  constexpr int N = NI + NB, NCOL = N + 1;
+  constexpr int SW = 8;                      // slab width (columns staged per pass)
+  constexpr int NS = (NCOL + SW - 1) / SW;   // slabs
+  static_assert(N <= 64, "wave kernel: one row per lane");
+  const int ni = NI, nb = NB, n = N;
+  const int lane = threadIdx.x;
+  __shared__ typename Gen::Shared s_gen;
+  __shared__ double s_slab[SW * 64];
+
+  for (int64_t cell = blockIdx.x; cell < ncells; cell += gridDim.x) {
+    gen.begin(cell, s_gen);
+    __syncthreads();
+    // generate SW columns at a time into LDS, pick this lane's row
+    double a[NCOL];
+#pragma unroll 1
+    for (int sl = 0; sl < NS; ++sl) {
+      for (int t = lane; t < N * SW; t += 64) {
+        const int cc = t / N, row = t - cc * N;
+        const int col = SW * sl + cc;
+        s_slab[cc * 64 + row] = col < NCOL ? GenCall<Gen>::entry(gen, s_gen, row, col, cell) : 0.0;
+      }
+      __syncthreads();
+      double v[SW];
+#pragma unroll
+      for (int cc = 0; cc < SW; ++cc) v[cc] = lane < N ? s_slab[cc * 64 + lane] : 0.0;
+#pragma unroll
+      for (int s2 = 0; s2 < NS; ++s2)
+        if (s2 == sl) {
+#pragma unroll
+          for (int cc = 0; cc < SW; ++cc)
+            if (SW * s2 + cc < NCOL) a[SW * s2 + cc] = v[cc];
+        }
+      __syncthreads();
     }
-    s_lam[t] = v;
+
+    bool active = lane < N;
+    int mypos = lane;        // LAPACK position of this row (interior rows)
+    int info = 0;
+    double diag_mine = 0.0;  // U(k,k) when this row is the pivot of step k
+#pragma unroll
+    for (int j = 0; j < NI; ++j) {
+      const double v = a[j];
+      const bool cand = active && lane < NI;
+      const double av = cand ? fabs(v) : -1.0;
+      double m = av;  // wave max of |v|
+#pragma unroll
+      for (int off = 32; off >= 1; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+      // among the maxima, the row with the smallest LAPACK position
+      int key = (cand && av == m) ? mypos : INT_MAX;
+#pragma unroll
+      for (int off = 32; off >= 1; off >>= 1) key = min(key, __shfl_xor(key, off, 64));
+      const uint64_t hit = __ballot(cand && av == m && mypos == key);
+      const int p = __builtin_amdgcn_readfirstlane(__ffsll((unsigned long long)hit) - 1);
+      const double pv = wave_readlane_f64(v, p);
+      if (pv == 0.0 && info == 0) info = j + 1;
+      const double rinv = pv != 0.0 ? 1.0 / pv : 0.0;
+      // LAPACK interchange of positions j and pos(p)
+      const int pos_p = __builtin_amdgcn_readlane(mypos, p);
+      if (mypos == j) mypos = pos_p;
+      if (lane == p) {
+        mypos = j;
+        active = false;
+        diag_mine = v;
+      }
+      const double l = active ? v * rinv : 0.0;
+#pragma unroll
+      for (int c = j + 1; c < NCOL; ++c) a[c] = fma(-l, wave_readlane_f64(a[c], p), a[c]);
+    }
+
+    if (BACKSUB) {
+      // backward map: single rhs column at index NI (NB == 0)
+#pragma unroll
+      for (int k = NI - 1; k >= 0; --k) {
+        const uint64_t own = __ballot(lane < NI && mypos == k);
+        const int p = __builtin_amdgcn_readfirstlane(__ffsll((unsigned long long)own) - 1);
+        const double d = wave_readlane_f64(diag_mine, p);
+        const double x = d != 0.0 ? wave_readlane_f64(a[NCOL - 1], p) / d : 0.0;
+        if (lane == p)
+          a[NCOL - 1] = x;
+        else if (lane < NI && mypos < k)
+          a[NCOL - 1] = fma(-a[k], x, a[NCOL - 1]);
+      }
+    }
+
+    // ---------------- outputs ----------------
+    if (lane < N) {
+      if (lane >= NI) {
+        const int br = lane - NI;
+#pragma unroll
+        for (int c = NI; c < N; ++c)
+          if (out.S) out.S[(cell * nb + (c - NI)) * nb + br] = a[c];
+        if (out.g) out.g[cell * nb + br] = a[N];
+      } else {
+        const int k = mypos;
+        const int64_t ubase = cell * ((int64_t)ni * (ni + 1) / 2);
+        if (out.U) {
+#pragma unroll
+          for (int c = 0; c < NI; ++c)
+            if (c >= k) out.U[ubase + c * (c + 1) / 2 + k] = a[c];
+        }
+        if (out.Y) {
+#pragma unroll
+          for (int c = NI; c < NCOL; ++c) out.Y[(cell * (nb + 1) + (c - NI)) * ni + k] = a[c];
+        }
+        if (out.X) out.X[cell * ni + k] = a[N];
+      }
+    }
+    (void)n;
+    if (lane == 0 && out.info) out.info[cell] = info;
+    __syncthreads();
   }
-  __syncthreads();
-  for (int t = threadIdx.x; t < cells_per_block * ni; t += blockDim.x) {
-    const int lc = t / ni, i = t - lc * ni;
-    const int64_t cell = c0 + lc;
-    if (cell >= ncells) continue;
-    const double* Wc = W + cell * (int64_t)(nb + 1) * ni;
-    double s = Wc[(int64_t)nb * ni + i];
-    for (int j = 0; j < nb; ++j) s = fma(-Wc[(int64_t)j * ni + i], s_lam[lc * nb + j], s);
-    if (i < na)
-      out[cell * na + i] = s;
-    else
-      out[ncells * na + cell * (ni - na) + (i - na)] = s;
+}
+
+// --------------------------------------------------------------------------
+// Recovery (BackwardStaticCondensationMap with the factors kept from the
+// forward pass): r = L^-1 P (b_i - A_ib lambda) = y - Y lambda, then
+// U x = r.  One wavefront per cell; the result is scattered into the
+// free-value vector [q of all cells; u of all cells; lambda_free].
+// --------------------------------------------------------------------------
+template <int KMAX>
+__global__ void __launch_bounds__(256)
+    recover_kernel(const double* __restrict__ U, const double* __restrict__ Y,
+                   const int32_t* __restrict__ cell_dofs, const double* __restrict__ lam_free,
+                   const double* __restrict__ lam_dir, double* __restrict__ out, int64_t ncells,
+                   int ni, int nb, int na) {
+  constexpr int WPB = 4;  // wavefronts per block
+  __shared__ double s_lam[WPB][256];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int64_t usz = (int64_t)ni * (ni + 1) / 2;
+  for (int64_t cell = (int64_t)blockIdx.x * WPB + w; cell < ncells;
+       cell += (int64_t)gridDim.x * WPB) {
+    for (int j = lane; j < nb; j += 64) {
+      const int32_t d = cell_dofs[cell * nb + j];
+      s_lam[w][j] = d >= 0 ? lam_free[d] : lam_dir[-(int64_t)d - 1];
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const double* Yc = Y + cell * (int64_t)(nb + 1) * ni;
+    double r[KMAX];
+#pragma unroll
+    for (int t = 0; t < KMAX; ++t) {
+      const int k = lane + 64 * t;
+      double s = 0.0;
+      if (k < ni) {
+        s = Yc[(int64_t)nb * ni + k];
+        for (int j = 0; j < nb; ++j) s = fma(-Yc[(int64_t)j * ni + k], s_lam[w][j], s);
+      }
+      r[t] = s;
+    }
+    const double* Uc = U + cell * usz;
+#pragma unroll 1
+    for (int c = ni - 1; c >= 0; --c) {
+      const int64_t col0 = (int64_t)c * (c + 1) / 2;
+      const double ucc = Uc[col0 + c];
+      const int tsel = c >> 6, lsel = c & 63;
+      double rc = 0.0;
+#pragma unroll
+      for (int t = 0; t < KMAX; ++t)
+        if (t == tsel) rc = r[t];
+      rc = __shfl(rc, lsel, 64);
+      const double x = ucc != 0.0 ? rc / ucc : 0.0;
+#pragma unroll
+      for (int t = 0; t < KMAX; ++t) {
+        const int k = lane + 64 * t;
+        if (k < c) r[t] = fma(-Uc[col0 + k], x, r[t]);
+        if (k == c) r[t] = x;
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < KMAX; ++t) {
+      const int i = lane + 64 * t;
+      if (i >= ni) continue;
+      if (i < na)
+        out[cell * na + i] = r[t];
+      else
+        out[ncells * na + cell * (ni - na) + (i - na)] = r[t];
+    }
+    __builtin_amdgcn_wave_barrier();
   }
 }
 
@@ -809,7 +894,8 @@ __global__ void rhs_kernel(PairTable P, const double* __restrict__ S, const doub
 }
 
 // Per-cell element systems (debug / parity entry point).
-__global__ void cell_systems_kernel(FusedGen gen, int64_t first, int64_t count, double* A,
+template <int PHYS>
+__global__ void cell_systems_kernel(FusedGen<PHYS> gen, int64_t first, int64_t count, double* A,
                                     double* b) {
   __shared__ FusedShared sh;
   const int n = gen.T.n;
@@ -838,7 +924,7 @@ struct Bucket {
 
 // Thread-grid / register-tile buckets by rows n and columns n+1.
 static bool pick_bucket(int n, Bucket& b) {
-  const Bucket list[] = {{8, 8, 4, 4}, {8, 8, 6, 6}, {8, 8, 9, 9}, {32, 16, 6, 11}, {32, 32, 8, 8}};
+  const Bucket list[] = {{8, 8, 4, 4}, {8, 8, 6, 6}, {8, 8, 9, 9}, {32, 16, 6, 11}};
   for (const auto& x : list)
     if (x.pr * x.r >= n && x.pc * x.cc >= n + 1) {
       b = x;
@@ -854,24 +940,41 @@ static int grid_for(int64_t ncells, int threads) {
   return (int)(ncells < target ? ncells : target);
 }
 
-template <class Gen>
+template <class Gen, bool BACKSUB>
 static hipError_t launch_condense(const Gen& gen, const CondenseOut& out, int64_t ncells, int ni,
                                   int nb, hipStream_t st) {
   if (ncells <= 0) return hipSuccess;
+  const int ncol = ni + nb + 1;
+  {
+    const int64_t target = 256 * 48;
+    const int grid = (int)(ncells < target ? ncells : target);
+#define HDG_WAVE(NI_, NB_)                                                                  \
+  if (ni == NI_ && nb == NB_) {                                                             \
+    hipLaunchKernelGGL((condense_wave_kernel<NI_, NB_, Gen, BACKSUB>), dim3(grid), dim3(64), \
+                       0, st, gen, out, ncells);                                            \
+    return hipGetLastError();                                                               \
+  }
+    // Darcy HDG Q_k classes (2D k=1..3, 3D k=1) and the reference P1/P0/P1 test
+    HDG_WAVE(12, 8)
+    HDG_WAVE(27, 12)
+    HDG_WAVE(48, 16)
+    HDG_WAVE(32, 24)
+    HDG_WAVE(7, 8)
+#undef HDG_WAVE
+  }
   Bucket b;
   if (!pick_bucket(ni + nb, b)) return hipErrorInvalidValue;
   const int grid = grid_for(ncells, b.pr * b.pc);
-#define HDG_LAUNCH(PR, PC, R, CC)                                                          \
-  if (b.pr == PR && b.pc == PC && b.r == R && b.cc == CC) {                                \
-    hipLaunchKernelGGL((condense_kernel<PR, PC, R, CC, Gen>), dim3(grid), dim3(PR * PC), 0, \
-                       st, gen, out, ncells, ni, nb);                                      \
-    return hipGetLastError();                                                              \
+#define HDG_LAUNCH(PR, PC, R, CC)                                                             \
+  if (b.pr == PR && b.pc == PC && b.r == R && b.cc == CC) {                                   \
+    hipLaunchKernelGGL((condense_kernel<PR, PC, R, CC, Gen, BACKSUB>), dim3(grid),             \
+                       dim3(PR * PC), 0, st, gen, out, ncells, ni, nb);                       \
+    return hipGetLastError();                                                                 \
   }
   HDG_LAUNCH(8, 8, 4, 4)
   HDG_LAUNCH(8, 8, 6, 6)
   HDG_LAUNCH(8, 8, 9, 9)
   HDG_LAUNCH(32, 16, 6, 11)
-  HDG_LAUNCH(32, 32, 8, 8)
 #undef HDG_LAUNCH
   return hipErrorInvalidValue;
 }
@@ -881,36 +984,61 @@ bool condense_supported(int n) {
   return pick_bucket(n, b);
 }
 
+static hipError_t launch_prep(const Tables& T, const MeshDev& M, hipStream_t st) {
+  if (M.ncells <= 0) return hipSuccess;
+  const int64_t blocks = (M.ncells + 127) / 128;
+  const int grid = (int)(blocks < 8192 ? blocks : 8192);
+  if (T.phys == kDarcy)
+    hipLaunchKernelGGL(prep_kernel<kDarcy>, dim3(grid), dim3(128), 0, st, T, M);
+  else
+    hipLaunchKernelGGL(prep_kernel<kElasticity>, dim3(grid), dim3(128), 0, st, T, M);
+  return hipGetLastError();
+}
+
 hipError_t launch_condense_fused(const Tables& T, const MeshDev& M, const CondenseOut& out,
                                  hipStream_t st) {
-  FusedGen gen{T, M};
-  return launch_condense(gen, out, M.ncells, T.ni, T.nb, st);
+  hipError_t e = launch_prep(T, M, st);
+  if (e != hipSuccess) return e;
+  if (T.phys == kDarcy) {
+    FusedGen<kDarcy> gen{T, M};
+    return launch_condense<FusedGen<kDarcy>, false>(gen, out, M.ncells, T.ni, T.nb, st);
+  }
+  FusedGen<kElasticity> gen{T, M};
+  return launch_condense<FusedGen<kElasticity>, false>(gen, out, M.ncells, T.ni, T.nb, st);
 }
 
 hipError_t launch_condense_dense(const DenseIn& in, int64_t ncells, const CondenseOut& out,
                                  hipStream_t st) {
   DenseGen gen{in};
-  return launch_condense(gen, out, ncells, in.ni, in.nb, st);
+  return launch_condense<DenseGen, false>(gen, out, ncells, in.ni, in.nb, st);
 }
 
 hipError_t launch_backward_dense(const DenseIn& in, const double* x, int64_t ncells, double* xi,
                                  int32_t* info, hipStream_t st) {
   BackwardGen gen{in, x};
-  CondenseOut out{nullptr, nullptr, xi, nullptr, info};
+  CondenseOut out{};
+  out.X = xi;
+  out.info = info;
   // interior-only system: nb = 0, the single rhs column is b_i - A_ib x
-  return launch_condense(gen, out, ncells, in.ni, 0, st);
+  return launch_condense<BackwardGen, true>(gen, out, ncells, in.ni, 0, st);
 }
 
-hipError_t launch_recover(const double* W, const int32_t* cell_dofs, const double* lam_free,
-                          const double* lam_dir, double* out, int64_t ncells, int ni, int nb,
-                          int na, int64_t nfree, hipStream_t st) {
+hipError_t launch_recover(const double* U, const double* Y, const int32_t* cell_dofs,
+                          const double* lam_free, const double* lam_dir, double* out,
+                          int64_t ncells, int ni, int nb, int na, int64_t nfree, hipStream_t st) {
+  if (nb > 256 || ni > 256) return hipErrorInvalidValue;
   if (ncells > 0) {
-    int cpb = 256 / ni;
-    if (cpb < 1) cpb = 1;
-    const int64_t grid = (ncells + cpb - 1) / cpb;
-    const size_t shm = sizeof(double) * (size_t)cpb * nb;
-    hipLaunchKernelGGL(recover_kernel, dim3((unsigned)grid), dim3(256), shm, st, W, cell_dofs,
-                       lam_free, lam_dir, out, ncells, ni, nb, na, cpb);
+    const int64_t blocks = (ncells + 3) / 4;
+    const int grid = (int)(blocks < 65536 ? blocks : 65536);
+    if (ni <= 64)
+      hipLaunchKernelGGL(recover_kernel<1>, dim3(grid), dim3(256), 0, st, U, Y, cell_dofs,
+                         lam_free, lam_dir, out, ncells, ni, nb, na);
+    else if (ni <= 128)
+      hipLaunchKernelGGL(recover_kernel<2>, dim3(grid), dim3(256), 0, st, U, Y, cell_dofs,
+                         lam_free, lam_dir, out, ncells, ni, nb, na);
+    else
+      hipLaunchKernelGGL(recover_kernel<4>, dim3(grid), dim3(256), 0, st, U, Y, cell_dofs,
+                         lam_free, lam_dir, out, ncells, ni, nb, na);
   }
   if (nfree > 0)
     hipLaunchKernelGGL(copy_kernel, dim3(1024), dim3(256), 0, st, lam_free, out + ncells * ni,
@@ -964,9 +1092,18 @@ hipError_t launch_count_nonzero(const int32_t* info, int64_t n, int64_t* count_d
 hipError_t launch_cell_systems(const Tables& T, const MeshDev& M, int64_t first, int64_t count,
                                double* A, double* b, hipStream_t st) {
   if (count <= 0) return hipSuccess;
-  FusedGen gen{T, M};
+  hipError_t e = launch_prep(T, M, st);
+  if (e != hipSuccess) return e;
   const int grid = (int)(count < 4096 ? count : 4096);
-  hipLaunchKernelGGL(cell_systems_kernel, dim3(grid), dim3(256), 0, st, gen, first, count, A, b);
+  if (T.phys == kDarcy) {
+    FusedGen<kDarcy> gen{T, M};
+    hipLaunchKernelGGL(cell_systems_kernel<kDarcy>, dim3(grid), dim3(256), 0, st, gen, first,
+                       count, A, b);
+  } else {
+    FusedGen<kElasticity> gen{T, M};
+    hipLaunchKernelGGL(cell_systems_kernel<kElasticity>, dim3(grid), dim3(256), 0, st, gen, first,
+                       count, A, b);
+  }
   return hipGetLastError();
 }
 

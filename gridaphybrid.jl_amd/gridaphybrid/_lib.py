@@ -41,6 +41,7 @@ EXPORTS = [
     "hdg_mesh_destroy", "hdg_mesh_get_info", "hdg_mesh_set_dirichlet",
     "hdg_mesh_set_dirichlet_mms", "hdg_mesh_set_source", "hdg_condense_assemble",
     "hdg_skeleton_system", "hdg_cell_blocks", "hdg_recover", "hdg_cell_systems",
+    "hdg_condense", "hdg_assemble", "hdg_mesh_singular_count", "hdg_mesh_interpolate_trace",
 ]
 
 
@@ -123,6 +124,10 @@ def lib():
         L.hdg_cell_blocks.argtypes = [vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp)]
         L.hdg_recover.argtypes = [vp, vp, vp]
         L.hdg_cell_systems.argtypes = [vp, i64, i64, vp, vp]
+        L.hdg_condense.argtypes = [vp, i32]
+        L.hdg_assemble.argtypes = [vp]
+        L.hdg_mesh_singular_count.argtypes = [vp, C.POINTER(i64)]
+        L.hdg_mesh_interpolate_trace.argtypes = [vp, vp]
         _ = dbl
         _lib = L
     return _lib

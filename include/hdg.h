@@ -187,6 +187,22 @@ int hdg_mesh_set_source(hdg_mesh* mesh, const double* src_qp);
  * may be NULL) receives the number of cells with a zero pivot. */
 int hdg_condense_assemble(hdg_mesh* mesh, int precision, int64_t* n_singular);
 
+/* The two halves of hdg_condense_assemble, enqueued asynchronously on the
+ * context stream with no host synchronisation (for timing and pipelining):
+ * hdg_condense runs the fused cell integration + condensation of every cell
+ * (StaticCondensationMap, src/StaticCondensationMap.jl:152-196, over
+ * lazy_map at src/HybridAffineFEOperators.jl:31); hdg_assemble the Dirichlet
+ * lifting and skeleton assembly (src/HybridAffineFEOperators.jl:38-46).
+ * hdg_mesh_singular_count synchronises and returns the number of cells whose
+ * last condensation met a zero pivot. */
+int hdg_condense(hdg_mesh* mesh, int precision);
+int hdg_assemble(hdg_mesh* mesh);
+int hdg_mesh_singular_count(hdg_mesh* mesh, int64_t* n_singular);
+
+/* Interpolates the built-in manufactured trace at the nodes of the free trace
+ * dofs, host [nfree] (Gridap's interpolate on the skeleton space). */
+int hdg_mesh_interpolate_trace(hdg_mesh* mesh, double* lam_free);
+
 /* Device pointers of the resident skeleton system (valid until the next
  * hdg_condense_assemble or destruction): colptr int64 [nfree+1], rowval int32
  * [nnz], nzval double [nnz], rhs double [nfree]. */
