@@ -1,0 +1,71 @@
+"""The C ABI library and the host mirror, without a GPU (CPU only)."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "hdg.h")
+
+
+def declared_functions():
+    src = open(HEADER).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(hdg_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_library_loads_and_exports_every_declared_symbol():
+    from gridaphybrid import _lib
+    L = _lib.lib()
+    decl = declared_functions()
+    assert len(decl) >= 20
+    missing = [f for f in decl if not hasattr(L, f)]
+    assert not missing, missing
+    assert sorted(_lib.EXPORTS) == decl
+
+
+def test_library_is_built_for_gfx950():
+    from gridaphybrid import _lib
+    blob = open(_lib.LIB_PATH, "rb").read()
+    assert b"gfx950" in blob
+
+
+def test_no_device_is_reported(monkeypatch):
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    from gridaphybrid import _lib
+    h = C.c_void_p()
+    assert _lib.lib().hdg_ctx_create(0, C.byref(h)) == _lib.HDG_ERR_NO_DEVICE
+
+
+def test_null_arguments_are_rejected():
+    from gridaphybrid import _lib
+    L = _lib.lib()
+    assert L.hdg_last_error(None) == b"null context"
+    info = _lib.MeshInfo()
+    assert L.hdg_mesh_get_info(None, C.byref(info)) == _lib.HDG_ERR_ARG
+    assert L.hdg_mesh_destroy(None) == _lib.HDG_ERR_ARG
+    assert L.hdg_ctx_destroy(None) == _lib.HDG_ERR_ARG
+    assert L.hdg_condense(None, 0) == _lib.HDG_ERR_ARG
+    assert L.hdg_recover(None, None, None) == _lib.HDG_ERR_ARG
+
+
+def test_host_mirror_checks_field_partition():
+    """StaticCondensationMap's _check_preconditions (src/StaticCondensationMap.jl:16-34)."""
+    import gridaphybrid as gh
+    gh.StaticCondensationMap([1, 2], [3])
+    gh.StaticCondensationMap([2], [1, 3])
+    for bad in (([1, 2], [2]), ([1, 1], [3]), ([1, 2], [4]), ([1], [3])):
+        with pytest.raises(gh.HDGError):
+            gh.StaticCondensationMap(*bad)
+
+
+def test_product_path_fails_loudly_without_gpu():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    import gridaphybrid as gh
+    with pytest.raises(gh.HDGError):
+        gh.Context(0)
