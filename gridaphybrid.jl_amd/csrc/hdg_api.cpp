@@ -804,6 +804,13 @@ int hdg_cell_blocks(hdg_mesh* m, const double** S, const double** g, const int32
   return HDG_OK;
 }
 
+int hdg_local_factors(hdg_mesh* m, const double** U, const double** Y) {
+  if (!m) return HDG_ERR_ARG;
+  if (U) *U = m->d_U.as<double>();
+  if (Y) *Y = m->d_Y.as<double>();
+  return m->condensed ? HDG_OK : fail(m->ctx, HDG_ERR_STATE, "no factors before condensation");
+}
+
 int hdg_recover(hdg_mesh* m, const double* lam_free, double* free_values) {
   if (!m) return HDG_ERR_ARG;
   hdg_ctx* ctx = m->ctx;

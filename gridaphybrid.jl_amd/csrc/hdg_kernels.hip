@@ -742,6 +742,304 @@ __global__ void __launch_bounds__(64)
 }
 
 // --------------------------------------------------------------------------
+// Column-per-wave elimination for medium systems (n <= 64*R rows, n+1 <= NW*CC
+// columns): lane l of wave w owns rows l + 64 r and columns w + NW c of
+// [A | b].  Every wave therefore holds the pivot row p in its own lane p % 64,
+// so the pivot row is read with readlane (no LDS, no barrier); only the
+// multiplier column crosses waves, through LDS, once per step.  The wave that
+// owns column j+1 updates it first and searches the pivot of step j+1 while
+// the other waves finish step j (look-ahead), so each step costs one barrier.
+// --------------------------------------------------------------------------
+__device__ inline int64_t dbits(double v) { return __double_as_longlong(v); }
+
+// Wave-wide maximum of a signed 64-bit key / minimum of a 32-bit int, with
+// DPP row rotations (row_ror 1,2,4,8) and the GFX9 row broadcasts
+// (row_bcast15, row_bcast31); the result is read from lane 63 and is
+// wave-uniform.  Lanes masked out of a DPP step keep their own value.
+template <int CTRL, int ROW_MASK>
+__device__ inline int64_t dpp_i64(int64_t x) {
+  const int lo = (int)(uint32_t)(uint64_t)x, hi = (int)((uint64_t)x >> 32);
+  const int lo2 = __builtin_amdgcn_update_dpp(lo, lo, CTRL, ROW_MASK, 0xf, false);
+  const int hi2 = __builtin_amdgcn_update_dpp(hi, hi, CTRL, ROW_MASK, 0xf, false);
+  return (int64_t)(((uint64_t)(uint32_t)hi2 << 32) | (uint64_t)(uint32_t)lo2);
+}
+__device__ inline int64_t wave_max_i64(int64_t x) {
+  int64_t y;
+  y = dpp_i64<0x121, 0xf>(x); x = y > x ? y : x;  // row_ror:1
+  y = dpp_i64<0x122, 0xf>(x); x = y > x ? y : x;  // row_ror:2
+  y = dpp_i64<0x124, 0xf>(x); x = y > x ? y : x;  // row_ror:4
+  y = dpp_i64<0x128, 0xf>(x); x = y > x ? y : x;  // row_ror:8
+  y = dpp_i64<0x142, 0xa>(x); x = y > x ? y : x;  // row_bcast:15
+  y = dpp_i64<0x143, 0xc>(x); x = y > x ? y : x;  // row_bcast:31
+  const int lo = __builtin_amdgcn_readlane((int)(uint32_t)(uint64_t)x, 63);
+  const int hi = __builtin_amdgcn_readlane((int)((uint64_t)x >> 32), 63);
+  return (int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint64_t)(uint32_t)lo);
+}
+__device__ inline int wave_min_i32(int x) {
+  int y;
+  y = __builtin_amdgcn_update_dpp(x, x, 0x121, 0xf, 0xf, false); x = min(x, y);
+  y = __builtin_amdgcn_update_dpp(x, x, 0x122, 0xf, 0xf, false); x = min(x, y);
+  y = __builtin_amdgcn_update_dpp(x, x, 0x124, 0xf, 0xf, false); x = min(x, y);
+  y = __builtin_amdgcn_update_dpp(x, x, 0x128, 0xf, 0xf, false); x = min(x, y);
+  y = __builtin_amdgcn_update_dpp(x, x, 0x142, 0xa, 0xf, false); x = min(x, y);
+  y = __builtin_amdgcn_update_dpp(x, x, 0x143, 0xc, 0xf, false); x = min(x, y);
+  return __builtin_amdgcn_readlane(x, 63);
+}
+
+template <int R, int CC, int NW, class Gen, bool BACKSUB>
+__global__ void __launch_bounds__(64 * NW)
+    condense_colwave_kernel(Gen gen, CondenseOut out, int64_t ncells, int ni, int nb) {
+  constexpr int NT = 64 * NW;
+  constexpr int NMAX = 64 * R;
+  static_assert(R <= 32, "row mask is 32 bits");
+  const int n = ni + nb, ncol = n + 1;
+  const int lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
+
+  __shared__ double s_lcol[2][NMAX];
+  __shared__ double s_diag[NMAX];
+  __shared__ int s_pos[NMAX];
+  __shared__ int s_at[NMAX];
+  __shared__ int s_piv[2];
+  __shared__ int s_info;
+  __shared__ typename Gen::Shared s_gen;
+  __shared__ double s_slab[NW][NMAX];
+  __shared__ double s_urow[NW][CC];
+  __shared__ double s_colv[NMAX];
+
+  for (int64_t cell = blockIdx.x; cell < ncells; cell += gridDim.x) {
+    gen.begin(cell, s_gen);
+    for (int i = threadIdx.x; i < NMAX; i += NT) {
+      s_pos[i] = i;
+      s_at[i] = i;
+    }
+    if (threadIdx.x == 0) s_info = 0;
+    __syncthreads();
+
+    // ---- generate: slab c holds local column c of every wave -------------
+    double a[R][CC];
+    uint32_t act = 0;
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+      if (lane + 64 * r < n) act |= 1u << r;
+#pragma unroll 1
+    for (int c = 0; c < CC; ++c) {
+      for (int t = threadIdx.x; t < n * NW; t += NT) {
+        const int cc = t / n, row = t - cc * n;
+        const int col = cc + NW * c;
+        s_slab[cc][row] = col < ncol ? GenCall<Gen>::entry(gen, s_gen, row, col, cell) : 0.0;
+      }
+      __syncthreads();
+      double v[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[r] = lane + 64 * r < n ? s_slab[w][lane + 64 * r] : 0.0;
+#pragma unroll
+      for (int c2 = 0; c2 < CC; ++c2)
+        if (c2 == c) {
+#pragma unroll
+          for (int r = 0; r < R; ++r) a[r][c2] = v[r];
+        }
+      __syncthreads();
+    }
+
+    // pivot search + multipliers + bookkeeping of step j on local column cj
+    // (executed by the wave owning column j; wave-uniform control flow)
+    auto pivot_step = [&](int j, int buf) {
+      double colv[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) colv[r] = s_colv[lane + 64 * r];
+      int64_t bkey = -1;  // |v| bit pattern (monotone for |v| >= 0); -1: no candidate
+      int bpos = INT_MAX;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int row = lane + 64 * r;
+        const bool cand = row < ni && ((act >> r) & 1u);
+        const int64_t key = cand ? dbits(fabs(colv[r])) : (int64_t)-1;
+        const int pr = cand ? s_pos[row] : INT_MAX;
+        const bool better = key > bkey || (key == bkey && pr < bpos);
+        bkey = better ? key : bkey;
+        bpos = better ? pr : bpos;
+      }
+      const int64_t m = wave_max_i64(bkey);
+      const int key = wave_min_i32(bkey == m ? bpos : INT_MAX);
+      const int p = __builtin_amdgcn_readfirstlane(s_at[key]);
+      const double pv = s_colv[p];
+      const double rinv = pv != 0.0 ? 1.0 / pv : 0.0;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int row = lane + 64 * r;
+        const bool use = ((act >> r) & 1u) && row != p;
+        s_lcol[buf][row] = use ? colv[r] * rinv : 0.0;
+      }
+      if (lane == 0) {
+        s_piv[buf] = p;
+        s_diag[j] = pv;
+        if (pv == 0.0 && s_info == 0) s_info = j + 1;
+        const int q = s_at[j];
+        s_at[key] = q;
+        s_pos[q] = key;
+        s_at[j] = p;
+        s_pos[p] = j;
+      }
+    };
+
+    if (ni > 0 && w == 0) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) s_colv[lane + 64 * r] = a[r][0];
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      pivot_step(0, 0);
+    }
+    __syncthreads();
+
+#pragma unroll 1
+    for (int j = 0; j < ni; ++j) {
+      const int buf = j & 1;
+      const int jl = j / NW;
+      const int p = __builtin_amdgcn_readfirstlane(s_piv[buf]);
+      const int pl = p >> 6, plane = p & 63;
+      if (lane == plane) act &= ~(1u << pl);
+      double l[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const double lv = s_lcol[buf][lane + 64 * r];
+        l[r] = ((act >> r) & 1u) ? lv : 0.0;
+      }
+      // pivot row of this wave's columns: its owner lane stores it in the
+      // wave's LDS row buffer (pl is wave-uniform: a scalar switch, no
+      // per-column selects), then every lane reads it back by broadcast
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+        if (r == pl) {  // wave-uniform branch
+          if (lane == plane) {
+#pragma unroll
+            for (int c = 0; c < CC; ++c) s_urow[w][c] = a[r][c];
+          }
+        }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      const int j1 = j + 1;
+      const bool look = j1 < ni && w == (j1 % NW);  // wave-uniform
+      const int c1 = j1 / NW;
+#pragma unroll
+      for (int c = 0; c < CC; ++c) {
+        if (c < jl) continue;  // local column fully eliminated
+        const double u = s_urow[w][c];
+#pragma unroll
+        for (int r = 0; r < R; ++r) a[r][c] = fma(-l[r], u, a[r][c]);
+        if (look && c == c1) {  // wave-uniform: stage column j+1 for its pivot search
+#pragma unroll
+          for (int r = 0; r < R; ++r) s_colv[lane + 64 * r] = a[r][c];
+        }
+      }
+      if (look) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        pivot_step(j1, buf ^ 1);
+      }
+      __syncthreads();
+    }
+    const int info = s_info;
+
+    // ---- back substitution (backward map only; single rhs column) -------
+    if (BACKSUB) {
+      const int rc = ni / NW, rw = ni % NW;  // rhs column ni lives in wave rw, local rc
+#pragma unroll 1
+      for (int k = ni - 1; k >= 0; --k) {
+        const int buf = k & 1;
+        const int p = __builtin_amdgcn_readfirstlane(s_at[k]);
+        const int pl = p >> 6, plane = p & 63;
+        const double d = s_diag[k];
+        // x_k in the rhs-owning wave, multipliers U(., k) from the column-k wave
+        if (w == rw) {
+#pragma unroll
+          for (int c = 0; c < CC; ++c)
+            if (c == rc) {
+              double sel = 0.0;
+#pragma unroll
+              for (int r = 0; r < R; ++r)
+                if (r == pl) sel = a[r][c];
+              const double x = d != 0.0 ? wave_readlane_f64(sel, plane) / d : 0.0;
+              if (lane == 0) s_lcol[buf][NMAX - 1] = x;  // not a row slot used here
+#pragma unroll
+              for (int r = 0; r < R; ++r)
+                if (r == pl && lane == plane) a[r][c] = x;
+            }
+        }
+        if (w == k % NW) {
+#pragma unroll
+          for (int c = 0; c < CC; ++c)
+            if (c == k / NW) {
+#pragma unroll
+              for (int r = 0; r < R; ++r) {
+                const int row = lane + 64 * r;
+                const bool use = row < ni && s_pos[row] < k;
+                if (row < NMAX - 1) s_lcol[buf][row] = use ? a[r][c] : 0.0;
+              }
+            }
+        }
+        __syncthreads();
+        if (w == rw) {
+          const double x = s_lcol[buf][NMAX - 1];
+#pragma unroll
+          for (int c = 0; c < CC; ++c)
+            if (c == rc) {
+#pragma unroll
+              for (int r = 0; r < R; ++r) {
+                const int row = lane + 64 * r;
+                if (row < ni && row < NMAX - 1) a[r][c] = fma(-s_lcol[buf][row], x, a[r][c]);
+              }
+            }
+        }
+        __syncthreads();
+      }
+    }
+
+    // ---- outputs, slab by slab --------------------------------------------
+#pragma unroll 1
+    for (int c = 0; c < CC; ++c) {
+      if (NW * c + NW <= ni && out.U == nullptr) continue;
+#pragma unroll
+      for (int c2 = 0; c2 < CC; ++c2)
+        if (c2 == c) {
+#pragma unroll
+          for (int r = 0; r < R; ++r) s_slab[w][lane + 64 * r] = a[r][c2];
+        }
+      __syncthreads();
+      for (int t = threadIdx.x; t < n * NW; t += NT) {
+        const int cc = t / n, row = t - cc * n;
+        const int col = cc + NW * c;
+        if (col >= ncol) continue;
+        const double v = s_slab[cc][row];
+        if (row >= ni) {
+          if (col < ni) continue;
+          if (col < n) {
+            if (out.S) out.S[(cell * nb + (col - ni)) * nb + (row - ni)] = v;
+          } else if (out.g) {
+            out.g[cell * nb + (row - ni)] = v;
+          }
+        } else {
+          const int k = s_pos[row];
+          if (col < ni) {
+            if (out.U && col >= k)
+              out.U[cell * ((int64_t)ni * (ni + 1) / 2) + col * (col + 1) / 2 + k] = v;
+          } else {
+            if (out.Y) out.Y[(cell * (nb + 1) + (col - ni)) * ni + k] = v;
+            if (out.X && col == n) out.X[cell * ni + k] = v;
+          }
+        }
+      }
+      __syncthreads();
+    }
+    if (threadIdx.x == 0 && out.info) out.info[cell] = info;
+    __syncthreads();
+  }
+}
+
+// --------------------------------------------------------------------------
 // Recovery (BackwardStaticCondensationMap with the factors kept from the
 // forward pass): r = L^-1 P (b_i - A_ib lambda) = y - Y lambda, then
 // U x = r.  One wavefront per cell; the result is scattered into the
@@ -961,6 +1259,12 @@ static hipError_t launch_condense(const Gen& gen, const CondenseOut& out, int64_
     HDG_WAVE(32, 24)
     HDG_WAVE(7, 8)
 #undef HDG_WAVE
+  }
+  if (ni + nb > 64 && ni + nb <= 192 && ni + nb + 1 <= 168) {
+    const int grid = (int)(ncells < 256 * 2 ? ncells : 256 * 2);
+    hipLaunchKernelGGL((condense_colwave_kernel<3, 21, 8, Gen, BACKSUB>), dim3(grid), dim3(512),
+                       0, st, gen, out, ncells, ni, nb);
+    return hipGetLastError();
   }
   Bucket b;
   if (!pick_bucket(ni + nb, b)) return hipErrorInvalidValue;

@@ -342,6 +342,21 @@ class HybridAffineFEOperator:
         _ = torch
         return Sh[0].transpose(0, 2, 1), Sh[1], Sh[2]
 
+    def local_factors(self):
+        """(U, Y) host copies: U (ncells, ni, ni) upper factor in LAPACK row order,
+        Y (ncells, ni, nb+1) = L^-1 P [A_ib b_i]."""
+        U, Y = C.c_void_p(), C.c_void_p()
+        check(lib().hdg_local_factors(self.mesh, C.byref(U), C.byref(Y)), self.ctx.h, "factors")
+        nc, ni, nb = self.info.ncells, self.info.ni, self.info.nb
+        Up = np.empty((nc, ni * (ni + 1) // 2))
+        Yh = np.empty((nc, nb + 1, ni))
+        check(lib().hdg_memcpy_d2h(self.ctx.h, Up.ctypes.data, U, Up.nbytes), self.ctx.h, "d2h")
+        check(lib().hdg_memcpy_d2h(self.ctx.h, Yh.ctypes.data, Y, Yh.nbytes), self.ctx.h, "d2h")
+        Uf = np.zeros((nc, ni, ni))
+        for c in range(ni):
+            Uf[:, :c + 1, c] = Up[:, c * (c + 1) // 2: c * (c + 1) // 2 + c + 1]
+        return Uf, Yh.transpose(0, 2, 1)
+
     def recover(self, lam_free):
         """BackwardStaticCondensationMap over all cells + scatter
         (src/HybridAffineFEOperators.jl:102-150): free values [q; u; lambda]."""

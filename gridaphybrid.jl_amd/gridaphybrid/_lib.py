@@ -42,6 +42,7 @@ EXPORTS = [
     "hdg_mesh_set_dirichlet_mms", "hdg_mesh_set_source", "hdg_condense_assemble",
     "hdg_skeleton_system", "hdg_cell_blocks", "hdg_recover", "hdg_cell_systems",
     "hdg_condense", "hdg_assemble", "hdg_mesh_singular_count", "hdg_mesh_interpolate_trace",
+    "hdg_local_factors",
 ]
 
 
@@ -128,6 +129,7 @@ def lib():
         L.hdg_assemble.argtypes = [vp]
         L.hdg_mesh_singular_count.argtypes = [vp, C.POINTER(i64)]
         L.hdg_mesh_interpolate_trace.argtypes = [vp, vp]
+        L.hdg_local_factors.argtypes = [vp, C.POINTER(vp), C.POINTER(vp)]
         _ = dbl
         _lib = L
     return _lib

@@ -213,6 +213,13 @@ int hdg_skeleton_system(hdg_mesh* mesh, const int64_t** colptr, const int32_t** 
 int hdg_cell_blocks(hdg_mesh* mesh, const double** S, const double** g,
                     const int32_t** cell_dofs);
 
+/* Device pointers of the resident local factors kept for recovery:
+ * U [ncells][ni(ni+1)/2], the upper LU factor of A_ii in LAPACK row order
+ * packed by columns (column c holds rows 0..c at c(c+1)/2 + k), and
+ * Y [ncells][nb+1][ni] = L^-1 P [A_ib b_i] in LAPACK row order.  The pivot
+ * sequence is getrf's (largest magnitude, first position on ties). */
+int hdg_local_factors(hdg_mesh* mesh, const double** U, const double** Y);
+
 /* solve!'s recovery (src/HybridAffineFEOperators.jl:89-150 with
  * BackwardStaticCondensationMap): lam_free device [nfree] (skeleton
  * solution), free_values device [nfree_total] = [q of all cells; u of all

@@ -143,6 +143,20 @@ def test_operator_parity(ctx, D, k, n, mms, logn):
     K, rhs = op.skeleton_system()
     assert np.array_equal(K.indptr, ref.colptr) and np.array_equal(K.indices, ref.rowval)
     assert rel(K.data, ref.nzval) < RTOL and rel(rhs, ref.rhs) < RTOL
+    # local factors: U and Y = L^-1 P [A_ib b_i] solve the local problems, and
+    # the pivots are partial-pivoting ones: A_ii U^-1 = P^T L with |L| <= 1
+    # (the pivot order itself may differ from another LU on exact ties)
+    import scipy.linalg as sl
+    U, Y = op.local_factors()
+    ni = op.info.ni
+    for c in range(min(nc, 16)):
+        Aii = A[c, :ni, :ni]
+        B = np.concatenate([A[c, :ni, ni:], b[c, :ni, None]], axis=1)
+        assert rel(sl.solve_triangular(U[c], Y[c]), np.linalg.solve(Aii, B)) < 1e-10
+        PtL = Aii @ np.linalg.inv(U[c])
+        assert np.max(np.abs(PtL)) < 1.0 + 1e-10
+        assert np.allclose(np.sort(np.max(np.abs(PtL), axis=1)), 1.0, atol=0) or \
+            np.count_nonzero(np.isclose(np.abs(PtL), 1.0, rtol=0, atol=1e-10)) >= ni
     # recovery from the oracle's skeleton solution
     fv = op.recover(ref.lam).cpu().numpy()
     assert rel(fv, ref.free_values) < RTOL
