@@ -1040,6 +1040,212 @@ __global__ void __launch_bounds__(64 * NW)
 }
 
 // --------------------------------------------------------------------------
+// Blocked elimination for large systems (up to 512 rows): the cell system
+// lives in a per-workgroup HBM workspace (column-major, leading dimension
+// ld); panels of KB columns are factored in LDS (partial pivoting over the
+// interior rows, LAPACK position bookkeeping), the panel's pivot rows are
+// brought to U form by forward substitution (U12), and the remaining active
+// rows get one rank-KB update per panel from register tiles.
+// --------------------------------------------------------------------------
+constexpr int kLargeMaxN = 512;
+constexpr int kLargeKB = 16;
+
+template <class Gen, bool BACKSUB>
+__global__ void __launch_bounds__(256)
+    condense_large_kernel(Gen gen, CondenseOut out, int64_t ncells, int ni, int nb,
+                          double* __restrict__ work, int ld) {
+  constexpr int NT = 256, KB = kLargeKB, NMAX = kLargeMaxN;
+  const int n = ni + nb, ncol = n + 1;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  double* M = work + (int64_t)blockIdx.x * ld * (int64_t)(NMAX + 1);
+
+  __shared__ double s_P[KB][NMAX];        // panel columns (all rows)
+  __shared__ double s_U[KB][NMAX + 1];    // pivot rows x trailing columns
+  __shared__ int s_pos[NMAX], s_at[NMAX];
+  __shared__ unsigned char s_act[NMAX];
+  __shared__ int s_piv[KB];
+  __shared__ double s_diag[NMAX];
+  __shared__ int64_t s_wkey[4];
+  __shared__ int s_wpos[4];
+  __shared__ int s_info;
+  __shared__ typename Gen::Shared s_gen;
+
+  for (int64_t cell = blockIdx.x; cell < ncells; cell += gridDim.x) {
+    gen.begin(cell, s_gen);
+    for (int i = tid; i < NMAX; i += NT) {
+      s_pos[i] = i;
+      s_at[i] = i;
+      s_act[i] = i < n ? 1 : 0;
+    }
+    if (tid == 0) s_info = 0;
+    __syncthreads();
+    // generate into the workspace (coalesced, column-major)
+    for (int64_t t = tid; t < (int64_t)n * ncol; t += NT) {
+      const int col = (int)(t / n), row = (int)(t - (int64_t)col * n);
+      M[(int64_t)col * ld + row] = GenCall<Gen>::entry(gen, s_gen, row, col, cell);
+    }
+    __syncthreads();
+
+    for (int j0 = 0; j0 < ni; j0 += KB) {
+      const int kb = min(KB, ni - j0);
+      const int jt = j0 + kb;  // first trailing column
+      // ---- load the panel ----
+      for (int t = tid; t < kb * n; t += NT) {
+        const int cc = t / n, row = t - cc * n;
+        s_P[cc][row] = M[(int64_t)(j0 + cc) * ld + row];
+      }
+      __syncthreads();
+      // ---- factor the panel in LDS ----
+      for (int jj = 0; jj < kb; ++jj) {
+        const int j = j0 + jj;
+        int64_t bkey = -1;
+        int bpos = INT_MAX;
+        for (int row = tid; row < ni; row += NT) {
+          if (!s_act[row]) continue;
+          const int64_t key = dbits(fabs(s_P[jj][row]));
+          const int pr = s_pos[row];
+          if (key > bkey || (key == bkey && pr < bpos)) {
+            bkey = key;
+            bpos = pr;
+          }
+        }
+        const int64_t wm = wave_max_i64(bkey);
+        const int wk = wave_min_i32(bkey == wm ? bpos : INT_MAX);
+        if (lane == 0) {
+          s_wkey[wv] = wm;
+          s_wpos[wv] = wk;
+        }
+        __syncthreads();
+        int64_t m = s_wkey[0];
+        for (int q = 1; q < 4; ++q) m = s_wkey[q] > m ? s_wkey[q] : m;
+        int key = INT_MAX;
+        for (int q = 0; q < 4; ++q)
+          if (s_wkey[q] == m) key = min(key, s_wpos[q]);
+        const int p = s_at[key];
+        const double pv = s_P[jj][p];
+        const double rinv = pv != 0.0 ? 1.0 / pv : 0.0;
+        // multipliers in place, then update the rest of the panel
+        for (int row = tid; row < n; row += NT) {
+          if (!s_act[row] || row == p) continue;
+          const double l = s_P[jj][row] * rinv;
+          s_P[jj][row] = l;
+          for (int cc = jj + 1; cc < kb; ++cc) s_P[cc][row] = fma(-l, s_P[cc][p], s_P[cc][row]);
+        }
+        __syncthreads();
+        if (tid == 0) {
+          s_piv[jj] = p;
+          s_diag[j] = pv;
+          if (pv == 0.0 && s_info == 0) s_info = j + 1;
+          s_act[p] = 0;
+          const int q = s_at[j];
+          s_at[key] = q;
+          s_pos[q] = key;
+          s_at[j] = p;
+          s_pos[p] = j;
+        }
+        __syncthreads();
+      }
+      // ---- pivot rows: panel part to the outputs, trailing part -> U12 ----
+      for (int t = tid; t < kb * KB; t += NT) {
+        const int jj = t / KB, cc = t - jj * KB;
+        if (cc >= kb || cc < jj) continue;
+        const int col = j0 + cc, k = j0 + jj;
+        const double v = s_P[cc][s_piv[jj]];
+        if (out.U) out.U[cell * ((int64_t)ni * (ni + 1) / 2) + col * (col + 1) / 2 + k] = v;
+        if (BACKSUB) M[(int64_t)col * ld + s_piv[jj]] = v;  // U row kept for back substitution
+      }
+      for (int c = jt + tid; c < ncol; c += NT) {
+        double u[KB];
+#pragma unroll
+        for (int jj = 0; jj < KB; ++jj) {
+          if (jj < kb) {
+            double v = M[(int64_t)c * ld + s_piv[jj]];
+            for (int ii = 0; ii < jj; ++ii) v = fma(-s_P[ii][s_piv[jj]], u[ii], v);
+            u[jj] = v;
+            s_U[jj][c - jt] = v;
+          }
+        }
+        for (int jj = 0; jj < kb; ++jj) {
+          const int k = j0 + jj;
+          if (c < ni) {
+            if (out.U) out.U[cell * ((int64_t)ni * (ni + 1) / 2) + c * (c + 1) / 2 + k] = u[jj];
+          } else {
+            if (out.Y) out.Y[(cell * (nb + 1) + (c - ni)) * ni + k] = u[jj];
+          }
+          if (BACKSUB) M[(int64_t)c * ld + s_piv[jj]] = u[jj];
+        }
+      }
+      __syncthreads();
+      // ---- rank-kb update of the active rows on the trailing columns ----
+      const int ntr = ncol - jt;
+      const int rt = (n + 3) / 4, ct = (ntr + 3) / 4;
+      for (int tile = tid; tile < rt * ct; tile += NT) {
+        const int tr4 = (tile % rt) * 4, tc4 = (tile / rt) * 4;
+        double acc[4][4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+          for (int q = 0; q < 4; ++q) acc[i][q] = 0.0;
+        bool rowok[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) rowok[i] = tr4 + i < n && s_act[tr4 + i];
+        if (!(rowok[0] | rowok[1] | rowok[2] | rowok[3])) continue;
+        for (int jj = 0; jj < kb; ++jj) {
+          double l[4], u[4];
+#pragma unroll
+          for (int i = 0; i < 4; ++i) l[i] = rowok[i] ? s_P[jj][tr4 + i] : 0.0;
+#pragma unroll
+          for (int q = 0; q < 4; ++q) u[q] = tc4 + q < ntr ? s_U[jj][tc4 + q] : 0.0;
+#pragma unroll
+          for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) acc[i][q] = fma(l[i], u[q], acc[i][q]);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int c = jt + tc4 + q;
+          if (c >= ncol) continue;
+#pragma unroll
+          for (int i = 0; i < 4; ++i)
+            if (rowok[i]) M[(int64_t)c * ld + tr4 + i] -= acc[i][q];
+        }
+      }
+      __syncthreads();
+    }
+
+    // ---- back substitution (backward map only: single rhs column) -------
+    if (BACKSUB) {
+      // rhs column n lives in M at the pivot rows (U12 values were stored)
+      if (tid == 0) {
+        for (int k = ni - 1; k >= 0; --k) {
+          const int p = s_at[k];
+          double x = M[(int64_t)n * ld + p];
+          for (int c = k + 1; c < ni; ++c)
+            x = fma(-M[(int64_t)c * ld + p], M[(int64_t)n * ld + s_at[c]], x);
+          const double d = s_diag[k];
+          x = d != 0.0 ? x / d : 0.0;
+          M[(int64_t)n * ld + p] = x;
+          if (out.X) out.X[cell * ni + k] = x;
+        }
+      }
+      __syncthreads();
+    }
+    // ---- boundary rows: S and g ----
+    for (int64_t t = tid; t < (int64_t)nb * (nb + 1); t += NT) {
+      const int cb = (int)(t / nb), rb = (int)(t - (int64_t)cb * nb);
+      const double v = M[(int64_t)(ni + cb) * ld + ni + rb];
+      if (cb < nb) {
+        if (out.S) out.S[(cell * nb + cb) * nb + rb] = v;
+      } else if (out.g) {
+        out.g[cell * nb + rb] = v;
+      }
+    }
+    if (tid == 0 && out.info) out.info[cell] = s_info;
+    __syncthreads();
+  }
+}
+
+// --------------------------------------------------------------------------
 // Recovery (BackwardStaticCondensationMap with the factors kept from the
 // forward pass): r = L^-1 P (b_i - A_ib lambda) = y - Y lambda, then
 // U x = r.  One wavefront per cell; the result is scattered into the
@@ -1238,10 +1444,41 @@ static int grid_for(int64_t ncells, int threads) {
   return (int)(ncells < target ? ncells : target);
 }
 
+// Workspace of the large-system kernel (grown on demand; one per process).
+static double* g_large_work = nullptr;
+static size_t g_large_bytes = 0;
+constexpr int kLargeGrid = 256 * 2;
+
+static hipError_t large_workspace(int n, double** ws, int* ld) {
+  const int l = (n + 1) & ~1;
+  const size_t need = (size_t)kLargeGrid * l * (kLargeMaxN + 1) * sizeof(double);
+  if (need > g_large_bytes) {
+    if (g_large_work) (void)hipFree(g_large_work);
+    g_large_work = nullptr;
+    g_large_bytes = 0;
+    hipError_t e = hipMalloc(&g_large_work, need);
+    if (e != hipSuccess) return e;
+    g_large_bytes = need;
+  }
+  *ws = g_large_work;
+  *ld = l;
+  return hipSuccess;
+}
+
 template <class Gen, bool BACKSUB>
 static hipError_t launch_condense(const Gen& gen, const CondenseOut& out, int64_t ncells, int ni,
                                   int nb, hipStream_t st) {
   if (ncells <= 0) return hipSuccess;
+  if (ni + nb > 192 && ni + nb <= kLargeMaxN) {
+    double* ws;
+    int ld;
+    hipError_t e = large_workspace(ni + nb, &ws, &ld);
+    if (e != hipSuccess) return e;
+    const int grid = (int)(ncells < kLargeGrid ? ncells : kLargeGrid);
+    hipLaunchKernelGGL((condense_large_kernel<Gen, BACKSUB>), dim3(grid), dim3(256), 0, st, gen,
+                       out, ncells, ni, nb, ws, ld);
+    return hipGetLastError();
+  }
   const int ncol = ni + nb + 1;
   {
     const int64_t target = 256 * 48;
@@ -1285,7 +1522,7 @@ static hipError_t launch_condense(const Gen& gen, const CondenseOut& out, int64_
 
 bool condense_supported(int n) {
   Bucket b;
-  return pick_bucket(n, b);
+  return pick_bucket(n, b) || (n > 192 && n <= kLargeMaxN);
 }
 
 static hipError_t launch_prep(const Tables& T, const MeshDev& M, hipStream_t st) {
