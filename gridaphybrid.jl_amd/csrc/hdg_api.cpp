@@ -27,6 +27,7 @@ struct hdg_ctx {
   int64_t* d_count = nullptr;  // scratch counter for singular-cell counts
   int32_t* d_perm = nullptr;   // field permutation of the map-level API
   int perm_cap = 0;
+  Workspace ws;                // large-system kernel workspace of this context
 };
 
 namespace {
@@ -92,6 +93,12 @@ struct hdg_mesh {
   // device
   DevBuf tables, d_x0, d_jac, d_sign, d_coef, d_cell_dofs, d_lam_dir;
   DevBuf d_S, d_g, d_U, d_Y, d_info, d_alpha, d_srcq;
+  // per-cell device state used by the kernels: the mesh's own buffers above,
+  // or slices of a batch's per-class buffers (hdg_batch_create)
+  double *pS = nullptr, *pg = nullptr, *pU = nullptr, *pY = nullptr;
+  double *palpha = nullptr, *psrcq = nullptr;
+  int32_t* pinfo = nullptr;
+  hdg_batch* batch = nullptr;
   DevBuf d_fptr, d_pptr, d_pfacet, d_pcell, d_plpos, d_gcell, d_glpos;
   DevBuf d_colptr, d_rowval, d_nzval, d_rhs;
   const double* src_qp = nullptr;
@@ -105,8 +112,8 @@ struct hdg_mesh {
     m.sign = d_sign.as<int8_t>();
     m.coef = d_coef.as<double>();
     m.src = src_qp;
-    m.alpha = d_alpha.as<double>();
-    m.srcq = d_srcq.as<double>();
+    m.alpha = palpha;
+    m.srcq = psrcq;
     m.source = prob.source;
     m.tau = prob.tau;
     m.young = prob.young;
@@ -127,6 +134,53 @@ struct hdg_mesh {
     return P;
   }
 };
+
+namespace {
+
+// Doubles per cell of each resident per-cell array.
+struct CellLayout {
+  size_t S, g, U, Y, alpha, srcq;
+  explicit CellLayout(const TablesHost& t)
+      : S((size_t)t.nb * t.nb),
+        g((size_t)t.nb),
+        U((size_t)t.ni * (t.ni + 1) / 2),
+        Y((size_t)(t.nb + 1) * t.ni),
+        alpha(72),
+        srcq((size_t)t.nqc * 4) {}
+};
+
+// Allocates the mesh's own per-cell state (S, g, U, Y, alpha, srcq, info) and
+// binds the kernel pointers to it.
+hipError_t own_cell_state(hdg_mesh* m) {
+  const CellLayout L(m->th);
+  const size_t nc = (size_t)std::max<int64_t>(m->ncells, 1);
+  hipError_t e;
+  if ((e = m->d_S.alloc(nc * L.S * sizeof(double))) != hipSuccess) return e;
+  if ((e = m->d_g.alloc(nc * L.g * sizeof(double))) != hipSuccess) return e;
+  if ((e = m->d_U.alloc(nc * L.U * sizeof(double))) != hipSuccess) return e;
+  if ((e = m->d_Y.alloc(nc * L.Y * sizeof(double))) != hipSuccess) return e;
+  if ((e = m->d_alpha.alloc(nc * L.alpha * sizeof(double))) != hipSuccess) return e;
+  if ((e = m->d_srcq.alloc(nc * L.srcq * sizeof(double))) != hipSuccess) return e;
+  if ((e = m->d_info.alloc(nc * sizeof(int32_t))) != hipSuccess) return e;
+  m->pS = m->d_S.as<double>();
+  m->pg = m->d_g.as<double>();
+  m->pU = m->d_U.as<double>();
+  m->pY = m->d_Y.as<double>();
+  m->palpha = m->d_alpha.as<double>();
+  m->psrcq = m->d_srcq.as<double>();
+  m->pinfo = m->d_info.as<int32_t>();
+  return hipSuccess;
+}
+
+// Frees the mesh's own per-cell state (the mesh is rebound to batch slices).
+void drop_cell_state(hdg_mesh* m) {
+  for (DevBuf* b : {&m->d_S, &m->d_g, &m->d_U, &m->d_Y, &m->d_alpha, &m->d_srcq, &m->d_info})
+    (void)b->alloc(0);
+  m->pS = m->pg = m->pU = m->pY = m->palpha = m->psrcq = nullptr;
+  m->pinfo = nullptr;
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -151,6 +205,7 @@ int hdg_ctx_destroy(hdg_ctx* ctx) {
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   if (ctx->d_count) (void)hipFree(ctx->d_count);
   if (ctx->d_perm) (void)hipFree(ctx->d_perm);
+  workspace_release(ctx->ws);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return HDG_OK;
@@ -265,7 +320,7 @@ int hdg_static_condensation(hdg_ctx* ctx, const hdg_fields* fields, int64_t ncel
   out.S = S;
   out.g = g;
   out.info = info;
-  HIP_TRY(ctx, launch_condense_dense(in, ncells, out, ctx->stream), "condense kernel");
+  HIP_TRY(ctx, launch_condense_dense(in, ncells, out, ctx->stream, ctx->ws), "condense kernel");
   int64_t bad = 0;
   if ((rc = count_singular(ctx, info, ncells, &bad))) return rc;
   if (bad) return fail(ctx, HDG_SINGULAR, std::to_string(bad) + " cell(s) with singular A_ii");
@@ -289,7 +344,8 @@ int hdg_backward_static_condensation(hdg_ctx* ctx, const hdg_fields* fields, int
   HIP_TRY(ctx, hipSetDevice(ctx->device), "hipSetDevice");
   if ((rc = upload_perm(ctx, perm))) return rc;
   DenseIn in{A, b, ctx->d_perm, ni + nb, ni, nb};
-  HIP_TRY(ctx, launch_backward_dense(in, x, ncells, xi, info, ctx->stream), "backward kernel");
+  HIP_TRY(ctx, launch_backward_dense(in, x, ncells, xi, info, ctx->stream, ctx->ws),
+          "backward kernel");
   int64_t bad = 0;
   if ((rc = count_singular(ctx, info, ncells, &bad))) return rc;
   if (bad) return fail(ctx, HDG_SINGULAR, std::to_string(bad) + " cell(s) with singular A_ii");
@@ -546,16 +602,7 @@ int hdg_mesh_create(hdg_ctx* ctx, const hdg_problem* prob, const hdg_mesh_desc* 
   std::vector<double> zeros((size_t)std::max<int64_t>(m->ndir, 1), 0.0);
   HIP_TRY(ctx, upload(m->d_lam_dir, zeros, st), "upload lam_dir");
   // resident outputs
-  HIP_TRY(ctx, m->d_S.alloc((size_t)nc * nb * nb * sizeof(double)), "alloc S");
-  HIP_TRY(ctx, m->d_g.alloc((size_t)nc * nb * sizeof(double)), "alloc g");
-  HIP_TRY(ctx, m->d_U.alloc((size_t)nc * ((size_t)th.ni * (th.ni + 1) / 2) * sizeof(double)),
-          "alloc U");
-  HIP_TRY(ctx, m->d_Y.alloc((size_t)nc * (nb + 1) * th.ni * sizeof(double)), "alloc Y");
-  HIP_TRY(ctx, m->d_alpha.alloc((size_t)std::max<int64_t>(nc, 1) * 72 * sizeof(double)),
-          "alloc alpha");
-  HIP_TRY(ctx, m->d_srcq.alloc((size_t)std::max<int64_t>(nc, 1) * th.nqc * 4 * sizeof(double)),
-          "alloc srcq");
-  HIP_TRY(ctx, m->d_info.alloc((size_t)std::max<int64_t>(nc, 1) * sizeof(int32_t)), "alloc info");
+  HIP_TRY(ctx, own_cell_state(m.get()), "alloc per-cell state");
   HIP_TRY(ctx, m->d_rowval.alloc((size_t)std::max<int64_t>(m->nnz, 1) * sizeof(int32_t)), "alloc rowval");
   HIP_TRY(ctx, m->d_nzval.alloc((size_t)std::max<int64_t>(m->nnz, 1) * sizeof(double)), "alloc nzval");
   HIP_TRY(ctx, m->d_rhs.alloc((size_t)std::max<int64_t>(m->nfree, 1) * sizeof(double)), "alloc rhs");
@@ -625,6 +672,8 @@ int hdg_mesh_create_cartesian(hdg_ctx* ctx, const hdg_problem* prob, const int64
 
 int hdg_mesh_destroy(hdg_mesh* mesh) {
   if (!mesh) return HDG_ERR_ARG;
+  if (mesh->batch)
+    return fail(mesh->ctx, HDG_ERR_STATE, "mesh belongs to a batch: destroy the batch first");
   (void)hipSetDevice(mesh->ctx->device);
   (void)hipStreamSynchronize(mesh->ctx->stream);
   delete mesh;
@@ -746,13 +795,15 @@ int hdg_condense(hdg_mesh* m, int precision) {
   if (m->prob.source == HDG_SOURCE_QP && !m->src_qp)
     return fail(ctx, HDG_ERR_STATE, "source values at quadrature points not set");
   HIP_TRY(ctx, hipSetDevice(ctx->device), "hipSetDevice");
+  if (!m->pS) HIP_TRY(ctx, own_cell_state(m), "alloc per-cell state");
   CondenseOut out;
-  out.S = m->d_S.as<double>();
-  out.g = m->d_g.as<double>();
-  out.U = m->d_U.as<double>();
-  out.Y = m->d_Y.as<double>();
-  out.info = m->d_info.as<int32_t>();
-  HIP_TRY(ctx, launch_condense_fused(m->tdev, m->meshdev(), out, ctx->stream), "condense kernel");
+  out.S = m->pS;
+  out.g = m->pg;
+  out.U = m->pU;
+  out.Y = m->pY;
+  out.info = m->pinfo;
+  HIP_TRY(ctx, launch_condense_fused(m->tdev, m->meshdev(), out, ctx->stream, ctx->ws),
+          "condense kernel");
   m->condensed = true;
   return HDG_OK;
 }
@@ -762,7 +813,7 @@ int hdg_assemble(hdg_mesh* m) {
   hdg_ctx* ctx = m->ctx;
   if (!m->condensed) return fail(ctx, HDG_ERR_STATE, "assemble called before condense");
   HIP_TRY(ctx, hipSetDevice(ctx->device), "hipSetDevice");
-  HIP_TRY(ctx, launch_assemble(m->pairs(), m->d_S.as<double>(), m->d_g.as<double>(),
+  HIP_TRY(ctx, launch_assemble(m->pairs(), m->pS, m->pg,
                                m->d_cell_dofs.as<int32_t>(), m->d_lam_dir.as<double>(),
                                m->d_nzval.as<double>(), m->d_rhs.as<double>(), ctx->stream),
           "assemble kernels");
@@ -772,7 +823,7 @@ int hdg_assemble(hdg_mesh* m) {
 int hdg_mesh_singular_count(hdg_mesh* m, int64_t* n_singular) {
   if (!m || !n_singular) return HDG_ERR_ARG;
   HIP_TRY(m->ctx, hipSetDevice(m->ctx->device), "hipSetDevice");
-  return count_singular(m->ctx, m->d_info.as<int32_t>(), m->ncells, n_singular);
+  return count_singular(m->ctx, m->pinfo, m->ncells, n_singular);
 }
 
 int hdg_condense_assemble(hdg_mesh* m, int precision, int64_t* n_singular) {
@@ -798,16 +849,16 @@ int hdg_skeleton_system(hdg_mesh* m, const int64_t** colptr, const int32_t** row
 
 int hdg_cell_blocks(hdg_mesh* m, const double** S, const double** g, const int32_t** cell_dofs) {
   if (!m) return HDG_ERR_ARG;
-  if (S) *S = m->d_S.as<double>();
-  if (g) *g = m->d_g.as<double>();
+  if (S) *S = m->pS;
+  if (g) *g = m->pg;
   if (cell_dofs) *cell_dofs = m->d_cell_dofs.as<int32_t>();
   return HDG_OK;
 }
 
 int hdg_local_factors(hdg_mesh* m, const double** U, const double** Y) {
   if (!m) return HDG_ERR_ARG;
-  if (U) *U = m->d_U.as<double>();
-  if (Y) *Y = m->d_Y.as<double>();
+  if (U) *U = m->pU;
+  if (Y) *Y = m->pY;
   return m->condensed ? HDG_OK : fail(m->ctx, HDG_ERR_STATE, "no factors before condensation");
 }
 
@@ -817,7 +868,7 @@ int hdg_recover(hdg_mesh* m, const double* lam_free, double* free_values) {
   if (!m->condensed) return fail(ctx, HDG_ERR_STATE, "recover called before condense");
   if ((m->nfree > 0 && !lam_free) || !free_values) return fail(ctx, HDG_ERR_ARG, "null buffer");
   HIP_TRY(ctx, hipSetDevice(ctx->device), "hipSetDevice");
-  HIP_TRY(ctx, launch_recover(m->d_U.as<double>(), m->d_Y.as<double>(),
+  HIP_TRY(ctx, launch_recover(m->pU, m->pY,
                               m->d_cell_dofs.as<int32_t>(), lam_free,
                               m->d_lam_dir.as<double>(), free_values, m->ncells, m->th.ni,
                               m->th.nb, m->th.na, m->nfree, ctx->stream),
@@ -833,8 +884,345 @@ int hdg_cell_systems(hdg_mesh* m, int64_t first, int64_t count, double* A, doubl
   if (m->prob.source == HDG_SOURCE_QP && !m->src_qp)
     return fail(ctx, HDG_ERR_STATE, "source values at quadrature points not set");
   HIP_TRY(ctx, hipSetDevice(ctx->device), "hipSetDevice");
+  if (!m->pS) HIP_TRY(ctx, own_cell_state(m), "alloc per-cell state");
   HIP_TRY(ctx, launch_cell_systems(m->tdev, m->meshdev(), first, count, A, b, ctx->stream),
           "cell systems kernel");
+  return HDG_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Batches of independent meshes: per-class contiguous per-cell state, one
+// condensation launch per class, everything captured into one HIP graph.
+// ---------------------------------------------------------------------------
+struct hdg_batch {
+  struct Class {
+    hdg_batch_class info{};
+    std::vector<int> members;
+    const hdg_mesh* lead = nullptr;  // reference tables shared by the members
+    DevBuf S, g, U, Y, alpha, srcq, infos;
+
+    MeshDev cells(int64_t count) const {
+      MeshDev M;
+      M.ncells = count;
+      M.alpha = alpha.as<double>();
+      M.srcq = srcq.as<double>();
+      return M;
+    }
+    CondenseOut out() const {
+      CondenseOut o;
+      o.S = S.as<double>();
+      o.g = g.as<double>();
+      o.U = U.as<double>();
+      o.Y = Y.as<double>();
+      o.info = infos.as<int32_t>();
+      return o;
+    }
+  };
+  hdg_ctx* ctx = nullptr;
+  std::vector<hdg_mesh*> meshes;
+  std::vector<std::unique_ptr<Class>> classes;
+  Workspace ws;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  std::vector<const double*> captured_src;  // member sources baked into the graph
+
+  void drop_graph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+  }
+};
+
+namespace {
+
+bool same_class(const hdg_problem& a, const hdg_problem& b) {
+  return a.physics == b.physics && a.dim == b.dim && a.space == b.space &&
+         a.k_flux == b.k_flux && a.k_pot == b.k_pot && a.k_trace == b.k_trace &&
+         a.quad_degree == b.quad_degree;
+}
+
+bool sources_ready(hdg_batch* b) {
+  for (hdg_mesh* m : b->meshes)
+    if (m->prob.source == HDG_SOURCE_QP && !m->src_qp) return false;
+  return true;
+}
+
+hipError_t enqueue_prep(hdg_batch* b, hipStream_t st) {
+  for (hdg_mesh* m : b->meshes) {
+    hipError_t e = launch_prep_cells(m->tdev, m->meshdev(), st);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+hipError_t enqueue_condense(hdg_batch* b, const hdg_batch::Class& c, int64_t count, int fam,
+                            int gridmul, hipStream_t st) {
+  return launch_condense_fused(c.lead->tdev, c.cells(count), c.out(), st, b->ws, false, fam,
+                               gridmul);
+}
+
+hipError_t enqueue_all(hdg_batch* b, hipStream_t st) {
+  hipError_t e = enqueue_prep(b, st);
+  for (size_t k = 0; e == hipSuccess && k < b->classes.size(); ++k) {
+    const hdg_batch::Class& c = *b->classes[k];
+    e = enqueue_condense(b, c, c.info.ncells, c.info.family, c.info.grid_per_cu, st);
+  }
+  for (size_t k = 0; e == hipSuccess && k < b->meshes.size(); ++k) {
+    hdg_mesh* m = b->meshes[k];
+    e = launch_assemble(m->pairs(), m->pS, m->pg, m->d_cell_dofs.as<int32_t>(),
+                        m->d_lam_dir.as<double>(), m->d_nzval.as<double>(),
+                        m->d_rhs.as<double>(), st);
+  }
+  return e;
+}
+
+// Times every (family, grid) able to condense each class on a bounded sample
+// of its own cells and keeps the fastest.  The per-cell preparation must have
+// run.
+int autotune(hdg_batch* b) {
+  hdg_ctx* ctx = b->ctx;
+  hipStream_t st = ctx->stream;
+  hipEvent_t e0, e1;
+  HIP_TRY(ctx, hipEventCreate(&e0), "event");
+  HIP_TRY(ctx, hipEventCreate(&e1), "event");
+  int rc = HDG_OK;
+  for (auto& cp : b->classes) {
+    hdg_batch::Class& c = *cp;
+    const int n = c.info.ni + c.info.nb;
+    int fams[8];
+    const int nf = condense_variants(c.info.ni, c.info.nb, fams, 8);
+    int gms[8];
+    int ncand = 0;
+    for (int i = 0; i < nf; ++i) ncand += condense_grid_options(fams[i], gms, 8);
+    if (ncand <= 1) continue;
+    const int64_t sample =
+        std::min<int64_t>(c.info.ncells, std::max<int64_t>(8192, (int64_t(1) << 26) / (n * n)));
+    float best = 0.f;
+    for (int i = 0; i < nf && rc == HDG_OK; ++i) {
+      const int ng = condense_grid_options(fams[i], gms, 8);
+      hipError_t e = workspace_reserve(b->ws, condense_workspace_bytes(c.info.ni, c.info.nb, fams[i]));
+      if (e != hipSuccess) {
+        rc = hipfail(ctx, e, "autotune workspace");
+        break;
+      }
+      for (int j = 0; j < ng; ++j) {
+        e = enqueue_condense(b, c, sample, fams[i], gms[j], st);  // warm-up
+        if (e == hipSuccess) e = hipEventRecord(e0, st);
+        for (int r = 0; r < 2 && e == hipSuccess; ++r)
+          e = enqueue_condense(b, c, sample, fams[i], gms[j], st);
+        if (e == hipSuccess) e = hipEventRecord(e1, st);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        float ms = 0.f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+        if (e != hipSuccess) {
+          rc = hipfail(ctx, e, "autotune launch");
+          break;
+        }
+        ms *= 0.5f;
+        if (best == 0.f || ms < best) {
+          best = ms;
+          c.info.family = fams[i];
+          c.info.grid_per_cu = gms[j];
+        }
+      }
+    }
+    // time of the pick, scaled from the sample to the whole class
+    c.info.tuned_ms = best * (double)c.info.ncells / (double)std::max<int64_t>(sample, 1);
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return rc;
+}
+
+int capture(hdg_batch* b) {
+  hdg_ctx* ctx = b->ctx;
+  hipStream_t st = ctx->stream;
+  b->drop_graph();
+  size_t need = 0;
+  for (auto& c : b->classes)
+    need = std::max(need, condense_workspace_bytes(c->info.ni, c->info.nb, c->info.family));
+  HIP_TRY(ctx, workspace_reserve(b->ws, need), "batch workspace");
+  HIP_TRY(ctx, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "begin capture");
+  hipError_t e = enqueue_all(b, st);
+  hipGraph_t g = nullptr;
+  hipError_t e2 = hipStreamEndCapture(st, &g);
+  if (e != hipSuccess || e2 != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    return hipfail(ctx, e != hipSuccess ? e : e2, "batch capture");
+  }
+  b->graph = g;
+  HIP_TRY(ctx, hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0), "graph instantiate");
+  b->captured_src.clear();
+  for (hdg_mesh* m : b->meshes) b->captured_src.push_back(m->src_qp);
+  return HDG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int hdg_batch_create(hdg_ctx* ctx, int nmesh, hdg_mesh* const* meshes, int flags,
+                     hdg_batch** out) {
+  if (!ctx) return HDG_ERR_ARG;
+  if (!out || !meshes || nmesh <= 0) return fail(ctx, HDG_ERR_ARG, "empty batch");
+  *out = nullptr;
+  if (flags & ~HDG_BATCH_AUTOTUNE) return fail(ctx, HDG_ERR_ARG, "unknown batch flags");
+  std::unique_ptr<hdg_batch> b(new hdg_batch());
+  b->ctx = ctx;
+  for (int i = 0; i < nmesh; ++i) {
+    hdg_mesh* m = meshes[i];
+    if (!m || m->ctx != ctx) return fail(ctx, HDG_ERR_ARG, "mesh of another context");
+    if (m->batch) return fail(ctx, HDG_ERR_STATE, "mesh already belongs to a batch");
+    if (std::find(b->meshes.begin(), b->meshes.end(), m) != b->meshes.end())
+      return fail(ctx, HDG_ERR_ARG, "mesh listed twice");
+    b->meshes.push_back(m);
+  }
+  // classes in order of first appearance
+  for (int i = 0; i < nmesh; ++i) {
+    hdg_mesh* m = b->meshes[i];
+    hdg_batch::Class* c = nullptr;
+    for (auto& cp : b->classes)
+      if (same_class(cp->lead->prob, m->prob)) c = cp.get();
+    if (!c) {
+      b->classes.emplace_back(new hdg_batch::Class());
+      c = b->classes.back().get();
+      c->lead = m;
+      c->info.ni = m->th.ni;
+      c->info.nb = m->th.nb;
+      c->info.physics = m->prob.physics;
+      c->info.dim = m->prob.dim;
+      c->info.family = condense_default_family(m->th.ni, m->th.nb);
+    }
+    c->members.push_back(i);
+    c->info.nmesh++;
+    c->info.ncells += m->ncells;
+  }
+  HIP_TRY(ctx, hipSetDevice(ctx->device), "hipSetDevice");
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream), "sync");
+  for (auto& cp : b->classes) {
+    hdg_batch::Class& c = *cp;
+    const CellLayout L(c.lead->th);
+    const size_t nc = (size_t)std::max<int64_t>(c.info.ncells, 1);
+    HIP_TRY(ctx, c.S.alloc(nc * L.S * sizeof(double)), "alloc batch S");
+    HIP_TRY(ctx, c.g.alloc(nc * L.g * sizeof(double)), "alloc batch g");
+    HIP_TRY(ctx, c.U.alloc(nc * L.U * sizeof(double)), "alloc batch U");
+    HIP_TRY(ctx, c.Y.alloc(nc * L.Y * sizeof(double)), "alloc batch Y");
+    HIP_TRY(ctx, c.alpha.alloc(nc * L.alpha * sizeof(double)), "alloc batch alpha");
+    HIP_TRY(ctx, c.srcq.alloc(nc * L.srcq * sizeof(double)), "alloc batch srcq");
+    HIP_TRY(ctx, c.infos.alloc(nc * sizeof(int32_t)), "alloc batch info");
+  }
+  // rebind the members to their slices (nothing fails from here on)
+  for (auto& cp : b->classes) {
+    hdg_batch::Class& c = *cp;
+    const CellLayout L(c.lead->th);
+    size_t off = 0;
+    for (int i : c.members) {
+      hdg_mesh* m = b->meshes[i];
+      drop_cell_state(m);
+      m->pS = c.S.as<double>() + off * L.S;
+      m->pg = c.g.as<double>() + off * L.g;
+      m->pU = c.U.as<double>() + off * L.U;
+      m->pY = c.Y.as<double>() + off * L.Y;
+      m->palpha = c.alpha.as<double>() + off * L.alpha;
+      m->psrcq = c.srcq.as<double>() + off * L.srcq;
+      m->pinfo = c.infos.as<int32_t>() + off;
+      m->batch = b.get();
+      m->condensed = false;
+      off += (size_t)m->ncells;
+    }
+  }
+  hdg_batch* bp = b.release();
+  if (flags & HDG_BATCH_AUTOTUNE) {
+    int rc = HDG_OK;
+    hipError_t e;
+    if (!sources_ready(bp))
+      rc = fail(ctx, HDG_ERR_STATE, "autotune needs every source set");
+    else if ((e = enqueue_prep(bp, ctx->stream)) != hipSuccess)
+      rc = hipfail(ctx, e, "prep kernels");
+    else
+      rc = autotune(bp);
+    if (rc) {
+      const std::string msg = ctx->err;
+      (void)hdg_batch_destroy(bp);
+      ctx->err = msg;
+      return rc;
+    }
+  }
+  *out = bp;
+  return HDG_OK;
+}
+
+int hdg_batch_destroy(hdg_batch* b) {
+  if (!b) return HDG_ERR_ARG;
+  hdg_ctx* ctx = b->ctx;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  b->drop_graph();
+  workspace_release(b->ws);
+  int rc = HDG_OK;
+  for (hdg_mesh* m : b->meshes) {
+    m->batch = nullptr;
+    m->condensed = false;
+    m->pS = m->pg = m->pU = m->pY = m->palpha = m->psrcq = nullptr;
+    m->pinfo = nullptr;
+    hipError_t e = own_cell_state(m);
+    if (e != hipSuccess && rc == HDG_OK) rc = hipfail(ctx, e, "alloc per-cell state");
+  }
+  delete b;
+  return rc;
+}
+
+int hdg_batch_get_classes(const hdg_batch* b, int max, hdg_batch_class* classes, int* nclass) {
+  if (!b || (max > 0 && !classes)) return HDG_ERR_ARG;
+  if (nclass) *nclass = (int)b->classes.size();
+  for (int k = 0; k < max && k < (int)b->classes.size(); ++k) classes[k] = b->classes[k]->info;
+  return HDG_OK;
+}
+
+int hdg_batch_launch(hdg_batch* b, int precision) {
+  if (!b) return HDG_ERR_ARG;
+  hdg_ctx* ctx = b->ctx;
+  if (precision != HDG_FP64 && precision != HDG_MIXED)
+    return fail(ctx, HDG_ERR_ARG, "unknown precision");
+  if (precision == HDG_MIXED) return fail(ctx, HDG_ERR_UNSUPPORTED, "mixed precision not built yet");
+  if (!sources_ready(b)) return fail(ctx, HDG_ERR_STATE, "source values at quadrature points not set");
+  HIP_TRY(ctx, hipSetDevice(ctx->device), "hipSetDevice");
+  bool stale = !b->exec;
+  for (size_t k = 0; !stale && k < b->meshes.size(); ++k)
+    stale = b->captured_src[k] != b->meshes[k]->src_qp;
+  if (stale) {
+    int rc = capture(b);
+    if (rc) return rc;
+  }
+  HIP_TRY(ctx, hipGraphLaunch(b->exec, ctx->stream), "graph launch");
+  for (hdg_mesh* m : b->meshes) m->condensed = true;
+  return HDG_OK;
+}
+
+int hdg_batch_condense_assemble(hdg_batch* b, int precision, int64_t* n_singular) {
+  int rc = hdg_batch_launch(b, precision);
+  if (rc) return rc;
+  int64_t bad = 0;
+  for (auto& c : b->classes) {
+    int64_t k = 0;
+    if ((rc = count_singular(b->ctx, c->infos.as<int32_t>(), c->info.ncells, &k))) return rc;
+    bad += k;
+  }
+  if (n_singular) *n_singular = bad;
+  if (bad) return fail(b->ctx, HDG_SINGULAR, std::to_string(bad) + " cell(s) with singular A_ii");
+  return HDG_OK;
+}
+
+int hdg_batch_recover(hdg_batch* b, const double* const* lam_free, double* const* free_values) {
+  if (!b) return HDG_ERR_ARG;
+  if (!lam_free || !free_values) return fail(b->ctx, HDG_ERR_ARG, "null pointer array");
+  for (size_t k = 0; k < b->meshes.size(); ++k) {
+    int rc = hdg_recover(b->meshes[k], lam_free[k], free_values[k]);
+    if (rc) return rc;
+  }
   return HDG_OK;
 }
 

@@ -122,14 +122,34 @@ struct PairTable {
 
 #include <hip/hip_runtime_api.h>
 namespace hdg {
+// Device workspace of the blocked large-system kernel.  Owned by a context or
+// a batch; it only grows, and only outside stream capture (a captured graph
+// keeps the address it saw).
+struct Workspace {
+  double* p = nullptr;
+  size_t bytes = 0;
+};
+hipError_t workspace_reserve(Workspace& w, size_t bytes);
+void workspace_release(Workspace& w);
+
 // Kernel launchers (hdg_kernels.hip)
 bool condense_supported(int n);
+// fam: kernel family (-1 = default for the size; see condense_variants);
+// gridmul: resident workgroups per CU (0 = the family's default)
 hipError_t launch_condense_fused(const Tables& T, const MeshDev& M, const CondenseOut& out,
-                                 hipStream_t st);
+                                 hipStream_t st, Workspace& ws, bool prep = true, int fam = -1,
+                                 int gridmul = 0);
+hipError_t launch_prep_cells(const Tables& T, const MeshDev& M, hipStream_t st);
+// kernel families able to condense an (ni, nb) system; returns their count
+int condense_variants(int ni, int nb, int* fams, int max);
+int condense_default_family(int ni, int nb);
+// grid sizes (resident workgroups per CU) worth timing for a family
+int condense_grid_options(int fam, int* gm, int max);
+size_t condense_workspace_bytes(int ni, int nb, int fam);
 hipError_t launch_condense_dense(const DenseIn& in, int64_t ncells, const CondenseOut& out,
-                                 hipStream_t st);
+                                 hipStream_t st, Workspace& ws);
 hipError_t launch_backward_dense(const DenseIn& in, const double* x, int64_t ncells, double* xi,
-                                 int32_t* info, hipStream_t st);
+                                 int32_t* info, hipStream_t st, Workspace& ws);
 hipError_t launch_recover(const double* U, const double* Y, const int32_t* cell_dofs,
                           const double* lam_free, const double* lam_dir, double* out,
                           int64_t ncells, int ni, int nb, int na, int64_t nfree, hipStream_t st);

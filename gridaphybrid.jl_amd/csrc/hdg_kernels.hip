@@ -1057,7 +1057,7 @@ __global__ void __launch_bounds__(256)
   constexpr int NT = 256, KB = kLargeKB, NMAX = kLargeMaxN;
   const int n = ni + nb, ncol = n + 1;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  double* M = work + (int64_t)blockIdx.x * ld * (int64_t)(NMAX + 1);
+  double* M = work + (int64_t)blockIdx.x * ld * (int64_t)ncol;
 
   __shared__ double s_P[KB][NMAX];        // panel columns (all rows)
   __shared__ double s_U[KB][NMAX + 1];    // pivot rows x trailing columns
@@ -1444,68 +1444,114 @@ static int grid_for(int64_t ncells, int threads) {
   return (int)(ncells < target ? ncells : target);
 }
 
-// Workspace of the large-system kernel (grown on demand; one per process).
-static double* g_large_work = nullptr;
-static size_t g_large_bytes = 0;
 constexpr int kLargeGrid = 256 * 2;
 
-static hipError_t large_workspace(int n, double** ws, int* ld) {
+// Bytes of the large-system kernel's workspace: one column-major (n+1)-column
+// system per resident workgroup, leading dimension rounded up to even.
+static size_t large_bytes(int n, int* ld) {
   const int l = (n + 1) & ~1;
-  const size_t need = (size_t)kLargeGrid * l * (kLargeMaxN + 1) * sizeof(double);
-  if (need > g_large_bytes) {
-    if (g_large_work) (void)hipFree(g_large_work);
-    g_large_work = nullptr;
-    g_large_bytes = 0;
-    hipError_t e = hipMalloc(&g_large_work, need);
-    if (e != hipSuccess) return e;
-    g_large_bytes = need;
+  if (ld) *ld = l;
+  return (size_t)kLargeGrid * l * (size_t)(n + 1) * sizeof(double);
+}
+
+hipError_t workspace_reserve(Workspace& w, size_t bytes) {
+  if (bytes <= w.bytes) return hipSuccess;
+  workspace_release(w);
+  hipError_t e = hipMalloc(&w.p, bytes);
+  if (e != hipSuccess) {
+    w.p = nullptr;
+    return e;
   }
-  *ws = g_large_work;
-  *ld = l;
+  w.bytes = bytes;
   return hipSuccess;
+}
+
+void workspace_release(Workspace& w) {
+  if (w.p) (void)hipFree(w.p);
+  w.p = nullptr;
+  w.bytes = 0;
+}
+
+// Kernel families for a local system size (see condense_variants()).
+enum Family { kFamWave = 0, kFamGrid8 = 1, kFamColwave = 2, kFamGrid32 = 3, kFamLarge = 4 };
+
+static bool wave_class(int ni, int nb) {
+  return (ni == 12 && nb == 8) || (ni == 27 && nb == 12) || (ni == 48 && nb == 16) ||
+         (ni == 32 && nb == 24) || (ni == 7 && nb == 8);
+}
+
+static bool family_ok(int fam, int ni, int nb) {
+  const int n = ni + nb;
+  Bucket b;
+  switch (fam) {
+    case kFamWave: return wave_class(ni, nb);
+    case kFamGrid8: return pick_bucket(n, b) && b.pr == 8;
+    case kFamColwave: return n > 64 && n <= 192 && n + 1 <= 168;
+    case kFamGrid32: return n <= 192 && n + 1 <= 176 && n > 71;
+    case kFamLarge: return n > 64 && n <= kLargeMaxN;
+  }
+  return false;
+}
+
+// default choice: the measured-fastest family for the size
+static int default_family(int ni, int nb) {
+  const int n = ni + nb;
+  if (wave_class(ni, nb)) return kFamWave;
+  if (n + 1 <= 72) return kFamGrid8;
+  if (n <= 192 && n + 1 <= 168) return kFamColwave;
+  return kFamLarge;
 }
 
 template <class Gen, bool BACKSUB>
 static hipError_t launch_condense(const Gen& gen, const CondenseOut& out, int64_t ncells, int ni,
-                                  int nb, hipStream_t st) {
+                                  int nb, hipStream_t st, Workspace& ws, int fam = -1,
+                                  int gridmul = 0) {
   if (ncells <= 0) return hipSuccess;
-  if (ni + nb > 192 && ni + nb <= kLargeMaxN) {
-    double* ws;
+  if (fam < 0) fam = default_family(ni, nb);
+  if (!family_ok(fam, ni, nb)) return hipErrorInvalidValue;
+  auto grid_of = [&](int per_cu_default) {
+    const int64_t per = gridmul > 0 ? gridmul : per_cu_default;
+    const int64_t target = 256 * per;
+    return (int)(ncells < target ? ncells : target);
+  };
+  if (fam == kFamLarge) {
     int ld;
-    hipError_t e = large_workspace(ni + nb, &ws, &ld);
+    hipError_t e = workspace_reserve(ws, large_bytes(ni + nb, &ld));
     if (e != hipSuccess) return e;
     const int grid = (int)(ncells < kLargeGrid ? ncells : kLargeGrid);
     hipLaunchKernelGGL((condense_large_kernel<Gen, BACKSUB>), dim3(grid), dim3(256), 0, st, gen,
-                       out, ncells, ni, nb, ws, ld);
+                       out, ncells, ni, nb, ws.p, ld);
     return hipGetLastError();
   }
-  const int ncol = ni + nb + 1;
-  {
-    const int64_t target = 256 * 48;
-    const int grid = (int)(ncells < target ? ncells : target);
+  if (fam == kFamWave) {
+    const int grid = grid_of(48);
 #define HDG_WAVE(NI_, NB_)                                                                  \
   if (ni == NI_ && nb == NB_) {                                                             \
     hipLaunchKernelGGL((condense_wave_kernel<NI_, NB_, Gen, BACKSUB>), dim3(grid), dim3(64), \
                        0, st, gen, out, ncells);                                            \
     return hipGetLastError();                                                               \
   }
-    // Darcy HDG Q_k classes (2D k=1..3, 3D k=1) and the reference P1/P0/P1 test
     HDG_WAVE(12, 8)
     HDG_WAVE(27, 12)
     HDG_WAVE(48, 16)
     HDG_WAVE(32, 24)
     HDG_WAVE(7, 8)
 #undef HDG_WAVE
+    return hipErrorInvalidValue;
   }
-  if (ni + nb > 64 && ni + nb <= 192 && ni + nb + 1 <= 168) {
-    const int grid = (int)(ncells < 256 * 2 ? ncells : 256 * 2);
+  if (fam == kFamColwave) {
+    const int grid = grid_of(2);
     hipLaunchKernelGGL((condense_colwave_kernel<3, 21, 8, Gen, BACKSUB>), dim3(grid), dim3(512),
                        0, st, gen, out, ncells, ni, nb);
     return hipGetLastError();
   }
   Bucket b;
-  if (!pick_bucket(ni + nb, b)) return hipErrorInvalidValue;
-  const int grid = grid_for(ncells, b.pr * b.pc);
+  if (fam == kFamGrid32) {
+    b = Bucket{32, 16, 6, 11};
+  } else if (!pick_bucket(ni + nb, b)) {
+    return hipErrorInvalidValue;
+  }
+  const int grid = gridmul > 0 ? grid_of(gridmul) : grid_for(ncells, b.pr * b.pc);
 #define HDG_LAUNCH(PR, PC, R, CC)                                                             \
   if (b.pr == PR && b.pc == PC && b.r == R && b.cc == CC) {                                   \
     hipLaunchKernelGGL((condense_kernel<PR, PC, R, CC, Gen, BACKSUB>), dim3(grid),             \
@@ -1520,9 +1566,41 @@ static hipError_t launch_condense(const Gen& gen, const CondenseOut& out, int64_
   return hipErrorInvalidValue;
 }
 
+int condense_variants(int ni, int nb, int* fams, int max) {
+  int k = 0;
+  for (int f = 0; f <= kFamLarge && k < max; ++f)
+    if (family_ok(f, ni, nb)) fams[k++] = f;
+  return k;
+}
+
+int condense_grid_options(int fam, int* gm, int max) {
+  // resident workgroups per CU worth timing for each family (0 = fixed grid)
+  static const int wave[] = {16, 32, 48, 64}, grid8[] = {20, 40, 80}, col[] = {1, 2, 3},
+                   g32[] = {2, 3, 4}, large[] = {0};
+  const int* v = large;
+  int n = 1;
+  switch (fam) {
+    case kFamWave: v = wave, n = 4; break;
+    case kFamGrid8: v = grid8, n = 3; break;
+    case kFamColwave: v = col, n = 3; break;
+    case kFamGrid32: v = g32, n = 3; break;
+    default: break;
+  }
+  int k = 0;
+  for (; k < n && k < max; ++k) gm[k] = v[k];
+  return k;
+}
+
+int condense_default_family(int ni, int nb) { return default_family(ni, nb); }
+
+size_t condense_workspace_bytes(int ni, int nb, int fam) {
+  if (fam < 0) fam = default_family(ni, nb);
+  return fam == kFamLarge ? large_bytes(ni + nb, nullptr) : 0;
+}
+
 bool condense_supported(int n) {
   Bucket b;
-  return pick_bucket(n, b) || (n > 192 && n <= kLargeMaxN);
+  return pick_bucket(n, b) || (n > 64 && n <= kLargeMaxN);
 }
 
 static hipError_t launch_prep(const Tables& T, const MeshDev& M, hipStream_t st) {
@@ -1536,32 +1614,41 @@ static hipError_t launch_prep(const Tables& T, const MeshDev& M, hipStream_t st)
   return hipGetLastError();
 }
 
+hipError_t launch_prep_cells(const Tables& T, const MeshDev& M, hipStream_t st) {
+  return launch_prep(T, M, st);
+}
+
 hipError_t launch_condense_fused(const Tables& T, const MeshDev& M, const CondenseOut& out,
-                                 hipStream_t st) {
-  hipError_t e = launch_prep(T, M, st);
-  if (e != hipSuccess) return e;
+                                 hipStream_t st, Workspace& ws, bool prep, int fam,
+                                 int gridmul) {
+  if (prep) {
+    hipError_t e = launch_prep(T, M, st);
+    if (e != hipSuccess) return e;
+  }
   if (T.phys == kDarcy) {
     FusedGen<kDarcy> gen{T, M};
-    return launch_condense<FusedGen<kDarcy>, false>(gen, out, M.ncells, T.ni, T.nb, st);
+    return launch_condense<FusedGen<kDarcy>, false>(gen, out, M.ncells, T.ni, T.nb, st, ws, fam,
+                                                    gridmul);
   }
   FusedGen<kElasticity> gen{T, M};
-  return launch_condense<FusedGen<kElasticity>, false>(gen, out, M.ncells, T.ni, T.nb, st);
+  return launch_condense<FusedGen<kElasticity>, false>(gen, out, M.ncells, T.ni, T.nb, st, ws,
+                                                       fam, gridmul);
 }
 
 hipError_t launch_condense_dense(const DenseIn& in, int64_t ncells, const CondenseOut& out,
-                                 hipStream_t st) {
+                                 hipStream_t st, Workspace& ws) {
   DenseGen gen{in};
-  return launch_condense<DenseGen, false>(gen, out, ncells, in.ni, in.nb, st);
+  return launch_condense<DenseGen, false>(gen, out, ncells, in.ni, in.nb, st, ws);
 }
 
 hipError_t launch_backward_dense(const DenseIn& in, const double* x, int64_t ncells, double* xi,
-                                 int32_t* info, hipStream_t st) {
+                                 int32_t* info, hipStream_t st, Workspace& ws) {
   BackwardGen gen{in, x};
   CondenseOut out{};
   out.X = xi;
   out.info = info;
   // interior-only system: nb = 0, the single rhs column is b_i - A_ib x
-  return launch_condense<BackwardGen, true>(gen, out, ncells, in.ni, 0, st);
+  return launch_condense<BackwardGen, true>(gen, out, ncells, in.ni, 0, st, ws);
 }
 
 hipError_t launch_recover(const double* U, const double* Y, const int32_t* cell_dofs,
@@ -1618,10 +1705,15 @@ __global__ void count_nonzero_kernel(const int32_t* __restrict__ v, int64_t n,
   if (local) atomicAdd(count, local);  // integer count: order-independent
 }
 
+__global__ void zero_count_kernel(unsigned long long* count) {
+  if (threadIdx.x == 0) count[threadIdx.x] = 0;
+}
+
 hipError_t launch_count_nonzero(const int32_t* info, int64_t n, int64_t* count_dev,
                                 hipStream_t st) {
-  hipError_t e = hipMemsetAsync(count_dev, 0, sizeof(int64_t), st);
-  if (e != hipSuccess) return e;
+  // a fill kernel rather than a memset node, so the sequence stays capturable
+  hipLaunchKernelGGL(zero_count_kernel, dim3(1), dim3(1), 0, st,
+                     reinterpret_cast<unsigned long long*>(count_dev));
   if (n > 0) {
     const int grid = (int)((n + 255) / 256 < 1024 ? (n + 255) / 256 : 1024);
     hipLaunchKernelGGL(count_nonzero_kernel, dim3(grid), dim3(256), 0, st, info, n,

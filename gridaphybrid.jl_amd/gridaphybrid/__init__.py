@@ -281,7 +281,7 @@ class HybridAffineFEOperator:
     device until ``solve``."""
 
     def __init__(self, weakform, trial, test, bulk_fields, skeleton_fields, precision=FP64,
-                 ctx=None):
+                 ctx=None, assemble=True):
         if list(bulk_fields) != [1, 2] or list(skeleton_fields) != [3]:
             raise HDGError(_lib.HDG_ERR_UNSUPPORTED,
                            "hybrid operators here hybridize fields [1,2] onto trace field [3]")
@@ -302,7 +302,9 @@ class HybridAffineFEOperator:
             check(lib().hdg_mesh_set_source(self.mesh, self._src_keep.data_ptr()), self.ctx.h,
                   "set_source")
         check(lib().hdg_mesh_set_dirichlet_mms(self.mesh), self.ctx.h, "dirichlet")
-        self.assemble()
+        self._batch = None
+        if assemble:
+            self.assemble()
 
     def assemble(self):
         nbad = C.c_int64(0)
@@ -369,10 +371,83 @@ class HybridAffineFEOperator:
         self.ctx.synchronize()
         return out
 
+    def interpolate_trace(self):
+        """Host interpolant of the built-in manufactured trace at the free
+        trace dofs (Gridap's interpolate on the skeleton space)."""
+        lam = np.empty(self.info.nfree)
+        check(lib().hdg_mesh_interpolate_trace(self.mesh, lam.ctypes.data), self.ctx.h,
+              "interpolate_trace")
+        return lam
+
     def close(self):
-        if getattr(self, "mesh", None):
+        if getattr(self, "_batch", None) is not None:
+            self._batch.close()
+        # a context closed first (interpreter shutdown) took the device with it
+        if getattr(self, "mesh", None) and self.ctx.h:
             lib().hdg_mesh_destroy(self.mesh)
-            self.mesh = None
+        self.mesh = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class HybridOperatorBatch:
+    """Many independent HybridAffineFEOperators condensed and assembled
+    together: one condensation launch per class of equal local systems, the
+    whole sequence replayed from one captured HIP graph per ``launch``.  Build
+    the operators with ``assemble=False``; afterwards their ``skeleton_system``,
+    ``cell_blocks`` and ``recover`` work as for a lone operator."""
+
+    def __init__(self, ops, autotune=True):
+        if not ops:
+            raise HDGError(_lib.HDG_ERR_ARG, "empty batch")
+        self.ops = list(ops)
+        self.ctx = self.ops[0].ctx
+        arr = (C.c_void_p * len(self.ops))(*[op.mesh for op in self.ops])
+        h = C.c_void_p()
+        flags = _lib.BATCH_AUTOTUNE if autotune else 0
+        check(lib().hdg_batch_create(self.ctx.h, len(self.ops), arr, flags, C.byref(h)),
+              self.ctx.h, "batch_create")
+        self.h = h
+        for op in self.ops:
+            op._batch = self
+
+    def classes(self):
+        n = C.c_int(0)
+        check(lib().hdg_batch_get_classes(self.h, 0, None, C.byref(n)), self.ctx.h, "classes")
+        arr = (_lib.BatchClass * n.value)()
+        check(lib().hdg_batch_get_classes(self.h, n.value, arr, C.byref(n)), self.ctx.h,
+              "classes")
+        return [{"ni": c.ni, "nb": c.nb, "physics": c.physics, "dim": c.dim,
+                 "nmesh": c.nmesh, "ncells": c.ncells,
+                 "family": _lib.FAMILY_NAMES.get(c.family, str(c.family)),
+                 "grid_per_cu": c.grid_per_cu, "tuned_ms": c.tuned_ms} for c in arr]
+
+    def launch(self, precision=FP64):
+        """Enqueue one graph replay (no host synchronisation)."""
+        check(lib().hdg_batch_launch(self.h, int(precision)), self.ctx.h, "batch_launch")
+
+    def assemble(self, precision=FP64):
+        nbad = C.c_int64(0)
+        check(lib().hdg_batch_condense_assemble(self.h, int(precision), C.byref(nbad)),
+              self.ctx.h, "batch_condense_assemble")
+
+    def recover(self, lams, outs):
+        """Device tensors: lams[m] (nfree of member m), outs[m] (nfree_total)."""
+        n = len(self.ops)
+        la = (C.c_void_p * n)(*[t.data_ptr() if t.numel() else None for t in lams])
+        oa = (C.c_void_p * n)(*[t.data_ptr() for t in outs])
+        check(lib().hdg_batch_recover(self.h, la, oa), self.ctx.h, "batch_recover")
+
+    def close(self):
+        if getattr(self, "h", None) and self.ctx.h:
+            lib().hdg_batch_destroy(self.h)
+        self.h = None
+        for op in getattr(self, "ops", []):
+            op._batch = None
 
     def __del__(self):
         try:

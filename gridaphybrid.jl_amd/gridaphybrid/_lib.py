@@ -42,8 +42,12 @@ EXPORTS = [
     "hdg_mesh_set_dirichlet_mms", "hdg_mesh_set_source", "hdg_condense_assemble",
     "hdg_skeleton_system", "hdg_cell_blocks", "hdg_recover", "hdg_cell_systems",
     "hdg_condense", "hdg_assemble", "hdg_mesh_singular_count", "hdg_mesh_interpolate_trace",
-    "hdg_local_factors",
+    "hdg_local_factors", "hdg_batch_create", "hdg_batch_destroy", "hdg_batch_get_classes",
+    "hdg_batch_launch", "hdg_batch_condense_assemble", "hdg_batch_recover",
 ]
+
+BATCH_AUTOTUNE = 1
+FAMILY_NAMES = {0: "wave", 1: "grid8x8", 2: "colwave", 3: "grid32x16", 4: "blocked"}
 
 
 class HDGError(RuntimeError):
@@ -85,6 +89,14 @@ class MeshInfo(C.Structure):
         ("n_trace_facet", C.c_int), ("nq_cell", C.c_int), ("ncells", C.c_int64),
         ("nfacets", C.c_int64), ("nfree", C.c_int64), ("ndir", C.c_int64), ("nnz", C.c_int64),
         ("nfree_total", C.c_int64),
+    ]
+
+
+class BatchClass(C.Structure):
+    _fields_ = [
+        ("ni", C.c_int), ("nb", C.c_int), ("physics", C.c_int), ("dim", C.c_int),
+        ("nmesh", C.c_int), ("ncells", C.c_int64), ("family", C.c_int),
+        ("grid_per_cu", C.c_int), ("tuned_ms", C.c_double),
     ]
 
 
@@ -130,6 +142,12 @@ def lib():
         L.hdg_mesh_singular_count.argtypes = [vp, C.POINTER(i64)]
         L.hdg_mesh_interpolate_trace.argtypes = [vp, vp]
         L.hdg_local_factors.argtypes = [vp, C.POINTER(vp), C.POINTER(vp)]
+        L.hdg_batch_create.argtypes = [vp, i32, C.POINTER(vp), i32, C.POINTER(vp)]
+        L.hdg_batch_destroy.argtypes = [vp]
+        L.hdg_batch_get_classes.argtypes = [vp, i32, C.POINTER(BatchClass), C.POINTER(i32)]
+        L.hdg_batch_launch.argtypes = [vp, i32]
+        L.hdg_batch_condense_assemble.argtypes = [vp, i32, C.POINTER(i64)]
+        L.hdg_batch_recover.argtypes = [vp, C.POINTER(vp), C.POINTER(vp)]
         _ = dbl
         _lib = L
     return _lib

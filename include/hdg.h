@@ -52,6 +52,7 @@ typedef enum {
 
 typedef struct hdg_ctx hdg_ctx;
 typedef struct hdg_mesh hdg_mesh;
+typedef struct hdg_batch hdg_batch;
 
 /* ------------------------------------------------------------------------ */
 /* Context                                                                  */
@@ -230,6 +231,57 @@ int hdg_recover(hdg_mesh* mesh, const double* lam_free, double* free_values);
  * device [ncells][n][n] / [ncells][n] for cells [first, first+count): a
  * debugging/parity entry point; not on the hot path. */
 int hdg_cell_systems(hdg_mesh* mesh, int64_t first, int64_t count, double* A, double* b);
+
+/* ---------------------------------------------------------------------------
+ * Batches of independent meshes.  The reference builds one
+ * HybridAffineFEOperator per mesh (src/HybridAffineFEOperators.jl:10-50) and
+ * condenses its cells one by one; a batch does the same work for many meshes
+ * at once.  Meshes whose cell systems share reference tables (same physics,
+ * dimension, orders and quadrature) form a class.  The per-cell state of each
+ * class is one contiguous allocation, so each class is condensed by ONE kernel
+ * launch over the cells of all its meshes.  The whole sequence -- per-mesh cell
+ * preparation, one condensation per class, per-mesh Dirichlet lifting and
+ * skeleton assembly -- is captured once into a HIP graph and replayed by every
+ * hdg_batch_launch.  Each mesh's per-cell state becomes a slice of its class
+ * buffers, so the per-mesh calls (hdg_recover, hdg_cell_blocks,
+ * hdg_skeleton_system, ...) keep working on it.
+ *
+ * A mesh belongs to at most one batch and cannot be destroyed before it.  The
+ * graph records the device pointers in use at creation; hdg_mesh_set_source
+ * on a member makes the next launch capture a new graph. */
+enum { HDG_BATCH_AUTOTUNE = 1 };
+
+typedef struct {
+  int ni, nb;          /* local system sizes of the class */
+  int physics, dim;
+  int nmesh;           /* member meshes */
+  int64_t ncells;      /* cells over all members */
+  int family;          /* condensation kernel family in use (see below) */
+  int grid_per_cu;     /* resident workgroups per CU (0 = family default) */
+  double tuned_ms;     /* measured condensation time of the pick (0 if untuned) */
+} hdg_batch_class;
+/* Kernel families: 0 one wavefront per cell (fixed sizes), 1 8x8 grid with
+ * register tiles, 2 column-per-wavefront with look-ahead, 3 32x16 grid,
+ * 4 blocked HBM-workspace elimination. */
+
+/* flags: HDG_BATCH_AUTOTUNE times every kernel family and grid size able to
+ * condense each class on the batch's own cells and keeps the fastest;
+ * otherwise each class uses the default family for its size. */
+int hdg_batch_create(hdg_ctx* ctx, int nmesh, hdg_mesh* const* meshes, int flags,
+                     hdg_batch** out);
+int hdg_batch_destroy(hdg_batch* batch);
+int hdg_batch_get_classes(const hdg_batch* batch, int max, hdg_batch_class* classes,
+                          int* nclass);
+/* Enqueues one replay of the batch graph (condensation + assembly of every
+ * member) on the context stream, without host synchronisation. */
+int hdg_batch_launch(hdg_batch* batch, int precision);
+/* hdg_batch_launch, then synchronises; *n_singular (may be NULL) receives the
+ * number of cells over all members with a zero pivot. */
+int hdg_batch_condense_assemble(hdg_batch* batch, int precision, int64_t* n_singular);
+/* hdg_recover for every member: lam_free[m] / free_values[m] are device
+ * pointers for member m (host arrays of nmesh pointers). */
+int hdg_batch_recover(hdg_batch* batch, const double* const* lam_free,
+                      double* const* free_values);
 
 #ifdef __cplusplus
 }
