@@ -92,13 +92,18 @@ struct hdg_mesh {
   std::vector<int8_t> facet_dir;
   // device
   DevBuf tables, d_x0, d_jac, d_sign, d_coef, d_cell_dofs, d_lam_dir;
-  DevBuf d_S, d_g, d_U, d_Y, d_info, d_alpha, d_srcq;
+  DevBuf d_S, d_g, d_U, d_Y, d_info, d_alpha, d_srcq, d_iters;
   // per-cell device state used by the kernels: the mesh's own buffers above,
   // or slices of a batch's per-class buffers (hdg_batch_create)
   double *pS = nullptr, *pg = nullptr, *pU = nullptr, *pY = nullptr;
   double *palpha = nullptr, *psrcq = nullptr;
   int32_t* pinfo = nullptr;
+  int32_t* piters = nullptr;  // refinement sweeps per cell (mixed precision)
   hdg_batch* batch = nullptr;
+  // the resident factors are W = A_ii^-1 [A_ib b_i] (mixed precision) rather
+  // than getrf's U and L^-1 P [A_ib b_i]
+  bool mixed = false;
+  bool fp64_fallback = false;
   DevBuf d_fptr, d_pptr, d_pfacet, d_pcell, d_plpos, d_gcell, d_glpos;
   DevBuf d_colptr, d_rowval, d_nzval, d_rhs;
   const double* src_qp = nullptr;
@@ -162,6 +167,7 @@ hipError_t own_cell_state(hdg_mesh* m) {
   if ((e = m->d_alpha.alloc(nc * L.alpha * sizeof(double))) != hipSuccess) return e;
   if ((e = m->d_srcq.alloc(nc * L.srcq * sizeof(double))) != hipSuccess) return e;
   if ((e = m->d_info.alloc(nc * sizeof(int32_t))) != hipSuccess) return e;
+  if ((e = m->d_iters.alloc(nc * sizeof(int32_t))) != hipSuccess) return e;
   m->pS = m->d_S.as<double>();
   m->pg = m->d_g.as<double>();
   m->pU = m->d_U.as<double>();
@@ -169,15 +175,17 @@ hipError_t own_cell_state(hdg_mesh* m) {
   m->palpha = m->d_alpha.as<double>();
   m->psrcq = m->d_srcq.as<double>();
   m->pinfo = m->d_info.as<int32_t>();
+  m->piters = m->d_iters.as<int32_t>();
   return hipSuccess;
 }
 
 // Frees the mesh's own per-cell state (the mesh is rebound to batch slices).
 void drop_cell_state(hdg_mesh* m) {
-  for (DevBuf* b : {&m->d_S, &m->d_g, &m->d_U, &m->d_Y, &m->d_alpha, &m->d_srcq, &m->d_info})
+  for (DevBuf* b :
+       {&m->d_S, &m->d_g, &m->d_U, &m->d_Y, &m->d_alpha, &m->d_srcq, &m->d_info, &m->d_iters})
     (void)b->alloc(0);
   m->pS = m->pg = m->pU = m->pY = m->palpha = m->psrcq = nullptr;
-  m->pinfo = nullptr;
+  m->pinfo = m->piters = nullptr;
 }
 
 }  // namespace
@@ -282,6 +290,16 @@ int upload_perm(hdg_ctx* ctx, const std::vector<int32_t>& perm) {
   HIP_TRY(ctx, hipMemcpyAsync(ctx->d_perm, perm.data(), perm.size() * sizeof(int32_t),
                               hipMemcpyHostToDevice, ctx->stream),
           "upload perm");
+  return HDG_OK;
+}
+
+int count_unconverged(hdg_ctx* ctx, const int32_t* iters, int64_t n, int64_t* out) {
+  HIP_TRY(ctx, launch_count_negative(iters, n, ctx->d_count, ctx->stream), "count kernel");
+  int64_t h = 0;
+  HIP_TRY(ctx, hipMemcpyAsync(&h, ctx->d_count, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream),
+          "count copy");
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream), "count sync");
+  *out = h;
   return HDG_OK;
 }
 
@@ -791,7 +809,6 @@ int hdg_condense(hdg_mesh* m, int precision) {
   hdg_ctx* ctx = m->ctx;
   if (precision != HDG_FP64 && precision != HDG_MIXED)
     return fail(ctx, HDG_ERR_ARG, "unknown precision");
-  if (precision == HDG_MIXED) return fail(ctx, HDG_ERR_UNSUPPORTED, "mixed precision not built yet");
   if (m->prob.source == HDG_SOURCE_QP && !m->src_qp)
     return fail(ctx, HDG_ERR_STATE, "source values at quadrature points not set");
   HIP_TRY(ctx, hipSetDevice(ctx->device), "hipSetDevice");
@@ -799,11 +816,20 @@ int hdg_condense(hdg_mesh* m, int precision) {
   CondenseOut out;
   out.S = m->pS;
   out.g = m->pg;
-  out.U = m->pU;
   out.Y = m->pY;
   out.info = m->pinfo;
-  HIP_TRY(ctx, launch_condense_fused(m->tdev, m->meshdev(), out, ctx->stream, ctx->ws),
-          "condense kernel");
+  m->fp64_fallback = false;
+  if (precision == HDG_MIXED && mixed_supported(m->th.ni, m->th.nb)) {
+    HIP_TRY(ctx, launch_condense_mixed(m->tdev, m->meshdev(), out, ctx->stream, ctx->ws, true,
+                                       kMaxRefinementSweeps, m->piters),
+            "mixed condense kernels");
+    m->mixed = true;
+  } else {
+    out.U = m->pU;
+    HIP_TRY(ctx, launch_condense_fused(m->tdev, m->meshdev(), out, ctx->stream, ctx->ws),
+            "condense kernel");
+    m->mixed = false;
+  }
   m->condensed = true;
   return HDG_OK;
 }
@@ -830,10 +856,42 @@ int hdg_condense_assemble(hdg_mesh* m, int precision, int64_t* n_singular) {
   int rc = hdg_condense(m, precision);
   if (rc) return rc;
   if ((rc = hdg_assemble(m))) return rc;
+  if (m->mixed) {
+    // cells whose refinement did not converge: redo the mesh in FP64
+    int64_t bad = 0;
+    if ((rc = count_unconverged(m->ctx, m->piters, m->ncells, &bad))) return rc;
+    if (bad) {
+      if ((rc = hdg_condense(m, HDG_FP64)) || (rc = hdg_assemble(m))) return rc;
+      m->fp64_fallback = true;
+    }
+  }
   int64_t bad = 0;
   if ((rc = hdg_mesh_singular_count(m, &bad))) return rc;
   if (n_singular) *n_singular = bad;
   if (bad) return fail(m->ctx, HDG_SINGULAR, std::to_string(bad) + " cell(s) with singular A_ii");
+  return HDG_OK;
+}
+
+int hdg_mesh_refinement_stats(hdg_mesh* m, hdg_refinement_stats* st) {
+  if (!m || !st) return HDG_ERR_ARG;
+  hdg_ctx* ctx = m->ctx;
+  std::memset(st, 0, sizeof(*st));
+  st->mixed = m->mixed ? 1 : 0;
+  st->fp64_fallback = m->fp64_fallback ? 1 : 0;
+  if (!m->mixed || !m->condensed || m->ncells == 0) return HDG_OK;
+  std::vector<int32_t> it((size_t)m->ncells);
+  HIP_TRY(ctx, hipSetDevice(ctx->device), "hipSetDevice");
+  HIP_TRY(ctx, hipMemcpyAsync(it.data(), m->piters, it.size() * sizeof(int32_t),
+                              hipMemcpyDeviceToHost, ctx->stream), "d2h sweeps");
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream), "sync");
+  int64_t sum = 0;
+  for (int32_t v : it) {
+    const int sweeps = v >= 0 ? v : -v - 1;
+    if (v < 0) st->unconverged++;
+    st->max_sweeps = std::max(st->max_sweeps, sweeps);
+    sum += sweeps;
+  }
+  st->mean_sweeps = (double)sum / (double)it.size();
   return HDG_OK;
 }
 
@@ -857,7 +915,7 @@ int hdg_cell_blocks(hdg_mesh* m, const double** S, const double** g, const int32
 
 int hdg_local_factors(hdg_mesh* m, const double** U, const double** Y) {
   if (!m) return HDG_ERR_ARG;
-  if (U) *U = m->pU;
+  if (U) *U = m->mixed ? nullptr : m->pU;
   if (Y) *Y = m->pY;
   return m->condensed ? HDG_OK : fail(m->ctx, HDG_ERR_STATE, "no factors before condensation");
 }
@@ -868,7 +926,7 @@ int hdg_recover(hdg_mesh* m, const double* lam_free, double* free_values) {
   if (!m->condensed) return fail(ctx, HDG_ERR_STATE, "recover called before condense");
   if ((m->nfree > 0 && !lam_free) || !free_values) return fail(ctx, HDG_ERR_ARG, "null buffer");
   HIP_TRY(ctx, hipSetDevice(ctx->device), "hipSetDevice");
-  HIP_TRY(ctx, launch_recover(m->pU, m->pY,
+  HIP_TRY(ctx, launch_recover(m->mixed ? nullptr : m->pU, m->pY,
                               m->d_cell_dofs.as<int32_t>(), lam_free,
                               m->d_lam_dir.as<double>(), free_values, m->ncells, m->th.ni,
                               m->th.nb, m->th.na, m->nfree, ctx->stream),
@@ -901,7 +959,8 @@ struct hdg_batch {
     hdg_batch_class info{};
     std::vector<int> members;
     const hdg_mesh* lead = nullptr;  // reference tables shared by the members
-    DevBuf S, g, U, Y, alpha, srcq, infos;
+    DevBuf S, g, U, Y, alpha, srcq, infos, iters;
+    bool mixed = false;  // condensed in mixed precision at the last launch
 
     MeshDev cells(int64_t count) const {
       MeshDev M;
@@ -927,6 +986,7 @@ struct hdg_batch {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   std::vector<const double*> captured_src;  // member sources baked into the graph
+  int captured_precision = -1;
 
   void drop_graph() {
     if (exec) (void)hipGraphExecDestroy(exec);
@@ -964,11 +1024,22 @@ hipError_t enqueue_condense(hdg_batch* b, const hdg_batch::Class& c, int64_t cou
                                gridmul);
 }
 
-hipError_t enqueue_all(hdg_batch* b, hipStream_t st) {
+bool class_mixed(const hdg_batch::Class& c, int precision) {
+  return precision == HDG_MIXED && mixed_supported(c.info.ni, c.info.nb);
+}
+
+hipError_t enqueue_all(hdg_batch* b, int precision, hipStream_t st) {
   hipError_t e = enqueue_prep(b, st);
   for (size_t k = 0; e == hipSuccess && k < b->classes.size(); ++k) {
     const hdg_batch::Class& c = *b->classes[k];
-    e = enqueue_condense(b, c, c.info.ncells, c.info.family, c.info.grid_per_cu, st);
+    if (class_mixed(c, precision)) {
+      CondenseOut o = c.out();
+      o.U = nullptr;
+      e = launch_condense_mixed(c.lead->tdev, c.cells(c.info.ncells), o, st, b->ws, false,
+                                kMaxRefinementSweeps, c.iters.as<int32_t>());
+    } else {
+      e = enqueue_condense(b, c, c.info.ncells, c.info.family, c.info.grid_per_cu, st);
+    }
   }
   for (size_t k = 0; e == hipSuccess && k < b->meshes.size(); ++k) {
     hdg_mesh* m = b->meshes[k];
@@ -1037,16 +1108,18 @@ int autotune(hdg_batch* b) {
   return rc;
 }
 
-int capture(hdg_batch* b) {
+int capture(hdg_batch* b, int precision) {
   hdg_ctx* ctx = b->ctx;
   hipStream_t st = ctx->stream;
   b->drop_graph();
   size_t need = 0;
   for (auto& c : b->classes)
-    need = std::max(need, condense_workspace_bytes(c->info.ni, c->info.nb, c->info.family));
+    need = std::max(need, class_mixed(*c, precision)
+                              ? mixed_workspace_bytes(c->info.ni, c->info.nb, c->info.ncells)
+                              : condense_workspace_bytes(c->info.ni, c->info.nb, c->info.family));
   HIP_TRY(ctx, workspace_reserve(b->ws, need), "batch workspace");
   HIP_TRY(ctx, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "begin capture");
-  hipError_t e = enqueue_all(b, st);
+  hipError_t e = enqueue_all(b, precision, st);
   hipGraph_t g = nullptr;
   hipError_t e2 = hipStreamEndCapture(st, &g);
   if (e != hipSuccess || e2 != hipSuccess) {
@@ -1057,6 +1130,7 @@ int capture(hdg_batch* b) {
   HIP_TRY(ctx, hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0), "graph instantiate");
   b->captured_src.clear();
   for (hdg_mesh* m : b->meshes) b->captured_src.push_back(m->src_qp);
+  b->captured_precision = precision;
   return HDG_OK;
 }
 
@@ -1113,6 +1187,7 @@ int hdg_batch_create(hdg_ctx* ctx, int nmesh, hdg_mesh* const* meshes, int flags
     HIP_TRY(ctx, c.alpha.alloc(nc * L.alpha * sizeof(double)), "alloc batch alpha");
     HIP_TRY(ctx, c.srcq.alloc(nc * L.srcq * sizeof(double)), "alloc batch srcq");
     HIP_TRY(ctx, c.infos.alloc(nc * sizeof(int32_t)), "alloc batch info");
+    HIP_TRY(ctx, c.iters.alloc(nc * sizeof(int32_t)), "alloc batch sweeps");
   }
   // rebind the members to their slices (nothing fails from here on)
   for (auto& cp : b->classes) {
@@ -1129,6 +1204,7 @@ int hdg_batch_create(hdg_ctx* ctx, int nmesh, hdg_mesh* const* meshes, int flags
       m->palpha = c.alpha.as<double>() + off * L.alpha;
       m->psrcq = c.srcq.as<double>() + off * L.srcq;
       m->pinfo = c.infos.as<int32_t>() + off;
+      m->piters = c.iters.as<int32_t>() + off;
       m->batch = b.get();
       m->condensed = false;
       off += (size_t)m->ncells;
@@ -1167,7 +1243,8 @@ int hdg_batch_destroy(hdg_batch* b) {
     m->batch = nullptr;
     m->condensed = false;
     m->pS = m->pg = m->pU = m->pY = m->palpha = m->psrcq = nullptr;
-    m->pinfo = nullptr;
+    m->pinfo = m->piters = nullptr;
+    m->mixed = false;
     hipError_t e = own_cell_state(m);
     if (e != hipSuccess && rc == HDG_OK) rc = hipfail(ctx, e, "alloc per-cell state");
   }
@@ -1187,24 +1264,41 @@ int hdg_batch_launch(hdg_batch* b, int precision) {
   hdg_ctx* ctx = b->ctx;
   if (precision != HDG_FP64 && precision != HDG_MIXED)
     return fail(ctx, HDG_ERR_ARG, "unknown precision");
-  if (precision == HDG_MIXED) return fail(ctx, HDG_ERR_UNSUPPORTED, "mixed precision not built yet");
   if (!sources_ready(b)) return fail(ctx, HDG_ERR_STATE, "source values at quadrature points not set");
   HIP_TRY(ctx, hipSetDevice(ctx->device), "hipSetDevice");
-  bool stale = !b->exec;
+  bool stale = !b->exec || b->captured_precision != precision;
   for (size_t k = 0; !stale && k < b->meshes.size(); ++k)
     stale = b->captured_src[k] != b->meshes[k]->src_qp;
   if (stale) {
-    int rc = capture(b);
+    int rc = capture(b, precision);
     if (rc) return rc;
   }
   HIP_TRY(ctx, hipGraphLaunch(b->exec, ctx->stream), "graph launch");
-  for (hdg_mesh* m : b->meshes) m->condensed = true;
+  for (auto& c : b->classes) {
+    c->mixed = class_mixed(*c, precision);
+    for (int i : c->members) {
+      hdg_mesh* m = b->meshes[i];
+      m->condensed = true;
+      m->mixed = c->mixed;
+      m->fp64_fallback = false;
+    }
+  }
   return HDG_OK;
 }
 
 int hdg_batch_condense_assemble(hdg_batch* b, int precision, int64_t* n_singular) {
   int rc = hdg_batch_launch(b, precision);
   if (rc) return rc;
+  // members with cells whose refinement did not converge are redone in FP64
+  for (hdg_mesh* m : b->meshes) {
+    if (!m->mixed) continue;
+    int64_t k = 0;
+    if ((rc = count_unconverged(b->ctx, m->piters, m->ncells, &k))) return rc;
+    if (k) {
+      if ((rc = hdg_condense(m, HDG_FP64)) || (rc = hdg_assemble(m))) return rc;
+      m->fp64_fallback = true;
+    }
+  }
   int64_t bad = 0;
   for (auto& c : b->classes) {
     int64_t k = 0;

@@ -140,6 +140,16 @@ hipError_t launch_condense_fused(const Tables& T, const MeshDev& M, const Conden
                                  hipStream_t st, Workspace& ws, bool prep = true, int fam = -1,
                                  int gridmul = 0);
 hipError_t launch_prep_cells(const Tables& T, const MeshDev& M, hipStream_t st);
+// Mixed precision (FP32 Gauss-Jordan inverse of A_ii, FP64 refinement of
+// W = A_ii^-1 [A_ib b_i], FP64 S and g).  out.Y receives W (natural interior
+// order, [nb+1][ni] per cell); out.U is not written.  iters (may be null) gets
+// the refinement sweeps per cell, negative when not converged.
+constexpr int kMaxRefinementSweeps = 10;
+bool mixed_supported(int ni, int nb);
+size_t mixed_workspace_bytes(int ni, int nb, int64_t ncells);
+hipError_t launch_condense_mixed(const Tables& T, const MeshDev& M, const CondenseOut& out,
+                                 hipStream_t st, Workspace& ws, bool prep, int maxit,
+                                 int32_t* iters);
 // kernel families able to condense an (ni, nb) system; returns their count
 int condense_variants(int ni, int nb, int* fams, int max);
 int condense_default_family(int ni, int nb);
@@ -161,6 +171,8 @@ hipError_t launch_cell_systems(const Tables& T, const MeshDev& M, int64_t first,
                                double* A, double* b, hipStream_t st);
 hipError_t launch_count_nonzero(const int32_t* info, int64_t n, int64_t* count_dev,
                                 hipStream_t st);
+hipError_t launch_count_negative(const int32_t* v, int64_t n, int64_t* count_dev,
+                                 hipStream_t st);
 
 // Host helpers (hdg_fe.cpp)
 int build_tables(int phys, int D, int space, int kp, int kq, int km, int quad_degree,

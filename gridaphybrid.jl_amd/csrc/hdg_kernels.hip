@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
 #include <climits>
 
 #include "hdg_internal.h"
@@ -1281,9 +1282,9 @@ __global__ void __launch_bounds__(256)
       }
       r[t] = s;
     }
-    const double* Uc = U + cell * usz;
+    const double* Uc = U ? U + cell * usz : nullptr;
 #pragma unroll 1
-    for (int c = ni - 1; c >= 0; --c) {
+    for (int c = U ? ni - 1 : -1; c >= 0; --c) {
       const int64_t col0 = (int64_t)c * (c + 1) / 2;
       const double ucc = Uc[col0 + c];
       const int tsel = c >> 6, lsel = c & 63;
@@ -1415,6 +1416,278 @@ __global__ void cell_systems_kernel(FusedGen<PHYS> gen, int64_t first, int64_t c
       else
         b[lc * n + row] = v;
     }
+    __syncthreads();
+  }
+}
+
+// --------------------------------------------------------------------------
+// Mixed precision.  W = A_ii^-1 [A_ib b_i] is computed from an FP32
+// Gauss-Jordan inverse X of A_ii and refined in FP64:
+//   W_0 = X B,   R_t = B - A_ii W_t (FP64),   W_{t+1} = W_t + X R_t (FP32
+// products, FP64 update), until the correction reaches FP64 rounding or
+// stops shrinking; then S = A_bb - A_bi W and g = b_b - A_bi w in FP64.
+// W (natural interior order) is kept for recovery: x_i = w - W_b lambda.
+// --------------------------------------------------------------------------
+constexpr int kMixedMaxNi = 128;
+__host__ __device__ inline int cdiv(int a, int b) { return (a + b - 1) / b; }
+constexpr int64_t kMixedChunkCells = 16384;  // cells per gen / inverse / refine round
+
+// Cell systems in FP64, [cell][col][row] with n rows and n+1 columns (the last
+// column is b), for cells [first, first+count).
+template <int PHYS>
+__global__ void __launch_bounds__(256)
+    gen_system_kernel(FusedGen<PHYS> gen, int64_t first, int64_t count, double* __restrict__ A64) {
+  __shared__ FusedShared sh;
+  const int n = gen.T.n;
+  const int per = n * (n + 1);
+  for (int64_t lc = blockIdx.x; lc < count; lc += gridDim.x) {
+    gen.begin(first + lc, sh);
+    __syncthreads();
+    double* A = A64 + lc * (int64_t)per;
+    for (int t = threadIdx.x; t < per; t += blockDim.x) {
+      const int col = t / n, row = t - col * n;
+      A[t] = gen.entry(sh, row, col);
+    }
+    __syncthreads();
+  }
+}
+
+// FP32 Gauss-Jordan inverse of A_ii with partial pivoting (largest magnitude,
+// lowest row on ties; rows are not moved).  A 16x16 thread grid holds the
+// matrix in registers: thread (tx, ty) owns rows tx + 16a and columns
+// ty + 16b.  Per step, the pivot column and the pivot row go through LDS.
+// Output X [cell][ni][ni] row-major with X = A_ii^-1; info[cell] = first zero
+// pivot step + 1 (0 if none).
+template <int RA>
+__global__ void __launch_bounds__(256)
+    gj_inverse_kernel(const double* __restrict__ A64, int n, int ni, int64_t count,
+                      float* __restrict__ X, int32_t* __restrict__ info) {
+  __shared__ float s_col[2][kMixedMaxNi];
+  __shared__ float s_row[kMixedMaxNi];
+  __shared__ int s_piv[kMixedMaxNi], s_inv[kMixedMaxNi];
+  __shared__ unsigned char s_used[kMixedMaxNi];
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4, lane = threadIdx.x & 63;
+  for (int64_t lc = blockIdx.x; lc < count; lc += gridDim.x) {
+    const double* A = A64 + lc * (int64_t)n * (n + 1);
+    float m[RA][RA];
+#pragma unroll
+    for (int a = 0; a < RA; ++a)
+#pragma unroll
+      for (int b = 0; b < RA; ++b) {
+        const int i = tx + 16 * a, j = ty + 16 * b;
+        m[a][b] = (i < ni && j < ni) ? (float)A[(int64_t)j * n + i] : 0.f;
+      }
+    if (threadIdx.x < kMixedMaxNi) s_used[threadIdx.x] = 0;
+    if (ty == 0) {
+#pragma unroll
+      for (int a = 0; a < RA; ++a)
+        if (tx + 16 * a < ni) s_col[0][tx + 16 * a] = m[a][0];
+    }
+    __syncthreads();
+    int bad = 0;
+    for (int k = 0; k < ni; ++k) {
+      const float* col = s_col[k & 1];
+      // pivot search, repeated by every wavefront (no broadcast needed)
+      int64_t best = -1;
+      for (int i = lane; i < ni; i += 64) {
+        if (s_used[i]) continue;
+        const int64_t key =
+            ((int64_t)__float_as_uint(fabsf(col[i])) << 32) | (int64_t)(0xFFFF - i);
+        best = key > best ? key : best;
+      }
+      best = wave_max_i64(best);
+      const int p = 0xFFFF - (int)(best & 0xFFFF);
+      if ((best >> 32) == 0 && !bad) bad = k + 1;
+      if (tx == (p & 15)) {
+        const int ap = p >> 4;
+#pragma unroll
+        for (int b = 0; b < RA; ++b) {
+          float v = 0.f;
+#pragma unroll
+          for (int a = 0; a < RA; ++a)
+            if (a == ap) v = m[a][b];
+          if (ty + 16 * b < ni) s_row[ty + 16 * b] = v;
+        }
+      }
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        s_piv[k] = p;
+        s_inv[p] = k;
+        s_used[p] = 1;
+      }
+      const float pv = s_row[k];
+      const float inv = pv != 0.f ? 1.f / pv : 0.f;
+      float c[RA], r[RA];
+#pragma unroll
+      for (int a = 0; a < RA; ++a) c[a] = tx + 16 * a < ni ? col[tx + 16 * a] : 0.f;
+#pragma unroll
+      for (int b = 0; b < RA; ++b) r[b] = ty + 16 * b < ni ? s_row[ty + 16 * b] * inv : 0.f;
+#pragma unroll
+      for (int a = 0; a < RA; ++a) {
+        const bool prow = tx + 16 * a == p;
+#pragma unroll
+        for (int b = 0; b < RA; ++b) {
+          const bool pcol = ty + 16 * b == k;
+          // other rows: m_ij - m_ik m_pj / m_pk, column k: -m_ik / m_pk;
+          // pivot row: m_pj / m_pk, pivot: 1 / m_pk
+          float v = pcol ? -c[a] * inv : fmaf(-c[a], r[b], m[a][b]);
+          if (prow) v = pcol ? inv : r[b];
+          m[a][b] = v;
+        }
+      }
+      // publish column k+1 for the next step
+      if (k + 1 < ni && ty == ((k + 1) & 15)) {
+        const int bk = (k + 1) >> 4;
+#pragma unroll
+        for (int a = 0; a < RA; ++a) {
+          float v = 0.f;
+#pragma unroll
+          for (int b = 0; b < RA; ++b)
+            if (b == bk) v = m[a][b];
+          if (tx + 16 * a < ni) s_col[(k + 1) & 1][tx + 16 * a] = v;
+        }
+      }
+      __syncthreads();
+    }
+    // m[i][c] holds (A^-1)[inv(i)][piv(c)]
+    float* Xc = X + lc * (int64_t)ni * ni;
+#pragma unroll
+    for (int a = 0; a < RA; ++a)
+#pragma unroll
+      for (int b = 0; b < RA; ++b) {
+        const int i = tx + 16 * a, c = ty + 16 * b;
+        if (i < ni && c < ni) Xc[(int64_t)s_inv[i] * ni + s_piv[c]] = m[a][b];
+      }
+    if (threadIdx.x == 0 && info) info[lc] = bad;
+    __syncthreads();
+  }
+}
+
+// FP64 refinement of W and the FP64 Schur complement.  Thread t works on
+// interior row r = t % ni and the contiguous columns [g JC, (g+1) JC) of W,
+// g = t / ni.  The working copy of W is row-major with ld = ng JC >= nb+1
+// (padding columns stay zero), so the inner loops run without guards.  X and
+// the FP32 residual live in LDS; A and W in the workspace.  The final W goes
+// to its output slot ([nb+1][ni] per cell).  iters[cell] (may be null) gets
+// the refinement sweeps used, negated when the last correction was still
+// above 1e-7 of W (not converged).
+template <int JC>
+__global__ void __launch_bounds__(256)
+    mixed_refine_kernel(const double* __restrict__ A64, const float* __restrict__ X,
+                        double* __restrict__ Wws, int ni, int nb, int64_t count,
+                        CondenseOut out, int maxit, int32_t* __restrict__ iters) {
+  extern __shared__ float s_dyn[];
+  const int n = ni + nb, ncol = nb + 1, ldx = ni + 1;
+  const int ng = cdiv(ncol, JC), ldr = ng * JC;
+  float* sX = s_dyn;             // [ni][ni+1]
+  float* sR = s_dyn + ni * ldx;  // [ni][ldr]
+  __shared__ double s_red[2][4];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int r = tid % ni, g = tid / ni, j0 = g * JC;
+  const bool act = g < ng;
+  for (int64_t lc = blockIdx.x; lc < count; lc += gridDim.x) {
+    const double* A = A64 + lc * (int64_t)n * (n + 1);
+    const float* Xc = X + lc * (int64_t)ni * ni;
+    double* W = Wws + lc * (int64_t)ni * ldr;  // W[k][j] at k*ldr + j
+    for (int t = tid; t < ni * ni; t += 256) {
+      const int rr = t / ni, cc = t - rr * ni;
+      sX[rr * ldx + cc] = Xc[t];
+    }
+    double acc[JC];
+    // B = [A_ib b_i] (zero in the padding columns); R_0 = B
+    if (act) {
+#pragma unroll
+      for (int jj = 0; jj < JC; ++jj) {
+        const int j = j0 + jj;
+        sR[r * ldr + j] = j < ncol ? (float)A[(int64_t)(ni + j) * n + r] : 0.f;
+      }
+    }
+    __syncthreads();
+    double dprev = 0.0;
+    int it = 0, conv = 1;
+    for (;;) {
+      // W += X R   (W_0 = X B)
+      float d[JC];
+#pragma unroll
+      for (int jj = 0; jj < JC; ++jj) d[jj] = 0.f;
+      double dmax = 0.0, wmax = 0.0;
+      if (act) {
+        const float* xr = sX + r * ldx;
+        const float* rr = sR + j0;
+        for (int k = 0; k < ni; ++k) {
+          const float xv = xr[k];
+#pragma unroll
+          for (int jj = 0; jj < JC; ++jj) d[jj] = fmaf(xv, rr[k * ldr + jj], d[jj]);
+        }
+        double* wr = W + (int64_t)r * ldr + j0;
+#pragma unroll
+        for (int jj = 0; jj < JC; ++jj) {
+          const double w = (it == 0 ? 0.0 : wr[jj]) + (double)d[jj];
+          wr[jj] = w;
+          dmax = fmax(dmax, fabs((double)d[jj]));
+          wmax = fmax(wmax, fabs(w));
+        }
+      }
+      // workgroup max of |correction| and |W|
+      dmax = __builtin_bit_cast(double, wave_max_i64(__builtin_bit_cast(int64_t, dmax)));
+      wmax = __builtin_bit_cast(double, wave_max_i64(__builtin_bit_cast(int64_t, wmax)));
+      if (lane == 0) {
+        s_red[0][wv] = dmax;
+        s_red[1][wv] = wmax;
+      }
+      __syncthreads();  // also publishes W for the residual
+      dmax = fmax(fmax(s_red[0][0], s_red[0][1]), fmax(s_red[0][2], s_red[0][3]));
+      wmax = fmax(fmax(s_red[1][0], s_red[1][1]), fmax(s_red[1][2], s_red[1][3]));
+      const bool done = it >= maxit || dmax <= 0x1p-50 * wmax ||
+                        (it >= 2 && dmax > 0.5 * dprev);
+      if (done) {
+        conv = dmax <= 1e-7 * wmax;
+        break;
+      }
+      dprev = dmax;
+      ++it;
+      // R = B - A_ii W  (FP64)
+      if (act) {
+#pragma unroll
+        for (int jj = 0; jj < JC; ++jj) {
+          const int j = j0 + jj;
+          acc[jj] = j < ncol ? A[(int64_t)(ni + j) * n + r] : 0.0;
+        }
+        const double* wk = W + j0;
+        for (int k = 0; k < ni; ++k) {
+          const double a = A[(int64_t)k * n + r];
+#pragma unroll
+          for (int jj = 0; jj < JC; ++jj) acc[jj] = fma(-a, wk[(int64_t)k * ldr + jj], acc[jj]);
+        }
+      }
+      __syncthreads();  // everyone is done with sR before it is overwritten
+      if (act) {
+#pragma unroll
+        for (int jj = 0; jj < JC; ++jj) sR[r * ldr + j0 + jj] = (float)acc[jj];
+      }
+      __syncthreads();
+    }
+    // S = A_bb - A_bi W, g = b_b - A_bi w  (FP64), one boundary row per thread
+    // and a strided set of columns
+    const int ngb = max(1, min(ncol, 256 / nb));
+    const int rb = tid % nb, gb = tid / nb;
+    if (gb < ngb) {
+      for (int j = gb; j < ncol; j += ngb) {
+        double v = A[(int64_t)(ni + j) * n + ni + rb];
+        for (int k = 0; k < ni; ++k) v = fma(-A[(int64_t)k * n + ni + rb], W[(int64_t)k * ldr + j], v);
+        if (j < nb) {
+          if (out.S) out.S[(lc * nb + j) * nb + rb] = v;
+        } else {
+          out.g[lc * nb + rb] = v;
+        }
+      }
+    }
+    // W to its output slot, column-major [nb+1][ni]
+    for (int t = tid; t < ncol * ni; t += 256) {
+      const int j = t / ni, k = t - j * ni;
+      out.Y[lc * (int64_t)ncol * ni + t] = W[(int64_t)k * ldr + j];
+    }
+    if (tid == 0 && iters) iters[lc] = conv ? it : -it - 1;
     __syncthreads();
   }
 }
@@ -1705,8 +1978,29 @@ __global__ void count_nonzero_kernel(const int32_t* __restrict__ v, int64_t n,
   if (local) atomicAdd(count, local);  // integer count: order-independent
 }
 
+__global__ void count_negative_kernel(const int32_t* __restrict__ v, int64_t n,
+                                      unsigned long long* __restrict__ count) {
+  unsigned long long local = 0;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+       t += (int64_t)gridDim.x * blockDim.x)
+    local += v[t] < 0;
+  if (local) atomicAdd(count, local);  // integer count: order-independent
+}
+
 __global__ void zero_count_kernel(unsigned long long* count) {
   if (threadIdx.x == 0) count[threadIdx.x] = 0;
+}
+
+hipError_t launch_count_negative(const int32_t* v, int64_t n, int64_t* count_dev,
+                                 hipStream_t st) {
+  hipLaunchKernelGGL(zero_count_kernel, dim3(1), dim3(1), 0, st,
+                     reinterpret_cast<unsigned long long*>(count_dev));
+  if (n > 0) {
+    const int grid = (int)((n + 255) / 256 < 1024 ? (n + 255) / 256 : 1024);
+    hipLaunchKernelGGL(count_negative_kernel, dim3(grid), dim3(256), 0, st, v, n,
+                       reinterpret_cast<unsigned long long*>(count_dev));
+  }
+  return hipGetLastError();
 }
 
 hipError_t launch_count_nonzero(const int32_t* info, int64_t n, int64_t* count_dev,
@@ -1720,6 +2014,110 @@ hipError_t launch_count_nonzero(const int32_t* info, int64_t n, int64_t* count_d
                        reinterpret_cast<unsigned long long*>(count_dev));
   }
   return hipGetLastError();
+}
+
+// columns of W per thread (mixed_refine_kernel's JC): the smallest of
+// 4, 8, 16, 32, 64 with ceil((nb+1) / JC) column groups fitting in 256 / ni
+static int mixed_jc(int ni, int nb) {
+  const int ncol = nb + 1, groups = std::max(1, 256 / ni);
+  for (int jc : {4, 8, 16, 32, 64})
+    if (cdiv(ncol, jc) <= groups) return jc;
+  return 0;
+}
+
+static size_t mixed_lds(int ni, int nb) {
+  const int jc = mixed_jc(ni, nb), ldr = cdiv(nb + 1, jc) * jc;
+  return ((size_t)ni * (ni + 1) + (size_t)ni * ldr) * sizeof(float);
+}
+
+bool mixed_supported(int ni, int nb) {
+  return ni >= 1 && ni <= kMixedMaxNi && nb >= 1 && mixed_jc(ni, nb) > 0 &&
+         mixed_lds(ni, nb) <= 150 * 1024;
+}
+
+// cells per chunk of the mixed path and the workspace they need: the FP64
+// systems, the FP32 inverses and the row-major working copies of W
+static int64_t mixed_chunk(int64_t ncells) { return std::min<int64_t>(ncells, kMixedChunkCells); }
+
+static size_t mixed_cell_bytes(int ni, int nb, size_t* a_off, size_t* x_off) {
+  const size_t n = ni + nb, jc = mixed_jc(ni, nb), ldr = cdiv(nb + 1, (int)jc) * jc;
+  const size_t a = n * (n + 1) * sizeof(double), x = (size_t)ni * ni * sizeof(float);
+  if (a_off) *a_off = a;
+  if (x_off) *x_off = x;
+  return a + ((x + 7) & ~(size_t)7) + (size_t)ni * ldr * sizeof(double);
+}
+
+size_t mixed_workspace_bytes(int ni, int nb, int64_t ncells) {
+  return (size_t)mixed_chunk(ncells) * mixed_cell_bytes(ni, nb, nullptr, nullptr);
+}
+
+template <int PHYS>
+static hipError_t launch_mixed_phys(const Tables& T, const MeshDev& M, const CondenseOut& out,
+                                    hipStream_t st, Workspace& ws, int maxit, int32_t* iters) {
+  const int ni = T.ni, nb = T.nb, n = ni + nb;
+  const int64_t chunk = mixed_chunk(M.ncells);
+  hipError_t e = workspace_reserve(ws, mixed_workspace_bytes(ni, nb, M.ncells));
+  if (e != hipSuccess) return e;
+  size_t abytes, xbytes;
+  (void)mixed_cell_bytes(ni, nb, &abytes, &xbytes);
+  char* base = reinterpret_cast<char*>(ws.p);
+  double* A64 = reinterpret_cast<double*>(base);
+  float* X = reinterpret_cast<float*>(base + (size_t)chunk * abytes);
+  double* Wws = reinterpret_cast<double*>(base + (size_t)chunk * abytes +
+                                          (((size_t)chunk * xbytes + 7) & ~(size_t)7));
+  FusedGen<PHYS> gen{T, M};
+  const int ra = cdiv(ni, 16), jc = mixed_jc(ni, nb);
+  const size_t lds = mixed_lds(ni, nb);
+  for (int64_t first = 0; first < M.ncells; first += chunk) {
+    const int64_t count = std::min<int64_t>(chunk, M.ncells - first);
+    const int g1 = (int)std::min<int64_t>(count, 256 * 16);
+    hipLaunchKernelGGL(gen_system_kernel<PHYS>, dim3(g1), dim3(256), 0, st, gen, first, count,
+                       A64);
+    const int g2 = (int)std::min<int64_t>(count, 256 * 8);
+    int32_t* info = out.info ? out.info + first : nullptr;
+    switch (ra) {
+#define HDG_GJ(R_)                                                                            \
+  case R_:                                                                                    \
+    hipLaunchKernelGGL(gj_inverse_kernel<R_>, dim3(g2), dim3(256), 0, st, A64, n, ni, count, X, \
+                       info);                                                                 \
+    break;
+      HDG_GJ(1) HDG_GJ(2) HDG_GJ(3) HDG_GJ(4) HDG_GJ(5) HDG_GJ(6) HDG_GJ(7) HDG_GJ(8)
+#undef HDG_GJ
+      default: return hipErrorInvalidValue;
+    }
+    CondenseOut o;
+    o.S = out.S ? out.S + first * nb * nb : nullptr;
+    o.g = out.g + first * nb;
+    o.Y = out.Y + first * (int64_t)(nb + 1) * ni;
+    int32_t* it = iters ? iters + first : nullptr;
+    const int g3 = (int)std::min<int64_t>(count, 256 * 2);
+#define HDG_REF(J_)                                                                           \
+  if (jc == J_) {                                                                             \
+    e = hipFuncSetAttribute((const void*)mixed_refine_kernel<J_>,                             \
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);            \
+    if (e != hipSuccess) return e;                                                            \
+    hipLaunchKernelGGL(mixed_refine_kernel<J_>, dim3(g3), dim3(256), lds, st, A64, X, Wws, ni, \
+                       nb, count, o, maxit, it);                                              \
+  } else
+    HDG_REF(4) HDG_REF(8) HDG_REF(16) HDG_REF(32) HDG_REF(64) return hipErrorInvalidValue;
+#undef HDG_REF
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+hipError_t launch_condense_mixed(const Tables& T, const MeshDev& M, const CondenseOut& out,
+                                 hipStream_t st, Workspace& ws, bool prep, int maxit,
+                                 int32_t* iters) {
+  if (M.ncells <= 0) return hipSuccess;
+  if (!mixed_supported(T.ni, T.nb) || !out.g || !out.Y) return hipErrorInvalidValue;
+  if (prep) {
+    hipError_t e = launch_prep(T, M, st);
+    if (e != hipSuccess) return e;
+  }
+  if (T.phys == kDarcy) return launch_mixed_phys<kDarcy>(T, M, out, st, ws, maxit, iters);
+  return launch_mixed_phys<kElasticity>(T, M, out, st, ws, maxit, iters);
 }
 
 hipError_t launch_cell_systems(const Tables& T, const MeshDev& M, int64_t first, int64_t count,

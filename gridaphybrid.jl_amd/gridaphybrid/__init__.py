@@ -350,6 +350,11 @@ class HybridAffineFEOperator:
         U, Y = C.c_void_p(), C.c_void_p()
         check(lib().hdg_local_factors(self.mesh, C.byref(U), C.byref(Y)), self.ctx.h, "factors")
         nc, ni, nb = self.info.ncells, self.info.ni, self.info.nb
+        if not U.value:   # mixed precision: W = A_ii^-1 [A_ib b_i], natural order
+            Yh = np.empty((nc, nb + 1, ni))
+            check(lib().hdg_memcpy_d2h(self.ctx.h, Yh.ctypes.data, Y, Yh.nbytes), self.ctx.h,
+                  "d2h")
+            return None, Yh.transpose(0, 2, 1)
         Up = np.empty((nc, ni * (ni + 1) // 2))
         Yh = np.empty((nc, nb + 1, ni))
         check(lib().hdg_memcpy_d2h(self.ctx.h, Up.ctypes.data, U, Up.nbytes), self.ctx.h, "d2h")
@@ -370,6 +375,14 @@ class HybridAffineFEOperator:
                                 out.data_ptr()), self.ctx.h, "recover")
         self.ctx.synchronize()
         return out
+
+    def refinement_stats(self):
+        """Refinement sweeps of the last condensation (mixed precision)."""
+        st = _lib.RefinementStats()
+        check(lib().hdg_mesh_refinement_stats(self.mesh, C.byref(st)), self.ctx.h, "stats")
+        return {"mixed": bool(st.mixed), "fp64_fallback": bool(st.fp64_fallback),
+                "max_sweeps": st.max_sweeps, "mean_sweeps": st.mean_sweeps,
+                "unconverged": st.unconverged}
 
     def interpolate_trace(self):
         """Host interpolant of the built-in manufactured trace at the free

@@ -44,6 +44,7 @@ EXPORTS = [
     "hdg_condense", "hdg_assemble", "hdg_mesh_singular_count", "hdg_mesh_interpolate_trace",
     "hdg_local_factors", "hdg_batch_create", "hdg_batch_destroy", "hdg_batch_get_classes",
     "hdg_batch_launch", "hdg_batch_condense_assemble", "hdg_batch_recover",
+    "hdg_mesh_refinement_stats",
 ]
 
 BATCH_AUTOTUNE = 1
@@ -100,6 +101,13 @@ class BatchClass(C.Structure):
     ]
 
 
+class RefinementStats(C.Structure):
+    _fields_ = [
+        ("mixed", C.c_int), ("fp64_fallback", C.c_int), ("max_sweeps", C.c_int),
+        ("mean_sweeps", C.c_double), ("unconverged", C.c_int64),
+    ]
+
+
 _lib = None
 
 
@@ -148,6 +156,7 @@ def lib():
         L.hdg_batch_launch.argtypes = [vp, i32]
         L.hdg_batch_condense_assemble.argtypes = [vp, i32, C.POINTER(i64)]
         L.hdg_batch_recover.argtypes = [vp, C.POINTER(vp), C.POINTER(vp)]
+        L.hdg_mesh_refinement_stats.argtypes = [vp, C.POINTER(RefinementStats)]
         _ = dbl
         _lib = L
     return _lib

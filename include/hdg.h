@@ -34,6 +34,12 @@ typedef enum {
   HDG_SINGULAR = 1          /* a local A_ii had a zero pivot (getrf info > 0) */
 } hdg_status;
 
+/* HDG_FP64: getrf-order elimination of the whole cell system in FP64.
+ * HDG_MIXED: A_ii is inverted in FP32 (Gauss-Jordan, partial pivoting),
+ * W = A_ii^-1 [A_ib b_i] is formed with FP32 products and refined with FP64
+ * residuals B - A_ii W until the correction reaches FP64 rounding or stops
+ * shrinking; S, g and the recovery are FP64.  Local systems with ni > 128 (or
+ * too large for LDS) use the FP64 path under HDG_MIXED. */
 typedef enum { HDG_FP64 = 0, HDG_MIXED = 1 } hdg_precision;
 typedef enum { HDG_DARCY = 0, HDG_ELASTICITY = 1 } hdg_physics;
 typedef enum { HDG_SPACE_Q = 0, HDG_SPACE_P = 1 } hdg_space;
@@ -185,8 +191,20 @@ int hdg_mesh_set_source(hdg_mesh* mesh, const double* src_qp);
  * fused cell integration + StaticCondensationMap for every cell, Dirichlet
  * lifting and deterministic assembly of the skeleton CSC matrix and rhs.
  * Keeps W = A_ii^-1 [A_ib b_i] resident for recovery.  *n_singular (host,
- * may be NULL) receives the number of cells with a zero pivot. */
+ * may be NULL) receives the number of cells with a zero pivot.  Under
+ * HDG_MIXED, a mesh with any cell whose refinement did not converge is
+ * condensed again in FP64 (reported by hdg_mesh_refinement_stats). */
 int hdg_condense_assemble(hdg_mesh* mesh, int precision, int64_t* n_singular);
+
+/* Refinement statistics of the last condensation (synchronises). */
+typedef struct {
+  int mixed;            /* the resident factors come from the mixed path */
+  int fp64_fallback;    /* the mesh was redone in FP64 after a non-converged cell */
+  int max_sweeps;       /* largest number of FP64 refinement sweeps of a cell */
+  double mean_sweeps;
+  int64_t unconverged;  /* cells whose last correction exceeded 1e-7 of W */
+} hdg_refinement_stats;
+int hdg_mesh_refinement_stats(hdg_mesh* mesh, hdg_refinement_stats* stats);
 
 /* The two halves of hdg_condense_assemble, enqueued asynchronously on the
  * context stream with no host synchronisation (for timing and pipelining):
@@ -218,7 +236,9 @@ int hdg_cell_blocks(hdg_mesh* mesh, const double** S, const double** g,
  * U [ncells][ni(ni+1)/2], the upper LU factor of A_ii in LAPACK row order
  * packed by columns (column c holds rows 0..c at c(c+1)/2 + k), and
  * Y [ncells][nb+1][ni] = L^-1 P [A_ib b_i] in LAPACK row order.  The pivot
- * sequence is getrf's (largest magnitude, first position on ties). */
+ * sequence is getrf's (largest magnitude, first position on ties).  After a
+ * mixed-precision condensation *U is NULL and Y holds W = A_ii^-1 [A_ib b_i]
+ * in natural interior order. */
 int hdg_local_factors(hdg_mesh* mesh, const double** U, const double** Y);
 
 /* solve!'s recovery (src/HybridAffineFEOperators.jl:89-150 with
