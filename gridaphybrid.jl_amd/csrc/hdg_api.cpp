@@ -229,6 +229,26 @@ int hdg_ctx_synchronize(hdg_ctx* ctx) {
   return HDG_OK;
 }
 
+int hdg_profile_start(hdg_ctx* ctx) {
+  if (!ctx) return HDG_ERR_ARG;
+  clock_start();
+  return HDG_OK;
+}
+
+int hdg_profile_stop(hdg_ctx* ctx, int max, hdg_kernel_time* times, int* n) {
+  if (!ctx || (max > 0 && !times)) return HDG_ERR_ARG;
+  std::vector<KernelTime> kt;
+  const int bad = clock_stop(kt);
+  if (n) *n = (int)kt.size();
+  for (int i = 0; i < max && i < (int)kt.size(); ++i) {
+    std::memset(&times[i], 0, sizeof(times[i]));
+    std::strncpy(times[i].name, kt[i].name.c_str(), sizeof(times[i].name) - 1);
+    times[i].launches = kt[i].launches;
+    times[i].total_ms = kt[i].total_ms;
+  }
+  return bad ? fail(ctx, HDG_ERR_HIP, "kernel timing events failed") : HDG_OK;
+}
+
 int hdg_memcpy_d2h(hdg_ctx* ctx, void* dst, const void* src, int64_t bytes) {
   if (!ctx || bytes < 0) return HDG_ERR_ARG;
   if (bytes == 0) return HDG_OK;

@@ -174,6 +174,15 @@ hipError_t launch_count_nonzero(const int32_t* info, int64_t n, int64_t* count_d
 hipError_t launch_count_negative(const int32_t* v, int64_t n, int64_t* count_dev,
                                  hipStream_t st);
 
+// Kernel clock (hdg_kernels.hip): per-name totals of event-timed launches.
+struct KernelTime {
+  std::string name;
+  int64_t launches;
+  double total_ms;
+};
+void clock_start();
+int clock_stop(std::vector<KernelTime>& out);  // returns the number of failed timings
+
 // Host helpers (hdg_fe.cpp)
 int build_tables(int phys, int D, int space, int kp, int kq, int km, int quad_degree,
                  TablesHost& t, std::string& err);

@@ -1563,37 +1563,43 @@ __global__ void __launch_bounds__(256)
   }
 }
 
-// FP64 refinement of W and the FP64 Schur complement.  Thread t works on
-// interior row r = t % ni and the contiguous columns [g JC, (g+1) JC) of W,
-// g = t / ni.  The working copy of W is row-major with ld = ng JC >= nb+1
-// (padding columns stay zero), so the inner loops run without guards.  X and
-// the FP32 residual live in LDS; A and W in the workspace.  The final W goes
+// FP64 refinement of W and the FP64 Schur complement, one cell per 512-thread
+// workgroup.  Threads form column groups of RP = 64 ceil(ni/64) rows (whole
+// wavefronts, so a wavefront reads the same W entries: LDS broadcasts): thread
+// t works on interior row t % RP and the JC contiguous columns of group
+// t / RP.  W is kept row-major with ld = ng JC >= nb+1 (padding columns stay
+// zero, so the inner loops need no guards), in LDS when WLDS and in the
+// workspace otherwise; X and the FP32 residual are in LDS.  The final W goes
 // to its output slot ([nb+1][ni] per cell).  iters[cell] (may be null) gets
 // the refinement sweeps used, negated when the last correction was still
-// above 1e-7 of W (not converged).
-template <int JC>
-__global__ void __launch_bounds__(256)
+// above 1e-10 of W (not converged).
+constexpr int kRefineThreads = 512;
+
+template <int JC, bool WLDS>
+__global__ void __launch_bounds__(kRefineThreads)
     mixed_refine_kernel(const double* __restrict__ A64, const float* __restrict__ X,
                         double* __restrict__ Wws, int ni, int nb, int64_t count,
                         CondenseOut out, int maxit, int32_t* __restrict__ iters) {
-  extern __shared__ float s_dyn[];
+  constexpr int NT = kRefineThreads, NW = NT / 64;
+  extern __shared__ double s_dyn[];
   const int n = ni + nb, ncol = nb + 1, ldx = ni + 1;
   const int ng = cdiv(ncol, JC), ldr = ng * JC;
-  float* sX = s_dyn;             // [ni][ni+1]
-  float* sR = s_dyn + ni * ldx;  // [ni][ldr]
-  __shared__ double s_red[2][4];
+  const int RP = 64 * cdiv(ni, 64);
+  double* sW = s_dyn;                                                  // [ni][ldr] (WLDS)
+  float* sX = reinterpret_cast<float*>(s_dyn + (WLDS ? ni * ldr : 0));  // [ni][ni+1]
+  float* sR = sX + ni * ldx;                                           // [ni][ldr]
+  __shared__ double s_red[2][NW];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  const int r = tid % ni, g = tid / ni, j0 = g * JC;
-  const bool act = g < ng;
+  const int r = tid % RP, g = tid / RP, j0 = g * JC;
+  const bool act = r < ni && g < ng;
   for (int64_t lc = blockIdx.x; lc < count; lc += gridDim.x) {
     const double* A = A64 + lc * (int64_t)n * (n + 1);
     const float* Xc = X + lc * (int64_t)ni * ni;
-    double* W = Wws + lc * (int64_t)ni * ldr;  // W[k][j] at k*ldr + j
-    for (int t = tid; t < ni * ni; t += 256) {
+    double* W = WLDS ? sW : Wws + (size_t)blockIdx.x * ni * ldr;  // W[k][j] at k*ldr + j
+    for (int t = tid; t < ni * ni; t += NT) {
       const int rr = t / ni, cc = t - rr * ni;
       sX[rr * ldx + cc] = Xc[t];
     }
-    double acc[JC];
     // B = [A_ib b_i] (zero in the padding columns); R_0 = B
     if (act) {
 #pragma unroll
@@ -1607,11 +1613,11 @@ __global__ void __launch_bounds__(256)
     int it = 0, conv = 1;
     for (;;) {
       // W += X R   (W_0 = X B)
-      float d[JC];
-#pragma unroll
-      for (int jj = 0; jj < JC; ++jj) d[jj] = 0.f;
       double dmax = 0.0, wmax = 0.0;
       if (act) {
+        float d[JC];
+#pragma unroll
+        for (int jj = 0; jj < JC; ++jj) d[jj] = 0.f;
         const float* xr = sX + r * ldx;
         const float* rr = sR + j0;
         for (int k = 0; k < ni; ++k) {
@@ -1619,7 +1625,7 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
           for (int jj = 0; jj < JC; ++jj) d[jj] = fmaf(xv, rr[k * ldr + jj], d[jj]);
         }
-        double* wr = W + (int64_t)r * ldr + j0;
+        double* wr = W + r * ldr + j0;
 #pragma unroll
         for (int jj = 0; jj < JC; ++jj) {
           const double w = (it == 0 ? 0.0 : wr[jj]) + (double)d[jj];
@@ -1636,17 +1642,26 @@ __global__ void __launch_bounds__(256)
         s_red[1][wv] = wmax;
       }
       __syncthreads();  // also publishes W for the residual
-      dmax = fmax(fmax(s_red[0][0], s_red[0][1]), fmax(s_red[0][2], s_red[0][3]));
-      wmax = fmax(fmax(s_red[1][0], s_red[1][1]), fmax(s_red[1][2], s_red[1][3]));
-      const bool done = it >= maxit || dmax <= 0x1p-50 * wmax ||
-                        (it >= 2 && dmax > 0.5 * dprev);
+      dmax = s_red[0][0];
+      wmax = s_red[1][0];
+#pragma unroll
+      for (int q = 1; q < NW; ++q) {
+        dmax = fmax(dmax, s_red[0][q]);
+        wmax = fmax(wmax, s_red[1][q]);
+      }
+      // converged to FP64 rounding, or stalled at the FP64 floor of the
+      // residual (a stall above 1e-10 of W is not accepted: transient growth of
+      // the correction is possible while I - X A_ii is far from normal)
+      const bool done = it >= maxit || dmax <= 0x1p-44 * wmax ||
+                        (it >= 2 && dmax <= 1e-10 * wmax && dmax > 0.5 * dprev);
       if (done) {
-        conv = dmax <= 1e-7 * wmax;
+        conv = dmax <= 1e-10 * wmax;
         break;
       }
       dprev = dmax;
       ++it;
       // R = B - A_ii W  (FP64)
+      double acc[JC];
       if (act) {
 #pragma unroll
         for (int jj = 0; jj < JC; ++jj) {
@@ -1657,39 +1672,104 @@ __global__ void __launch_bounds__(256)
         for (int k = 0; k < ni; ++k) {
           const double a = A[(int64_t)k * n + r];
 #pragma unroll
-          for (int jj = 0; jj < JC; ++jj) acc[jj] = fma(-a, wk[(int64_t)k * ldr + jj], acc[jj]);
+          for (int jj = 0; jj < JC; ++jj) acc[jj] = fma(-a, wk[k * ldr + jj], acc[jj]);
         }
       }
-      __syncthreads();  // everyone is done with sR before it is overwritten
+      __syncthreads();  // everyone is done with sR (and s_red) before they change
       if (act) {
 #pragma unroll
         for (int jj = 0; jj < JC; ++jj) sR[r * ldr + j0 + jj] = (float)acc[jj];
       }
       __syncthreads();
     }
-    // S = A_bb - A_bi W, g = b_b - A_bi w  (FP64), one boundary row per thread
-    // and a strided set of columns
-    const int ngb = max(1, min(ncol, 256 / nb));
-    const int rb = tid % nb, gb = tid / nb;
-    if (gb < ngb) {
-      for (int j = gb; j < ncol; j += ngb) {
-        double v = A[(int64_t)(ni + j) * n + ni + rb];
-        for (int k = 0; k < ni; ++k) v = fma(-A[(int64_t)k * n + ni + rb], W[(int64_t)k * ldr + j], v);
-        if (j < nb) {
-          if (out.S) out.S[(lc * nb + j) * nb + rb] = v;
-        } else {
-          out.g[lc * nb + rb] = v;
+    // S = A_bb - A_bi W, g = b_b - A_bi w  (FP64): boundary row per thread, a
+    // wavefront-uniform strided set of columns
+    {
+      const int RB = 64 * cdiv(nb, 64), ngb = NT / RB;
+      const int rb = tid % RB, gb = tid / RB;
+      if (rb < nb) {
+        for (int j = gb; j < ncol; j += ngb) {
+          double v = A[(int64_t)(ni + j) * n + ni + rb];
+          for (int k = 0; k < ni; ++k) v = fma(-A[(int64_t)k * n + ni + rb], W[k * ldr + j], v);
+          if (j < nb) {
+            if (out.S) out.S[(lc * nb + j) * nb + rb] = v;
+          } else {
+            out.g[lc * nb + rb] = v;
+          }
         }
       }
     }
     // W to its output slot, column-major [nb+1][ni]
-    for (int t = tid; t < ncol * ni; t += 256) {
+    for (int t = tid; t < ncol * ni; t += NT) {
       const int j = t / ni, k = t - j * ni;
-      out.Y[lc * (int64_t)ncol * ni + t] = W[(int64_t)k * ldr + j];
+      out.Y[lc * (int64_t)ncol * ni + t] = W[k * ldr + j];
     }
     if (tid == 0 && iters) iters[lc] = conv ? it : -it - 1;
     __syncthreads();
   }
+}
+
+// --------------------------------------------------------------------------
+// Kernel clock: while enabled, every launch outside stream capture is
+// bracketed by two HIP events on its own stream, and the elapsed times are
+// summed per kernel name when the clock is read.
+// --------------------------------------------------------------------------
+namespace {
+struct ClockMark {
+  const char* name;
+  hipEvent_t a, b;
+};
+std::vector<ClockMark>* g_clock = nullptr;
+
+struct ClockScope {
+  hipStream_t st;
+  ClockMark mark{nullptr, nullptr, nullptr};
+  ClockScope(hipStream_t s, const char* name) : st(s) {
+    if (!g_clock) return;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return;
+    if (hipEventCreate(&mark.a) != hipSuccess) return;
+    if (hipEventCreate(&mark.b) != hipSuccess) {
+      (void)hipEventDestroy(mark.a);
+      return;
+    }
+    mark.name = name;
+    (void)hipEventRecord(mark.a, s);
+  }
+  ~ClockScope() {
+    if (!mark.name) return;
+    (void)hipEventRecord(mark.b, st);
+    g_clock->push_back(mark);
+  }
+};
+}  // namespace
+
+void clock_start() {
+  if (!g_clock) g_clock = new std::vector<ClockMark>();
+}
+
+int clock_stop(std::vector<KernelTime>& out) {
+  out.clear();
+  if (!g_clock) return 0;
+  int bad = 0;
+  for (ClockMark& m : *g_clock) {
+    float ms = 0.f;
+    if (hipEventSynchronize(m.b) != hipSuccess || hipEventElapsedTime(&ms, m.a, m.b) != hipSuccess)
+      ++bad;
+    (void)hipEventDestroy(m.a);
+    (void)hipEventDestroy(m.b);
+    auto it = std::find_if(out.begin(), out.end(),
+                           [&](const KernelTime& k) { return k.name == m.name; });
+    if (it == out.end()) {
+      out.push_back(KernelTime{m.name, 0, 0.0});
+      it = out.end() - 1;
+    }
+    it->launches++;
+    it->total_ms += ms;
+  }
+  delete g_clock;
+  g_clock = nullptr;
+  return bad;
 }
 
 // --------------------------------------------------------------------------
@@ -1792,16 +1872,19 @@ static hipError_t launch_condense(const Gen& gen, const CondenseOut& out, int64_
     hipError_t e = workspace_reserve(ws, large_bytes(ni + nb, &ld));
     if (e != hipSuccess) return e;
     const int grid = (int)(ncells < kLargeGrid ? ncells : kLargeGrid);
-    hipLaunchKernelGGL((condense_large_kernel<Gen, BACKSUB>), dim3(grid), dim3(256), 0, st, gen,
+    {
+      ClockScope clk_(st, "condense_blocked");
+      hipLaunchKernelGGL((condense_large_kernel<Gen, BACKSUB>), dim3(grid), dim3(256), 0, st, gen,
                        out, ncells, ni, nb, ws.p, ld);
+    }
     return hipGetLastError();
   }
   if (fam == kFamWave) {
     const int grid = grid_of(48);
 #define HDG_WAVE(NI_, NB_)                                                                  \
   if (ni == NI_ && nb == NB_) {                                                             \
-    hipLaunchKernelGGL((condense_wave_kernel<NI_, NB_, Gen, BACKSUB>), dim3(grid), dim3(64), \
-                       0, st, gen, out, ncells);                                            \
+    { ClockScope clk_(st, "condense_wave"); hipLaunchKernelGGL((condense_wave_kernel<NI_, NB_, Gen, BACKSUB>), dim3(grid), dim3(64), \
+                       0, st, gen, out, ncells); }                                            \
     return hipGetLastError();                                                               \
   }
     HDG_WAVE(12, 8)
@@ -1814,8 +1897,11 @@ static hipError_t launch_condense(const Gen& gen, const CondenseOut& out, int64_
   }
   if (fam == kFamColwave) {
     const int grid = grid_of(2);
-    hipLaunchKernelGGL((condense_colwave_kernel<3, 21, 8, Gen, BACKSUB>), dim3(grid), dim3(512),
+    {
+      ClockScope clk_(st, "condense_colwave");
+      hipLaunchKernelGGL((condense_colwave_kernel<3, 21, 8, Gen, BACKSUB>), dim3(grid), dim3(512),
                        0, st, gen, out, ncells, ni, nb);
+    }
     return hipGetLastError();
   }
   Bucket b;
@@ -1827,8 +1913,8 @@ static hipError_t launch_condense(const Gen& gen, const CondenseOut& out, int64_
   const int grid = gridmul > 0 ? grid_of(gridmul) : grid_for(ncells, b.pr * b.pc);
 #define HDG_LAUNCH(PR, PC, R, CC)                                                             \
   if (b.pr == PR && b.pc == PC && b.r == R && b.cc == CC) {                                   \
-    hipLaunchKernelGGL((condense_kernel<PR, PC, R, CC, Gen, BACKSUB>), dim3(grid),             \
-                       dim3(PR * PC), 0, st, gen, out, ncells, ni, nb);                       \
+    { ClockScope clk_(st, "condense_grid"); hipLaunchKernelGGL((condense_kernel<PR, PC, R, CC, Gen, BACKSUB>), dim3(grid),             \
+                       dim3(PR * PC), 0, st, gen, out, ncells, ni, nb); }                       \
     return hipGetLastError();                                                                 \
   }
   HDG_LAUNCH(8, 8, 4, 4)
@@ -1881,9 +1967,15 @@ static hipError_t launch_prep(const Tables& T, const MeshDev& M, hipStream_t st)
   const int64_t blocks = (M.ncells + 127) / 128;
   const int grid = (int)(blocks < 8192 ? blocks : 8192);
   if (T.phys == kDarcy)
-    hipLaunchKernelGGL(prep_kernel<kDarcy>, dim3(grid), dim3(128), 0, st, T, M);
+    {
+      ClockScope clk_(st, "prep");
+      hipLaunchKernelGGL(prep_kernel<kDarcy>, dim3(grid), dim3(128), 0, st, T, M);
+    }
   else
-    hipLaunchKernelGGL(prep_kernel<kElasticity>, dim3(grid), dim3(128), 0, st, T, M);
+    {
+      ClockScope clk_(st, "prep");
+      hipLaunchKernelGGL(prep_kernel<kElasticity>, dim3(grid), dim3(128), 0, st, T, M);
+    }
   return hipGetLastError();
 }
 
@@ -1932,25 +2024,40 @@ hipError_t launch_recover(const double* U, const double* Y, const int32_t* cell_
     const int64_t blocks = (ncells + 3) / 4;
     const int grid = (int)(blocks < 65536 ? blocks : 65536);
     if (ni <= 64)
+      {
+      ClockScope clk_(st, "recover");
       hipLaunchKernelGGL(recover_kernel<1>, dim3(grid), dim3(256), 0, st, U, Y, cell_dofs,
                          lam_free, lam_dir, out, ncells, ni, nb, na);
+    }
     else if (ni <= 128)
+      {
+      ClockScope clk_(st, "recover");
       hipLaunchKernelGGL(recover_kernel<2>, dim3(grid), dim3(256), 0, st, U, Y, cell_dofs,
                          lam_free, lam_dir, out, ncells, ni, nb, na);
+    }
     else
+      {
+      ClockScope clk_(st, "recover");
       hipLaunchKernelGGL(recover_kernel<4>, dim3(grid), dim3(256), 0, st, U, Y, cell_dofs,
                          lam_free, lam_dir, out, ncells, ni, nb, na);
+    }
   }
   if (nfree > 0)
-    hipLaunchKernelGGL(copy_kernel, dim3(1024), dim3(256), 0, st, lam_free, out + ncells * ni,
+    {
+      ClockScope clk_(st, "recover_copy");
+      hipLaunchKernelGGL(copy_kernel, dim3(1024), dim3(256), 0, st, lam_free, out + ncells * ni,
                        nfree);
+    }
   return hipGetLastError();
 }
 
 hipError_t launch_symbolic(const PairTable& P, int32_t* rowval, hipStream_t st) {
   if (P.nff > 0) {
     const int grid = (int)(P.nff < 65536 ? P.nff : 65536);
-    hipLaunchKernelGGL(symbolic_kernel, dim3(grid), dim3(256), 0, st, P, rowval);
+    {
+      ClockScope clk_(st, "symbolic");
+      hipLaunchKernelGGL(symbolic_kernel, dim3(grid), dim3(256), 0, st, P, rowval);
+    }
   }
   return hipGetLastError();
 }
@@ -1960,11 +2067,17 @@ hipError_t launch_assemble(const PairTable& P, const double* S, const double* g,
                            double* rhs, hipStream_t st) {
   if (P.nff > 0) {
     const int grid = (int)(P.nff < 65536 ? P.nff : 65536);
-    hipLaunchKernelGGL(assemble_kernel, dim3(grid), dim3(256), 0, st, P, S, nzval);
+    {
+      ClockScope clk_(st, "assemble");
+      hipLaunchKernelGGL(assemble_kernel, dim3(grid), dim3(256), 0, st, P, S, nzval);
+    }
     const int64_t tot = P.nff * P.ntf;
     const int grid2 = (int)((tot + 255) / 256 < 65536 ? (tot + 255) / 256 : 65536);
-    hipLaunchKernelGGL(rhs_kernel, dim3(grid2), dim3(256), 0, st, P, S, g, cell_dofs, lam_dir,
+    {
+      ClockScope clk_(st, "assemble_rhs");
+      hipLaunchKernelGGL(rhs_kernel, dim3(grid2), dim3(256), 0, st, P, S, g, cell_dofs, lam_dir,
                        rhs);
+    }
   }
   return hipGetLastError();
 }
@@ -1993,12 +2106,18 @@ __global__ void zero_count_kernel(unsigned long long* count) {
 
 hipError_t launch_count_negative(const int32_t* v, int64_t n, int64_t* count_dev,
                                  hipStream_t st) {
-  hipLaunchKernelGGL(zero_count_kernel, dim3(1), dim3(1), 0, st,
+  {
+      ClockScope clk_(st, "count");
+      hipLaunchKernelGGL(zero_count_kernel, dim3(1), dim3(1), 0, st,
                      reinterpret_cast<unsigned long long*>(count_dev));
+    }
   if (n > 0) {
     const int grid = (int)((n + 255) / 256 < 1024 ? (n + 255) / 256 : 1024);
-    hipLaunchKernelGGL(count_negative_kernel, dim3(grid), dim3(256), 0, st, v, n,
+    {
+      ClockScope clk_(st, "count");
+      hipLaunchKernelGGL(count_negative_kernel, dim3(grid), dim3(256), 0, st, v, n,
                        reinterpret_cast<unsigned long long*>(count_dev));
+    }
   }
   return hipGetLastError();
 }
@@ -2006,49 +2125,66 @@ hipError_t launch_count_negative(const int32_t* v, int64_t n, int64_t* count_dev
 hipError_t launch_count_nonzero(const int32_t* info, int64_t n, int64_t* count_dev,
                                 hipStream_t st) {
   // a fill kernel rather than a memset node, so the sequence stays capturable
-  hipLaunchKernelGGL(zero_count_kernel, dim3(1), dim3(1), 0, st,
+  {
+      ClockScope clk_(st, "count");
+      hipLaunchKernelGGL(zero_count_kernel, dim3(1), dim3(1), 0, st,
                      reinterpret_cast<unsigned long long*>(count_dev));
+    }
   if (n > 0) {
     const int grid = (int)((n + 255) / 256 < 1024 ? (n + 255) / 256 : 1024);
-    hipLaunchKernelGGL(count_nonzero_kernel, dim3(grid), dim3(256), 0, st, info, n,
+    {
+      ClockScope clk_(st, "count");
+      hipLaunchKernelGGL(count_nonzero_kernel, dim3(grid), dim3(256), 0, st, info, n,
                        reinterpret_cast<unsigned long long*>(count_dev));
+    }
   }
   return hipGetLastError();
 }
 
-// columns of W per thread (mixed_refine_kernel's JC): the smallest of
-// 4, 8, 16, 32, 64 with ceil((nb+1) / JC) column groups fitting in 256 / ni
+// columns of W per thread (mixed_refine_kernel's JC): the smallest of 4, 8,
+// 16, 32 whose ceil((nb+1) / JC) column groups fit in the workgroup
 static int mixed_jc(int ni, int nb) {
-  const int ncol = nb + 1, groups = std::max(1, 256 / ni);
-  for (int jc : {4, 8, 16, 32, 64})
-    if (cdiv(ncol, jc) <= groups) return jc;
+  const int groups = kRefineThreads / (64 * cdiv(ni, 64));
+  for (int jc : {4, 8, 16, 32})
+    if (cdiv(nb + 1, jc) <= groups) return jc;
   return 0;
 }
 
-static size_t mixed_lds(int ni, int nb) {
-  const int jc = mixed_jc(ni, nb), ldr = cdiv(nb + 1, jc) * jc;
-  return ((size_t)ni * (ni + 1) + (size_t)ni * ldr) * sizeof(float);
+static int mixed_ldr(int ni, int nb) {
+  const int jc = mixed_jc(ni, nb);
+  return cdiv(nb + 1, jc) * jc;
 }
 
+// LDS of mixed_refine_kernel: X and the residual in FP32, W in FP64 if wlds
+static size_t mixed_lds(int ni, int nb, bool wlds) {
+  const size_t ldr = mixed_ldr(ni, nb);
+  return ((size_t)ni * (ni + 1) + (size_t)ni * ldr) * sizeof(float) +
+         (wlds ? (size_t)ni * ldr * sizeof(double) : 0);
+}
+
+constexpr size_t kMixedMaxLds = 150 * 1024;
+
 bool mixed_supported(int ni, int nb) {
-  return ni >= 1 && ni <= kMixedMaxNi && nb >= 1 && mixed_jc(ni, nb) > 0 &&
-         mixed_lds(ni, nb) <= 150 * 1024;
+  return ni >= 1 && ni <= kMixedMaxNi && nb >= 1 && nb <= 512 && mixed_jc(ni, nb) > 0 &&
+         mixed_lds(ni, nb, false) <= kMixedMaxLds;
 }
 
 // cells per chunk of the mixed path and the workspace they need: the FP64
-// systems, the FP32 inverses and the row-major working copies of W
+// systems and the FP32 inverses per cell, plus one W per refine workgroup
 static int64_t mixed_chunk(int64_t ncells) { return std::min<int64_t>(ncells, kMixedChunkCells); }
+constexpr int kRefineGrid = 256 * 2;
 
 static size_t mixed_cell_bytes(int ni, int nb, size_t* a_off, size_t* x_off) {
-  const size_t n = ni + nb, jc = mixed_jc(ni, nb), ldr = cdiv(nb + 1, (int)jc) * jc;
+  const size_t n = ni + nb;
   const size_t a = n * (n + 1) * sizeof(double), x = (size_t)ni * ni * sizeof(float);
   if (a_off) *a_off = a;
   if (x_off) *x_off = x;
-  return a + ((x + 7) & ~(size_t)7) + (size_t)ni * ldr * sizeof(double);
+  return a + x;
 }
 
 size_t mixed_workspace_bytes(int ni, int nb, int64_t ncells) {
-  return (size_t)mixed_chunk(ncells) * mixed_cell_bytes(ni, nb, nullptr, nullptr);
+  return (size_t)mixed_chunk(ncells) * mixed_cell_bytes(ni, nb, nullptr, nullptr) + 8 +
+         (size_t)kRefineGrid * ni * mixed_ldr(ni, nb) * sizeof(double);
 }
 
 template <int PHYS>
@@ -2067,19 +2203,23 @@ static hipError_t launch_mixed_phys(const Tables& T, const MeshDev& M, const Con
                                           (((size_t)chunk * xbytes + 7) & ~(size_t)7));
   FusedGen<PHYS> gen{T, M};
   const int ra = cdiv(ni, 16), jc = mixed_jc(ni, nb);
-  const size_t lds = mixed_lds(ni, nb);
+  const bool wlds = mixed_lds(ni, nb, true) <= kMixedMaxLds;
+  const size_t lds = mixed_lds(ni, nb, wlds);
   for (int64_t first = 0; first < M.ncells; first += chunk) {
     const int64_t count = std::min<int64_t>(chunk, M.ncells - first);
     const int g1 = (int)std::min<int64_t>(count, 256 * 16);
-    hipLaunchKernelGGL(gen_system_kernel<PHYS>, dim3(g1), dim3(256), 0, st, gen, first, count,
+    {
+      ClockScope clk_(st, "mixed_gen_system");
+      hipLaunchKernelGGL(gen_system_kernel<PHYS>, dim3(g1), dim3(256), 0, st, gen, first, count,
                        A64);
+    }
     const int g2 = (int)std::min<int64_t>(count, 256 * 8);
     int32_t* info = out.info ? out.info + first : nullptr;
     switch (ra) {
 #define HDG_GJ(R_)                                                                            \
   case R_:                                                                                    \
-    hipLaunchKernelGGL(gj_inverse_kernel<R_>, dim3(g2), dim3(256), 0, st, A64, n, ni, count, X, \
-                       info);                                                                 \
+    { ClockScope clk_(st, "mixed_gj_inverse"); hipLaunchKernelGGL(gj_inverse_kernel<R_>, dim3(g2), dim3(256), 0, st, A64, n, ni, count, X, \
+                       info); }                                                                 \
     break;
       HDG_GJ(1) HDG_GJ(2) HDG_GJ(3) HDG_GJ(4) HDG_GJ(5) HDG_GJ(6) HDG_GJ(7) HDG_GJ(8)
 #undef HDG_GJ
@@ -2090,16 +2230,18 @@ static hipError_t launch_mixed_phys(const Tables& T, const MeshDev& M, const Con
     o.g = out.g + first * nb;
     o.Y = out.Y + first * (int64_t)(nb + 1) * ni;
     int32_t* it = iters ? iters + first : nullptr;
-    const int g3 = (int)std::min<int64_t>(count, 256 * 2);
-#define HDG_REF(J_)                                                                           \
-  if (jc == J_) {                                                                             \
-    e = hipFuncSetAttribute((const void*)mixed_refine_kernel<J_>,                             \
+    const int g3 = (int)std::min<int64_t>(count, kRefineGrid);
+#define HDG_REF(J_, L_)                                                                       \
+  if (jc == J_ && wlds == L_) {                                                               \
+    e = hipFuncSetAttribute((const void*)mixed_refine_kernel<J_, L_>,                         \
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);            \
     if (e != hipSuccess) return e;                                                            \
-    hipLaunchKernelGGL(mixed_refine_kernel<J_>, dim3(g3), dim3(256), lds, st, A64, X, Wws, ni, \
-                       nb, count, o, maxit, it);                                              \
+    { ClockScope clk_(st, "mixed_refine"); hipLaunchKernelGGL((mixed_refine_kernel<J_, L_>), dim3(g3), dim3(kRefineThreads), lds, st, \
+                       A64, X, Wws, ni, nb, count, o, maxit, it); }                             \
   } else
-    HDG_REF(4) HDG_REF(8) HDG_REF(16) HDG_REF(32) HDG_REF(64) return hipErrorInvalidValue;
+    HDG_REF(4, true) HDG_REF(8, true) HDG_REF(16, true) HDG_REF(32, true)
+    HDG_REF(4, false) HDG_REF(8, false) HDG_REF(16, false) HDG_REF(32, false)
+    return hipErrorInvalidValue;
 #undef HDG_REF
     e = hipGetLastError();
     if (e != hipSuccess) return e;
@@ -2128,12 +2270,18 @@ hipError_t launch_cell_systems(const Tables& T, const MeshDev& M, int64_t first,
   const int grid = (int)(count < 4096 ? count : 4096);
   if (T.phys == kDarcy) {
     FusedGen<kDarcy> gen{T, M};
-    hipLaunchKernelGGL(cell_systems_kernel<kDarcy>, dim3(grid), dim3(256), 0, st, gen, first,
+    {
+      ClockScope clk_(st, "cell_systems");
+      hipLaunchKernelGGL(cell_systems_kernel<kDarcy>, dim3(grid), dim3(256), 0, st, gen, first,
                        count, A, b);
+    }
   } else {
     FusedGen<kElasticity> gen{T, M};
-    hipLaunchKernelGGL(cell_systems_kernel<kElasticity>, dim3(grid), dim3(256), 0, st, gen, first,
+    {
+      ClockScope clk_(st, "cell_systems");
+      hipLaunchKernelGGL(cell_systems_kernel<kElasticity>, dim3(grid), dim3(256), 0, st, gen, first,
                        count, A, b);
+    }
   }
   return hipGetLastError();
 }

@@ -81,6 +81,18 @@ class Context:
     def synchronize(self):
         check(lib().hdg_ctx_synchronize(self.h), self.h, "synchronize")
 
+    def profile_start(self):
+        """Time every library kernel launch with HIP events until profile_stop."""
+        check(lib().hdg_profile_start(self.h), self.h, "profile_start")
+
+    def profile_stop(self):
+        """{kernel family: (launches, total ms)} since profile_start."""
+        n = C.c_int(0)
+        arr = (_lib.KernelTime * 64)()
+        check(lib().hdg_profile_stop(self.h, 64, arr, C.byref(n)), self.h, "profile_stop")
+        return {arr[i].name.decode(): (arr[i].launches, arr[i].total_ms)
+                for i in range(min(n.value, 64))}
+
     def close(self):
         if self.h:
             lib().hdg_ctx_destroy(self.h)

@@ -44,7 +44,7 @@ EXPORTS = [
     "hdg_condense", "hdg_assemble", "hdg_mesh_singular_count", "hdg_mesh_interpolate_trace",
     "hdg_local_factors", "hdg_batch_create", "hdg_batch_destroy", "hdg_batch_get_classes",
     "hdg_batch_launch", "hdg_batch_condense_assemble", "hdg_batch_recover",
-    "hdg_mesh_refinement_stats",
+    "hdg_mesh_refinement_stats", "hdg_profile_start", "hdg_profile_stop",
 ]
 
 BATCH_AUTOTUNE = 1
@@ -99,6 +99,10 @@ class BatchClass(C.Structure):
         ("nmesh", C.c_int), ("ncells", C.c_int64), ("family", C.c_int),
         ("grid_per_cu", C.c_int), ("tuned_ms", C.c_double),
     ]
+
+
+class KernelTime(C.Structure):
+    _fields_ = [("name", C.c_char * 32), ("launches", C.c_int64), ("total_ms", C.c_double)]
 
 
 class RefinementStats(C.Structure):
@@ -157,6 +161,8 @@ def lib():
         L.hdg_batch_condense_assemble.argtypes = [vp, i32, C.POINTER(i64)]
         L.hdg_batch_recover.argtypes = [vp, C.POINTER(vp), C.POINTER(vp)]
         L.hdg_mesh_refinement_stats.argtypes = [vp, C.POINTER(RefinementStats)]
+        L.hdg_profile_start.argtypes = [vp]
+        L.hdg_profile_stop.argtypes = [vp, i32, C.POINTER(KernelTime), C.POINTER(i32)]
         _ = dbl
         _lib = L
     return _lib

@@ -72,6 +72,19 @@ const char* hdg_last_error(const hdg_ctx* ctx);
 /* The stream all work of the context is enqueued on (a hipStream_t). */
 void* hdg_ctx_stream(hdg_ctx* ctx);
 int hdg_ctx_synchronize(hdg_ctx* ctx);
+/* Kernel timing.  Between hdg_profile_start and hdg_profile_stop every kernel
+ * launch of the library (outside stream capture) is bracketed by two HIP
+ * events on the stream it runs on; hdg_profile_stop synchronises and returns
+ * the launch count and summed duration per kernel family (up to max entries;
+ * *n receives the number of families).  Process-wide: one profile at a time. */
+typedef struct {
+  char name[32];
+  int64_t launches;
+  double total_ms;
+} hdg_kernel_time;
+int hdg_profile_start(hdg_ctx* ctx);
+int hdg_profile_stop(hdg_ctx* ctx, int max, hdg_kernel_time* times, int* n);
+
 /* Copies between host and device on the context stream (synchronous). */
 int hdg_memcpy_d2h(hdg_ctx* ctx, void* host_dst, const void* dev_src, int64_t bytes);
 int hdg_memcpy_h2d(hdg_ctx* ctx, void* dev_dst, const void* host_src, int64_t bytes);
@@ -202,7 +215,7 @@ typedef struct {
   int fp64_fallback;    /* the mesh was redone in FP64 after a non-converged cell */
   int max_sweeps;       /* largest number of FP64 refinement sweeps of a cell */
   double mean_sweeps;
-  int64_t unconverged;  /* cells whose last correction exceeded 1e-7 of W */
+  int64_t unconverged;  /* cells whose last correction exceeded 1e-10 of W */
 } hdg_refinement_stats;
 int hdg_mesh_refinement_stats(hdg_mesh* mesh, hdg_refinement_stats* stats);
 
