@@ -25,6 +25,17 @@ def test_library_loads_and_exports_every_declared_symbol():
     assert sorted(_lib.EXPORTS) == decl
 
 
+def test_julia_shim_binds_only_exported_symbols():
+    """Every libhdg symbol the Julia binding ccalls is declared and exported."""
+    from gridaphybrid import _lib
+    shim = os.path.join(ROOT, "gridaphybrid.jl_amd", "julia", "GridapHybridMI355X.jl")
+    used = set(re.findall(r"ccall\(\(:(hdg_[a-z0-9_]+), libhdg\)", open(shim).read()))
+    assert len(used) >= 8
+    assert used <= set(declared_functions())
+    L = _lib.lib()
+    assert all(hasattr(L, f) for f in used)
+
+
 def test_library_is_built_for_gfx950():
     from gridaphybrid import _lib
     blob = open(_lib.LIB_PATH, "rb").read()
