@@ -639,14 +639,58 @@ int hdg_mesh_create(hdg_ctx* ctx, const hdg_problem* prob, const hdg_mesh_desc* 
   HIP_TRY(ctx, upload(m->d_colptr, colptr, st), "upload colptr");
   std::vector<double> zeros((size_t)std::max<int64_t>(m->ndir, 1), 0.0);
   HIP_TRY(ctx, upload(m->d_lam_dir, zeros, st), "upload lam_dir");
-  // resident outputs
-  HIP_TRY(ctx, own_cell_state(m.get()), "alloc per-cell state");
+  // the per-cell state (S, g, factors, ...) is allocated at the first
+  // condensation, or comes from a batch (hdg_batch_create)
   HIP_TRY(ctx, m->d_rowval.alloc((size_t)std::max<int64_t>(m->nnz, 1) * sizeof(int32_t)), "alloc rowval");
   HIP_TRY(ctx, m->d_nzval.alloc((size_t)std::max<int64_t>(m->nnz, 1) * sizeof(double)), "alloc nzval");
   HIP_TRY(ctx, m->d_rhs.alloc((size_t)std::max<int64_t>(m->nfree, 1) * sizeof(double)), "alloc rhs");
   HIP_TRY(ctx, launch_symbolic(m->pairs(), m->d_rowval.as<int32_t>(), st), "symbolic kernel");
   HIP_TRY(ctx, hipStreamSynchronize(st), "symbolic sync");
   *out = m.release();
+  return HDG_OK;
+}
+
+int hdg_footprint_cartesian(const hdg_problem* prob, const int64_t* cells, int precision,
+                            int64_t* device_bytes) {
+  if (!prob || !cells || !device_bytes) return HDG_ERR_ARG;
+  const int D = prob->dim;
+  if (D != 2 && D != 3) return HDG_ERR_ARG;
+  TablesHost th;
+  std::string err;
+  if (build_tables(prob->physics, D, prob->space, prob->k_flux, prob->k_pot, prob->k_trace,
+                   prob->quad_degree, th, err))
+    return HDG_ERR_ARG;
+  int64_t nc = 1, nfac = 0, nbnd = 0;
+  for (int a = 0; a < D; ++a) {
+    if (cells[a] <= 0) return HDG_ERR_ARG;
+    nc *= cells[a];
+  }
+  for (int a = 0; a < D; ++a) {
+    int64_t across = 1;
+    for (int b = 0; b < D; ++b)
+      if (b != a) across *= cells[b];
+    nfac += (cells[a] + 1) * across;
+    nbnd += 2 * across;
+  }
+  const int64_t nff = nfac - nbnd;               // every boundary facet is Dirichlet
+  const int64_t ntf = th.ntf, nb = th.nb, ni = th.ni;
+  const int64_t nfree = nff * ntf, ndir = nbnd * ntf;
+  const int64_t npairs = nff * (4 * D - 1);      // facets coupled to a free facet (bound)
+  const int64_t nnz = npairs * ntf * ntf;
+  const CellLayout L(th);
+  int64_t b = 0;
+  b += nc * (D + D * D + 1) * 8 + nc * 2 * D + nc * nb * 4;              // geometry, dofs
+  b += std::max<int64_t>(ndir, 1) * 8;                                   // Dirichlet values
+  b += (nff + 1) * 12 + npairs * (4 + 8 + 2) + nff * (8 + 2);           // facet-pair table
+  b += (nfree + 1) * 8 + nnz * (4 + 8) + nfree * 8;                      // CSC + rhs
+  b += nc * (int64_t)(L.S + L.g + L.U + L.Y + L.alpha + L.srcq) * 8 + nc * 8;  // per cell
+  b += (int64_t)1 << 20;                                                 // reference tables
+  if (precision == HDG_MIXED && mixed_supported(th.ni, th.nb))
+    b += (int64_t)mixed_workspace_bytes(th.ni, th.nb, nc);
+  else
+    b += (int64_t)condense_workspace_bytes(th.ni, th.nb, -1);
+  (void)ni;
+  *device_bytes = b;
   return HDG_OK;
 }
 
@@ -868,6 +912,7 @@ int hdg_assemble(hdg_mesh* m) {
 
 int hdg_mesh_singular_count(hdg_mesh* m, int64_t* n_singular) {
   if (!m || !n_singular) return HDG_ERR_ARG;
+  if (!m->condensed) return fail(m->ctx, HDG_ERR_STATE, "no condensation yet");
   HIP_TRY(m->ctx, hipSetDevice(m->ctx->device), "hipSetDevice");
   return count_singular(m->ctx, m->pinfo, m->ncells, n_singular);
 }
@@ -1262,11 +1307,10 @@ int hdg_batch_destroy(hdg_batch* b) {
   for (hdg_mesh* m : b->meshes) {
     m->batch = nullptr;
     m->condensed = false;
+    // own per-cell state is allocated again at the next condensation
     m->pS = m->pg = m->pU = m->pY = m->palpha = m->psrcq = nullptr;
     m->pinfo = m->piters = nullptr;
     m->mixed = false;
-    hipError_t e = own_cell_state(m);
-    if (e != hipSuccess && rc == HDG_OK) rc = hipfail(ctx, e, "alloc per-cell state");
   }
   delete b;
   return rc;

@@ -45,6 +45,7 @@ EXPORTS = [
     "hdg_local_factors", "hdg_batch_create", "hdg_batch_destroy", "hdg_batch_get_classes",
     "hdg_batch_launch", "hdg_batch_condense_assemble", "hdg_batch_recover",
     "hdg_mesh_refinement_stats", "hdg_profile_start", "hdg_profile_stop",
+    "hdg_footprint_cartesian",
 ]
 
 BATCH_AUTOTUNE = 1
@@ -162,6 +163,7 @@ def lib():
         L.hdg_batch_recover.argtypes = [vp, C.POINTER(vp), C.POINTER(vp)]
         L.hdg_mesh_refinement_stats.argtypes = [vp, C.POINTER(RefinementStats)]
         L.hdg_profile_start.argtypes = [vp]
+        L.hdg_footprint_cartesian.argtypes = [C.POINTER(Problem), vp, i32, C.POINTER(i64)]
         L.hdg_profile_stop.argtypes = [vp, i32, C.POINTER(KernelTime), C.POINTER(i32)]
         _ = dbl
         _lib = L

@@ -1,0 +1,274 @@
+"""Continuous-batching request loop for HDG solves.
+
+Requests are independent HDG solves (Darcy or elasticity, 2D or 3D, orders
+1..3, Cartesian meshes of different sizes).  The loop
+
+  * admits waiting requests in arrival order while their device footprint
+    (hdg_footprint_cartesian, computed on the host before anything is
+    allocated) fits the free HBM budget;
+  * condenses and assembles every request admitted in one round with one
+    batch-graph launch (hdg_batch_launch: one condensation launch per class of
+    equal local systems), asynchronously, on one of several contexts (HIP
+    streams), so rounds admitted while others are on the GPU run beside them;
+  * keeps each request's mesh, dof maps and factors resident until its
+    recovery (hdg_batch_recover) has run, then copies the free values back and
+    retires the request.
+
+The skeleton solve of a request is either the host sparse direct solve
+(``skeleton="spsolve"``, the reference's solve) or, for workloads where that
+host solve would dominate, the interpolant of the manufactured trace
+(``skeleton="interpolant"``), as in the 64^3 benchmark.
+
+Freeing device memory synchronises the device, so retired requests are
+released only when no round is in flight (or when admission needs the space).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import time
+from collections import deque
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from ._lib import FP64, MIXED, check, lib
+
+
+@dataclass
+class SolveRequest:
+    rid: int
+    arrival: float          # seconds after the start of the run
+    physics: int            # 0 Darcy, 1 elasticity
+    dim: int
+    order: int
+    n: int                  # cells per direction
+    seed: int
+    admitted: float = None
+    done: float = None
+    cells: int = 0
+    footprint: int = 0
+    result: np.ndarray = None
+    sweeps: float = 0.0
+    fp64_fallback: bool = False
+    extra: dict = field(default_factory=dict)
+
+    @property
+    def latency(self):
+        return None if self.done is None else self.done - self.arrival
+
+
+def draw_requests(count, rate, seed, start=0.0):
+    """Seeded Poisson stream of solves with the uneven-batch distribution:
+    half Darcy (2D with n in [8,128] or 3D with n in [4,32]), half 3D
+    elasticity (n in [4,32]); order uniform in {1,2,3}."""
+    rng = np.random.default_rng(seed)
+    t = start + np.cumsum(rng.exponential(1.0 / rate, count))
+    out = []
+    for i in range(count):
+        phys = int(rng.integers(0, 2))
+        D = int(rng.integers(2, 4)) if phys == 0 else 3
+        n = int(rng.integers(8, 129)) if D == 2 else int(rng.integers(4, 33))
+        k = int(rng.integers(1, 4))
+        out.append(SolveRequest(i, float(t[i]), phys, D, k, n, int(rng.integers(0, 2 ** 31 - 1))))
+    return out
+
+
+def _problem(req):
+    el = req.physics == 1
+    src = _lib.SOURCE_ELAS_SIN if el else _lib.SOURCE_DARCY_SINPROD
+    return _lib.Problem(int(req.physics), req.dim, _lib.SPACE_P if el else _lib.SPACE_Q, req.order,
+                        req.order + 1 if el else req.order, req.order, 2 * (req.order + 1),
+                        float(req.n) if el else 1.0, 1.0, src)
+
+
+def footprint(req, precision):
+    prob = _problem(req)
+    cells = (C.c_int64 * 3)(*([req.n] * req.dim + [1] * (3 - req.dim)))
+    b = C.c_int64(0)
+    check(lib().hdg_footprint_cartesian(C.byref(prob), cells, int(precision), C.byref(b)), None,
+          "footprint")
+    return int(b.value)
+
+
+def build_operator(gh, ctx, req):
+    """HybridAffineFEOperator of a request (not yet condensed): kappa
+    lognormal(0,1) per cell (Darcy) or nu uniform in [0.2,0.4] (elasticity)
+    from the request's seed."""
+    D, n = req.dim, req.n
+    model = gh.CartesianDiscreteModel(tuple([0.0, 1.0] * D), tuple([n] * D))
+    rng = np.random.default_rng(req.seed)
+    nc = n ** D
+    if req.physics == 0:
+        X = gh.HDGSpaces.darcy_baseline(model, req.order)
+        wf = gh.DarcyHDG(tau=1.0, kappa=np.exp(rng.standard_normal(nc)),
+                         source=_lib.SOURCE_DARCY_SINPROD)
+    else:
+        X = gh.HDGSpaces.elasticity(model, req.order)
+        wf = gh.ElasticityHDG(tau=float(n), nu=rng.uniform(0.2, 0.4, nc), E=1.0,
+                              source=_lib.SOURCE_ELAS_SIN)
+    return gh.HybridAffineFEOperator(wf, X, X, [1, 2], [3], ctx=ctx, assemble=False)
+
+
+class _Round:
+    """Requests admitted together: one batch graph on one context."""
+
+    def __init__(self, server, ctx, reqs, now):
+        import torch
+        gh = server.gh
+        self.server, self.ctx, self.reqs = server, ctx, reqs
+        self.ops = [build_operator(gh, ctx, r) for r in reqs]
+        for r, op in zip(reqs, self.ops):
+            r.admitted = now
+            r.cells = op.info.ncells
+        self.batch = gh.HybridOperatorBatch(self.ops, autotune=False)
+        self.batch.launch(server.precision)
+        self.ev = torch.cuda.Event()
+        self.ev.record(ctx.torch_stream)
+        self.stage = "condense"
+        self.bytes = sum(r.footprint for r in reqs)
+
+    def advance(self, now):
+        """Moves the round on when its GPU work is done; True when finished."""
+        import torch
+        if not self.ev.query():
+            return False
+        ctx = self.ctx
+        if self.stage == "condense":
+            lams, outs = [], []
+            with ctx.stream():
+                for op in self.ops:
+                    if self.server.skeleton == "spsolve" and op.info.nfree:
+                        import scipy.sparse.linalg as spla
+                        K, rhs = op.skeleton_system()
+                        lam = spla.spsolve(K.tocsc(), rhs)
+                    else:
+                        lam = op.interpolate_trace()
+                    lams.append(torch.as_tensor(lam, dtype=torch.float64, device="cuda"))
+                    outs.append(torch.empty(op.info.nfree_total, dtype=torch.float64,
+                                            device="cuda"))
+                self.batch.recover(lams, outs)
+                self.outs = outs
+                self.ev = torch.cuda.Event()
+                self.ev.record(ctx.torch_stream)
+            self.lams = lams
+            self.stage = "recover"
+            return False
+        for r, op, out in zip(self.reqs, self.ops, self.outs):
+            r.result = out.cpu().numpy()
+            st = op.refinement_stats()
+            r.sweeps = st["mean_sweeps"]
+            if st["unconverged"]:
+                # the mixed refinement did not converge somewhere: redo in FP64
+                check(lib().hdg_condense_assemble(op.mesh, int(FP64), None), ctx.h, "fallback")
+                r.result = op.recover(self.lams[self.ops.index(op)]).cpu().numpy()
+                r.fp64_fallback = True
+            r.done = self.server.clock()
+        return True
+
+    def release(self):
+        self.batch.close()
+        for op in self.ops:
+            op.close()
+
+
+class Server:
+    """Continuous-batching HDG solve server on one GPU."""
+
+    def __init__(self, gh, hbm_budget=200 << 30, precision=MIXED, streams=2, max_batch=64,
+                 skeleton="interpolant", batching=True):
+        self.gh = gh
+        self.ctxs = [gh.Context(0) for _ in range(streams)]
+        self.budget = int(hbm_budget)
+        self.precision = precision
+        self.max_batch = max_batch if batching else 1
+        self.skeleton = skeleton
+        self.t0 = None
+
+    def clock(self):
+        return time.perf_counter() - self.t0
+
+    def run(self, requests, progress=None):
+        """Serves `requests` (arrival times relative to the start) and returns
+        them with latencies filled in; requests whose footprint exceeds the
+        whole budget are rejected (done stays None)."""
+        pending = deque(sorted(requests, key=lambda r: r.arrival))
+        waiting = deque()
+        inflight, retired = [], []
+        used = 0
+        busy = set()
+        self.rejected = []
+        self.rounds = []
+        self.t0 = time.perf_counter()
+        last_report = 0.0
+        while pending or waiting or inflight:
+            now = self.clock()
+            while pending and pending[0].arrival <= now:
+                r = pending.popleft()
+                r.footprint = footprint(r, self.precision)
+                if r.footprint > self.budget:
+                    self.rejected.append(r)
+                else:
+                    waiting.append(r)
+            progressed = False
+            for rd in list(inflight):
+                if rd.advance(self.clock()):
+                    inflight.remove(rd)
+                    busy.discard(id(rd.ctx))
+                    retired.append(rd)
+                    progressed = True
+            if not inflight and retired:
+                for rd in retired:
+                    rd.release()
+                    used -= rd.bytes
+                retired = []
+            free = [c for c in self.ctxs if id(c) not in busy]
+            if waiting and free:
+                take, need = [], 0
+                while (waiting and len(take) < self.max_batch and
+                       used + need + waiting[0].footprint <= self.budget):
+                    r = waiting.popleft()
+                    take.append(r)
+                    need += r.footprint
+                if not take and retired and not inflight:
+                    continue  # space is released at the top of the next pass
+                if take:
+                    ctx = free[0]
+                    rd = _Round(self, ctx, take, self.clock())
+                    self.rounds.append(len(take))
+                    inflight.append(rd)
+                    busy.add(id(ctx))
+                    used += rd.bytes
+                    progressed = True
+            if progress and self.clock() - last_report > 5.0:
+                last_report = self.clock()
+                progress(f"t={last_report:.1f}s done={sum(r.done is not None for r in requests)} "
+                         f"waiting={len(waiting)} inflight={len(inflight)}")
+            if not progressed:
+                wait = 2e-4
+                if pending and not waiting and not inflight:
+                    wait = max(0.0, min(0.05, pending[0].arrival - self.clock()))
+                time.sleep(wait)
+        for rd in retired:
+            rd.release()
+        for c in self.ctxs:
+            c.synchronize()
+        return requests
+
+    def close(self):
+        for c in self.ctxs:
+            c.close()
+
+
+def summarize(requests, wall=None):
+    lat = np.array([r.latency for r in requests if r.latency is not None])
+    cells = sum(r.cells for r in requests if r.done is not None)
+    done = [r for r in requests if r.done is not None]
+    span = wall if wall is not None else (max(r.done for r in done) - min(r.arrival for r in done))
+    return dict(requests=len(done), rejected=len(requests) - len(done),
+                p50_ms=float(np.percentile(lat, 50) * 1e3) if len(lat) else None,
+                p99_ms=float(np.percentile(lat, 99) * 1e3) if len(lat) else None,
+                mean_ms=float(lat.mean() * 1e3) if len(lat) else None,
+                cells=int(cells), cells_per_s=cells / span if span > 0 else None,
+                span_s=span,
+                fp64_fallbacks=sum(r.fp64_fallback for r in done))

@@ -188,6 +188,13 @@ int hdg_mesh_create(hdg_ctx* ctx, const hdg_problem* prob, const hdg_mesh_desc* 
 int hdg_mesh_create_cartesian(hdg_ctx* ctx, const hdg_problem* prob, const int64_t* cells,
                               const double* domain, const double* coef, hdg_mesh** out);
 int hdg_mesh_destroy(hdg_mesh* mesh);
+/* Upper bound of the device bytes hdg_mesh_create_cartesian + a condensation
+ * in `precision` would hold for this problem and partition (per-cell state,
+ * skeleton pattern bound, dof maps, workspaces), computed on the host without
+ * touching the device: the admission test of a serving loop.  No reference
+ * counterpart (Gridap allocates lazily). */
+int hdg_footprint_cartesian(const hdg_problem* prob, const int64_t* cells, int precision,
+                            int64_t* device_bytes);
 int hdg_mesh_get_info(const hdg_mesh* mesh, hdg_mesh_info* info);
 
 /* Dirichlet trace values, host [ndir] (Gridap's dirichlet_values of the trial
