@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -555,8 +556,10 @@ int hdg_mesh_create(hdg_ctx* ctx, const hdg_problem* prob, const hdg_mesh_desc* 
     size_t nd = 0;
     for (auto* v : dv) nd += v->size();
     size_t bytes = nd * sizeof(double) + th.prog_val.size() * sizeof(double) +
-                   th.prog_off.size() * sizeof(int32_t) + th.pmask.size() + th.qmask.size() +
-                   th.prog_id.size() + 16;
+                   (th.prog_val0.size() + th.prog_xval.size()) * sizeof(double) +
+                   th.prog_off.size() * sizeof(int32_t) + th.prog_head.size() * sizeof(uint32_t) +
+                   th.pmask.size() + th.qmask.size() + th.prog_id.size() + th.prog_xid.size() +
+                   16;
     std::vector<char> host(bytes, 0);
     HIP_TRY(ctx, m->tables.alloc(bytes), "alloc tables");
     size_t off = 0;
@@ -569,9 +572,18 @@ int hdg_mesh_create(hdg_ctx* ctx, const hdg_problem* prob, const hdg_mesh_desc* 
     size_t off_pv = off;
     std::memcpy(host.data() + off, th.prog_val.data(), th.prog_val.size() * sizeof(double));
     off += th.prog_val.size() * sizeof(double);
+    size_t off_v0 = off;
+    std::memcpy(host.data() + off, th.prog_val0.data(), th.prog_val0.size() * sizeof(double));
+    off += th.prog_val0.size() * sizeof(double);
+    size_t off_xv = off;
+    std::memcpy(host.data() + off, th.prog_xval.data(), th.prog_xval.size() * sizeof(double));
+    off += th.prog_xval.size() * sizeof(double);
     size_t off_po = off;
     std::memcpy(host.data() + off, th.prog_off.data(), th.prog_off.size() * sizeof(int32_t));
     off += th.prog_off.size() * sizeof(int32_t);
+    size_t off_ph = off;
+    std::memcpy(host.data() + off, th.prog_head.data(), th.prog_head.size() * sizeof(uint32_t));
+    off += th.prog_head.size() * sizeof(uint32_t);
     size_t off_pm = off;
     std::memcpy(host.data() + off, th.pmask.data(), th.pmask.size());
     off += th.pmask.size();
@@ -580,6 +592,9 @@ int hdg_mesh_create(hdg_ctx* ctx, const hdg_problem* prob, const hdg_mesh_desc* 
     off += th.qmask.size();
     size_t off_pi = off;
     std::memcpy(host.data() + off, th.prog_id.data(), th.prog_id.size());
+    off += th.prog_id.size();
+    size_t off_xi = off;
+    std::memcpy(host.data() + off, th.prog_xid.data(), th.prog_xid.size());
     HIP_TRY(ctx, hipMemcpyAsync(m->tables.p, host.data(), bytes, hipMemcpyHostToDevice, st),
             "upload tables");
     HIP_TRY(ctx, hipStreamSynchronize(st), "sync");
@@ -610,6 +625,10 @@ int hdg_mesh_create(hdg_ctx* ctx, const hdg_problem* prob, const hdg_mesh_desc* 
     T.prog_val = reinterpret_cast<const double*>(base + off_pv);
     T.prog_off = reinterpret_cast<const int32_t*>(base + off_po);
     T.prog_id = reinterpret_cast<const uint8_t*>(base + off_pi);
+    T.prog_head = reinterpret_cast<const uint32_t*>(base + off_ph);
+    T.prog_val0 = reinterpret_cast<const double*>(base + off_v0);
+    T.prog_xid = reinterpret_cast<const uint8_t*>(base + off_xi);
+    T.prog_xval = reinterpret_cast<const double*>(base + off_xv);
   }
   m->x0.assign(desc->x0, desc->x0 + nc * D);
   m->jac.assign(desc->jac, desc->jac + nc * D * D);
@@ -1047,6 +1066,7 @@ struct hdg_batch {
   hdg_ctx* ctx = nullptr;
   std::vector<hdg_mesh*> meshes;
   std::vector<std::unique_ptr<Class>> classes;
+  bool tune_log = false;  // HDG_AUTOTUNE_LOG set: print every candidate's time
   Workspace ws;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -1158,6 +1178,9 @@ int autotune(hdg_batch* b) {
           break;
         }
         ms *= 0.5f;
+        if (b->tune_log)
+          std::fprintf(stderr, "hdg autotune: ni=%d nb=%d family=%d grid_per_cu=%d %.3f ms / %lld cells\n",
+                       c.info.ni, c.info.nb, fams[i], gms[j], ms, (long long)sample);
         if (best == 0.f || ms < best) {
           best = ms;
           c.info.family = fams[i];
@@ -1211,6 +1234,7 @@ int hdg_batch_create(hdg_ctx* ctx, int nmesh, hdg_mesh* const* meshes, int flags
   if (flags & ~HDG_BATCH_AUTOTUNE) return fail(ctx, HDG_ERR_ARG, "unknown batch flags");
   std::unique_ptr<hdg_batch> b(new hdg_batch());
   b->ctx = ctx;
+  b->tune_log = std::getenv("HDG_AUTOTUNE_LOG") != nullptr;
   for (int i = 0; i < nmesh; ++i) {
     hdg_mesh* m = meshes[i];
     if (!m || m->ctx != ctx) return fail(ctx, HDG_ERR_ARG, "mesh of another context");

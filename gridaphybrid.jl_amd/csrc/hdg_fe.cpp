@@ -21,6 +21,7 @@
 
 namespace hdg {
 void build_program(TablesHost& t);
+int inline_program(TablesHost& t, std::string& err);
 namespace {
 
 // Gauss-Legendre nodes/weights on [0,1]: Newton on P_n from the asymptotic
@@ -402,7 +403,7 @@ int build_tables(int phys, int D, int space, int kp, int kq, int km, int quad_de
   for (int i = 0; i < nm; ++i)
     for (int a = 0; a < D - 1; ++a) t.trace_nodes[(size_t)i * (D - 1) + a] = bm.node(i, a);
   build_program(t);
-  return 0;
+  return inline_program(t, err);
 }
 
 // ---------------------------------------------------------------------------
@@ -573,6 +574,37 @@ void build_program(TablesHost& t) {
         t.prog_off.push_back((int32_t)t.prog_id.size());
       }
   }
+}
+
+// Inline-first-term form of the entry program (prog_head / prog_val0 /
+// prog_xid / prog_xval), same terms in the same order.
+int inline_program(TablesHost& t, std::string& err) {
+  const size_t ne = t.prog_off.empty() ? 0 : t.prog_off.size() - 1;
+  t.prog_head.assign(ne, 0u);
+  t.prog_val0.assign(ne, 0.0);
+  t.prog_xid.clear();
+  t.prog_xval.clear();
+  for (size_t e = 0; e < ne; ++e) {
+    const int k0 = t.prog_off[e], k1 = t.prog_off[e + 1], cnt = k1 - k0;
+    if (cnt > 15) {
+      err = "entry program term count above 15";
+      return 1;
+    }
+    const size_t x = t.prog_xid.size();
+    if (x >= (1u << 20)) {
+      err = "entry program too long";
+      return 1;
+    }
+    if (cnt > 0) {
+      t.prog_head[e] = (uint32_t)t.prog_id[k0] | ((uint32_t)cnt << 8) | ((uint32_t)x << 12);
+      t.prog_val0[e] = t.prog_val[k0];
+      for (int k = k0 + 1; k < k1; ++k) {
+        t.prog_xid.push_back(t.prog_id[k]);
+        t.prog_xval.push_back(t.prog_val[k]);
+      }
+    }
+  }
+  return 0;
 }
 
 }  // namespace hdg

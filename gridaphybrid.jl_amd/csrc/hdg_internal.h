@@ -43,6 +43,13 @@ struct TablesHost {
   std::vector<int32_t> prog_off;  // [n*n+1]
   std::vector<uint8_t> prog_id;   // [nterms] coefficient index
   std::vector<double> prog_val;   // [nterms] reference-integral value
+  // the same program with the first term of each entry inline (one load of
+  // head and one of val0 per entry; most entries have at most one term):
+  // head = id0 | count << 8 | (offset of the further terms in x*) << 12
+  std::vector<uint32_t> prog_head;  // [n*n]
+  std::vector<double> prog_val0;    // [n*n]
+  std::vector<uint8_t> prog_xid;    // further terms
+  std::vector<double> prog_xval;
 };
 
 // Device view of the tables (pointers into one device allocation).
@@ -68,6 +75,10 @@ struct Tables {
   const int32_t* prog_off;
   const uint8_t* prog_id;
   const double* prog_val;
+  const uint32_t* prog_head;
+  const double* prog_val0;
+  const uint8_t* prog_xid;
+  const double* prog_xval;
 };
 
 // Device view of a mesh for the fused element generator.

@@ -272,9 +272,14 @@ struct FusedGen {
     const int n = T.n;
     if (col < n) {
       const int e = col * n + row;
-      const int k0 = T.prog_off[e], k1 = T.prog_off[e + 1];
-      double acc = 0.0;
-      for (int k = k0; k < k1; ++k) acc = fma(sh.alpha[T.prog_id[k]], T.prog_val[k], acc);
+      const uint32_t h = T.prog_head[e];
+      const double v0 = T.prog_val0[e];
+      const int cnt = (h >> 8) & 15;
+      if (cnt == 0) return 0.0;
+      double acc = fma(sh.alpha[h & 255], v0, 0.0);
+      const int x0 = (int)(h >> 12);
+      for (int k = 0; k < cnt - 1; ++k)
+        acc = fma(sh.alpha[T.prog_xid[x0 + k]], T.prog_xval[x0 + k], acc);
       return acc;
     }
     return rhs(sh, row);
@@ -1247,6 +1252,461 @@ __global__ void __launch_bounds__(256)
 }
 
 // --------------------------------------------------------------------------
+// Blocked elimination with MFMA trailing updates (64 < n <= NMAX).  The cell
+// system lives row-major in a per-workgroup HBM workspace (ld = ldc).  Per
+// panel of 16 interior columns: the panel is copied to LDS and factored by
+// wavefront 0 alone (pivot search by DPP reductions, getrf's choice and
+// LAPACK position bookkeeping, no workgroup barriers), the pivot rows are
+// brought to U form over the trailing columns (U12, one column per thread),
+// and the trailing matrix gets the rank-16 update C -= L21 U12 from
+// v_mfma_f64_16x16x4f64 tiles (rows of eliminated pivots carry zero
+// multipliers, so every tile is updated in full).
+// --------------------------------------------------------------------------
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+
+__device__ inline void wave_sync_lds() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <int NMAX, class Gen, bool BACKSUB>
+__global__ void __launch_bounds__(256)
+    condense_panel_kernel(Gen gen, CondenseOut out, int64_t ncells, int ni, int nb,
+                          double* __restrict__ work, int ldc) {
+  constexpr int NT = 256, KB = 16, LDP = NMAX + 16;
+  const int n = ni + nb, ncol = n + 1;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  double* M = work + (int64_t)blockIdx.x * (int64_t)n * ldc;  // M[row * ldc + col]
+
+  __shared__ double s_P[KB][LDP];  // panel columns (all rows), then multipliers
+  __shared__ double s_U[KB][LDP];  // U12: pivot rows x trailing columns
+  __shared__ int s_pos[NMAX], s_at[NMAX];
+  __shared__ unsigned char s_act[NMAX];
+  __shared__ int s_piv[KB];
+  __shared__ double s_diag[NMAX];
+  __shared__ int s_info;
+  __shared__ typename Gen::Shared s_gen;
+
+  for (int64_t cell = blockIdx.x; cell < ncells; cell += gridDim.x) {
+    gen.begin(cell, s_gen);
+    for (int i = tid; i < NMAX; i += NT) {
+      s_pos[i] = i;
+      s_at[i] = i;
+      s_act[i] = i < n ? 1 : 0;
+    }
+    if (tid == 0) s_info = 0;
+    __syncthreads();
+    for (int64_t t = tid; t < (int64_t)n * ncol; t += NT) {
+      const int row = (int)(t / ncol), col = (int)(t - (int64_t)row * ncol);
+      M[(int64_t)row * ldc + col] = GenCall<Gen>::entry(gen, s_gen, row, col, cell);
+    }
+    __syncthreads();
+
+    for (int j0 = 0; j0 < ni; j0 += KB) {
+      const int kb = min(KB, ni - j0);
+      const int jt = j0 + kb;  // first trailing column
+      for (int t = tid; t < n * KB; t += NT) {
+        const int row = t / KB, cc = t - row * KB;
+        s_P[cc][row] = cc < kb ? M[(int64_t)row * ldc + j0 + cc] : 0.0;
+      }
+      __syncthreads();
+      // ---- factor the panel: wavefront 0 only ----
+      if (wv == 0) {
+        for (int jj = 0; jj < kb; ++jj) {
+          const int j = j0 + jj;
+          int64_t bkey = -1;
+          int bpos = INT_MAX;
+          for (int row = lane; row < ni; row += 64) {
+            if (!s_act[row]) continue;
+            const int64_t key = dbits(fabs(s_P[jj][row]));
+            const int pr = s_pos[row];
+            if (key > bkey || (key == bkey && pr < bpos)) {
+              bkey = key;
+              bpos = pr;
+            }
+          }
+          const int64_t wm = wave_max_i64(bkey);
+          const int key = wave_min_i32(bkey == wm ? bpos : INT_MAX);
+          const int p = s_at[key];
+          const double pv = s_P[jj][p];
+          wave_sync_lds();
+          if (lane == 0) {
+            s_piv[jj] = p;
+            s_diag[j] = pv;
+            if (pv == 0.0 && s_info == 0) s_info = j + 1;
+            s_act[p] = 0;
+            const int q = s_at[j];
+            s_at[key] = q;
+            s_pos[q] = key;
+            s_at[j] = p;
+            s_pos[p] = j;
+          }
+          wave_sync_lds();
+          const double rinv = pv != 0.0 ? 1.0 / pv : 0.0;
+          for (int row = lane; row < n; row += 64) {
+            if (!s_act[row]) continue;
+            const double l = s_P[jj][row] * rinv;
+            s_P[jj][row] = l;
+            for (int cc = jj + 1; cc < kb; ++cc) s_P[cc][row] = fma(-l, s_P[cc][p], s_P[cc][row]);
+          }
+          wave_sync_lds();
+        }
+      }
+      __syncthreads();
+      // ---- pivot rows: panel part to the outputs, trailing part -> U12 ----
+      for (int t = tid; t < kb * KB; t += NT) {
+        const int jj = t / KB, cc = t - jj * KB;
+        if (cc >= kb || cc < jj) continue;
+        const int col = j0 + cc, k = j0 + jj;
+        const double v = s_P[cc][s_piv[jj]];
+        if (out.U) out.U[cell * ((int64_t)ni * (ni + 1) / 2) + col * (col + 1) / 2 + k] = v;
+        if (BACKSUB) M[(int64_t)s_piv[jj] * ldc + col] = v;
+      }
+      for (int c = jt + tid; c < ncol; c += NT) {
+        double u[KB], mv[KB];
+#pragma unroll
+        for (int jj = 0; jj < KB; ++jj)  // all loads in flight before the substitution
+          mv[jj] = jj < kb ? M[(int64_t)s_piv[jj] * ldc + c] : 0.0;
+#pragma unroll
+        for (int jj = 0; jj < KB; ++jj) {
+          if (jj < kb) {
+            const int pr = s_piv[jj];
+            double v = mv[jj];
+            for (int ii = 0; ii < jj; ++ii) v = fma(-s_P[ii][pr], u[ii], v);
+            u[jj] = v;
+            s_U[jj][c - jt] = v;
+            const int k = j0 + jj;
+            if (c < ni) {
+              if (out.U) out.U[cell * ((int64_t)ni * (ni + 1) / 2) + c * (c + 1) / 2 + k] = v;
+            } else if (out.Y) {
+              out.Y[(cell * (nb + 1) + (c - ni)) * ni + k] = v;
+            }
+            if (BACKSUB) M[(int64_t)pr * ldc + c] = v;
+          }
+        }
+      }
+      __syncthreads();
+      // ---- trailing update C -= L21 U12 with FP64 MFMA tiles ----
+      const int ntr = ncol - jt;
+      const int rtiles = (n + 15) >> 4, ctiles = (ntr + 15) >> 4;
+      const int i16 = lane & 15, q4 = lane >> 4;
+      const int ntile = rtiles * ctiles;
+      // groups of TG tiles per wavefront: every load of a group is in flight
+      // before its MFMAs
+      constexpr int TG = 4;
+      for (int t0 = wv * TG; t0 < ntile; t0 += (NT / 64) * TG) {
+        dbl4 acc[TG];
+        bool live[TG];
+#pragma unroll
+        for (int g = 0; g < TG; ++g) {
+          const int tile = t0 + g;
+          const int r0 = (tile % rtiles) << 4, c0 = (tile / rtiles) << 4;
+          const int ra = r0 + i16, cc = c0 + i16;
+          live[g] = tile < ntile && __ballot(ra < n && s_act[ra]) != 0;
+          const bool cok = cc < ntr;
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int r = r0 + q4 + 4 * v;
+            acc[g][v] = (live[g] && cok && r < n) ? M[(int64_t)r * ldc + jt + cc] : 0.0;
+          }
+        }
+#pragma unroll
+        for (int g = 0; g < TG; ++g) {
+          if (!live[g]) continue;
+          const int tile = t0 + g;
+          const int r0 = (tile % rtiles) << 4, c0 = (tile / rtiles) << 4;
+          const int ra = r0 + i16, cc = c0 + i16;
+          const bool aok = ra < n && s_act[ra], cok = cc < ntr;
+#pragma unroll
+          for (int k4 = 0; k4 < KB / 4; ++k4) {
+            const int k = 4 * k4 + q4;
+            const double a = (aok && k < kb) ? -s_P[k][ra] : 0.0;
+            const double b = (cok && k < kb) ? s_U[k][cc] : 0.0;
+            acc[g] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[g], 0, 0, 0);
+          }
+        }
+#pragma unroll
+        for (int g = 0; g < TG; ++g) {
+          if (!live[g]) continue;
+          const int tile = t0 + g;
+          const int r0 = (tile % rtiles) << 4, c0 = (tile / rtiles) << 4;
+          const int cc = c0 + i16;
+          if (cc >= ntr) continue;
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int r = r0 + q4 + 4 * v;
+            if (r < n) M[(int64_t)r * ldc + jt + cc] = acc[g][v];
+          }
+        }
+      }
+      __syncthreads();
+    }
+
+    // ---- back substitution (backward map only: single rhs column) -------
+    if (BACKSUB) {
+      if (tid == 0) {
+        for (int k = ni - 1; k >= 0; --k) {
+          const int p = s_at[k];
+          double x = M[(int64_t)p * ldc + n];
+          for (int c = k + 1; c < ni; ++c)
+            x = fma(-M[(int64_t)p * ldc + c], M[(int64_t)s_at[c] * ldc + n], x);
+          const double d = s_diag[k];
+          x = d != 0.0 ? x / d : 0.0;
+          M[(int64_t)p * ldc + n] = x;
+          if (out.X) out.X[cell * ni + k] = x;
+        }
+      }
+      __syncthreads();
+    }
+    // ---- boundary rows: S and g ----
+    for (int64_t t = tid; t < (int64_t)nb * (nb + 1); t += NT) {
+      const int cb = (int)(t / nb), rb = (int)(t - (int64_t)cb * nb);
+      const double v = M[(int64_t)(ni + rb) * ldc + ni + cb];
+      if (cb < nb) {
+        if (out.S) out.S[(cell * nb + cb) * nb + rb] = v;
+      } else if (out.g) {
+        out.g[cell * nb + rb] = v;
+      }
+    }
+    if (tid == 0 && out.info) out.info[cell] = s_info;
+    __syncthreads();
+  }
+}
+
+// --------------------------------------------------------------------------
+// Register-resident blocked elimination (n <= 16 * 11): the cell system is a
+// grid of 16x16 FP64 tiles held in MFMA accumulator layout by the 16
+// wavefronts of the workgroup (tile t belongs to wavefront t % 16, register
+// slot t / 16; lane l, register v of a tile hold row 16 rt + l/16 + 4v,
+// column 16 ct + l%16).  Per panel of 16 interior columns: the panel's tile
+// column goes to LDS and is factored by wavefront 0 (getrf's pivots, LAPACK
+// position bookkeeping); the pivot rows' trailing parts are gathered from
+// the tiles into LDS and brought to U form (U12); every tile right of the
+// panel then takes C -= L21 U12 through v_mfma_f64_16x16x4f64.  Nothing of
+// the cell system goes through HBM; S and g are written from the tiles.
+// --------------------------------------------------------------------------
+constexpr int kTileMaxN = 16 * 11 - 1;  // n + 1 columns in at most 11 tile columns
+
+__device__ inline double dbl4_at(const dbl4& a, int v) {
+  return v == 0 ? a[0] : (v == 1 ? a[1] : (v == 2 ? a[2] : a[3]));
+}
+
+constexpr int kTileThreads = 512;
+
+template <int TPW, class Gen>
+__global__ void __launch_bounds__(kTileThreads, 1)
+    condense_tile_kernel(Gen gen, CondenseOut out, int64_t ncells, int ni, int nb) {
+  constexpr int NT = kTileThreads, NW = NT / 64, KB = 16, NMAX = 16 * 11, LDP = NMAX + 16;
+  const int n = ni + nb, ncol = n + 1;
+  const int RT = (n + 15) >> 4, CT = (ncol + 15) >> 4, ntiles = RT * CT;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int i16 = lane & 15, q4 = lane >> 4;
+
+  __shared__ double s_P[KB][LDP];  // panel columns (all rows), then multipliers
+  __shared__ double s_U[KB][LDP];  // pivot rows x trailing columns (U12)
+  __shared__ double s_stage[NW][4][64];  // one generated tile per wavefront
+  __shared__ int s_pos[NMAX], s_at[NMAX];
+  __shared__ unsigned char s_act[NMAX];
+  __shared__ int s_piv[KB];
+  __shared__ int s_info;
+  __shared__ short s_rc[NW * TPW];  // tile (rt, ct) of slot (wavefront, i): rt | ct << 8
+  __shared__ typename Gen::Shared s_gen;
+  for (int t = tid; t < NW * TPW; t += NT) {
+    const int tt = (t % NW) + NW * (t / NW);
+    s_rc[t] = (short)(tt < ntiles ? (tt % RT) | ((tt / RT) << 8) : 0x7f7f);
+  }
+
+  for (int64_t cell = blockIdx.x; cell < ncells; cell += gridDim.x) {
+    gen.begin(cell, s_gen);
+    for (int i = tid; i < NMAX; i += NT) {
+      s_pos[i] = i;
+      s_at[i] = i;
+      s_act[i] = i < n ? 1 : 0;
+    }
+    if (tid == 0) s_info = 0;
+    __syncthreads();
+    // generate the wavefront's tiles one at a time through LDS (one call
+    // site of the entry program keeps the register budget for the tiles)
+    dbl4 acc[TPW];
+#pragma unroll 1
+    for (int i = 0; i < TPW; ++i) {
+      const int t = wv + NW * i;
+      const int rc = s_rc[wv + NW * i], rt = rc & 0xff, ct = rc >> 8;
+#pragma unroll 1
+      for (int v = 0; v < 4; ++v) {
+        const int row = 16 * rt + q4 + 4 * v, col = 16 * ct + i16;
+        s_stage[wv][v][lane] = (t < ntiles && row < n && col < ncol)
+                                   ? GenCall<Gen>::entry(gen, s_gen, row, col, cell)
+                                   : 0.0;
+      }
+      wave_sync_lds();
+      dbl4 g;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) g[v] = s_stage[wv][v][lane];
+#pragma unroll
+      for (int ii = 0; ii < TPW; ++ii)
+        if (ii == i) acc[ii] = g;
+      wave_sync_lds();
+    }
+
+    for (int j0 = 0; j0 < ni; j0 += KB) {
+      const int kb = min(KB, ni - j0);
+      const int jt = j0 + kb;  // first trailing column
+      const int pc = j0 >> 4;  // tile column of the panel
+      // ---- panel tile column -> LDS ----
+#pragma unroll
+      for (int i = 0; i < TPW; ++i) {
+        const int t = wv + NW * i;
+        const int rc = s_rc[wv + NW * i], rt = rc & 0xff, ct = rc >> 8;
+        if (t < ntiles && ct == pc && i16 < kb) {
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int row = 16 * rt + q4 + 4 * v;
+            if (row < n) s_P[i16][row] = acc[i][v];
+          }
+        }
+      }
+      __syncthreads();
+      // ---- factor the panel: wavefront 0 only ----
+      if (wv == 0) {
+        for (int jj = 0; jj < kb; ++jj) {
+          const int j = j0 + jj;
+          int64_t bkey = -1;
+          int bpos = INT_MAX;
+          for (int row = lane; row < ni; row += 64) {
+            if (!s_act[row]) continue;
+            const int64_t key = dbits(fabs(s_P[jj][row]));
+            const int pr = s_pos[row];
+            if (key > bkey || (key == bkey && pr < bpos)) {
+              bkey = key;
+              bpos = pr;
+            }
+          }
+          const int64_t wm = wave_max_i64(bkey);
+          const int key = wave_min_i32(bkey == wm ? bpos : INT_MAX);
+          const int p = s_at[key];
+          const double pv = s_P[jj][p];
+          wave_sync_lds();
+          if (lane == 0) {
+            s_piv[jj] = p;
+            if (pv == 0.0 && s_info == 0) s_info = j + 1;
+            s_act[p] = 0;
+            const int q = s_at[j];
+            s_at[key] = q;
+            s_pos[q] = key;
+            s_at[j] = p;
+            s_pos[p] = j;
+          }
+          wave_sync_lds();
+          const double rinv = pv != 0.0 ? 1.0 / pv : 0.0;
+          // each active row with all its loads in flight
+          for (int row = lane; row < n; row += 64) {
+            if (!s_act[row]) continue;
+            double r[KB];
+#pragma unroll
+            for (int cc = 0; cc < KB; ++cc) r[cc] = cc >= jj && cc < kb ? s_P[cc][row] : 0.0;
+            const double l = r[jj % KB] * rinv;
+#pragma unroll
+            for (int cc = 0; cc < KB; ++cc) {
+              if (cc == jj) s_P[cc][row] = l;
+              if (cc > jj && cc < kb) s_P[cc][row] = fma(-l, s_P[cc][p], r[cc]);
+            }
+          }
+          wave_sync_lds();
+        }
+      }
+      __syncthreads();
+      // ---- pivot rows: panel part to U; trailing parts gathered into U12 ----
+      if (out.U) {
+        for (int t = tid; t < kb * KB; t += NT) {
+          const int jj = t / KB, cc = t - jj * KB;
+          if (cc >= kb || cc < jj) continue;
+          const int col = j0 + cc, k = j0 + jj;
+          out.U[cell * ((int64_t)ni * (ni + 1) / 2) + col * (col + 1) / 2 + k] =
+              s_P[cc][s_piv[jj]];
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < TPW; ++i) {
+        const int t = wv + NW * i;
+        const int rc = s_rc[wv + NW * i], rt = rc & 0xff, ct = rc >> 8;
+        if (t >= ntiles || 16 * ct + 15 < jt) continue;
+        const int col = 16 * ct + i16;
+        for (int jj = 0; jj < kb; ++jj) {
+          const int rr = s_piv[jj] - 16 * rt;
+          if (rr < 0 || rr >= 16) continue;
+          if (q4 == (rr & 3) && col >= jt && col < ncol) s_U[jj][col - jt] = dbl4_at(acc[i], rr >> 2);
+        }
+      }
+      __syncthreads();
+      for (int c = jt + tid; c < ncol; c += NT) {
+        double u[KB];
+#pragma unroll
+        for (int jj = 0; jj < KB; ++jj) u[jj] = jj < kb ? s_U[jj][c - jt] : 0.0;
+#pragma unroll
+        for (int jj = 0; jj < KB; ++jj) {
+          if (jj < kb) {
+            const int pr = s_piv[jj];
+            double v = u[jj];
+#pragma unroll
+            for (int ii = 0; ii < jj; ++ii) v = fma(-s_P[ii][pr], u[ii], v);
+            u[jj] = v;
+            s_U[jj][c - jt] = v;
+            const int k = j0 + jj;
+            if (c < ni) {
+              if (out.U) out.U[cell * ((int64_t)ni * (ni + 1) / 2) + c * (c + 1) / 2 + k] = v;
+            } else if (out.Y) {
+              out.Y[(cell * (nb + 1) + (c - ni)) * ni + k] = v;
+            }
+          }
+        }
+      }
+      __syncthreads();
+      // ---- trailing update C -= L21 U12 on every tile right of the panel ----
+#pragma unroll
+      for (int i = 0; i < TPW; ++i) {
+        const int t = wv + NW * i;
+        const int rc = s_rc[wv + NW * i], rt = rc & 0xff, ct = rc >> 8;
+        if (t >= ntiles || 16 * ct + 15 < jt) continue;
+        const int ra = 16 * rt + i16, col = 16 * ct + i16;
+        const bool aok = ra < n && s_act[ra];
+        if (__ballot(aok) == 0) continue;
+        const bool cok = col >= jt && col < ncol;
+#pragma unroll
+        for (int k4 = 0; k4 < KB / 4; ++k4) {
+          const int k = 4 * k4 + q4;
+          const double a = (aok && k < kb) ? -s_P[k][ra] : 0.0;
+          const double b = (cok && k < kb) ? s_U[k][col - jt] : 0.0;
+          acc[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[i], 0, 0, 0);
+        }
+      }
+      __syncthreads();
+    }
+    // ---- boundary rows: S and g straight from the tiles ----
+#pragma unroll
+    for (int i = 0; i < TPW; ++i) {
+      const int t = wv + NW * i;
+      const int rc = s_rc[wv + NW * i], rt = rc & 0xff, ct = rc >> 8;
+      if (t >= ntiles) continue;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int row = 16 * rt + q4 + 4 * v, col = 16 * ct + i16;
+        if (row < ni || row >= n || col < ni || col >= ncol) continue;
+        const int rb = row - ni, cb = col - ni;
+        if (cb < nb) {
+          if (out.S) out.S[(cell * nb + cb) * nb + rb] = acc[i][v];
+        } else if (out.g) {
+          out.g[cell * nb + rb] = acc[i][v];
+        }
+      }
+    }
+    if (tid == 0 && out.info) out.info[cell] = s_info;
+    __syncthreads();
+  }
+}
+
+// --------------------------------------------------------------------------
 // Recovery (BackwardStaticCondensationMap with the factors kept from the
 // forward pass): r = L^-1 P (b_i - A_ib lambda) = y - Y lambda, then
 // U x = r.  One wavefront per cell; the result is scattered into the
@@ -1807,6 +2267,23 @@ static size_t large_bytes(int n, int* ld) {
   return (size_t)kLargeGrid * l * (size_t)(n + 1) * sizeof(double);
 }
 
+constexpr int kPanelMaxN = 512;
+
+static int panel_nmax(int n) { return n <= 192 ? 192 : (n <= 256 ? 256 : 512); }
+
+// Row-major workspace of the panel kernel: n rows, ld = n+1 rounded to even,
+// one system per resident workgroup.
+static size_t panel_bytes(int n, int grid, int* ld) {
+  const int l = (n + 2) & ~1;
+  if (ld) *ld = l;
+  return (size_t)grid * (size_t)n * l * sizeof(double);
+}
+
+static int panel_grid(int n, int64_t ncells, int gridmul) {
+  const int per = gridmul > 0 ? gridmul : (panel_nmax(n) <= 256 ? 2 : 1);
+  return (int)std::min<int64_t>(ncells, 256 * per);
+}
+
 hipError_t workspace_reserve(Workspace& w, size_t bytes) {
   if (bytes <= w.bytes) return hipSuccess;
   workspace_release(w);
@@ -1826,7 +2303,16 @@ void workspace_release(Workspace& w) {
 }
 
 // Kernel families for a local system size (see condense_variants()).
-enum Family { kFamWave = 0, kFamGrid8 = 1, kFamColwave = 2, kFamGrid32 = 3, kFamLarge = 4 };
+enum Family {
+  kFamWave = 0,
+  kFamGrid8 = 1,
+  kFamColwave = 2,
+  kFamGrid32 = 3,
+  kFamLarge = 4,
+  kFamPanel = 5,
+  kFamTile = 6,
+  kFamCount = 7
+};
 
 static bool wave_class(int ni, int nb) {
   return (ni == 12 && nb == 8) || (ni == 27 && nb == 12) || (ni == 48 && nb == 16) ||
@@ -1842,6 +2328,8 @@ static bool family_ok(int fam, int ni, int nb) {
     case kFamColwave: return n > 64 && n <= 192 && n + 1 <= 168;
     case kFamGrid32: return n <= 192 && n + 1 <= 176 && n > 71;
     case kFamLarge: return n > 64 && n <= kLargeMaxN;
+    case kFamPanel: return n > 64 && n <= kPanelMaxN;
+    case kFamTile: return n > 64 && n <= kTileMaxN;
   }
   return false;
 }
@@ -1851,7 +2339,8 @@ static int default_family(int ni, int nb) {
   const int n = ni + nb;
   if (wave_class(ni, nb)) return kFamWave;
   if (n + 1 <= 72) return kFamGrid8;
-  if (n <= 192 && n + 1 <= 168) return kFamColwave;
+  if (n > 128 && n <= 192 && n + 1 <= 176) return kFamGrid32;
+  if (n <= 256) return kFamPanel;
   return kFamLarge;
 }
 
@@ -1878,6 +2367,44 @@ static hipError_t launch_condense(const Gen& gen, const CondenseOut& out, int64_
                        out, ncells, ni, nb, ws.p, ld);
     }
     return hipGetLastError();
+  }
+  if (fam == kFamTile && BACKSUB) fam = kFamPanel;
+  if (fam == kFamTile) {
+    const int n = ni + nb;
+    const int tiles = ((n + 15) >> 4) * ((n + 16) >> 4);
+    const int grid = (int)std::min<int64_t>(ncells, 256 * (gridmul > 0 ? gridmul : 1));
+#define HDG_TILE(TPW_)                                                                        \
+  if (tiles <= (kTileThreads / 64) * TPW_) {                                                  \
+    ClockScope clk_(st, "condense_tile");                                                     \
+    hipLaunchKernelGGL((condense_tile_kernel<TPW_, Gen>), dim3(grid), dim3(kTileThreads), 0,  \
+                       st, gen, out, ncells, ni, nb);                                         \
+    return hipGetLastError();                                                                 \
+  }
+    HDG_TILE(4)
+    HDG_TILE(8)
+    HDG_TILE(12)
+    HDG_TILE(16)
+#undef HDG_TILE
+    return hipErrorInvalidValue;
+  }
+  if (fam == kFamPanel) {
+    const int n = ni + nb, grid = panel_grid(n, ncells, gridmul);
+    int ld;
+    hipError_t e = workspace_reserve(ws, panel_bytes(n, grid, &ld));
+    if (e != hipSuccess) return e;
+    const int nmax = panel_nmax(n);
+#define HDG_PANEL(NM_)                                                                        \
+  if (nmax == NM_) {                                                                          \
+    ClockScope clk_(st, "condense_panel");                                                    \
+    hipLaunchKernelGGL((condense_panel_kernel<NM_, Gen, BACKSUB>), dim3(grid), dim3(256), 0,  \
+                       st, gen, out, ncells, ni, nb, ws.p, ld);                               \
+    return hipGetLastError();                                                                 \
+  }
+    HDG_PANEL(192)
+    HDG_PANEL(256)
+    HDG_PANEL(512)
+#undef HDG_PANEL
+    return hipErrorInvalidValue;
   }
   if (fam == kFamWave) {
     const int grid = grid_of(48);
@@ -1927,7 +2454,7 @@ static hipError_t launch_condense(const Gen& gen, const CondenseOut& out, int64_
 
 int condense_variants(int ni, int nb, int* fams, int max) {
   int k = 0;
-  for (int f = 0; f <= kFamLarge && k < max; ++f)
+  for (int f = 0; f < kFamCount && k < max; ++f)
     if (family_ok(f, ni, nb)) fams[k++] = f;
   return k;
 }
@@ -1935,7 +2462,7 @@ int condense_variants(int ni, int nb, int* fams, int max) {
 int condense_grid_options(int fam, int* gm, int max) {
   // resident workgroups per CU worth timing for each family (0 = fixed grid)
   static const int wave[] = {16, 32, 48, 64}, grid8[] = {20, 40, 80}, col[] = {1, 2, 3},
-                   g32[] = {2, 3, 4}, large[] = {0};
+                   g32[] = {2, 3, 4}, pan[] = {1, 2}, til[] = {1}, large[] = {0};
   const int* v = large;
   int n = 1;
   switch (fam) {
@@ -1943,6 +2470,8 @@ int condense_grid_options(int fam, int* gm, int max) {
     case kFamGrid8: v = grid8, n = 3; break;
     case kFamColwave: v = col, n = 3; break;
     case kFamGrid32: v = g32, n = 3; break;
+    case kFamPanel: v = pan, n = 2; break;
+    case kFamTile: v = til, n = 1; break;
     default: break;
   }
   int k = 0;
@@ -1954,6 +2483,10 @@ int condense_default_family(int ni, int nb) { return default_family(ni, nb); }
 
 size_t condense_workspace_bytes(int ni, int nb, int fam) {
   if (fam < 0) fam = default_family(ni, nb);
+  if (fam == kFamPanel) {
+    const int n = ni + nb;
+    return panel_bytes(n, panel_grid(n, INT64_MAX, 0) * 2, nullptr);  // any grid multiplier <= 2x
+  }
   return fam == kFamLarge ? large_bytes(ni + nb, nullptr) : 0;
 }
 

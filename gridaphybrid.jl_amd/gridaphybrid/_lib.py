@@ -49,7 +49,8 @@ EXPORTS = [
 ]
 
 BATCH_AUTOTUNE = 1
-FAMILY_NAMES = {0: "wave", 1: "grid8x8", 2: "colwave", 3: "grid32x16", 4: "blocked"}
+FAMILY_NAMES = {0: "wave", 1: "grid8x8", 2: "colwave", 3: "grid32x16", 4: "blocked",
+                5: "panel_mfma", 6: "tile_mfma"}
 
 
 class HDGError(RuntimeError):

@@ -302,7 +302,10 @@ typedef struct {
 } hdg_batch_class;
 /* Kernel families: 0 one wavefront per cell (fixed sizes), 1 8x8 grid with
  * register tiles, 2 column-per-wavefront with look-ahead, 3 32x16 grid,
- * 4 blocked HBM-workspace elimination. */
+ * 4 blocked HBM-workspace elimination, 5 blocked elimination with a
+ * single-wavefront panel and FP64 MFMA trailing updates.  With the
+ * environment variable HDG_AUTOTUNE_LOG set, the autotuner prints every
+ * candidate's time to stderr. */
 
 /* flags: HDG_BATCH_AUTOTUNE times every kernel family and grid size able to
  * condense each class on the batch's own cells and keeps the fastest;
