@@ -2674,6 +2674,204 @@ hipError_t launch_count_nonzero(const int32_t* info, int64_t n, int64_t* count_d
   return hipGetLastError();
 }
 
+// MFMA form of the refinement, for ni <= 128 and nb + 1 <= 64 (one cell per
+// 512-thread workgroup).  Both products of a sweep are 16x16 tile GEMMs:
+//   R = B - A_ii W         v_mfma_f64_16x16x4f64, A_ii from the workspace,
+//                          W (FP64) from LDS;
+//   W += X R               v_mfma_f32_16x16x4f32, X and R (FP32) from LDS;
+// and S, g = [A_bb b_b] - A_bi W with FP64 MFMA after the last sweep.  The 16
+// rows x 16 columns output tiles are dealt to the 8 wavefronts; a wavefront
+// updates only its own tiles of W, so a sweep needs two barriers.
+typedef float flt4 __attribute__((ext_vector_type(4)));
+constexpr int kRefMaxRT = 8;   // row tiles of W (ni <= 128)
+constexpr int kRefMaxCT = 4;   // column tiles of W (nb + 1 <= 64)
+
+template <int CTM>
+__global__ void __launch_bounds__(512)
+    mixed_refine_mfma_kernel(const double* __restrict__ A64, const float* __restrict__ X,
+                             int ni, int nb, int64_t count, CondenseOut out, int maxit,
+                             int32_t* __restrict__ iters) {
+  // wavefront w owns row strip w (16 rows) of W and all its CT <= CTM column
+  // tiles, so each k-step loads its A_ii (or X) operand once for CT MFMAs
+  constexpr int NT = 512, NW = 8;
+  const int n = ni + nb, ncol = nb + 1;
+  const int RT = (ni + 15) >> 4, CT = (ncol + 15) >> 4;
+  const int LDX = 16 * RT + 4, LDW = 16 * CT + 2, LDR = 16 * CT + 4;
+  const int KS = (ni + 3) >> 2;  // k-steps of 4
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int i16 = lane & 15, q4 = lane >> 4;
+  extern __shared__ double s_dyn[];
+  double* sW = s_dyn;                                        // [16 RT][LDW]
+  float* sX = reinterpret_cast<float*>(sW + 16 * RT * LDW);  // [16 RT][LDX]
+  float* sR = sX + 16 * RT * LDX;                            // [16 RT][LDR]
+  __shared__ double s_red[2][NW];
+  const bool strip = wv < RT;
+  const int r0 = wv << 4, ra = r0 + i16;
+
+  for (int64_t lc = blockIdx.x; lc < count; lc += gridDim.x) {
+    const double* A = A64 + lc * (int64_t)n * (n + 1);  // A[col * n + row]
+    const float* Xc = X + lc * (int64_t)ni * ni;
+    // X, W = 0 and R_0 = B into LDS (zero padding outside ni x ncol)
+    for (int t = tid; t < 16 * RT * 16 * RT; t += NT) {
+      const int r = t / (16 * RT), c = t - r * (16 * RT);
+      sX[r * LDX + c] = (r < ni && c < ni) ? Xc[r * ni + c] : 0.f;
+    }
+    for (int t = tid; t < 16 * RT * 16 * CT; t += NT) {
+      const int r = t / (16 * CT), c = t - r * (16 * CT);
+      sW[r * LDW + c] = 0.0;
+      sR[r * LDR + c] = (r < ni && c < ncol) ? (float)A[(int64_t)(ni + c) * n + r] : 0.f;
+    }
+    __syncthreads();
+    double dprev = 0.0;
+    int it = 0, conv = 1;
+    for (;;) {
+      // ---- W += X R (FP32 products, FP64 update) ----
+      double dmax = 0.0, wmax = 0.0;
+      if (strip) {
+        flt4 d[CTM];
+#pragma unroll
+        for (int ct = 0; ct < CTM; ++ct) d[ct] = flt4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll 2
+        for (int ks = 0; ks < KS; ++ks) {
+          const int k = 4 * ks + q4;
+          const float a = sX[ra * LDX + k];
+#pragma unroll
+          for (int ct = 0; ct < CTM; ++ct)
+            if (ct < CT)
+              d[ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, sR[k * LDR + 16 * ct + i16], d[ct],
+                                                          0, 0, 0);
+        }
+#pragma unroll
+        for (int ct = 0; ct < CTM; ++ct) {
+          if (ct >= CT) continue;
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int r = r0 + 4 * q4 + v, c = 16 * ct + i16;
+            if (r < ni && c < ncol) {
+              const double w = sW[r * LDW + c] + (double)d[ct][v];
+              sW[r * LDW + c] = w;
+              dmax = fmax(dmax, fabs((double)d[ct][v]));
+              wmax = fmax(wmax, fabs(w));
+            }
+          }
+        }
+      }
+      dmax = __builtin_bit_cast(double, wave_max_i64(__builtin_bit_cast(int64_t, dmax)));
+      wmax = __builtin_bit_cast(double, wave_max_i64(__builtin_bit_cast(int64_t, wmax)));
+      if (lane == 0) {
+        s_red[0][wv] = dmax;
+        s_red[1][wv] = wmax;
+      }
+      __syncthreads();  // W complete for the residual; the maxima published
+      dmax = s_red[0][0];
+      wmax = s_red[1][0];
+#pragma unroll
+      for (int q = 1; q < NW; ++q) {
+        dmax = fmax(dmax, s_red[0][q]);
+        wmax = fmax(wmax, s_red[1][q]);
+      }
+      const bool done = it >= maxit || dmax <= 0x1p-44 * wmax ||
+                        (it >= 2 && dmax <= 1e-10 * wmax && dmax > 0.5 * dprev);
+      if (done) {
+        conv = dmax <= 1e-10 * wmax;
+        break;
+      }
+      dprev = dmax;
+      ++it;
+      // ---- R = B - A_ii W (FP64), then to LDS in FP32 ----
+      dbl4 acc[CTM];
+      if (strip) {
+#pragma unroll
+        for (int ct = 0; ct < CTM; ++ct) {
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int r = r0 + q4 + 4 * v, c = 16 * ct + i16;
+            acc[ct][v] = (ct < CT && r < ni && c < ncol) ? A[(int64_t)(ni + c) * n + r] : 0.0;
+          }
+        }
+#pragma unroll 2
+        for (int ks = 0; ks < KS; ++ks) {
+          const int k = 4 * ks + q4;
+          const double a = (ra < ni && k < ni) ? -A[(int64_t)k * n + ra] : 0.0;
+#pragma unroll
+          for (int ct = 0; ct < CTM; ++ct)
+            if (ct < CT)
+              acc[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, sW[k * LDW + 16 * ct + i16],
+                                                             acc[ct], 0, 0, 0);
+        }
+      }
+      __syncthreads();  // everyone is done reading R before it is overwritten
+      if (strip) {
+#pragma unroll
+        for (int ct = 0; ct < CTM; ++ct) {
+          if (ct >= CT) continue;
+#pragma unroll
+          for (int v = 0; v < 4; ++v)
+            sR[(r0 + q4 + 4 * v) * LDR + 16 * ct + i16] = (float)acc[ct][v];
+        }
+      }
+      __syncthreads();
+    }
+    // ---- S = A_bb - A_bi W, g = b_b - A_bi w (FP64 MFMA), row strip per wavefront ----
+    const int BT = (nb + 15) >> 4;
+    for (int bs = wv; bs < BT; bs += NW) {
+      const int b0 = bs << 4, rb_a = b0 + i16;
+      dbl4 acc[CTM];
+#pragma unroll
+      for (int ct = 0; ct < CTM; ++ct) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int r = b0 + q4 + 4 * v, c = 16 * ct + i16;
+          acc[ct][v] = (ct < CT && r < nb && c < ncol) ? A[(int64_t)(ni + c) * n + ni + r] : 0.0;
+        }
+      }
+#pragma unroll 2
+      for (int ks = 0; ks < KS; ++ks) {
+        const int k = 4 * ks + q4;
+        const double a = (rb_a < nb && k < ni) ? -A[(int64_t)k * n + ni + rb_a] : 0.0;
+#pragma unroll
+        for (int ct = 0; ct < CTM; ++ct)
+          if (ct < CT)
+            acc[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, sW[k * LDW + 16 * ct + i16],
+                                                           acc[ct], 0, 0, 0);
+      }
+#pragma unroll
+      for (int ct = 0; ct < CTM; ++ct) {
+        if (ct >= CT) continue;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int rb = b0 + q4 + 4 * v, c = 16 * ct + i16;
+          if (rb >= nb || c >= ncol) continue;
+          if (c < nb) {
+            if (out.S) out.S[(lc * nb + c) * nb + rb] = acc[ct][v];
+          } else {
+            out.g[lc * nb + rb] = acc[ct][v];
+          }
+        }
+      }
+    }
+    // W to its output slot, column-major [nb+1][ni]
+    for (int t = tid; t < ncol * ni; t += NT) {
+      const int j = t / ni, k = t - j * ni;
+      out.Y[lc * (int64_t)ncol * ni + t] = sW[k * LDW + j];
+    }
+    if (tid == 0 && iters) iters[lc] = conv ? it : -it - 1;
+    __syncthreads();
+  }
+}
+
+static size_t refine_mfma_lds(int ni, int nb) {
+  const int RT = cdiv(ni, 16), CT = cdiv(nb + 1, 16);
+  return (size_t)16 * RT *
+         ((16 * CT + 2) * sizeof(double) + (16 * RT + 4) * sizeof(float) +
+          (16 * CT + 4) * sizeof(float));
+}
+
+static bool refine_mfma_ok(int ni, int nb) {
+  return ni <= 16 * kRefMaxRT && nb + 1 <= 16 * kRefMaxCT && nb >= 1 &&
+         refine_mfma_lds(ni, nb) <= 150 * 1024;
+}
+
 // columns of W per thread (mixed_refine_kernel's JC): the smallest of 4, 8,
 // 16, 32 whose ceil((nb+1) / JC) column groups fit in the workgroup
 static int mixed_jc(int ni, int nb) {
@@ -2764,6 +2962,24 @@ static hipError_t launch_mixed_phys(const Tables& T, const MeshDev& M, const Con
     o.Y = out.Y + first * (int64_t)(nb + 1) * ni;
     int32_t* it = iters ? iters + first : nullptr;
     const int g3 = (int)std::min<int64_t>(count, kRefineGrid);
+    if (refine_mfma_ok(ni, nb)) {
+      const int ctiles = cdiv(nb + 1, 16);
+      const size_t lds3 = refine_mfma_lds(ni, nb);
+#define HDG_RMF(T_)                                                                           \
+  if (ctiles <= T_) {                                                                         \
+    e = hipFuncSetAttribute((const void*)mixed_refine_mfma_kernel<T_>,                        \
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds3);           \
+    if (e != hipSuccess) return e;                                                            \
+    ClockScope clk_(st, "mixed_refine");                                                      \
+    hipLaunchKernelGGL(mixed_refine_mfma_kernel<T_>, dim3(g3), dim3(512), lds3, st, A64, X,   \
+                       ni, nb, count, o, maxit, it);                                          \
+  } else
+      HDG_RMF(1) HDG_RMF(2) HDG_RMF(4) return hipErrorInvalidValue;
+#undef HDG_RMF
+      e = hipGetLastError();
+      if (e != hipSuccess) return e;
+      continue;
+    }
 #define HDG_REF(J_, L_)                                                                       \
   if (jc == J_ && wlds == L_) {                                                               \
     e = hipFuncSetAttribute((const void*)mixed_refine_kernel<J_, L_>,                         \
