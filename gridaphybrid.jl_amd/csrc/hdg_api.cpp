@@ -93,13 +93,14 @@ struct hdg_mesh {
   std::vector<int8_t> facet_dir;
   // device
   DevBuf tables, d_x0, d_jac, d_sign, d_coef, d_cell_dofs, d_lam_dir;
-  DevBuf d_S, d_g, d_U, d_Y, d_info, d_alpha, d_srcq, d_iters;
+  DevBuf d_S, d_g, d_U, d_Y, d_info, d_alpha, d_srcq, d_iters, d_Xinv;
   // per-cell device state used by the kernels: the mesh's own buffers above,
   // or slices of a batch's per-class buffers (hdg_batch_create)
   double *pS = nullptr, *pg = nullptr, *pU = nullptr, *pY = nullptr;
   double *palpha = nullptr, *psrcq = nullptr;
   int32_t* pinfo = nullptr;
   int32_t* piters = nullptr;  // refinement sweeps per cell (mixed precision)
+  float* pXinv = nullptr;     // FP32 inverses of A_ii kept for recovery (mixed precision)
   hdg_batch* batch = nullptr;
   // the resident factors are W = A_ii^-1 [A_ib b_i] (mixed precision) rather
   // than getrf's U and L^-1 P [A_ib b_i]
@@ -182,11 +183,12 @@ hipError_t own_cell_state(hdg_mesh* m) {
 
 // Frees the mesh's own per-cell state (the mesh is rebound to batch slices).
 void drop_cell_state(hdg_mesh* m) {
-  for (DevBuf* b :
-       {&m->d_S, &m->d_g, &m->d_U, &m->d_Y, &m->d_alpha, &m->d_srcq, &m->d_info, &m->d_iters})
+  for (DevBuf* b : {&m->d_S, &m->d_g, &m->d_U, &m->d_Y, &m->d_alpha, &m->d_srcq, &m->d_info,
+                    &m->d_iters, &m->d_Xinv})
     (void)b->alloc(0);
   m->pS = m->pg = m->pU = m->pY = m->palpha = m->psrcq = nullptr;
   m->pinfo = m->piters = nullptr;
+  m->pXinv = nullptr;
 }
 
 }  // namespace
@@ -705,7 +707,7 @@ int hdg_footprint_cartesian(const hdg_problem* prob, const int64_t* cells, int p
   b += nc * (int64_t)(L.S + L.g + L.U + L.Y + L.alpha + L.srcq) * 8 + nc * 8;  // per cell
   b += (int64_t)1 << 20;                                                 // reference tables
   if (precision == HDG_MIXED && mixed_supported(th.ni, th.nb))
-    b += (int64_t)mixed_workspace_bytes(th.ni, th.nb, nc);
+    b += (int64_t)mixed_workspace_bytes(th.ni, th.nb, nc) + nc * (int64_t)th.ni * th.ni * 4;
   else
     b += (int64_t)condense_workspace_bytes(th.ni, th.nb, -1);
   (void)ni;
@@ -903,6 +905,14 @@ int hdg_condense(hdg_mesh* m, int precision) {
   out.info = m->pinfo;
   m->fp64_fallback = false;
   if (precision == HDG_MIXED && mixed_supported(m->th.ni, m->th.nb)) {
+    if (!m->pXinv) {
+      HIP_TRY(ctx,
+              m->d_Xinv.alloc((size_t)std::max<int64_t>(m->ncells, 1) * m->th.ni * m->th.ni *
+                              sizeof(float)),
+              "alloc FP32 inverses");
+      m->pXinv = m->d_Xinv.as<float>();
+    }
+    out.Xinv = m->pXinv;
     HIP_TRY(ctx, launch_condense_mixed(m->tdev, m->meshdev(), out, ctx->stream, ctx->ws, true,
                                        kMaxRefinementSweeps, m->piters),
             "mixed condense kernels");
@@ -1010,7 +1020,15 @@ int hdg_recover(hdg_mesh* m, const double* lam_free, double* free_values) {
   if (!m->condensed) return fail(ctx, HDG_ERR_STATE, "recover called before condense");
   if ((m->nfree > 0 && !lam_free) || !free_values) return fail(ctx, HDG_ERR_ARG, "null buffer");
   HIP_TRY(ctx, hipSetDevice(ctx->device), "hipSetDevice");
-  HIP_TRY(ctx, launch_recover(m->mixed ? nullptr : m->pU, m->pY,
+  if (m->mixed) {
+    HIP_TRY(ctx, launch_recover_mixed(m->tdev, m->meshdev(), m->pY, m->pXinv,
+                                      m->d_cell_dofs.as<int32_t>(), lam_free,
+                                      m->d_lam_dir.as<double>(), free_values, m->nfree,
+                                      ctx->stream),
+            "mixed recover kernel");
+    return HDG_OK;
+  }
+  HIP_TRY(ctx, launch_recover(m->pU, m->pY,
                               m->d_cell_dofs.as<int32_t>(), lam_free,
                               m->d_lam_dir.as<double>(), free_values, m->ncells, m->th.ni,
                               m->th.nb, m->th.na, m->nfree, ctx->stream),
@@ -1043,7 +1061,7 @@ struct hdg_batch {
     hdg_batch_class info{};
     std::vector<int> members;
     const hdg_mesh* lead = nullptr;  // reference tables shared by the members
-    DevBuf S, g, U, Y, alpha, srcq, infos, iters;
+    DevBuf S, g, U, Y, alpha, srcq, infos, iters, xinv;
     bool mixed = false;  // condensed in mixed precision at the last launch
 
     MeshDev cells(int64_t count) const {
@@ -1120,6 +1138,7 @@ hipError_t enqueue_all(hdg_batch* b, int precision, hipStream_t st) {
     if (class_mixed(c, precision)) {
       CondenseOut o = c.out();
       o.U = nullptr;
+      o.Xinv = c.xinv.as<float>();
       e = launch_condense_mixed(c.lead->tdev, c.cells(c.info.ncells), o, st, b->ws, false,
                                 kMaxRefinementSweeps, c.iters.as<int32_t>());
     } else {
@@ -1206,6 +1225,20 @@ int capture(hdg_batch* b, int precision) {
                               ? mixed_workspace_bytes(c->info.ni, c->info.nb, c->info.ncells)
                               : condense_workspace_bytes(c->info.ni, c->info.nb, c->info.family));
   HIP_TRY(ctx, workspace_reserve(b->ws, need), "batch workspace");
+  // FP32 inverses kept for the recovery of mixed classes (allocated once)
+  for (auto& cp : b->classes) {
+    hdg_batch::Class& c = *cp;
+    if (!class_mixed(c, precision) || c.xinv.p) continue;
+    const size_t per = (size_t)c.info.ni * c.info.ni;
+    HIP_TRY(ctx, c.xinv.alloc((size_t)std::max<int64_t>(c.info.ncells, 1) * per * sizeof(float)),
+            "alloc batch FP32 inverses");
+    size_t off = 0;
+    for (int i : c.members) {
+      hdg_mesh* m = b->meshes[i];
+      m->pXinv = c.xinv.as<float>() + off * per;
+      off += (size_t)m->ncells;
+    }
+  }
   HIP_TRY(ctx, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "begin capture");
   hipError_t e = enqueue_all(b, precision, st);
   hipGraph_t g = nullptr;
@@ -1334,6 +1367,7 @@ int hdg_batch_destroy(hdg_batch* b) {
     // own per-cell state is allocated again at the next condensation
     m->pS = m->pg = m->pU = m->pY = m->palpha = m->psrcq = nullptr;
     m->pinfo = m->piters = nullptr;
+    m->pXinv = nullptr;
     m->mixed = false;
   }
   delete b;

@@ -104,6 +104,7 @@ struct CondenseOut {
   double* Y = nullptr;     // [ncells][nb+1][ni] L^-1 P [A_ib b_i], LAPACK row order
   double* X = nullptr;     // [ncells][ni] back-substituted rhs (backward map)
   int32_t* info = nullptr; // [ncells]
+  float* Xinv = nullptr;   // [ncells][ni][ni] FP32 inverse of A_ii (mixed path, kept)
 };
 
 // Dense cell systems in HBM for the map-level API.
@@ -156,6 +157,7 @@ hipError_t launch_prep_cells(const Tables& T, const MeshDev& M, hipStream_t st);
 // order, [nb+1][ni] per cell); out.U is not written.  iters (may be null) gets
 // the refinement sweeps per cell, negative when not converged.
 constexpr int kMaxRefinementSweeps = 10;
+constexpr int kMixedMinNi = 64;  // smaller interiors: the FP64 path is faster
 bool mixed_supported(int ni, int nb);
 size_t mixed_workspace_bytes(int ni, int nb, int64_t ncells);
 hipError_t launch_condense_mixed(const Tables& T, const MeshDev& M, const CondenseOut& out,
@@ -174,6 +176,14 @@ hipError_t launch_backward_dense(const DenseIn& in, const double* x, int64_t nce
 hipError_t launch_recover(const double* U, const double* Y, const int32_t* cell_dofs,
                           const double* lam_free, const double* lam_dir, double* out,
                           int64_t ncells, int ni, int nb, int na, int64_t nfree, hipStream_t st);
+// Recovery after a mixed-precision condensation: x0 = w - W lambda, then one
+// FP64 refinement x = x0 + X (b_i - A_ib lambda - A_ii x0) with the cell
+// system regenerated from the per-cell coefficients and X the kept FP32
+// inverse of A_ii.
+hipError_t launch_recover_mixed(const Tables& T, const MeshDev& M, const double* W,
+                                const float* Xinv, const int32_t* cell_dofs,
+                                const double* lam_free, const double* lam_dir, double* out,
+                                int64_t nfree, hipStream_t st);
 hipError_t launch_symbolic(const PairTable& P, int32_t* rowval, hipStream_t st);
 hipError_t launch_assemble(const PairTable& P, const double* S, const double* g,
                            const int32_t* cell_dofs, const double* lam_dir, double* nzval,

@@ -1889,6 +1889,7 @@ __global__ void cell_systems_kernel(FusedGen<PHYS> gen, int64_t first, int64_t c
 // W (natural interior order) is kept for recovery: x_i = w - W_b lambda.
 // --------------------------------------------------------------------------
 constexpr int kMixedMaxNi = 128;
+constexpr int kRecoverMaxNb = 512;
 __host__ __device__ inline int cdiv(int a, int b) { return (a + b - 1) / b; }
 constexpr int64_t kMixedChunkCells = 16384;  // cells per gen / inverse / refine round
 
@@ -2584,6 +2585,85 @@ hipError_t launch_recover(const double* U, const double* Y, const int32_t* cell_
   return hipGetLastError();
 }
 
+template <int PHYS>
+__global__ void __launch_bounds__(256)
+    recover_mixed_kernel(FusedGen<PHYS> gen, const double* __restrict__ W,
+                         const float* __restrict__ Xinv, const int32_t* __restrict__ cell_dofs,
+                         const double* __restrict__ lam_free, const double* __restrict__ lam_dir,
+                         double* __restrict__ out) {
+  __shared__ FusedShared sh;
+  __shared__ double s_lam[kRecoverMaxNb], s_x[kMixedMaxNi], s_r[kMixedMaxNi];
+  const int ni = gen.T.ni, nb = gen.T.nb, na = gen.T.na;
+  const int64_t ncells = gen.M.ncells;
+  const int tid = threadIdx.x;
+  for (int64_t cell = blockIdx.x; cell < ncells; cell += gridDim.x) {
+    gen.begin(cell, sh);
+    for (int j = tid; j < nb; j += blockDim.x) {
+      const int32_t d = cell_dofs[cell * nb + j];
+      s_lam[j] = d >= 0 ? lam_free[d] : lam_dir[-(int64_t)d - 1];
+    }
+    __syncthreads();
+    const double* Wc = W + cell * (int64_t)(nb + 1) * ni;
+    // x0 = w - W lambda and rhs = b_i - A_ib lambda, one interior row per thread
+    for (int r = tid; r < ni; r += blockDim.x) {
+      double x0 = Wc[(int64_t)nb * ni + r];
+      double rhs = gen.rhs(sh, r);
+      for (int j = 0; j < nb; ++j) {
+        x0 = fma(-Wc[(int64_t)j * ni + r], s_lam[j], x0);
+        rhs = fma(-gen.entry(sh, r, ni + j), s_lam[j], rhs);
+      }
+      s_x[r] = x0;
+      s_r[r] = rhs;
+    }
+    __syncthreads();
+    // residual of the interior equations at x0 (FP64)
+    for (int r = tid; r < ni; r += blockDim.x) {
+      double res = s_r[r];
+      for (int k = 0; k < ni; ++k) res = fma(-gen.entry(sh, r, k), s_x[k], res);
+      s_r[r] = res;  // each thread rewrites only its own row
+    }
+    __syncthreads();
+    // x = x0 + X res, scattered to [q of all cells; u of all cells]
+    const float* Xc = Xinv + cell * (int64_t)ni * ni;
+    for (int r = tid; r < ni; r += blockDim.x) {
+      double acc = 0.0;
+      for (int k = 0; k < ni; ++k) acc = fma((double)Xc[(int64_t)r * ni + k], s_r[k], acc);
+      const double x = s_x[r] + acc;
+      if (r < na)
+        out[cell * na + r] = x;
+      else
+        out[ncells * na + cell * (ni - na) + (r - na)] = x;
+    }
+    __syncthreads();
+  }
+}
+
+hipError_t launch_recover_mixed(const Tables& T, const MeshDev& M, const double* W,
+                                const float* Xinv, const int32_t* cell_dofs,
+                                const double* lam_free, const double* lam_dir, double* out,
+                                int64_t nfree, hipStream_t st) {
+  if (T.ni > kMixedMaxNi || T.nb > kRecoverMaxNb) return hipErrorInvalidValue;
+  if (M.ncells > 0) {
+    const int grid = (int)std::min<int64_t>(M.ncells, 256 * 8);
+    ClockScope clk_(st, "recover_mixed");
+    if (T.phys == kDarcy) {
+      FusedGen<kDarcy> gen{T, M};
+      hipLaunchKernelGGL(recover_mixed_kernel<kDarcy>, dim3(grid), dim3(256), 0, st, gen, W,
+                         Xinv, cell_dofs, lam_free, lam_dir, out);
+    } else {
+      FusedGen<kElasticity> gen{T, M};
+      hipLaunchKernelGGL(recover_mixed_kernel<kElasticity>, dim3(grid), dim3(256), 0, st, gen,
+                         W, Xinv, cell_dofs, lam_free, lam_dir, out);
+    }
+  }
+  if (nfree > 0) {
+    ClockScope clk_(st, "recover_copy");
+    hipLaunchKernelGGL(copy_kernel, dim3(1024), dim3(256), 0, st, lam_free,
+                       out + M.ncells * T.ni, nfree);
+  }
+  return hipGetLastError();
+}
+
 hipError_t launch_symbolic(const PairTable& P, int32_t* rowval, hipStream_t st) {
   if (P.nff > 0) {
     const int grid = (int)(P.nff < 65536 ? P.nff : 65536);
@@ -2896,8 +2976,10 @@ static size_t mixed_lds(int ni, int nb, bool wlds) {
 constexpr size_t kMixedMaxLds = 150 * 1024;
 
 bool mixed_supported(int ni, int nb) {
-  return ni >= 1 && ni <= kMixedMaxNi && nb >= 1 && nb <= 512 && mixed_jc(ni, nb) > 0 &&
-         mixed_lds(ni, nb, false) <= kMixedMaxLds;
+  // below kMixedMinNi the FP64 kernels are bound by HBM traffic and faster
+  // than the FP32 inverse + refinement, so mixed requests use them
+  return ni >= kMixedMinNi && ni <= kMixedMaxNi && nb >= 1 && nb <= 512 &&
+         mixed_jc(ni, nb) > 0 && mixed_lds(ni, nb, false) <= kMixedMaxLds;
 }
 
 // cells per chunk of the mixed path and the workspace they need: the FP64
@@ -2929,7 +3011,7 @@ static hipError_t launch_mixed_phys(const Tables& T, const MeshDev& M, const Con
   (void)mixed_cell_bytes(ni, nb, &abytes, &xbytes);
   char* base = reinterpret_cast<char*>(ws.p);
   double* A64 = reinterpret_cast<double*>(base);
-  float* X = reinterpret_cast<float*>(base + (size_t)chunk * abytes);
+  float* Xws = reinterpret_cast<float*>(base + (size_t)chunk * abytes);
   double* Wws = reinterpret_cast<double*>(base + (size_t)chunk * abytes +
                                           (((size_t)chunk * xbytes + 7) & ~(size_t)7));
   FusedGen<PHYS> gen{T, M};
@@ -2938,6 +3020,8 @@ static hipError_t launch_mixed_phys(const Tables& T, const MeshDev& M, const Con
   const size_t lds = mixed_lds(ni, nb, wlds);
   for (int64_t first = 0; first < M.ncells; first += chunk) {
     const int64_t count = std::min<int64_t>(chunk, M.ncells - first);
+    // the FP32 inverses go straight to their per-cell slot when one is given
+    float* X = out.Xinv ? out.Xinv + first * (int64_t)ni * ni : Xws;
     const int g1 = (int)std::min<int64_t>(count, 256 * 16);
     {
       ClockScope clk_(st, "mixed_gen_system");

@@ -38,8 +38,9 @@ typedef enum {
  * HDG_MIXED: A_ii is inverted in FP32 (Gauss-Jordan, partial pivoting),
  * W = A_ii^-1 [A_ib b_i] is formed with FP32 products and refined with FP64
  * residuals B - A_ii W until the correction reaches FP64 rounding or stops
- * shrinking; S, g and the recovery are FP64.  Local systems with ni > 128 (or
- * too large for LDS) use the FP64 path under HDG_MIXED. */
+ * shrinking; S, g and the recovery are FP64.  Local systems with ni < 64
+ * (where the FP64 kernels are HBM-bound and faster) or ni > 128 use the FP64
+ * path under HDG_MIXED. */
 typedef enum { HDG_FP64 = 0, HDG_MIXED = 1 } hdg_precision;
 typedef enum { HDG_DARCY = 0, HDG_ELASTICITY = 1 } hdg_physics;
 typedef enum { HDG_SPACE_Q = 0, HDG_SPACE_P = 1 } hdg_space;

@@ -44,8 +44,10 @@ def _elasticity(ctx, k, n, seed, precision):
     return op, ref
 
 
-def _check(op, ref, mixed_expected=True):
+def _check(op, ref, mixed_expected=None):
     st = op.refinement_stats()
+    if mixed_expected is None:   # the mixed path serves 64 <= ni <= 128
+        mixed_expected = 64 <= op.info.ni <= 128
     assert st["mixed"] == mixed_expected, st
     assert st["unconverged"] == 0 and not st["fp64_fallback"], st
     S, g, cd = op.cell_blocks()
@@ -65,12 +67,16 @@ def _check(op, ref, mixed_expected=True):
     return st
 
 
-@pytest.mark.parametrize("D,k,n", [(2, 1, 8), (2, 2, 16), (2, 3, 6), (3, 1, 4), (3, 2, 4)])
+@pytest.mark.parametrize("D,k,n", [(2, 1, 8), (2, 2, 16), (2, 3, 6), (3, 1, 4), (3, 2, 4),
+                                   (3, 2, 6)])
 def test_mixed_darcy(ctx, D, k, n):
     from gridaphybrid import MIXED
     print(f"mixed darcy D={D} k={k} n={n}", flush=True)
     op, ref = _darcy(ctx, D, k, n, seed=D * 10 + k, precision=MIXED)
     st = _check(op, ref)
+    if not st["mixed"]:
+        op.close()
+        return
     assert 1 <= st["max_sweeps"] <= 10
     U, W = op.local_factors()
     assert U is None
@@ -124,7 +130,7 @@ def test_mixed_batch_matches_single(ctx):
     for op, (S0, g0, ref) in zip(ops, alone):
         S, g, _ = op.cell_blocks()
         assert np.array_equal(S, S0) and np.array_equal(g, g0)
-        assert op.refinement_stats()["mixed"]
+        assert op.refinement_stats()["mixed"] == (op.info.ni >= 64)
         fv = op.recover(ref.lam).cpu().numpy()
         assert rel(fv, ref.free_values) < RTOL
     batch.close()
