@@ -558,10 +558,12 @@ int hdg_mesh_create(hdg_ctx* ctx, const hdg_problem* prob, const hdg_mesh_desc* 
     size_t nd = 0;
     for (auto* v : dv) nd += v->size();
     size_t bytes = nd * sizeof(double) + th.prog_val.size() * sizeof(double) +
-                   (th.prog_val0.size() + th.prog_xval.size()) * sizeof(double) +
-                   th.prog_off.size() * sizeof(int32_t) + th.prog_head.size() * sizeof(uint32_t) +
+                   (th.prog_val0.size() + th.prog_xval.size() + th.prog_rval.size()) *
+                       sizeof(double) +
+                   (th.prog_off.size() + th.prog_rptr.size()) * sizeof(int32_t) +
+                   th.prog_head.size() * sizeof(uint32_t) + th.prog_rcol.size() * sizeof(uint16_t) +
                    th.pmask.size() + th.qmask.size() + th.prog_id.size() + th.prog_xid.size() +
-                   16;
+                   th.prog_rid.size() + 16;
     std::vector<char> host(bytes, 0);
     HIP_TRY(ctx, m->tables.alloc(bytes), "alloc tables");
     size_t off = 0;
@@ -580,12 +582,21 @@ int hdg_mesh_create(hdg_ctx* ctx, const hdg_problem* prob, const hdg_mesh_desc* 
     size_t off_xv = off;
     std::memcpy(host.data() + off, th.prog_xval.data(), th.prog_xval.size() * sizeof(double));
     off += th.prog_xval.size() * sizeof(double);
+    size_t off_rv = off;
+    std::memcpy(host.data() + off, th.prog_rval.data(), th.prog_rval.size() * sizeof(double));
+    off += th.prog_rval.size() * sizeof(double);
     size_t off_po = off;
     std::memcpy(host.data() + off, th.prog_off.data(), th.prog_off.size() * sizeof(int32_t));
     off += th.prog_off.size() * sizeof(int32_t);
+    size_t off_rp = off;
+    std::memcpy(host.data() + off, th.prog_rptr.data(), th.prog_rptr.size() * sizeof(int32_t));
+    off += th.prog_rptr.size() * sizeof(int32_t);
     size_t off_ph = off;
     std::memcpy(host.data() + off, th.prog_head.data(), th.prog_head.size() * sizeof(uint32_t));
     off += th.prog_head.size() * sizeof(uint32_t);
+    size_t off_rc = off;
+    std::memcpy(host.data() + off, th.prog_rcol.data(), th.prog_rcol.size() * sizeof(uint16_t));
+    off += th.prog_rcol.size() * sizeof(uint16_t);
     size_t off_pm = off;
     std::memcpy(host.data() + off, th.pmask.data(), th.pmask.size());
     off += th.pmask.size();
@@ -597,6 +608,9 @@ int hdg_mesh_create(hdg_ctx* ctx, const hdg_problem* prob, const hdg_mesh_desc* 
     off += th.prog_id.size();
     size_t off_xi = off;
     std::memcpy(host.data() + off, th.prog_xid.data(), th.prog_xid.size());
+    off += th.prog_xid.size();
+    size_t off_ri = off;
+    std::memcpy(host.data() + off, th.prog_rid.data(), th.prog_rid.size());
     HIP_TRY(ctx, hipMemcpyAsync(m->tables.p, host.data(), bytes, hipMemcpyHostToDevice, st),
             "upload tables");
     HIP_TRY(ctx, hipStreamSynchronize(st), "sync");
@@ -631,6 +645,10 @@ int hdg_mesh_create(hdg_ctx* ctx, const hdg_problem* prob, const hdg_mesh_desc* 
     T.prog_val0 = reinterpret_cast<const double*>(base + off_v0);
     T.prog_xid = reinterpret_cast<const uint8_t*>(base + off_xi);
     T.prog_xval = reinterpret_cast<const double*>(base + off_xv);
+    T.prog_rptr = reinterpret_cast<const int32_t*>(base + off_rp);
+    T.prog_rcol = reinterpret_cast<const uint16_t*>(base + off_rc);
+    T.prog_rid = reinterpret_cast<const uint8_t*>(base + off_ri);
+    T.prog_rval = reinterpret_cast<const double*>(base + off_rv);
   }
   m->x0.assign(desc->x0, desc->x0 + nc * D);
   m->jac.assign(desc->jac, desc->jac + nc * D * D);
@@ -1230,8 +1248,8 @@ int capture(hdg_batch* b, int precision) {
     hdg_batch::Class& c = *cp;
     if (!class_mixed(c, precision) || c.xinv.p) continue;
     const size_t per = (size_t)c.info.ni * c.info.ni;
-    HIP_TRY(ctx, c.xinv.alloc((size_t)std::max<int64_t>(c.info.ncells, 1) * per * sizeof(float)),
-            "alloc batch FP32 inverses");
+    const size_t nc = (size_t)std::max<int64_t>(c.info.ncells, 1);
+    HIP_TRY(ctx, c.xinv.alloc(nc * per * sizeof(float)), "alloc batch FP32 inverses");
     size_t off = 0;
     for (int i : c.members) {
       hdg_mesh* m = b->meshes[i];

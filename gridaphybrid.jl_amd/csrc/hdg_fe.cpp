@@ -604,6 +604,30 @@ int inline_program(TablesHost& t, std::string& err) {
       }
     }
   }
+  // row-major terms of the interior rows (entries are column-major: e = col n + row)
+  const int n = t.n, ni = t.ni;
+  t.prog_rptr.assign((size_t)ni + 1, 0);
+  for (int col = 0; col < n; ++col)
+    for (int row = 0; row < ni; ++row) {
+      const int e = col * n + row;
+      t.prog_rptr[row + 1] += t.prog_off[e + 1] - t.prog_off[e];
+    }
+  for (int r = 0; r < ni; ++r) t.prog_rptr[r + 1] += t.prog_rptr[r];
+  const size_t nt = (size_t)t.prog_rptr[ni];
+  t.prog_rcol.assign(nt, 0);
+  t.prog_rid.assign(nt, 0);
+  t.prog_rval.assign(nt, 0.0);
+  std::vector<int32_t> fill(t.prog_rptr.begin(), t.prog_rptr.end() - 1);
+  for (int col = 0; col < n; ++col)
+    for (int row = 0; row < ni; ++row) {
+      const int e = col * n + row;
+      for (int k = t.prog_off[e]; k < t.prog_off[e + 1]; ++k) {
+        const int32_t at = fill[row]++;
+        t.prog_rcol[at] = (uint16_t)col;
+        t.prog_rid[at] = t.prog_id[k];
+        t.prog_rval[at] = t.prog_val[k];
+      }
+    }
   return 0;
 }
 

@@ -50,6 +50,12 @@ struct TablesHost {
   std::vector<double> prog_val0;    // [n*n]
   std::vector<uint8_t> prog_xid;    // further terms
   std::vector<double> prog_xval;
+  // the interior rows of the program as a row-major sparse matrix of terms
+  // (row r: terms rptr[r]..rptr[r+1] with column, coefficient index, value)
+  std::vector<int32_t> prog_rptr;   // [ni+1]
+  std::vector<uint16_t> prog_rcol;
+  std::vector<uint8_t> prog_rid;
+  std::vector<double> prog_rval;
 };
 
 // Device view of the tables (pointers into one device allocation).
@@ -79,6 +85,10 @@ struct Tables {
   const double* prog_val0;
   const uint8_t* prog_xid;
   const double* prog_xval;
+  const int32_t* prog_rptr;
+  const uint16_t* prog_rcol;
+  const uint8_t* prog_rid;
+  const double* prog_rval;
 };
 
 // Device view of a mesh for the fused element generator.
@@ -177,8 +187,10 @@ hipError_t launch_recover(const double* U, const double* Y, const int32_t* cell_
                           const double* lam_free, const double* lam_dir, double* out,
                           int64_t ncells, int ni, int nb, int na, int64_t nfree, hipStream_t st);
 // Recovery after a mixed-precision condensation: x0 = w - W lambda, then one
-// FP64 refinement x = x0 + X (b_i - A_ib lambda - A_ii x0) with the cell
-// system regenerated from the per-cell coefficients and X the kept FP32
+// FP64 refinement x = x0 + X (b_i - A_ib lambda - A_ii x0) with the interior
+// rows of the cell system applied term by term from the per-cell
+// coefficients (the residual of the combined right-hand side keeps the
+// cancellation error at the level of the direct solve) and X the kept FP32
 // inverse of A_ii.
 hipError_t launch_recover_mixed(const Tables& T, const MeshDev& M, const double* W,
                                 const float* Xinv, const int32_t* cell_dofs,

@@ -2585,56 +2585,71 @@ hipError_t launch_recover(const double* U, const double* Y, const int32_t* cell_
   return hipGetLastError();
 }
 
-template <int PHYS>
+// One wavefront per cell; lane l works on interior rows l + 64 t.
+template <int PHYS, int KMAX>
 __global__ void __launch_bounds__(256)
     recover_mixed_kernel(FusedGen<PHYS> gen, const double* __restrict__ W,
                          const float* __restrict__ Xinv, const int32_t* __restrict__ cell_dofs,
                          const double* __restrict__ lam_free, const double* __restrict__ lam_dir,
                          double* __restrict__ out) {
-  __shared__ FusedShared sh;
-  __shared__ double s_lam[kRecoverMaxNb], s_x[kMixedMaxNi], s_r[kMixedMaxNi];
-  const int ni = gen.T.ni, nb = gen.T.nb, na = gen.T.na;
+  constexpr int WPB = 4;
+  __shared__ FusedShared s_sh[WPB];
+  __shared__ double s_v[WPB][kMixedMaxNi + kRecoverMaxNb];  // [x0; lambda]
+  __shared__ double s_r[WPB][kMixedMaxNi];
+  const Tables& T = gen.T;
+  const int ni = T.ni, nb = T.nb, na = T.na, nqc = T.nqc, ncol = nb + 1;
   const int64_t ncells = gen.M.ncells;
-  const int tid = threadIdx.x;
-  for (int64_t cell = blockIdx.x; cell < ncells; cell += gridDim.x) {
-    gen.begin(cell, sh);
-    for (int j = tid; j < nb; j += blockDim.x) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int64_t cell = (int64_t)blockIdx.x * WPB + w; cell < ncells;
+       cell += (int64_t)gridDim.x * WPB) {
+    for (int t = lane; t < kMaxAlpha; t += 64) s_sh[w].alpha[t] = gen.M.alpha[cell * kMaxAlpha + t];
+    for (int t = lane; t < nqc * 4; t += 64)
+      s_sh[w].src[t >> 2][t & 3] = gen.M.srcq[cell * (int64_t)nqc * 4 + t];
+    for (int j = lane; j < nb; j += 64) {
       const int32_t d = cell_dofs[cell * nb + j];
-      s_lam[j] = d >= 0 ? lam_free[d] : lam_dir[-(int64_t)d - 1];
+      s_v[w][ni + j] = d >= 0 ? lam_free[d] : lam_dir[-(int64_t)d - 1];
     }
-    __syncthreads();
-    const double* Wc = W + cell * (int64_t)(nb + 1) * ni;
-    // x0 = w - W lambda and rhs = b_i - A_ib lambda, one interior row per thread
-    for (int r = tid; r < ni; r += blockDim.x) {
-      double x0 = Wc[(int64_t)nb * ni + r];
-      double rhs = gen.rhs(sh, r);
-      for (int j = 0; j < nb; ++j) {
-        x0 = fma(-Wc[(int64_t)j * ni + r], s_lam[j], x0);
-        rhs = fma(-gen.entry(sh, r, ni + j), s_lam[j], rhs);
+    wave_sync_lds();
+    const double* Wc = W + cell * (int64_t)ncol * ni;
+    double x0[KMAX];
+#pragma unroll
+    for (int t = 0; t < KMAX; ++t) {
+      const int r = lane + 64 * t;
+      double x = 0.0;
+      if (r < ni) {
+        x = Wc[(int64_t)nb * ni + r];
+        for (int j = 0; j < nb; ++j) x = fma(-Wc[(int64_t)j * ni + r], s_v[w][ni + j], x);
+        s_v[w][r] = x;
       }
-      s_x[r] = x0;
-      s_r[r] = rhs;
+      x0[t] = x;
     }
-    __syncthreads();
-    // residual of the interior equations at x0 (FP64)
-    for (int r = tid; r < ni; r += blockDim.x) {
-      double res = s_r[r];
-      for (int k = 0; k < ni; ++k) res = fma(-gen.entry(sh, r, k), s_x[k], res);
-      s_r[r] = res;  // each thread rewrites only its own row
+    wave_sync_lds();
+    // residual of the interior equations at [x0; lambda], term by term (FP64)
+#pragma unroll
+    for (int t = 0; t < KMAX; ++t) {
+      const int r = lane + 64 * t;
+      if (r >= ni) continue;
+      double res = gen.rhs(s_sh[w], r);
+      const int k1 = T.prog_rptr[r + 1];
+      for (int k = T.prog_rptr[r]; k < k1; ++k)
+        res = fma(-(s_sh[w].alpha[T.prog_rid[k]] * T.prog_rval[k]), s_v[w][T.prog_rcol[k]], res);
+      s_r[w][r] = res;
     }
-    __syncthreads();
-    // x = x0 + X res, scattered to [q of all cells; u of all cells]
+    wave_sync_lds();
     const float* Xc = Xinv + cell * (int64_t)ni * ni;
-    for (int r = tid; r < ni; r += blockDim.x) {
+#pragma unroll
+    for (int t = 0; t < KMAX; ++t) {
+      const int r = lane + 64 * t;
+      if (r >= ni) continue;
       double acc = 0.0;
-      for (int k = 0; k < ni; ++k) acc = fma((double)Xc[(int64_t)r * ni + k], s_r[k], acc);
-      const double x = s_x[r] + acc;
+      for (int k = 0; k < ni; ++k) acc = fma((double)Xc[(int64_t)r * ni + k], s_r[w][k], acc);
+      const double x = x0[t] + acc;
       if (r < na)
         out[cell * na + r] = x;
       else
         out[ncells * na + cell * (ni - na) + (r - na)] = x;
     }
-    __syncthreads();
+    wave_sync_lds();
   }
 }
 
@@ -2642,19 +2657,20 @@ hipError_t launch_recover_mixed(const Tables& T, const MeshDev& M, const double*
                                 const float* Xinv, const int32_t* cell_dofs,
                                 const double* lam_free, const double* lam_dir, double* out,
                                 int64_t nfree, hipStream_t st) {
-  if (T.ni > kMixedMaxNi || T.nb > kRecoverMaxNb) return hipErrorInvalidValue;
+  if (T.ni > kMixedMaxNi || T.nb > kRecoverMaxNb || !W || !Xinv) return hipErrorInvalidValue;
   if (M.ncells > 0) {
-    const int grid = (int)std::min<int64_t>(M.ncells, 256 * 8);
+    const int64_t blocks = (M.ncells + 3) / 4;
+    const int grid = (int)(blocks < 65536 ? blocks : 65536);
     ClockScope clk_(st, "recover_mixed");
+#define HDG_RM(P_, K_)                                                                        \
+  hipLaunchKernelGGL((recover_mixed_kernel<P_, K_>), dim3(grid), dim3(256), 0, st,            \
+                     FusedGen<P_>{T, M}, W, Xinv, cell_dofs, lam_free, lam_dir, out)
     if (T.phys == kDarcy) {
-      FusedGen<kDarcy> gen{T, M};
-      hipLaunchKernelGGL(recover_mixed_kernel<kDarcy>, dim3(grid), dim3(256), 0, st, gen, W,
-                         Xinv, cell_dofs, lam_free, lam_dir, out);
+      if (T.ni <= 64) HDG_RM(kDarcy, 1); else HDG_RM(kDarcy, 2);
     } else {
-      FusedGen<kElasticity> gen{T, M};
-      hipLaunchKernelGGL(recover_mixed_kernel<kElasticity>, dim3(grid), dim3(256), 0, st, gen,
-                         W, Xinv, cell_dofs, lam_free, lam_dir, out);
+      if (T.ni <= 64) HDG_RM(kElasticity, 1); else HDG_RM(kElasticity, 2);
     }
+#undef HDG_RM
   }
   if (nfree > 0) {
     ClockScope clk_(st, "recover_copy");
