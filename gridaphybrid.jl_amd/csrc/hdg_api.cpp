@@ -106,6 +106,10 @@ struct hdg_mesh {
   // than getrf's U and L^-1 P [A_ib b_i]
   bool mixed = false;
   bool fp64_fallback = false;
+  // kernel family of FP64 condensations (-1: the default for the class) and
+  // its grid; the family of the last condensation decides the recovery
+  int family = -1, gridmul = 0;
+  int used_family = -1;
   DevBuf d_fptr, d_pptr, d_pfacet, d_pcell, d_plpos, d_gcell, d_glpos;
   DevBuf d_colptr, d_rowval, d_nzval, d_rhs;
   const double* src_qp = nullptr;
@@ -150,7 +154,7 @@ struct CellLayout {
   explicit CellLayout(const TablesHost& t)
       : S((size_t)t.nb * t.nb),
         g((size_t)t.nb),
-        U((size_t)t.ni * (t.ni + 1) / 2),
+        U(factor_slot(sys_shape(t))),
         Y((size_t)(t.nb + 1) * t.ni),
         alpha(72),
         srcq((size_t)t.nqc * 4) {}
@@ -554,13 +558,14 @@ int hdg_mesh_create(hdg_ctx* ctx, const hdg_problem* prob, const hdg_mesh_desc* 
     // tables: one allocation, doubles first then bytes
     std::vector<const std::vector<double>*> dv = {&th.Mpp, &th.Gpq, &th.Gqp, &th.Fqp, &th.Fqq,
                                                    &th.Fpm, &th.Fqm, &th.Fmm, &th.Bp, &th.Bq,
-                                                   &th.xq};
+                                                   &th.xq, &th.fb_minv, &th.fb_kfrag};
     size_t nd = 0;
     for (auto* v : dv) nd += v->size();
     size_t bytes = nd * sizeof(double) + th.prog_val.size() * sizeof(double) +
                    (th.prog_val0.size() + th.prog_xval.size() + th.prog_rval.size()) *
                        sizeof(double) +
-                   (th.prog_off.size() + th.prog_rptr.size()) * sizeof(int32_t) +
+                   (th.prog_off.size() + th.prog_rptr.size() + th.prog_rbsplit.size()) *
+                       sizeof(int32_t) +
                    th.prog_head.size() * sizeof(uint32_t) + th.prog_rcol.size() * sizeof(uint16_t) +
                    th.pmask.size() + th.qmask.size() + th.prog_id.size() + th.prog_xid.size() +
                    th.prog_rid.size() + 16;
@@ -591,6 +596,9 @@ int hdg_mesh_create(hdg_ctx* ctx, const hdg_problem* prob, const hdg_mesh_desc* 
     size_t off_rp = off;
     std::memcpy(host.data() + off, th.prog_rptr.data(), th.prog_rptr.size() * sizeof(int32_t));
     off += th.prog_rptr.size() * sizeof(int32_t);
+    size_t off_rs = off;
+    std::memcpy(host.data() + off, th.prog_rbsplit.data(), th.prog_rbsplit.size() * sizeof(int32_t));
+    off += th.prog_rbsplit.size() * sizeof(int32_t);
     size_t off_ph = off;
     std::memcpy(host.data() + off, th.prog_head.data(), th.prog_head.size() * sizeof(uint32_t));
     off += th.prog_head.size() * sizeof(uint32_t);
@@ -633,7 +641,7 @@ int hdg_mesh_create(hdg_ctx* ctx, const hdg_problem* prob, const hdg_mesh_desc* 
     T.n = th.n;
     T.nqc = th.nqc;
     const double** ptrs[] = {&T.Mpp, &T.Gpq, &T.Gqp, &T.Fqp, &T.Fqq, &T.Fpm,
-                             &T.Fqm, &T.Fmm, &T.Bp,  &T.Bq,  &T.xq};
+                             &T.Fqm, &T.Fmm, &T.Bp,  &T.Bq,  &T.xq, &T.fb_minv, &T.fb_kfrag};
     for (size_t i = 0; i < dv.size(); ++i) *ptrs[i] = reinterpret_cast<const double*>(base + offs[i]);
     T.pmask = reinterpret_cast<const uint8_t*>(base + off_pm);
     T.qmask = reinterpret_cast<const uint8_t*>(base + off_qm);
@@ -646,9 +654,14 @@ int hdg_mesh_create(hdg_ctx* ctx, const hdg_problem* prob, const hdg_mesh_desc* 
     T.prog_xid = reinterpret_cast<const uint8_t*>(base + off_xi);
     T.prog_xval = reinterpret_cast<const double*>(base + off_xv);
     T.prog_rptr = reinterpret_cast<const int32_t*>(base + off_rp);
+    T.prog_rbsplit = reinterpret_cast<const int32_t*>(base + off_rs);
     T.prog_rcol = reinterpret_cast<const uint16_t*>(base + off_rc);
     T.prog_rid = reinterpret_cast<const uint8_t*>(base + off_ri);
     T.prog_rval = reinterpret_cast<const double*>(base + off_rv);
+    T.fb_nc = th.fb_nc;
+    std::memcpy(T.fb_cid, th.fb_cid, sizeof(T.fb_cid));
+    if (th.fb_minv.empty()) T.fb_minv = nullptr;
+    if (th.fb_kfrag.empty()) T.fb_kfrag = nullptr;
   }
   m->x0.assign(desc->x0, desc->x0 + nc * D);
   m->jac.assign(desc->jac, desc->jac + nc * D * D);
@@ -727,7 +740,7 @@ int hdg_footprint_cartesian(const hdg_problem* prob, const int64_t* cells, int p
   if (precision == HDG_MIXED && mixed_supported(th.ni, th.nb))
     b += (int64_t)mixed_workspace_bytes(th.ni, th.nb, nc) + nc * (int64_t)th.ni * th.ni * 4;
   else
-    b += (int64_t)condense_workspace_bytes(th.ni, th.nb, -1);
+    b += (int64_t)condense_workspace_bytes(sys_shape(th), -1, nc);
   (void)ni;
   *device_bytes = b;
   return HDG_OK;
@@ -921,6 +934,11 @@ int hdg_condense(hdg_mesh* m, int precision) {
   out.g = m->pg;
   out.Y = m->pY;
   out.info = m->pinfo;
+  {
+    const CellLayout L(m->th);
+    out.u_stride = (int64_t)L.U;
+    out.y_stride = (int64_t)L.Y;
+  }
   m->fp64_fallback = false;
   if (precision == HDG_MIXED && mixed_supported(m->th.ni, m->th.nb)) {
     if (!m->pXinv) {
@@ -937,9 +955,12 @@ int hdg_condense(hdg_mesh* m, int precision) {
     m->mixed = true;
   } else {
     out.U = m->pU;
-    HIP_TRY(ctx, launch_condense_fused(m->tdev, m->meshdev(), out, ctx->stream, ctx->ws),
+    const int fam = m->family >= 0 ? m->family : condense_default_family(sys_shape(m->tdev));
+    HIP_TRY(ctx, launch_condense_fused(m->tdev, m->meshdev(), out, ctx->stream, ctx->ws, true,
+                                       fam, m->gridmul),
             "condense kernel");
     m->mixed = false;
+    m->used_family = fam;
   }
   m->condensed = true;
   return HDG_OK;
@@ -1032,6 +1053,33 @@ int hdg_local_factors(hdg_mesh* m, const double** U, const double** Y) {
   return m->condensed ? HDG_OK : fail(m->ctx, HDG_ERR_STATE, "no factors before condensation");
 }
 
+int hdg_mesh_factor_layout(hdg_mesh* m, hdg_factor_layout* lay) {
+  if (!m || !lay) return HDG_ERR_ARG;
+  if (!m->condensed) return fail(m->ctx, HDG_ERR_STATE, "no factors before condensation");
+  const bool flux = !m->mixed && m->used_family == kFamilyFluxSchur;
+  const CellLayout L(m->th);
+  lay->rows = flux ? m->th.ni - m->th.na : m->th.ni;
+  lay->family = m->mixed ? -1 : m->used_family;
+  lay->u_stride = (int64_t)L.U;
+  lay->y_stride = (int64_t)L.Y;
+  return HDG_OK;
+}
+
+int hdg_mesh_set_family(hdg_mesh* m, int family, int grid_per_cu) {
+  if (!m) return HDG_ERR_ARG;
+  if (m->batch) return fail(m->ctx, HDG_ERR_STATE, "the batch chooses the family of its members");
+  if (family >= 0) {
+    int fams[16];
+    const int nf = condense_variants(sys_shape(m->tdev), fams, 16);
+    if (std::find(fams, fams + nf, family) == fams + nf)
+      return fail(m->ctx, HDG_ERR_ARG, "kernel family cannot condense this class");
+  }
+  if (grid_per_cu < 0) return fail(m->ctx, HDG_ERR_ARG, "negative grid");
+  m->family = family;
+  m->gridmul = grid_per_cu;
+  return HDG_OK;
+}
+
 int hdg_recover(hdg_mesh* m, const double* lam_free, double* free_values) {
   if (!m) return HDG_ERR_ARG;
   hdg_ctx* ctx = m->ctx;
@@ -1046,10 +1094,20 @@ int hdg_recover(hdg_mesh* m, const double* lam_free, double* free_values) {
             "mixed recover kernel");
     return HDG_OK;
   }
+  if (m->used_family == kFamilyFluxSchur) {
+    HIP_TRY(ctx, launch_recover_flux(m->tdev, m->meshdev(), m->pU, m->pY,
+                                     m->d_cell_dofs.as<int32_t>(), lam_free,
+                                     m->d_lam_dir.as<double>(), free_values, m->nfree,
+                                     ctx->stream),
+            "flux recover kernels");
+    return HDG_OK;
+  }
+  const CellLayout L(m->th);
   HIP_TRY(ctx, launch_recover(m->pU, m->pY,
                               m->d_cell_dofs.as<int32_t>(), lam_free,
                               m->d_lam_dir.as<double>(), free_values, m->ncells, m->th.ni,
-                              m->th.nb, m->th.na, m->nfree, ctx->stream),
+                              m->th.nb, m->th.na, m->nfree, ctx->stream, (int64_t)L.U,
+                              (int64_t)L.Y),
           "recover kernel");
   return HDG_OK;
 }
@@ -1096,6 +1154,9 @@ struct hdg_batch {
       o.U = U.as<double>();
       o.Y = Y.as<double>();
       o.info = infos.as<int32_t>();
+      const CellLayout L(lead->th);
+      o.u_stride = (int64_t)L.U;
+      o.y_stride = (int64_t)L.Y;
       return o;
     }
   };
@@ -1185,8 +1246,9 @@ int autotune(hdg_batch* b) {
   for (auto& cp : b->classes) {
     hdg_batch::Class& c = *cp;
     const int n = c.info.ni + c.info.nb;
-    int fams[8];
-    const int nf = condense_variants(c.info.ni, c.info.nb, fams, 8);
+    int fams[16];
+    const SysShape sh = sys_shape(c.lead->tdev);
+    const int nf = condense_variants(sh, fams, 16);
     int gms[8];
     int ncand = 0;
     for (int i = 0; i < nf; ++i) ncand += condense_grid_options(fams[i], gms, 8);
@@ -1196,7 +1258,7 @@ int autotune(hdg_batch* b) {
     float best = 0.f;
     for (int i = 0; i < nf && rc == HDG_OK; ++i) {
       const int ng = condense_grid_options(fams[i], gms, 8);
-      hipError_t e = workspace_reserve(b->ws, condense_workspace_bytes(c.info.ni, c.info.nb, fams[i]));
+      hipError_t e = workspace_reserve(b->ws, condense_workspace_bytes(sh, fams[i], sample));
       if (e != hipSuccess) {
         rc = hipfail(ctx, e, "autotune workspace");
         break;
@@ -1241,7 +1303,8 @@ int capture(hdg_batch* b, int precision) {
   for (auto& c : b->classes)
     need = std::max(need, class_mixed(*c, precision)
                               ? mixed_workspace_bytes(c->info.ni, c->info.nb, c->info.ncells)
-                              : condense_workspace_bytes(c->info.ni, c->info.nb, c->info.family));
+                              : condense_workspace_bytes(sys_shape(c->lead->tdev), c->info.family,
+                                                         c->info.ncells));
   HIP_TRY(ctx, workspace_reserve(b->ws, need), "batch workspace");
   // FP32 inverses kept for the recovery of mixed classes (allocated once)
   for (auto& cp : b->classes) {
@@ -1308,7 +1371,7 @@ int hdg_batch_create(hdg_ctx* ctx, int nmesh, hdg_mesh* const* meshes, int flags
       c->info.nb = m->th.nb;
       c->info.physics = m->prob.physics;
       c->info.dim = m->prob.dim;
-      c->info.family = condense_default_family(m->th.ni, m->th.nb);
+      c->info.family = condense_default_family(sys_shape(m->tdev));
     }
     c->members.push_back(i);
     c->info.nmesh++;
@@ -1420,6 +1483,7 @@ int hdg_batch_launch(hdg_batch* b, int precision) {
       hdg_mesh* m = b->meshes[i];
       m->condensed = true;
       m->mixed = c->mixed;
+      m->used_family = c->mixed ? -1 : c->info.family;
       m->fp64_fallback = false;
     }
   }

@@ -16,12 +16,14 @@
 // Vandermonde solve in centred monomials).
 #include <cmath>
 #include <cstring>
+#include <utility>
 
 #include "hdg_internal.h"
 
 namespace hdg {
 void build_program(TablesHost& t);
 int inline_program(TablesHost& t, std::string& err);
+void detect_flux_block(TablesHost& t);
 namespace {
 
 // Gauss-Legendre nodes/weights on [0,1]: Newton on P_n from the asymptotic
@@ -403,6 +405,7 @@ int build_tables(int phys, int D, int space, int kp, int kq, int km, int quad_de
   for (int i = 0; i < nm; ++i)
     for (int a = 0; a < D - 1; ++a) t.trace_nodes[(size_t)i * (D - 1) + a] = bm.node(i, a);
   build_program(t);
+  detect_flux_block(t);
   return inline_program(t, err);
 }
 
@@ -628,7 +631,117 @@ int inline_program(TablesHost& t, std::string& err) {
         t.prog_rval[at] = t.prog_val[k];
       }
     }
+  // terms are in column order within a row: the boundary columns are a suffix
+  t.prog_rbsplit.assign((size_t)ni, 0);
+  for (int r = 0; r < ni; ++r) {
+    int k = t.prog_rptr[r];
+    while (k < t.prog_rptr[r + 1] && t.prog_rcol[k] < ni) ++k;
+    t.prog_rbsplit[r] = k;
+  }
   return 0;
 }
 
+}  // namespace hdg
+
+namespace hdg {
+// ---------------------------------------------------------------------------
+// Kronecker flux block.  Scans the entry program of the leading na x na block
+// of A_ii: every entry must be either empty or one term whose value is the
+// scalar mass-matrix entry M[i][j] (Mpp) and whose coefficient id depends on
+// the component pair (a, b) only.  Then A_qq = C (x) M per cell with
+// C[a][b] = alpha[cid(a,b)], and A_qq^-1 = C^-1 (x) M^-1.  M^-1 is computed
+// once here: Gauss-Jordan with partial pivoting in long double, then two
+// steps of iterative refinement of each column.
+// ---------------------------------------------------------------------------
+void detect_flux_block(TablesHost& t) {
+  t.fb_nc = 0;
+  t.fb_minv.clear();
+  t.fb_kfrag.clear();
+  const int np = t.np, na = t.na, n = t.n;
+  if (np <= 0 || na <= 0 || na % np != 0 || na >= t.ni) return;
+  const int nc = na / np;
+  if (nc > 6) return;
+  int cid[36];
+  for (int& c : cid) c = -2;  // -2: not seen yet
+  for (int col = 0; col < na; ++col)
+    for (int row = 0; row < na; ++row) {
+      const int a = row / np, i = row % np, b = col / np, j = col % np;
+      const double m = t.Mpp[(size_t)i * np + j];
+      const size_t e = (size_t)col * n + row;
+      const int k0 = t.prog_off[e], k1 = t.prog_off[e + 1];
+      int& c = cid[a * 6 + b];
+      if (k1 == k0) {
+        if (m != 0.0) {
+          if (c >= 0) return;
+          c = -1;
+        }
+        continue;
+      }
+      if (k1 - k0 != 1 || t.prog_val[k0] != m) return;
+      const int id = t.prog_id[k0];
+      if (c == -1 || (c >= 0 && c != id)) return;
+      c = id;
+    }
+  std::vector<long double> A((size_t)np * np), X((size_t)np * np, 0.0L);
+  for (int i = 0; i < np * np; ++i) A[i] = t.Mpp[i];
+  for (int i = 0; i < np; ++i) X[(size_t)i * np + i] = 1.0L;
+  std::vector<long double> W = A;
+  for (int c = 0; c < np; ++c) {
+    int p = c;
+    for (int r = c + 1; r < np; ++r)
+      if (std::fabs(W[(size_t)r * np + c]) > std::fabs(W[(size_t)p * np + c])) p = r;
+    if (W[(size_t)p * np + c] == 0.0L) return;
+    if (p != c)
+      for (int k = 0; k < np; ++k) {
+        std::swap(W[(size_t)p * np + k], W[(size_t)c * np + k]);
+        std::swap(X[(size_t)p * np + k], X[(size_t)c * np + k]);
+      }
+    const long double d = W[(size_t)c * np + c];
+    for (int k = 0; k < np; ++k) {
+      W[(size_t)c * np + k] /= d;
+      X[(size_t)c * np + k] /= d;
+    }
+    for (int r = 0; r < np; ++r) {
+      if (r == c) continue;
+      const long double f = W[(size_t)r * np + c];
+      if (f == 0.0L) continue;
+      for (int k = 0; k < np; ++k) {
+        W[(size_t)r * np + k] -= f * W[(size_t)c * np + k];
+        X[(size_t)r * np + k] -= f * X[(size_t)c * np + k];
+      }
+    }
+  }
+  // X <- X + X (I - A X), twice
+  for (int it = 0; it < 2; ++it) {
+    std::vector<long double> R((size_t)np * np), D((size_t)np * np);
+    for (int i = 0; i < np; ++i)
+      for (int j = 0; j < np; ++j) {
+        long double s = i == j ? 1.0L : 0.0L;
+        for (int k = 0; k < np; ++k) s -= A[(size_t)i * np + k] * X[(size_t)k * np + j];
+        R[(size_t)i * np + j] = s;
+      }
+    for (int i = 0; i < np; ++i)
+      for (int j = 0; j < np; ++j) {
+        long double s = 0.0L;
+        for (int k = 0; k < np; ++k) s += X[(size_t)i * np + k] * R[(size_t)k * np + j];
+        D[(size_t)i * np + j] = s;
+      }
+    for (int i = 0; i < np * np; ++i) X[i] += D[i];
+  }
+  t.fb_minv.resize((size_t)np * np);
+  for (int i = 0; i < np * np; ++i) t.fb_minv[i] = (double)X[i];
+  // fragment order of the f64 MFMA A operand: lane l of k-step s of row tile
+  // rt holds row 16 rt + l % 16, column 4 s + l / 16
+  const int KS = (na + 3) / 4, RTZ = (KS + 3) / 4;
+  t.fb_kfrag.assign((size_t)RTZ * KS * 64, 0.0);
+  for (int rt = 0; rt < RTZ; ++rt)
+    for (int s = 0; s < KS; ++s)
+      for (int l = 0; l < 64; ++l) {
+        const int i = 16 * rt + (l & 15), k = 4 * s + (l >> 4);
+        if (i < na && k < na && i / np == k / np)
+          t.fb_kfrag[((size_t)rt * KS + s) * 64 + l] = t.fb_minv[(size_t)(i % np) * np + k % np];
+      }
+  for (int k = 0; k < 36; ++k) t.fb_cid[k] = (int8_t)(cid[k] >= 0 ? cid[k] : -1);
+  t.fb_nc = nc;
+}
 }  // namespace hdg

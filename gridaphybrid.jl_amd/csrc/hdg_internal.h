@@ -56,6 +56,18 @@ struct TablesHost {
   std::vector<uint16_t> prog_rcol;
   std::vector<uint8_t> prog_rid;
   std::vector<double> prog_rval;
+  std::vector<int32_t> prog_rbsplit;  // [ni] first term of row r with a boundary column
+  // Leading flux block of A_ii with Kronecker structure C (x) M: entry
+  // ((a,i),(b,j)) = C[a][b] M[i][j] with C[a][b] = alpha[fb_cid[6a+b]] per
+  // cell (fb_cid < 0: identically zero) and M the np x np reference mass
+  // matrix of the scalar flux basis.  fb_nc = 0 when the program has no such
+  // block.  fb_minv = M^-1, row-major.
+  // fb_kfrag: the A operands of (I (x) M^-1) in MFMA fragment order,
+  // [row tile t][k-step s][lane] = (I (x) M^-1)[16t + lane%16][4s + lane/16].
+  int fb_nc = 0;
+  int8_t fb_cid[36] = {};
+  std::vector<double> fb_minv;
+  std::vector<double> fb_kfrag;
 };
 
 // Device view of the tables (pointers into one device allocation).
@@ -89,6 +101,11 @@ struct Tables {
   const uint16_t* prog_rcol;
   const uint8_t* prog_rid;
   const double* prog_rval;
+  const int32_t* prog_rbsplit;
+  int fb_nc;              // flux-block components (0: no Kronecker flux block)
+  int8_t fb_cid[36];      // coefficient id of C[a][b] (< 0: zero)
+  const double* fb_minv;  // [np][np] inverse of the scalar flux mass matrix
+  const double* fb_kfrag; // (I (x) M^-1) in f64 MFMA A-operand fragment order
 };
 
 // Device view of a mesh for the fused element generator.
@@ -115,6 +132,16 @@ struct CondenseOut {
   double* X = nullptr;     // [ncells][ni] back-substituted rhs (backward map)
   int32_t* info = nullptr; // [ncells]
   float* Xinv = nullptr;   // [ncells][ni][ni] FP32 inverse of A_ii (mixed path, kept)
+  // per-cell strides of U and Y (0: packed for this ni, i.e. ni(ni+1)/2 and
+  // (nb+1) ni); the flux-block path stores the reduced factors in the
+  // per-cell slots of the full interior so every family shares one layout
+  int64_t u_stride = 0, y_stride = 0;
+  // (grid elimination kernels) the multipliers and the pivot order, for
+  // solves with other right-hand sides: L[cell * u_stride + j * ni + r] =
+  // multiplier of interior row r at step j (rows pivoted later than j),
+  // P[cell * u_stride + k] = the row at LAPACK position k
+  double* L = nullptr;
+  double* P = nullptr;
 };
 
 // Dense cell systems in HBM for the map-level API.
@@ -173,19 +200,50 @@ size_t mixed_workspace_bytes(int ni, int nb, int64_t ncells);
 hipError_t launch_condense_mixed(const Tables& T, const MeshDev& M, const CondenseOut& out,
                                  hipStream_t st, Workspace& ws, bool prep, int maxit,
                                  int32_t* iters);
-// kernel families able to condense an (ni, nb) system; returns their count
-int condense_variants(int ni, int nb, int* fams, int max);
-int condense_default_family(int ni, int nb);
+// What the kernel families need to know of a class: the local sizes and
+// whether the Kronecker flux-block elimination applies (flux_na = n_flux).
+struct SysShape {
+  int ni = 0, nb = 0;
+  int flux_na = 0;
+  bool flux = false;
+};
+SysShape sys_shape(const Tables& T);
+SysShape sys_shape(const TablesHost& t);
+// Doubles per cell of the factor slot ("U"): ni(ni+1)/2 for the elimination
+// families; for classes where the flux-block path applies, room for its
+// reduced factors and the interior blocks its recovery refines with:
+//   [U m(m+1)/2][L m*m][P m][A_uu m*m][A_qu na*m][A_uq m*na],  m = ni - na.
+size_t factor_slot(const SysShape& sh);
+// Family id of the flux-block Schur complement: the flux unknowns are
+// eliminated through C^-1 (x) M^-1 and the reduced system of the remaining
+// ni - n_flux interior unknowns is condensed by an elimination kernel.  The
+// resident factors are then those of the reduced system.
+constexpr int kFamilyFluxSchur = 8;
+// kernel families able to condense a class; returns their count
+int condense_variants(const SysShape& sh, int* fams, int max);
+int condense_default_family(const SysShape& sh);
 // grid sizes (resident workgroups per CU) worth timing for a family
 int condense_grid_options(int fam, int* gm, int max);
-size_t condense_workspace_bytes(int ni, int nb, int fam);
+size_t condense_workspace_bytes(const SysShape& sh, int fam, int64_t ncells);
+hipError_t launch_condense_flux(const Tables& T, const MeshDev& M, const CondenseOut& out,
+                                hipStream_t st, Workspace& ws, int gridmul);
+// Recovery after a flux-block condensation: the reduced back substitution
+// (U, Y of the reduced system) gives x_u, x_q = (C^-1 (x) M^-1)(b_q - A_qu x_u
+// - A_qb lambda); then FP64 refinement against the residual of the full
+// interior system (the reduced system is worse conditioned than A_ii).
+hipError_t launch_recover_flux(const Tables& T, const MeshDev& M, const double* U,
+                               const double* Y, const int32_t* cell_dofs, const double* lam_free,
+                               const double* lam_dir, double* out, int64_t nfree,
+                               hipStream_t st);
+constexpr int kFluxRefineSteps = 2;  // FP64 refinement steps of the flux recovery (at most)
 hipError_t launch_condense_dense(const DenseIn& in, int64_t ncells, const CondenseOut& out,
                                  hipStream_t st, Workspace& ws);
 hipError_t launch_backward_dense(const DenseIn& in, const double* x, int64_t ncells, double* xi,
                                  int32_t* info, hipStream_t st, Workspace& ws);
 hipError_t launch_recover(const double* U, const double* Y, const int32_t* cell_dofs,
                           const double* lam_free, const double* lam_dir, double* out,
-                          int64_t ncells, int ni, int nb, int na, int64_t nfree, hipStream_t st);
+                          int64_t ncells, int ni, int nb, int na, int64_t nfree, hipStream_t st,
+                          int64_t u_stride = 0, int64_t y_stride = 0);
 // Recovery after a mixed-precision condensation: x0 = w - W lambda, then one
 // FP64 refinement x = x0 + X (b_i - A_ib lambda - A_ii x0) with the interior
 // rows of the cell system applied term by term from the per-cell

@@ -249,6 +249,14 @@ __global__ void __launch_bounds__(128) prep_kernel(Tables T, MeshDev M) {
   }
 }
 
+// Per-cell strides of the factor outputs (see CondenseOut).
+__device__ inline int64_t ustride(const CondenseOut& o, int ni) {
+  return o.u_stride ? o.u_stride : (int64_t)ni * (ni + 1) / 2;
+}
+__device__ inline int64_t ystride(const CondenseOut& o, int ni, int nb) {
+  return o.y_stride ? o.y_stride : (int64_t)(nb + 1) * ni;
+}
+
 struct FusedShared {
   double alpha[kMaxAlpha];  // per-cell coefficients of the entry program
   double src[kMaxQP][4];    // |det J| * source values at cell quadrature points
@@ -480,7 +488,9 @@ __global__ void __launch_bounds__(PR * PC)
         for (int r = 0; r < R; ++r) {
           const int row = tr + PR * r;
           const bool use = ((act >> r) & 1u) && row != p;
-          s_lcol[buf][row] = use ? colv[r] * rinv : 0.0;
+          const double lv = use ? colv[r] * rinv : 0.0;
+          s_lcol[buf][row] = lv;
+          if (out.L && use && row < ni) out.L[cell * out.u_stride + (int64_t)j * ni + row] = lv;
         }
         if (tr == 0) {
           s_piv[buf] = p;
@@ -525,6 +535,8 @@ __global__ void __launch_bounds__(PR * PC)
     }
     __syncthreads();  // last step's pivot row / multipliers fully consumed
     const int info = s_info;
+    if (out.P)
+      for (int k = tid; k < ni; k += NT) out.P[cell * out.u_stride + k] = (double)s_at[k];
 
     // ---------------- back substitution (backward map only) ---------------
     if (BACKSUB) {
@@ -603,9 +615,9 @@ __global__ void __launch_bounds__(PR * PC)
           const int k = s_pos[row];
           if (col < ni) {
             if (out.U && col >= k)
-              out.U[cell * ((int64_t)ni * (ni + 1) / 2) + col * (col + 1) / 2 + k] = v;
+              out.U[cell * ustride(out, ni) + col * (col + 1) / 2 + k] = v;
           } else {
-            if (out.Y) out.Y[(cell * (nb + 1) + (col - ni)) * ni + k] = v;
+            if (out.Y) out.Y[cell * ystride(out, ni, nb) + (int64_t)(col - ni) * ni + k] = v;
             if (out.X && col == n) out.X[cell * ni + k] = v;
           }
         }
@@ -728,7 +740,7 @@ __global__ void __launch_bounds__(64)
         if (out.g) out.g[cell * nb + br] = a[N];
       } else {
         const int k = mypos;
-        const int64_t ubase = cell * ((int64_t)ni * (ni + 1) / 2);
+        const int64_t ubase = cell * ustride(out, ni);
         if (out.U) {
 #pragma unroll
           for (int c = 0; c < NI; ++c)
@@ -736,7 +748,7 @@ __global__ void __launch_bounds__(64)
         }
         if (out.Y) {
 #pragma unroll
-          for (int c = NI; c < NCOL; ++c) out.Y[(cell * (nb + 1) + (c - NI)) * ni + k] = a[c];
+          for (int c = NI; c < NCOL; ++c) out.Y[cell * ystride(out, ni, nb) + (int64_t)(c - NI) * ni + k] = a[c];
         }
         if (out.X) out.X[cell * ni + k] = a[N];
       }
@@ -1031,9 +1043,9 @@ __global__ void __launch_bounds__(64 * NW)
           const int k = s_pos[row];
           if (col < ni) {
             if (out.U && col >= k)
-              out.U[cell * ((int64_t)ni * (ni + 1) / 2) + col * (col + 1) / 2 + k] = v;
+              out.U[cell * ustride(out, ni) + col * (col + 1) / 2 + k] = v;
           } else {
-            if (out.Y) out.Y[(cell * (nb + 1) + (col - ni)) * ni + k] = v;
+            if (out.Y) out.Y[cell * ystride(out, ni, nb) + (int64_t)(col - ni) * ni + k] = v;
             if (out.X && col == n) out.X[cell * ni + k] = v;
           }
         }
@@ -1157,7 +1169,7 @@ __global__ void __launch_bounds__(256)
         if (cc >= kb || cc < jj) continue;
         const int col = j0 + cc, k = j0 + jj;
         const double v = s_P[cc][s_piv[jj]];
-        if (out.U) out.U[cell * ((int64_t)ni * (ni + 1) / 2) + col * (col + 1) / 2 + k] = v;
+        if (out.U) out.U[cell * ustride(out, ni) + col * (col + 1) / 2 + k] = v;
         if (BACKSUB) M[(int64_t)col * ld + s_piv[jj]] = v;  // U row kept for back substitution
       }
       for (int c = jt + tid; c < ncol; c += NT) {
@@ -1174,9 +1186,9 @@ __global__ void __launch_bounds__(256)
         for (int jj = 0; jj < kb; ++jj) {
           const int k = j0 + jj;
           if (c < ni) {
-            if (out.U) out.U[cell * ((int64_t)ni * (ni + 1) / 2) + c * (c + 1) / 2 + k] = u[jj];
+            if (out.U) out.U[cell * ustride(out, ni) + c * (c + 1) / 2 + k] = u[jj];
           } else {
-            if (out.Y) out.Y[(cell * (nb + 1) + (c - ni)) * ni + k] = u[jj];
+            if (out.Y) out.Y[cell * ystride(out, ni, nb) + (int64_t)(c - ni) * ni + k] = u[jj];
           }
           if (BACKSUB) M[(int64_t)c * ld + s_piv[jj]] = u[jj];
         }
@@ -1360,7 +1372,7 @@ __global__ void __launch_bounds__(256)
         if (cc >= kb || cc < jj) continue;
         const int col = j0 + cc, k = j0 + jj;
         const double v = s_P[cc][s_piv[jj]];
-        if (out.U) out.U[cell * ((int64_t)ni * (ni + 1) / 2) + col * (col + 1) / 2 + k] = v;
+        if (out.U) out.U[cell * ustride(out, ni) + col * (col + 1) / 2 + k] = v;
         if (BACKSUB) M[(int64_t)s_piv[jj] * ldc + col] = v;
       }
       for (int c = jt + tid; c < ncol; c += NT) {
@@ -1378,9 +1390,9 @@ __global__ void __launch_bounds__(256)
             s_U[jj][c - jt] = v;
             const int k = j0 + jj;
             if (c < ni) {
-              if (out.U) out.U[cell * ((int64_t)ni * (ni + 1) / 2) + c * (c + 1) / 2 + k] = v;
+              if (out.U) out.U[cell * ustride(out, ni) + c * (c + 1) / 2 + k] = v;
             } else if (out.Y) {
-              out.Y[(cell * (nb + 1) + (c - ni)) * ni + k] = v;
+              out.Y[cell * ystride(out, ni, nb) + (int64_t)(c - ni) * ni + k] = v;
             }
             if (BACKSUB) M[(int64_t)pr * ldc + c] = v;
           }
@@ -1623,7 +1635,7 @@ __global__ void __launch_bounds__(kTileThreads, 1)
           const int jj = t / KB, cc = t - jj * KB;
           if (cc >= kb || cc < jj) continue;
           const int col = j0 + cc, k = j0 + jj;
-          out.U[cell * ((int64_t)ni * (ni + 1) / 2) + col * (col + 1) / 2 + k] =
+          out.U[cell * ustride(out, ni) + col * (col + 1) / 2 + k] =
               s_P[cc][s_piv[jj]];
         }
       }
@@ -1655,9 +1667,9 @@ __global__ void __launch_bounds__(kTileThreads, 1)
             s_U[jj][c - jt] = v;
             const int k = j0 + jj;
             if (c < ni) {
-              if (out.U) out.U[cell * ((int64_t)ni * (ni + 1) / 2) + c * (c + 1) / 2 + k] = v;
+              if (out.U) out.U[cell * ustride(out, ni) + c * (c + 1) / 2 + k] = v;
             } else if (out.Y) {
-              out.Y[(cell * (nb + 1) + (c - ni)) * ni + k] = v;
+              out.Y[cell * ystride(out, ni, nb) + (int64_t)(c - ni) * ni + k] = v;
             }
           }
         }
@@ -1717,11 +1729,10 @@ __global__ void __launch_bounds__(256)
     recover_kernel(const double* __restrict__ U, const double* __restrict__ Y,
                    const int32_t* __restrict__ cell_dofs, const double* __restrict__ lam_free,
                    const double* __restrict__ lam_dir, double* __restrict__ out, int64_t ncells,
-                   int ni, int nb, int na) {
+                   int ni, int nb, int na, int64_t usz, int64_t ysz) {
   constexpr int WPB = 4;  // wavefronts per block
   __shared__ double s_lam[WPB][256];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int64_t usz = (int64_t)ni * (ni + 1) / 2;
   for (int64_t cell = (int64_t)blockIdx.x * WPB + w; cell < ncells;
        cell += (int64_t)gridDim.x * WPB) {
     for (int j = lane; j < nb; j += 64) {
@@ -1730,7 +1741,7 @@ __global__ void __launch_bounds__(256)
     }
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    const double* Yc = Y + cell * (int64_t)(nb + 1) * ni;
+    const double* Yc = Y + cell * ysz;
     double r[KMAX];
 #pragma unroll
     for (int t = 0; t < KMAX; ++t) {
@@ -2312,7 +2323,9 @@ enum Family {
   kFamLarge = 4,
   kFamPanel = 5,
   kFamTile = 6,
-  kFamCount = 7
+  kFamGrid16 = 7,
+  kFamFlux = kFamilyFluxSchur,
+  kFamCount = 9
 };
 
 static bool wave_class(int ni, int nb) {
@@ -2331,8 +2344,9 @@ static bool family_ok(int fam, int ni, int nb) {
     case kFamLarge: return n > 64 && n <= kLargeMaxN;
     case kFamPanel: return n > 64 && n <= kPanelMaxN;
     case kFamTile: return n > 64 && n <= kTileMaxN;
+    case kFamGrid16: return n >= 48 && n + 1 <= 96;
   }
-  return false;
+  return false;  // kFamFlux depends on the tables: launch_condense_fused
 }
 
 // default choice: the measured-fastest family for the size
@@ -2435,6 +2449,8 @@ static hipError_t launch_condense(const Gen& gen, const CondenseOut& out, int64_
   Bucket b;
   if (fam == kFamGrid32) {
     b = Bucket{32, 16, 6, 11};
+  } else if (fam == kFamGrid16) {
+    b = Bucket{16, 16, 6, 6};
   } else if (!pick_bucket(ni + nb, b)) {
     return hipErrorInvalidValue;
   }
@@ -2449,21 +2465,23 @@ static hipError_t launch_condense(const Gen& gen, const CondenseOut& out, int64_
   HDG_LAUNCH(8, 8, 6, 6)
   HDG_LAUNCH(8, 8, 9, 9)
   HDG_LAUNCH(32, 16, 6, 11)
+  HDG_LAUNCH(16, 16, 6, 6)
 #undef HDG_LAUNCH
   return hipErrorInvalidValue;
 }
 
-int condense_variants(int ni, int nb, int* fams, int max) {
+int condense_variants(const SysShape& sh, int* fams, int max) {
   int k = 0;
   for (int f = 0; f < kFamCount && k < max; ++f)
-    if (family_ok(f, ni, nb)) fams[k++] = f;
+    if (family_ok(f, sh.ni, sh.nb) || (f == kFamFlux && sh.flux)) fams[k++] = f;
   return k;
 }
 
 int condense_grid_options(int fam, int* gm, int max) {
   // resident workgroups per CU worth timing for each family (0 = fixed grid)
   static const int wave[] = {16, 32, 48, 64}, grid8[] = {20, 40, 80}, col[] = {1, 2, 3},
-                   g32[] = {2, 3, 4}, pan[] = {1, 2}, til[] = {1}, large[] = {0};
+                   g32[] = {2, 3, 4}, pan[] = {1, 2}, til[] = {1}, large[] = {0},
+                   g16[] = {2, 4, 6};
   const int* v = large;
   int n = 1;
   switch (fam) {
@@ -2473,6 +2491,7 @@ int condense_grid_options(int fam, int* gm, int max) {
     case kFamGrid32: v = g32, n = 3; break;
     case kFamPanel: v = pan, n = 2; break;
     case kFamTile: v = til, n = 1; break;
+    case kFamGrid16: v = g16, n = 3; break;
     default: break;
   }
   int k = 0;
@@ -2480,10 +2499,19 @@ int condense_grid_options(int fam, int* gm, int max) {
   return k;
 }
 
-int condense_default_family(int ni, int nb) { return default_family(ni, nb); }
+static size_t flux_workspace_bytes(int nr, int64_t ncells);
 
-size_t condense_workspace_bytes(int ni, int nb, int fam) {
-  if (fam < 0) fam = default_family(ni, nb);
+int condense_default_family(const SysShape& sh) {
+  // small systems stay with the register-resident kernels (one wavefront or
+  // an 8x8 grid per cell), which no two-stage scheme beats
+  if (sh.flux && !wave_class(sh.ni, sh.nb) && sh.ni + sh.nb > 96) return kFamFlux;
+  return default_family(sh.ni, sh.nb);
+}
+
+size_t condense_workspace_bytes(const SysShape& sh, int fam, int64_t ncells) {
+  const int ni = sh.ni, nb = sh.nb;
+  if (fam < 0) fam = condense_default_family(sh);
+  if (fam == kFamFlux) return flux_workspace_bytes(ni - sh.flux_na + nb, ncells);
   if (fam == kFamPanel) {
     const int n = ni + nb;
     return panel_bytes(n, panel_grid(n, INT64_MAX, 0) * 2, nullptr);  // any grid multiplier <= 2x
@@ -2524,6 +2552,8 @@ hipError_t launch_condense_fused(const Tables& T, const MeshDev& M, const Conden
     hipError_t e = launch_prep(T, M, st);
     if (e != hipSuccess) return e;
   }
+  if (fam < 0) fam = condense_default_family(sys_shape(T));
+  if (fam == kFamFlux) return launch_condense_flux(T, M, out, st, ws, gridmul);
   if (T.phys == kDarcy) {
     FusedGen<kDarcy> gen{T, M};
     return launch_condense<FusedGen<kDarcy>, false>(gen, out, M.ncells, T.ni, T.nb, st, ws, fam,
@@ -2552,8 +2582,11 @@ hipError_t launch_backward_dense(const DenseIn& in, const double* x, int64_t nce
 
 hipError_t launch_recover(const double* U, const double* Y, const int32_t* cell_dofs,
                           const double* lam_free, const double* lam_dir, double* out,
-                          int64_t ncells, int ni, int nb, int na, int64_t nfree, hipStream_t st) {
+                          int64_t ncells, int ni, int nb, int na, int64_t nfree, hipStream_t st,
+                          int64_t u_stride, int64_t y_stride) {
   if (nb > 256 || ni > 256) return hipErrorInvalidValue;
+  const int64_t usz = u_stride ? u_stride : (int64_t)ni * (ni + 1) / 2;
+  const int64_t ysz = y_stride ? y_stride : (int64_t)(nb + 1) * ni;
   if (ncells > 0) {
     const int64_t blocks = (ncells + 3) / 4;
     const int grid = (int)(blocks < 65536 ? blocks : 65536);
@@ -2561,19 +2594,19 @@ hipError_t launch_recover(const double* U, const double* Y, const int32_t* cell_
       {
       ClockScope clk_(st, "recover");
       hipLaunchKernelGGL(recover_kernel<1>, dim3(grid), dim3(256), 0, st, U, Y, cell_dofs,
-                         lam_free, lam_dir, out, ncells, ni, nb, na);
+                         lam_free, lam_dir, out, ncells, ni, nb, na, usz, ysz);
     }
     else if (ni <= 128)
       {
       ClockScope clk_(st, "recover");
       hipLaunchKernelGGL(recover_kernel<2>, dim3(grid), dim3(256), 0, st, U, Y, cell_dofs,
-                         lam_free, lam_dir, out, ncells, ni, nb, na);
+                         lam_free, lam_dir, out, ncells, ni, nb, na, usz, ysz);
     }
     else
       {
       ClockScope clk_(st, "recover");
       hipLaunchKernelGGL(recover_kernel<4>, dim3(grid), dim3(256), 0, st, U, Y, cell_dofs,
-                         lam_free, lam_dir, out, ncells, ni, nb, na);
+                         lam_free, lam_dir, out, ncells, ni, nb, na, usz, ysz);
     }
   }
   if (nfree > 0)
@@ -2671,6 +2704,691 @@ hipError_t launch_recover_mixed(const Tables& T, const MeshDev& M, const double*
       if (T.ni <= 64) HDG_RM(kElasticity, 1); else HDG_RM(kElasticity, 2);
     }
 #undef HDG_RM
+  }
+  if (nfree > 0) {
+    ClockScope clk_(st, "recover_copy");
+    hipLaunchKernelGGL(copy_kernel, dim3(1024), dim3(256), 0, st, lam_free,
+                       out + M.ncells * T.ni, nfree);
+  }
+  return hipGetLastError();
+}
+
+// --------------------------------------------------------------------------
+// Flux-block Schur complement (family "flux_schur").  When the leading flux
+// block of A_ii is A_qq = C (x) M (per-cell C, fixed scalar mass matrix M;
+// see detect_flux_block in hdg_fe.cpp), the flux unknowns are eliminated by
+// two dense products instead of na pivot steps:
+//   Z = A_qq^-1 [A_qu A_qb b_q] = (I (x) M^-1) (C^-1 (x) I) G1   (na x nc)
+//   R = [A_uu A_ub b_u; A_bu A_bb b_b] - [A_uq; A_bq] Z            (nr x nc)
+// with nr = ni - na + nb and nc = nr + 1.  R is the augmented system of the
+// remaining interior unknowns and the boundary; the regular elimination
+// kernels condense it (ReducedGen) into S, g and the reduced factors that the
+// recovery back-substitutes.  This is exact block Gaussian elimination; A_qq
+// is C (x) M with M symmetric positive definite, so the flux block needs no
+// pivoting.
+//
+// One workgroup (NW wavefronts) per cell.  Both products run on
+// v_mfma_f64_16x16x4f64:
+//  * G1 is generated into LDS ([k][col], odd leading dimension), mixed over
+//    components by C^-1 in place, and read as B operands.  The A operands of
+//    (I (x) M^-1) are the same for every cell: the host stores them in
+//    fragment order (one coalesced 512-byte load per k-step), and blocks
+//    between different components are skipped (zmask).
+//  * Wavefront w owns column tiles w, w + NW, ... of Z and R.  Its Z tiles
+//    stay in registers: the accumulator layout of the f64 MFMA (row
+//    4v + lane/16 of the tile in register v) is the B-operand layout of
+//    k-steps 4t..4t+3 of the second product.
+//  * Rows na + 16 rr .. of the cell system ([A_uq A_uu A_ub b_u; ...]) are
+//    generated tile by tile into LDS ([col][17], double buffered, aliasing
+//    the dead G1 region) and shared by all wavefronts: the first na columns
+//    are the A operands (negated), the rest initialises the accumulators.
+// --------------------------------------------------------------------------
+constexpr int kFluxWaves = 6;
+constexpr int kFluxMaxNp = 64;
+constexpr int kFluxTileLd = 17;  // leading dimension of a generated row tile
+
+struct FluxPlan {
+  int nr, nc, KS;     // reduced rows, columns (incl. rhs), k-steps of the flux block
+  int ldg;            // leading dimension of G1 in LDS (odd)
+  int ncols;          // columns of a row tile: 4 KS + 16 ceil(nc / 16)
+  uint32_t zmask[8];  // bit s of zmask[t]: block (row tile t, k-step s) of I (x) M^-1
+                      // is not identically zero
+};
+
+// Inverse of the ncomp x ncomp coefficient matrix C[a][b] = alpha[cid[6a+b]]
+// (Gauss-Jordan with partial pivoting by one thread, in the LDS work array
+// w[6][12]).  Returns 1 when singular.
+__device__ inline int flux_cinv(int ncomp, const int8_t* cid, const double* alpha, double* w,
+                                double* cinv) {
+  for (int r = 0; r < 6; ++r)
+    for (int c = 0; c < 12; ++c) w[12 * r + c] = 0.0;
+  for (int r = 0; r < ncomp; ++r) {
+    for (int c = 0; c < ncomp; ++c) {
+      const int id = cid[6 * r + c];
+      w[12 * r + c] = id >= 0 ? alpha[id] : 0.0;
+    }
+    w[12 * r + 6 + r] = 1.0;
+  }
+  for (int c = 0; c < ncomp; ++c) {
+    int p = c;
+    for (int r = c + 1; r < ncomp; ++r)
+      if (fabs(w[12 * r + c]) > fabs(w[12 * p + c])) p = r;
+    if (w[12 * p + c] == 0.0) return 1;
+    if (p != c)
+      for (int k = 0; k < 12; ++k) {
+        const double t = w[12 * p + k];
+        w[12 * p + k] = w[12 * c + k];
+        w[12 * c + k] = t;
+      }
+    const double d = 1.0 / w[12 * c + c];
+    for (int k = 0; k < 12; ++k) w[12 * c + k] *= d;
+    for (int r = 0; r < ncomp; ++r) {
+      if (r == c) continue;
+      const double f = w[12 * r + c];
+      if (f == 0.0) continue;
+      for (int k = 0; k < 12; ++k) w[12 * r + k] = fma(-f, w[12 * c + k], w[12 * r + k]);
+    }
+  }
+  for (int r = 0; r < 6; ++r)
+    for (int c = 0; c < 6; ++c) cinv[6 * r + c] = (r < ncomp && c < ncomp) ? w[12 * r + 6 + c] : 0.0;
+  return 0;
+}
+
+template <int PHYS, int CT, int KSM>
+__global__ void __launch_bounds__(64 * kFluxWaves)
+    flux_schur_kernel(FusedGen<PHYS> gen, const double* __restrict__ kfrag, int64_t first,
+                      int64_t count, double* __restrict__ R, FluxPlan fp,
+                      double* __restrict__ blocks, int64_t bstride) {
+  constexpr int NW = kFluxWaves, NT = 64 * NW, LT = kFluxTileLd;
+  constexpr int RTZ = (KSM + 3) / 4;  // row tiles of Z
+  const Tables& T = gen.T;
+  const int na = T.na, np = T.np, ncomp = T.fb_nc;
+  const int nr = fp.nr, nc = fp.nc, KS = fp.KS, ldg = fp.ldg, ncols = fp.ncols;
+  const int nu = T.ni - na;
+  const int nk = 4 * KS;  // flux rows incl. zero padding
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int i16 = lane & 15, q4 = lane >> 4;
+  extern __shared__ double s_dyn[];  // G1 [nk][ldg], later row tiles [2][ncols][LT]
+  __shared__ FusedShared s_gen;
+  __shared__ double s_cinv[36];
+  __shared__ double s_cw[72];
+  __shared__ int s_bad;
+  const int tsz = ncols * LT;
+
+  for (int64_t lc = blockIdx.x; lc < count; lc += gridDim.x) {
+    gen.begin(first + lc, s_gen);
+    __syncthreads();
+    if (tid == 0) s_bad = flux_cinv(ncomp, T.fb_cid, s_gen.alpha, s_cw, s_cinv);
+    double* Rc = R + lc * (int64_t)nc * nr;  // [nc][nr] column-major
+    // ---- G1 = [A_qu A_qb b_q] into LDS ------------------------------------
+    for (int t = tid; t < nk * ldg; t += NT) {
+      const int c = t / nk, k = t - c * nk;
+      if (c >= ldg) continue;
+      s_dyn[k * ldg + c] = (k < na && c < nc) ? gen.entry(s_gen, k, na + c) : 0.0;
+    }
+    __syncthreads();
+    // interior blocks kept for the recovery: [A_uu][A_qu][A_uq], column-major
+    double* Bk = blocks + lc * bstride;
+    for (int t = tid; t < na * nu; t += NT) {
+      const int c = t / na, k = t - c * na;
+      Bk[nu * nu + t] = s_dyn[k * ldg + c];  // A_qu [u column][flux row]
+    }
+    __syncthreads();  // G1 read before the component mix below rewrites it
+    if (s_bad) {
+      // singular C: a zero system makes the reduced elimination report the cell
+      for (int t = tid; t < nc * nr; t += NT) Rc[t] = 0.0;
+      __syncthreads();
+      continue;
+    }
+    // ---- G1 <- (C^-1 (x) I) G1, one (scalar index, column) pair per thread --
+    for (int t = tid; t < np * nc; t += NT) {
+      const int i = t / nc, c = t - i * nc;
+      double v[6];
+#pragma unroll
+      for (int b = 0; b < 6; ++b) v[b] = b < ncomp ? s_dyn[(b * np + i) * ldg + c] : 0.0;
+#pragma unroll
+      for (int a = 0; a < 6; ++a) {
+        if (a >= ncomp) break;
+        double x = 0.0;
+#pragma unroll
+        for (int b = 0; b < 6; ++b) x = fma(s_cinv[6 * a + b], v[b], x);
+        s_dyn[(a * np + i) * ldg + c] = x;
+      }
+    }
+    __syncthreads();
+
+    // ---- Z = (I (x) M^-1) G1, one 16-row tile at a time --------------------
+    double zf[CT][KSM];
+#pragma unroll
+    for (int rt = 0; rt < RTZ; ++rt) {
+      dbl4 acc[CT];
+#pragma unroll
+      for (int ct = 0; ct < CT; ++ct) acc[ct] = dbl4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll 2
+      for (int s = 0; s < KS; ++s) {
+        if (!((fp.zmask[rt] >> s) & 1u)) continue;  // wave-uniform
+        const double a = kfrag[((int64_t)rt * KS + s) * 64 + lane];
+        const double* brow = s_dyn + (4 * s + q4) * ldg + i16;
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct)
+          if (16 * (w + NW * ct) < nc)  // wave-uniform: tiles past the last column stay idle
+            acc[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, brow[16 * (w + NW * ct)], acc[ct],
+                                                           0, 0, 0);
+      }
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+        if (4 * rt + v < KSM) {
+#pragma unroll
+          for (int ct = 0; ct < CT; ++ct) zf[ct][4 * rt + v] = acc[ct][v];
+        }
+    }
+    __syncthreads();  // G1 fully read: its region now holds the row tiles
+
+    // ---- R = R0 - [A_uq; A_bq] Z, one 16-row tile of R at a time ----------
+    const int RTR = (nr + 15) >> 4;
+#pragma unroll 1
+    for (int rr = 0; rr < RTR; ++rr) {
+      double* g = s_dyn + (rr & 1) * tsz;  // [ncols][LT]: cols < nk: -A_(u,b)q, then R0
+      for (int t = tid; t < 16 * ncols; t += NT) {
+        const int col = t >> 4, i = t & 15, row = 16 * rr + i;
+        double v = 0.0;
+        if (row < nr) {
+          if (col < nk) {
+            if (col < na) v = -gen.entry(s_gen, na + row, col);
+          } else if (col - nk < nc) {
+            v = gen.entry(s_gen, na + row, na + col - nk);
+          }
+        }
+        g[col * LT + i] = v;
+      }
+      __syncthreads();
+      if (16 * rr < nu)  // rows of the remaining interior unknowns: A_uq and A_uu
+        for (int t = tid; t < 16 * (na + nu); t += NT) {
+          const int col = t >> 4, i = t & 15, row = 16 * rr + i;
+          if (row >= nu) continue;
+          if (col < na)
+            Bk[nu * nu + na * nu + (int64_t)col * nu + row] = -g[col * LT + i];
+          else
+            Bk[(int64_t)(col - na) * nu + row] = g[(nk + col - na) * LT + i];
+        }
+#pragma unroll
+      for (int ct = 0; ct < CT; ++ct) {
+        const int c0 = 16 * (w + NW * ct);
+        if (c0 >= nc) continue;  // wave-uniform
+        const int col = c0 + i16;
+        dbl4 acc;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) acc[v] = g[(nk + col) * LT + q4 + 4 * v];
+#pragma unroll
+        for (int s = 0; s < KSM; ++s)
+          if (s < KS)
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(g[(4 * s + q4) * LT + i16], zf[ct][s], acc,
+                                                       0, 0, 0);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int row = 16 * rr + q4 + 4 * v;
+          if (row < nr && col < nc) Rc[(int64_t)col * nr + row] = acc[v];
+        }
+      }
+    }
+    __syncthreads();  // LDS reuse by the next cell
+  }
+}
+
+// The reduced augmented system R of a chunk of cells as an element generator
+// for the regular elimination kernels (rows and columns in reduced order, the
+// last column is the right-hand side).
+struct ReducedGen {
+  const double* R;  // [count][nc][nr]
+  int nr;
+  using Shared = DenseShared;
+  __device__ void begin(int64_t, Shared&) const {}
+  __device__ double entry(const Shared&, int row, int col, int64_t cell) const {
+    return R[(cell * (nr + 1) + col) * (int64_t)nr + row];
+  }
+};
+
+// Elimination family for the reduced system: the grid kernels (no workspace;
+// they also keep the multipliers and the pivot order for the recovery).
+static int reduced_family(int ni, int nb) {
+  const int n = ni + nb;
+  if (n + 1 <= 72) return kFamGrid8;
+  if (n + 1 <= 96) return kFamGrid16;
+  if (n <= 192 && n + 1 <= 176) return kFamGrid32;
+  return -1;
+}
+
+// Register tiling of the flux kernel for a size: column tiles per wavefront
+// and the k-step bound of the unrolled loops.
+static bool flux_fits(int fb_nc, int na, int np, int ni, int nb, int* ct, int* ksm) {
+  if (fb_nc <= 0 || np > kFluxMaxNp || na <= 0 || na >= ni) return false;
+  const int nr = ni - na + nb, KS = (na + 3) / 4;
+  const int c = ((nr + 1 + 15) / 16 + kFluxWaves - 1) / kFluxWaves;
+  const int k = KS <= 8 ? 8 : KS <= 16 ? 16 : KS <= 24 ? 24 : 0;
+  if (k == 0 || c > 2 || c * k > 32) return false;
+  if (reduced_family(ni - na, nb) < 0) return false;
+  const int nu = ni - na;
+  if (nu > 64 || na > 128 || nb > 192) return false;  // one wavefront per cell in the recovery
+  const size_t ncp = 16 * (size_t)((nr + 1 + 15) / 16);
+  const size_t g1 = (size_t)4 * KS * (ncp | 1), tiles = 2 * (4 * (size_t)KS + ncp) * kFluxTileLd;
+  if (std::max(g1, tiles) * sizeof(double) > 150 * 1024) return false;
+  if (ct) *ct = c;
+  if (ksm) *ksm = k;
+  return true;
+}
+
+static bool flux_plan(const Tables& T, FluxPlan& fp, int* ct, int* ksm) {
+  if (!T.fb_minv || !T.fb_kfrag || !flux_fits(T.fb_nc, T.na, T.np, T.ni, T.nb, ct, ksm))
+    return false;
+  const int na = T.na, np = T.np;
+  fp.nr = T.ni - na + T.nb;
+  fp.nc = fp.nr + 1;
+  fp.KS = (na + 3) / 4;
+  const int ncp = 16 * ((fp.nc + 15) / 16);
+  fp.ldg = ncp | 1;
+  fp.ncols = 4 * fp.KS + ncp;
+  for (int t = 0; t < 8; ++t) {
+    fp.zmask[t] = 0;
+    for (int s = 0; s < fp.KS && s < 32; ++s) {
+      bool nz = false;
+      for (int i = 16 * t; i < std::min(16 * t + 16, na) && !nz; ++i)
+        for (int k = 4 * s; k < std::min(4 * s + 4, na) && !nz; ++k) nz = i / np == k / np;
+      if (nz) fp.zmask[t] |= 1u << s;
+    }
+  }
+  return true;
+}
+
+static size_t flux_lds(const FluxPlan& fp) {
+  const size_t g1 = (size_t)4 * fp.KS * fp.ldg, tiles = (size_t)2 * fp.ncols * kFluxTileLd;
+  return std::max(g1, tiles) * sizeof(double);
+}
+
+// Cells per flux chunk: the reduced systems of one chunk live in the workspace.
+static int64_t flux_chunk(int nr) {
+  const int64_t per = (int64_t)nr * (nr + 1) * (int64_t)sizeof(double);
+  const int64_t c = ((int64_t)512 << 20) / per;
+  return std::max<int64_t>(1024, std::min<int64_t>(16384, c & ~(int64_t)255));
+}
+
+static size_t flux_workspace_bytes(int nr, int64_t ncells) {
+  const int64_t chunk = std::min<int64_t>(std::max<int64_t>(ncells, 1), flux_chunk(nr));
+  return (size_t)chunk * nr * (nr + 1) * sizeof(double);
+}
+
+
+template <int PHYS>
+static hipError_t launch_flux_phys(const Tables& T, const MeshDev& M, const CondenseOut& out,
+                                   hipStream_t st, Workspace& ws, int gridmul) {
+  FluxPlan fp;
+  int ct = 0, ksm = 0;
+  if (!flux_plan(T, fp, &ct, &ksm)) return hipErrorInvalidValue;
+  const int nu = T.ni - T.na, nb = T.nb, nr = fp.nr;
+  const int rfam = reduced_family(nu, nb);
+  if (rfam < 0) return hipErrorInvalidValue;
+  const int64_t ncells = M.ncells;
+  if (ncells <= 0) return hipSuccess;
+  if (!out.U) return hipErrorInvalidValue;
+  const int64_t chunk = std::min<int64_t>(ncells, flux_chunk(nr));
+  hipError_t e = workspace_reserve(ws, flux_workspace_bytes(nr, ncells));
+  if (e != hipSuccess) return e;
+  const size_t lds = flux_lds(fp);
+  if (lds > 160 * 1024 - sizeof(FusedShared) - 1024) return hipErrorInvalidValue;
+  FusedGen<PHYS> gen{T, M};
+  // reduced factors and kept blocks in the per-cell slots of the class
+  const int64_t usz = out.u_stride, ysz = out.y_stride;
+  if (usz < (int64_t)factor_slot(sys_shape(T)) || ysz < (int64_t)(nb + 1) * nu)
+    return hipErrorInvalidValue;
+  const int64_t boff = (int64_t)nu * (nu + 1) / 2 + (int64_t)nu * nu + nu;  // after U, L, P
+  for (int64_t first = 0; first < ncells; first += chunk) {
+    const int64_t count = std::min<int64_t>(chunk, ncells - first);
+    const int grid = (int)std::min<int64_t>(count, 256 * 4);
+    {
+      ClockScope clk_(st, "flux_schur");
+#define HDG_FLUX(CT_, KS_)                                                                   \
+  if (ct == CT_ && ksm == KS_)                                                               \
+    hipLaunchKernelGGL((flux_schur_kernel<PHYS, CT_, KS_>), dim3(grid), dim3(64 * kFluxWaves), \
+                       lds, st, gen, T.fb_kfrag, first, count, ws.p, fp,                      \
+                       out.U + first * usz + boff, usz);
+      HDG_FLUX(1, 8)
+      HDG_FLUX(1, 16)
+      HDG_FLUX(1, 24)
+      HDG_FLUX(2, 8)
+      HDG_FLUX(2, 16)
+#undef HDG_FLUX
+      e = hipGetLastError();
+      if (e != hipSuccess) return e;
+    }
+    CondenseOut o;
+    o.S = out.S ? out.S + first * nb * nb : nullptr;
+    o.g = out.g ? out.g + first * nb : nullptr;
+    o.U = out.U ? out.U + first * usz : nullptr;
+    o.Y = out.Y ? out.Y + first * ysz : nullptr;
+    o.u_stride = usz;
+    o.y_stride = ysz;
+    o.L = o.U ? o.U + (int64_t)nu * (nu + 1) / 2 : nullptr;
+    o.P = o.L ? o.L + (int64_t)nu * nu : nullptr;
+    o.info = out.info ? out.info + first : nullptr;
+    ReducedGen rg{ws.p, nr};
+    Workspace none;
+    e = launch_condense<ReducedGen, false>(rg, o, count, nu, nb, st, none, rfam, gridmul);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+hipError_t launch_condense_flux(const Tables& T, const MeshDev& M, const CondenseOut& out,
+                                hipStream_t st, Workspace& ws, int gridmul) {
+  return T.phys == kDarcy ? launch_flux_phys<kDarcy>(T, M, out, st, ws, gridmul)
+                          : launch_flux_phys<kElasticity>(T, M, out, st, ws, gridmul);
+}
+
+size_t factor_slot(const SysShape& sh) {
+  const size_t dense = (size_t)sh.ni * (sh.ni + 1) / 2;
+  if (!sh.flux) return dense;
+  const size_t m = (size_t)(sh.ni - sh.flux_na), na = (size_t)sh.flux_na;
+  return std::max(dense, m * (m + 1) / 2 + 2 * m * m + m + 2 * na * m);
+}
+
+SysShape sys_shape(const Tables& T) {
+  SysShape sh;
+  sh.ni = T.ni;
+  sh.nb = T.nb;
+  if (T.fb_minv && flux_fits(T.fb_nc, T.na, T.np, T.ni, T.nb, nullptr, nullptr)) sh.flux_na = T.na;
+  sh.flux = sh.flux_na > 0;
+  return sh;
+}
+
+SysShape sys_shape(const TablesHost& t) {
+  SysShape sh;
+  sh.ni = t.ni;
+  sh.nb = t.nb;
+  if (!t.fb_minv.empty() && flux_fits(t.fb_nc, t.na, t.np, t.ni, t.nb, nullptr, nullptr))
+    sh.flux_na = t.na;
+  sh.flux = sh.flux_na > 0;
+  return sh;
+}
+
+// Flux recovery.  One wavefront per cell, with the reduced factors and the
+// interior blocks A_uu, A_qu, A_uq the condensation kept in the cell's slot:
+//   c   = b_i - A_ib lambda                 (boundary columns of the program)
+//   x_u = U^-1 (y - Y lambda)               (reduced factors)
+//   x_q = A_qq^-1 (c_q - A_qu x_u)          (A_qq^-1 = C^-1 (x) M^-1)
+// then up to kFluxRefineSteps FP64 refinement steps against the residual of
+// the full interior system, r = c - A_ii x, with the structured solve as the
+// approximate inverse:
+//   z = A_qq^-1 r_q,  d_u = R_uu^-1 (r_u - A_uq z),  d_q = z - A_qq^-1 A_qu d_u
+// (R_uu^-1 through the kept multipliers, pivot order and U).  The reduced
+// system R_uu is worse conditioned than A_ii when the flux block is small
+// (large kappa): the first x_u can be off by cond(R_uu) eps in the near-null
+// mode of A_uq A_qq^-1 A_qu, and the refinement brings it to the accuracy of
+// a direct FP64 solve of A_ii.  It stops once a correction is below 2^-48 of
+// the solution.
+constexpr int kFluxMaxNb = 192;
+
+template <int PHYS, int KQ>
+__global__ void __launch_bounds__(256)
+    recover_flux_kernel(FusedGen<PHYS> gen, const double* __restrict__ U,
+                        const double* __restrict__ Y, int64_t usz, int64_t ysz,
+                        const int32_t* __restrict__ cell_dofs, const double* __restrict__ lam_free,
+                        const double* __restrict__ lam_dir, double* __restrict__ out) {
+  constexpr int WPB = 4;
+  constexpr int NQ = 64 * KQ;  // flux rows per wavefront, at most
+  __shared__ FusedShared s_sh[WPB];
+  __shared__ double s_lam[WPB][kFluxMaxNb];
+  __shared__ double s_xq[WPB][NQ];  // x_q
+  __shared__ double s_xu[WPB][64];  // x_u, then corrections d_u
+  __shared__ double s_cq[WPB][NQ];  // c_q
+  __shared__ double s_q[WPB][NQ];   // flux work vector
+  __shared__ double s_z[WPB][NQ];
+  __shared__ double s_u[WPB][64];   // reduced work vector
+  __shared__ double s_c6[WPB][2][36];  // C and C^-1
+  __shared__ double s_cw[WPB][72];
+  const Tables& T = gen.T;
+  const int ni = T.ni, nb = T.nb, na = T.na, np = T.np, nu = ni - na, nqc = T.nqc;
+  const int ncomp = T.fb_nc;
+  const int64_t ncells = gen.M.ncells;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  double* lam = s_lam[w];
+  double* xq = s_xq[w];
+  double* xuv = s_xu[w];
+  double* cq = s_cq[w];
+  double* qv = s_q[w];
+  double* zv = s_z[w];
+  double* uv = s_u[w];
+
+  // v <- (Cm (x) Mm) v over the flux rows (v in LDS): Mm = M^-1 with Cm = C^-1
+  // for a solve, Mm = M with Cm = C for a product
+  auto flux_apply = [&](double* v, const double* Mm, const double* Cm) {
+    double y[KQ];
+#pragma unroll
+    for (int t = 0; t < KQ; ++t) {
+      const int r = lane + 64 * t;
+      double acc = 0.0;
+      if (r < na) {
+        const int a = r / np, i = r - a * np;
+        const double* mr = Mm + (int64_t)i * np;
+        const double* vr = v + a * np;
+        for (int j = 0; j < np; ++j) acc = fma(mr[j], vr[j], acc);
+      }
+      y[t] = acc;
+    }
+    wave_sync_lds();
+#pragma unroll
+    for (int t = 0; t < KQ; ++t) {
+      const int r = lane + 64 * t;
+      if (r < na) v[r] = y[t];
+    }
+    wave_sync_lds();
+#pragma unroll
+    for (int t = 0; t < KQ; ++t) {
+      const int r = lane + 64 * t;
+      double acc = 0.0;
+      if (r < na) {
+        const int a = r / np, i = r - a * np;
+        for (int b = 0; b < ncomp; ++b) acc = fma(Cm[6 * a + b], v[b * np + i], acc);
+      }
+      y[t] = acc;
+    }
+    wave_sync_lds();
+#pragma unroll
+    for (int t = 0; t < KQ; ++t) {
+      const int r = lane + 64 * t;
+      if (r < na) v[r] = y[t];
+    }
+    wave_sync_lds();
+  };
+  // U x = y over the reduced unknowns, lane k holding y_k / x_k (k < nu <= 64)
+  auto back_subst = [&](const double* Uc, double x) {
+#pragma unroll 1
+    for (int c = nu - 1; c >= 0; --c) {
+      const int64_t col0 = (int64_t)c * (c + 1) / 2;
+      const double ucc = Uc[col0 + c];
+      const double xc0 = __shfl(x, c, 64);
+      const double xc = ucc != 0.0 ? xc0 / ucc : 0.0;
+      if (lane < c) x = fma(-Uc[col0 + lane], xc, x);
+      if (lane == c) x = xc;
+    }
+    return x;
+  };
+
+  for (int64_t cell = (int64_t)blockIdx.x * WPB + w; cell < ncells;
+       cell += (int64_t)gridDim.x * WPB) {
+    for (int t = lane; t < kMaxAlpha; t += 64) s_sh[w].alpha[t] = gen.M.alpha[cell * kMaxAlpha + t];
+    for (int t = lane; t < nqc * 4; t += 64)
+      s_sh[w].src[t >> 2][t & 3] = gen.M.srcq[cell * (int64_t)nqc * 4 + t];
+    for (int j = lane; j < nb; j += 64) {
+      const int32_t d = cell_dofs[cell * nb + j];
+      lam[j] = d >= 0 ? lam_free[d] : lam_dir[-(int64_t)d - 1];
+    }
+    wave_sync_lds();
+    if (lane < 36) {
+      const int a = lane / 6, b = lane - 6 * (lane / 6);
+      const int id = T.fb_cid[lane];
+      s_c6[w][0][lane] = (a < ncomp && b < ncomp && id >= 0) ? s_sh[w].alpha[id] : 0.0;
+    }
+    if (lane == 0 && flux_cinv(ncomp, T.fb_cid, s_sh[w].alpha, s_cw[w], s_c6[w][1]))
+      for (int t = 0; t < 36; ++t) s_c6[w][1][t] = 0.0;
+    const double* Uc = U + cell * usz;
+    const double* Lc = Uc + (int64_t)nu * (nu + 1) / 2;  // multipliers [step][row]
+    const double* Pc = Lc + (int64_t)nu * nu;             // row at each position
+    const double* Auu = Pc + nu;                          // [u col][u row]
+    const double* Aqu = Auu + (int64_t)nu * nu;           // [u col][flux row]
+    const double* Auq = Aqu + (int64_t)na * nu;           // [flux col][u row]
+    const double* Yc = Y + cell * ysz;
+    // ---- c = b_i - A_ib lambda ------------------------------------------------
+    double cu = 0.0;
+#pragma unroll
+    for (int t = 0; t < KQ + 1; ++t) {
+      const int r = lane + 64 * t;
+      if (r < ni) {
+        double v = gen.rhs(s_sh[w], r);
+        const int k1 = T.prog_rptr[r + 1];
+        for (int k = T.prog_rbsplit[r]; k < k1; ++k)
+          v = fma(-(s_sh[w].alpha[T.prog_rid[k]] * T.prog_rval[k]), lam[T.prog_rcol[k] - ni], v);
+        if (r < na)
+          cq[r] = v;
+        else
+          uv[r - na] = v;
+      }
+    }
+    wave_sync_lds();
+    if (lane < nu) cu = uv[lane];
+    // ---- x_u = U^-1 (y - Y lambda) --------------------------------------------
+    double xu = 0.0;
+    if (lane < nu) {
+      xu = Yc[(int64_t)nb * nu + lane];
+      for (int j = 0; j < nb; ++j) xu = fma(-Yc[(int64_t)j * nu + lane], lam[j], xu);
+    }
+    xu = back_subst(Uc, xu);
+    if (lane < nu) xuv[lane] = xu;
+    wave_sync_lds();
+    // ---- x_q = A_qq^-1 (c_q - A_qu x_u) ----------------------------------------
+#pragma unroll
+    for (int t = 0; t < KQ; ++t) {
+      const int r = lane + 64 * t;
+      if (r < na) {
+        double v = cq[r];
+        for (int c = 0; c < nu; ++c) v = fma(-Aqu[(int64_t)c * na + r], xuv[c], v);
+        qv[r] = v;
+      }
+    }
+    wave_sync_lds();
+    flux_apply(qv, T.fb_minv, s_c6[w][1]);
+#pragma unroll
+    for (int t = 0; t < KQ; ++t) {
+      const int r = lane + 64 * t;
+      if (r < na) xq[r] = qv[r];
+    }
+    wave_sync_lds();
+    const int at = lane < nu ? (int)Pc[lane] : 0;
+    // ---- FP64 refinement against the full interior system ---------------------
+#pragma unroll 1
+    for (int it = 0; it < kFluxRefineSteps; ++it) {
+      // r_q = c_q - (C (x) M) x_q - A_qu x_u  (into zv);  r_u = c_u - A_uq x_q - A_uu x_u
+#pragma unroll
+      for (int t = 0; t < KQ; ++t) {
+        const int r = lane + 64 * t;
+        if (r < na) zv[r] = xq[r];
+      }
+      wave_sync_lds();
+      flux_apply(zv, T.Mpp, s_c6[w][0]);
+#pragma unroll
+      for (int t = 0; t < KQ; ++t) {
+        const int r = lane + 64 * t;
+        if (r < na) {
+          double v = cq[r] - zv[r];
+          for (int c = 0; c < nu; ++c) v = fma(-Aqu[(int64_t)c * na + r], xuv[c], v);
+          zv[r] = v;
+        }
+      }
+      double ru = 0.0;
+      if (lane < nu) {
+        ru = cu;
+        for (int j = 0; j < na; ++j) ru = fma(-Auq[(int64_t)j * nu + lane], xq[j], ru);
+        for (int c = 0; c < nu; ++c) ru = fma(-Auu[(int64_t)c * nu + lane], xuv[c], ru);
+      }
+      wave_sync_lds();
+      flux_apply(zv, T.fb_minv, s_c6[w][1]);  // z = A_qq^-1 r_q
+      // t = r_u - A_uq z, in LAPACK order, then d_u = U^-1 L^-1 P t
+      if (lane < nu) {
+        for (int j = 0; j < na; ++j) ru = fma(-Auq[(int64_t)j * nu + lane], zv[j], ru);
+        uv[lane] = ru;
+      }
+      wave_sync_lds();
+      double y = lane < nu ? uv[at] : 0.0;
+#pragma unroll 1
+      for (int i = 0; i < nu; ++i) {
+        const double yi = __shfl(y, i, 64);
+        if (lane > i && lane < nu) y = fma(-Lc[(int64_t)i * nu + at], yi, y);
+      }
+      const double du = back_subst(Uc, y);
+      if (lane < nu) uv[lane] = du;
+      wave_sync_lds();
+      // d_q = z - A_qq^-1 (A_qu d_u)
+#pragma unroll
+      for (int t = 0; t < KQ; ++t) {
+        const int r = lane + 64 * t;
+        if (r < na) {
+          double v = 0.0;
+          for (int c = 0; c < nu; ++c) v = fma(Aqu[(int64_t)c * na + r], uv[c], v);
+          qv[r] = v;
+        }
+      }
+      wave_sync_lds();
+      flux_apply(qv, T.fb_minv, s_c6[w][1]);
+      double dmax = lane < nu ? fabs(du) : 0.0, xmax = 0.0;
+#pragma unroll
+      for (int t = 0; t < KQ; ++t) {
+        const int r = lane + 64 * t;
+        if (r < na) {
+          const double d = zv[r] - qv[r];
+          const double x = xq[r] + d;
+          xq[r] = x;
+          dmax = fmax(dmax, fabs(d));
+          xmax = fmax(xmax, fabs(x));
+        }
+      }
+      if (lane < nu) {
+        const double x = xuv[lane] + du;
+        xuv[lane] = x;
+        xmax = fmax(xmax, fabs(x));
+      }
+      wave_sync_lds();
+      dmax = __builtin_bit_cast(double, wave_max_i64(__builtin_bit_cast(int64_t, dmax)));
+      xmax = __builtin_bit_cast(double, wave_max_i64(__builtin_bit_cast(int64_t, xmax)));
+      if (dmax <= 0x1p-48 * xmax) break;  // wave-uniform
+    }
+#pragma unroll
+    for (int t = 0; t < KQ; ++t) {
+      const int r = lane + 64 * t;
+      if (r < na) out[cell * na + r] = xq[r];
+    }
+    if (lane < nu) out[ncells * na + cell * nu + lane] = xuv[lane];
+    wave_sync_lds();
+  }
+}
+
+hipError_t launch_recover_flux(const Tables& T, const MeshDev& M, const double* U,
+                               const double* Y, const int32_t* cell_dofs, const double* lam_free,
+                               const double* lam_dir, double* out, int64_t nfree,
+                               hipStream_t st) {
+  const int na = T.na, nu = T.ni - na;
+  if (na > kMixedMaxNi || nu > 64 || T.nb > kFluxMaxNb) return hipErrorInvalidValue;
+  const int64_t usz = (int64_t)factor_slot(sys_shape(T)), ysz = (int64_t)(T.nb + 1) * T.ni;
+  if (M.ncells > 0) {
+    const int64_t blocks = (M.ncells + 3) / 4;
+    const int grid = (int)(blocks < 65536 ? blocks : 65536);
+    ClockScope clk_(st, "recover_flux");
+#define HDG_RF(P_, K_)                                                                      \
+  hipLaunchKernelGGL((recover_flux_kernel<P_, K_>), dim3(grid), dim3(256), 0, st,           \
+                     FusedGen<P_>{T, M}, U, Y, usz, ysz, cell_dofs, lam_free, lam_dir, out)
+    if (T.phys == kDarcy) {
+      if (na <= 64) HDG_RF(kDarcy, 1); else HDG_RF(kDarcy, 2);
+    } else {
+      if (na <= 64) HDG_RF(kElasticity, 1); else HDG_RF(kElasticity, 2);
+    }
+#undef HDG_RF
   }
   if (nfree > 0) {
     ClockScope clk_(st, "recover_copy");

@@ -293,7 +293,7 @@ class HybridAffineFEOperator:
     device until ``solve``."""
 
     def __init__(self, weakform, trial, test, bulk_fields, skeleton_fields, precision=FP64,
-                 ctx=None, assemble=True):
+                 ctx=None, assemble=True, family=None, grid_per_cu=0):
         if list(bulk_fields) != [1, 2] or list(skeleton_fields) != [3]:
             raise HDGError(_lib.HDG_ERR_UNSUPPORTED,
                            "hybrid operators here hybridize fields [1,2] onto trace field [3]")
@@ -315,8 +315,29 @@ class HybridAffineFEOperator:
                   "set_source")
         check(lib().hdg_mesh_set_dirichlet_mms(self.mesh), self.ctx.h, "dirichlet")
         self._batch = None
+        if family is not None:
+            self.set_family(family, grid_per_cu)
         if assemble:
             self.assemble()
+
+    def set_family(self, family, grid_per_cu=0):
+        """Kernel family of the FP64 condensation (None or -1: the default for
+        the class); see _lib.FAMILY_NAMES."""
+        fam = -1 if family is None else int(family)
+        check(lib().hdg_mesh_set_family(self.mesh, fam, int(grid_per_cu)), self.ctx.h,
+              "set_family")
+
+    def factor_layout(self):
+        """hdg_mesh_factor_layout: rows of the resident factors, kernel family of
+        the last condensation, per-cell strides of the U and Y slots."""
+        lay = _lib.FactorLayout()
+        check(lib().hdg_mesh_factor_layout(self.mesh, C.byref(lay)), self.ctx.h, "factor_layout")
+        return lay
+
+    def factor_rows(self):
+        """(rows of the resident factors, kernel family of the last condensation)."""
+        lay = self.factor_layout()
+        return int(lay.rows), int(lay.family)
 
     def assemble(self):
         nbad = C.c_int64(0)
@@ -357,20 +378,28 @@ class HybridAffineFEOperator:
         return Sh[0].transpose(0, 2, 1), Sh[1], Sh[2]
 
     def local_factors(self):
-        """(U, Y) host copies: U (ncells, ni, ni) upper factor in LAPACK row order,
-        Y (ncells, ni, nb+1) = L^-1 P [A_ib b_i]."""
+        """(U, Y) host copies: U (ncells, m, m) upper factor in LAPACK row order,
+        Y (ncells, m, nb+1) = L^-1 P [B b] of the system the recovery
+        back-substitutes: A_ii (m = ni) for the elimination families, the
+        reduced system after the flux block (m = ni - n_flux) for flux_schur."""
         U, Y = C.c_void_p(), C.c_void_p()
         check(lib().hdg_local_factors(self.mesh, C.byref(U), C.byref(Y)), self.ctx.h, "factors")
-        nc, ni, nb = self.info.ncells, self.info.ni, self.info.nb
+        nc, nb = self.info.ncells, self.info.nb
+        lay = self.factor_layout()
+        ni = int(lay.rows)
         if not U.value:   # mixed precision: W = A_ii^-1 [A_ib b_i], natural order
             Yh = np.empty((nc, nb + 1, ni))
             check(lib().hdg_memcpy_d2h(self.ctx.h, Yh.ctypes.data, Y, Yh.nbytes), self.ctx.h,
                   "d2h")
             return None, Yh.transpose(0, 2, 1)
-        Up = np.empty((nc, ni * (ni + 1) // 2))
-        Yh = np.empty((nc, nb + 1, ni))
-        check(lib().hdg_memcpy_d2h(self.ctx.h, Up.ctypes.data, U, Up.nbytes), self.ctx.h, "d2h")
-        check(lib().hdg_memcpy_d2h(self.ctx.h, Yh.ctypes.data, Y, Yh.nbytes), self.ctx.h, "d2h")
+        # per-cell slots are sized for the full interior; the factors of a
+        # reduced system sit at the start of each slot
+        Us = np.empty((nc, int(lay.u_stride)))
+        Ys = np.empty((nc, int(lay.y_stride)))
+        check(lib().hdg_memcpy_d2h(self.ctx.h, Us.ctypes.data, U, Us.nbytes), self.ctx.h, "d2h")
+        check(lib().hdg_memcpy_d2h(self.ctx.h, Ys.ctypes.data, Y, Ys.nbytes), self.ctx.h, "d2h")
+        Up = Us[:, :ni * (ni + 1) // 2]
+        Yh = Ys[:, :(nb + 1) * ni].reshape(nc, nb + 1, ni)
         Uf = np.zeros((nc, ni, ni))
         for c in range(ni):
             Uf[:, :c + 1, c] = Up[:, c * (c + 1) // 2: c * (c + 1) // 2 + c + 1]

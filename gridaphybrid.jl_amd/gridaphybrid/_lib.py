@@ -45,12 +45,13 @@ EXPORTS = [
     "hdg_local_factors", "hdg_batch_create", "hdg_batch_destroy", "hdg_batch_get_classes",
     "hdg_batch_launch", "hdg_batch_condense_assemble", "hdg_batch_recover",
     "hdg_mesh_refinement_stats", "hdg_profile_start", "hdg_profile_stop",
-    "hdg_footprint_cartesian",
+    "hdg_footprint_cartesian", "hdg_mesh_factor_layout", "hdg_mesh_set_family",
 ]
 
 BATCH_AUTOTUNE = 1
 FAMILY_NAMES = {0: "wave", 1: "grid8x8", 2: "colwave", 3: "grid32x16", 4: "blocked",
-                5: "panel_mfma", 6: "tile_mfma"}
+                5: "panel_mfma", 6: "tile_mfma", 7: "grid16x16", 8: "flux_schur"}
+FAMILY_FLUX_SCHUR = 8
 
 
 class HDGError(RuntimeError):
@@ -61,6 +62,11 @@ class HDGError(RuntimeError):
 
 class SingularError(HDGError):
     """A local A_ii had a zero pivot (the reference's @check info==0)."""
+
+
+class FactorLayout(C.Structure):
+    _fields_ = [("rows", C.c_int), ("family", C.c_int), ("u_stride", C.c_int64),
+                ("y_stride", C.c_int64)]
 
 
 class Fields(C.Structure):
@@ -156,6 +162,8 @@ def lib():
         L.hdg_mesh_singular_count.argtypes = [vp, C.POINTER(i64)]
         L.hdg_mesh_interpolate_trace.argtypes = [vp, vp]
         L.hdg_local_factors.argtypes = [vp, C.POINTER(vp), C.POINTER(vp)]
+        L.hdg_mesh_factor_layout.argtypes = [vp, C.POINTER(FactorLayout)]
+        L.hdg_mesh_set_family.argtypes = [vp, i32, i32]
         L.hdg_batch_create.argtypes = [vp, i32, C.POINTER(vp), i32, C.POINTER(vp)]
         L.hdg_batch_destroy.argtypes = [vp]
         L.hdg_batch_get_classes.argtypes = [vp, i32, C.POINTER(BatchClass), C.POINTER(i32)]

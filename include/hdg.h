@@ -259,8 +259,27 @@ int hdg_cell_blocks(hdg_mesh* mesh, const double** S, const double** g,
  * Y [ncells][nb+1][ni] = L^-1 P [A_ib b_i] in LAPACK row order.  The pivot
  * sequence is getrf's (largest magnitude, first position on ties).  After a
  * mixed-precision condensation *U is NULL and Y holds W = A_ii^-1 [A_ib b_i]
- * in natural interior order. */
+ * in natural interior order.  After a flux-block condensation (family 8) U
+ * and Y are the factors of the reduced system of the m = ni - n_flux
+ * non-flux interior unknowns, at the start of each cell's slot.  Cell c's
+ * slots start at c * u_stride and c * y_stride (hdg_mesh_factor_layout). */
 int hdg_local_factors(hdg_mesh* mesh, const double** U, const double** Y);
+/* Layout of the resident factors: rows m of the factored system (ni, or
+ * ni - n_flux after a flux-block condensation), the kernel family of the last
+ * condensation (-1 after a mixed-precision one) and the per-cell strides (in
+ * doubles) of the U and Y slots.  No reference counterpart. */
+typedef struct {
+  int rows;
+  int family;
+  int64_t u_stride;
+  int64_t y_stride;
+} hdg_factor_layout;
+int hdg_mesh_factor_layout(hdg_mesh* mesh, hdg_factor_layout* layout);
+/* Kernel family (and resident workgroups per CU, 0 = the family's default)
+ * of this mesh's FP64 condensations; family -1 restores the default for the
+ * class.  HDG_ERR_ARG when the family cannot condense the class.  Not for
+ * batch members (the batch picks per class).  No reference counterpart. */
+int hdg_mesh_set_family(hdg_mesh* mesh, int family, int grid_per_cu);
 
 /* solve!'s recovery (src/HybridAffineFEOperators.jl:89-150 with
  * BackwardStaticCondensationMap): lam_free device [nfree] (skeleton
@@ -304,9 +323,14 @@ typedef struct {
 /* Kernel families: 0 one wavefront per cell (fixed sizes), 1 8x8 grid with
  * register tiles, 2 column-per-wavefront with look-ahead, 3 32x16 grid,
  * 4 blocked HBM-workspace elimination, 5 blocked elimination with a
- * single-wavefront panel and FP64 MFMA trailing updates.  With the
- * environment variable HDG_AUTOTUNE_LOG set, the autotuner prints every
- * candidate's time to stderr. */
+ * single-wavefront panel and FP64 MFMA trailing updates, 6 register-tile
+ * blocked elimination with MFMA updates, 7 16x16 grid, 8 flux-block Schur
+ * complement: where the flux block of A_ii is C (x) M (per-cell coefficient
+ * matrix C, fixed scalar mass matrix M; both weak forms here), the flux
+ * unknowns are eliminated with C^-1 (x) M^-1 in two FP64 MFMA products and
+ * the reduced system is condensed by one of the families above (the default
+ * where it applies).  With the environment variable HDG_AUTOTUNE_LOG set,
+ * the autotuner prints every candidate's time to stderr. */
 
 /* flags: HDG_BATCH_AUTOTUNE times every kernel family and grid size able to
  * condense each class on the batch's own cells and keeps the fastest;

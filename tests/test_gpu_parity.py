@@ -112,6 +112,36 @@ def _operator(ctx, D, k, n, kappa=None, source=None, spaces=None, mms=O.MMS_LINE
     return gh.HybridAffineFEOperator(wf, X, X, [1, 2], [3], ctx=ctx)
 
 
+def check_local_factors(op, A, b, U, Y, ncheck):
+    """U and Y = L^-1 P [B c] are partial-pivoting LU factors of the system the
+    recovery back-substitutes: A_ii itself, or, after the flux-block
+    elimination, R_uu of the reduced augmented system
+    [R_uu R_ub r_u] = [A_(u,b)(u,b) b_(u,b)] - A_(u,b)q A_qq^-1 [A_q(u,b) b_q]
+    formed here in numpy from the cell systems.  (The pivot order itself may
+    differ from another LU on exact ties.)"""
+    import scipy.linalg as sl
+    ni = op.info.ni
+    rows, fam = op.factor_rows()
+    na = ni - rows
+    for c in range(ncheck):
+        Ab = np.concatenate([A[c], b[c][:, None]], axis=1)
+        if na:
+            assert fam == 8
+            Z = np.linalg.solve(Ab[:na, :na], Ab[:na, na:])
+            M = Ab[na:, na:] - Ab[na:, :na] @ Z
+        else:
+            M = Ab
+        Aii = M[:rows, :rows]
+        B = M[:rows, rows:]   # nb boundary columns and the right-hand side
+        assert rel(sl.solve_triangular(U[c], Y[c]), np.linalg.solve(Aii, B)) < 1e-10
+        # |L| <= 1 up to the rounding of inv(U) (cond(U) times eps; a factor
+        # without partial pivoting has multipliers far above 1)
+        PtL = Aii @ np.linalg.inv(U[c])
+        assert np.max(np.abs(PtL)) < 1.0 + 1e-8
+        assert np.allclose(np.sort(np.max(np.abs(PtL), axis=1)), 1.0, atol=0) or \
+            np.count_nonzero(np.isclose(np.abs(PtL), 1.0, rtol=0, atol=1e-8)) >= rows
+
+
 CASES = [
     # (D, k, n, mms, lognormal kappa)
     (2, 1, 8, O.MMS_LINEAR, False),     # cfg1
@@ -148,15 +178,7 @@ def test_operator_parity(ctx, D, k, n, mms, logn):
     # (the pivot order itself may differ from another LU on exact ties)
     import scipy.linalg as sl
     U, Y = op.local_factors()
-    ni = op.info.ni
-    for c in range(min(nc, 16)):
-        Aii = A[c, :ni, :ni]
-        B = np.concatenate([A[c, :ni, ni:], b[c, :ni, None]], axis=1)
-        assert rel(sl.solve_triangular(U[c], Y[c]), np.linalg.solve(Aii, B)) < 1e-10
-        PtL = Aii @ np.linalg.inv(U[c])
-        assert np.max(np.abs(PtL)) < 1.0 + 1e-10
-        assert np.allclose(np.sort(np.max(np.abs(PtL), axis=1)), 1.0, atol=0) or \
-            np.count_nonzero(np.isclose(np.abs(PtL), 1.0, rtol=0, atol=1e-10)) >= ni
+    check_local_factors(op, A, b, U, Y, min(nc, 16))
     # recovery from the oracle's skeleton solution
     fv = op.recover(ref.lam).cpu().numpy()
     assert rel(fv, ref.free_values) < RTOL
