@@ -157,7 +157,7 @@ struct CellLayout {
         U(factor_slot(sys_shape(t))),
         Y((size_t)(t.nb + 1) * t.ni),
         alpha(72),
-        srcq((size_t)t.nqc * 4) {}
+        srcq((size_t)t.nqc * 4 + t.ni) {}  // source at quadrature points, then b_i
 };
 
 // Allocates the mesh's own per-cell state (S, g, U, Y, alpha, srcq, info) and
@@ -558,7 +558,7 @@ int hdg_mesh_create(hdg_ctx* ctx, const hdg_problem* prob, const hdg_mesh_desc* 
     // tables: one allocation, doubles first then bytes
     std::vector<const std::vector<double>*> dv = {&th.Mpp, &th.Gpq, &th.Gqp, &th.Fqp, &th.Fqq,
                                                    &th.Fpm, &th.Fqm, &th.Fmm, &th.Bp, &th.Bq,
-                                                   &th.xq, &th.fb_minv, &th.fb_kfrag};
+                                                   &th.xq, &th.fb_minv};
     size_t nd = 0;
     for (auto* v : dv) nd += v->size();
     size_t bytes = nd * sizeof(double) + th.prog_val.size() * sizeof(double) +
@@ -641,7 +641,7 @@ int hdg_mesh_create(hdg_ctx* ctx, const hdg_problem* prob, const hdg_mesh_desc* 
     T.n = th.n;
     T.nqc = th.nqc;
     const double** ptrs[] = {&T.Mpp, &T.Gpq, &T.Gqp, &T.Fqp, &T.Fqq, &T.Fpm,
-                             &T.Fqm, &T.Fmm, &T.Bp,  &T.Bq,  &T.xq, &T.fb_minv, &T.fb_kfrag};
+                             &T.Fqm, &T.Fmm, &T.Bp,  &T.Bq,  &T.xq, &T.fb_minv};
     for (size_t i = 0; i < dv.size(); ++i) *ptrs[i] = reinterpret_cast<const double*>(base + offs[i]);
     T.pmask = reinterpret_cast<const uint8_t*>(base + off_pm);
     T.qmask = reinterpret_cast<const uint8_t*>(base + off_qm);
@@ -661,7 +661,6 @@ int hdg_mesh_create(hdg_ctx* ctx, const hdg_problem* prob, const hdg_mesh_desc* 
     T.fb_nc = th.fb_nc;
     std::memcpy(T.fb_cid, th.fb_cid, sizeof(T.fb_cid));
     if (th.fb_minv.empty()) T.fb_minv = nullptr;
-    if (th.fb_kfrag.empty()) T.fb_kfrag = nullptr;
   }
   m->x0.assign(desc->x0, desc->x0 + nc * D);
   m->jac.assign(desc->jac, desc->jac + nc * D * D);

@@ -656,7 +656,6 @@ namespace hdg {
 void detect_flux_block(TablesHost& t) {
   t.fb_nc = 0;
   t.fb_minv.clear();
-  t.fb_kfrag.clear();
   const int np = t.np, na = t.na, n = t.n;
   if (np <= 0 || na <= 0 || na % np != 0 || na >= t.ni) return;
   const int nc = na / np;
@@ -730,17 +729,6 @@ void detect_flux_block(TablesHost& t) {
   }
   t.fb_minv.resize((size_t)np * np);
   for (int i = 0; i < np * np; ++i) t.fb_minv[i] = (double)X[i];
-  // fragment order of the f64 MFMA A operand: lane l of k-step s of row tile
-  // rt holds row 16 rt + l % 16, column 4 s + l / 16
-  const int KS = (na + 3) / 4, RTZ = (KS + 3) / 4;
-  t.fb_kfrag.assign((size_t)RTZ * KS * 64, 0.0);
-  for (int rt = 0; rt < RTZ; ++rt)
-    for (int s = 0; s < KS; ++s)
-      for (int l = 0; l < 64; ++l) {
-        const int i = 16 * rt + (l & 15), k = 4 * s + (l >> 4);
-        if (i < na && k < na && i / np == k / np)
-          t.fb_kfrag[((size_t)rt * KS + s) * 64 + l] = t.fb_minv[(size_t)(i % np) * np + k % np];
-      }
   for (int k = 0; k < 36; ++k) t.fb_cid[k] = (int8_t)(cid[k] >= 0 ? cid[k] : -1);
   t.fb_nc = nc;
 }

@@ -62,12 +62,9 @@ struct TablesHost {
   // cell (fb_cid < 0: identically zero) and M the np x np reference mass
   // matrix of the scalar flux basis.  fb_nc = 0 when the program has no such
   // block.  fb_minv = M^-1, row-major.
-  // fb_kfrag: the A operands of (I (x) M^-1) in MFMA fragment order,
-  // [row tile t][k-step s][lane] = (I (x) M^-1)[16t + lane%16][4s + lane/16].
   int fb_nc = 0;
   int8_t fb_cid[36] = {};
   std::vector<double> fb_minv;
-  std::vector<double> fb_kfrag;
 };
 
 // Device view of the tables (pointers into one device allocation).
@@ -105,7 +102,6 @@ struct Tables {
   int fb_nc;              // flux-block components (0: no Kronecker flux block)
   int8_t fb_cid[36];      // coefficient id of C[a][b] (< 0: zero)
   const double* fb_minv;  // [np][np] inverse of the scalar flux mass matrix
-  const double* fb_kfrag; // (I (x) M^-1) in f64 MFMA A-operand fragment order
 };
 
 // Device view of a mesh for the fused element generator.

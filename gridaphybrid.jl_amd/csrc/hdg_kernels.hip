@@ -166,6 +166,12 @@ constexpr int kMaxAlpha = 72;
 // affine pull-back coefficients alpha of the entry program (layout fixed by
 // build_program in hdg_fe.cpp) and the source values at the cell quadrature
 // points, pre-multiplied by |det J|.  One thread per cell.
+// Per-cell stride of the source scratch: |det J| * source at the cell
+// quadrature points [nqc][4], then the interior right-hand side b_i [ni].
+__device__ __host__ inline int64_t srcq_stride(const Tables& T) {
+  return (int64_t)T.nqc * 4 + T.ni;
+}
+
 template <int PHYS>
 __global__ void __launch_bounds__(128) prep_kernel(Tables T, MeshDev M) {
   const int D = T.D;
@@ -218,7 +224,7 @@ __global__ void __launch_bounds__(128) prep_kernel(Tables T, MeshDev M) {
       }
     }
     const int ncomp = PHYS == kDarcy ? D + 1 : D;
-    double* sq = M.srcq + cell * (int64_t)T.nqc * 4;
+    double* sq = M.srcq + cell * srcq_stride(T);
     for (int q = 0; q < T.nqc; ++q) {
       double v[4] = {0.0, 0.0, 0.0, 0.0};
       if (M.source == 100) {
@@ -257,10 +263,13 @@ __device__ inline int64_t ystride(const CondenseOut& o, int ni, int nb) {
   return o.y_stride ? o.y_stride : (int64_t)(nb + 1) * ni;
 }
 
+constexpr int kMaxRhsRows = 256;  // interior rows of the fused generators, at most
+
 struct FusedShared {
   double alpha[kMaxAlpha];  // per-cell coefficients of the entry program
-  double src[kMaxQP][4];    // |det J| * source values at cell quadrature points
+  double b[kMaxRhsRows];    // right-hand side b_i of the interior rows (rhs_kernel)
 };
+using CellCoefs = FusedShared;
 
 template <int PHYS>
 struct FusedGen {
@@ -272,8 +281,8 @@ struct FusedGen {
   __device__ void begin(int64_t cell, Shared& sh) const {
     for (int t = threadIdx.x; t < kMaxAlpha; t += blockDim.x)
       sh.alpha[t] = M.alpha[cell * kMaxAlpha + t];
-    for (int t = threadIdx.x; t < T.nqc * 4; t += blockDim.x)
-      sh.src[t >> 2][t & 3] = M.srcq[cell * (int64_t)T.nqc * 4 + t];
+    const double* sq = M.srcq + cell * srcq_stride(T);
+    for (int t = threadIdx.x; t < T.ni; t += blockDim.x) sh.b[t] = sq[T.nqc * 4 + t];
   }
 
   __device__ double entry(const Shared& sh, int row, int col) const {
@@ -290,27 +299,64 @@ struct FusedGen {
         acc = fma(sh.alpha[T.prog_xid[x0 + k]], T.prog_xval[x0 + k], acc);
       return acc;
     }
-    return rhs(sh, row);
+    return row < T.ni ? sh.b[row] : 0.0;
   }
 
-  // right-hand side: per-cell quadrature of the source against the test basis
-  __device__ double rhs(const Shared& sh, int row) const {
+  // U entries at once: every head, first value and first extra term is loaded
+  // before any is used, so U L2 round trips overlap.  Same terms in the same
+  // order as entry(); ok[u] false gives 0.
+  template <int U>
+  __device__ void entries(const Shared& sh, const int* row, const int* col, const bool* ok,
+                          double* out) const {
+    const int n = T.n;
+    uint32_t h[U];
+    double v0[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const bool p = ok[u] && col[u] < n;
+      const int e = p ? col[u] * n + row[u] : 0;
+      h[u] = p ? T.prog_head[e] : 0u;
+      v0[u] = p ? T.prog_val0[e] : 0.0;
+    }
+    int i1[U];
+    double v1[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int cnt = (h[u] >> 8) & 15, x0 = (int)(h[u] >> 12);
+      i1[u] = cnt >= 2 ? (int)T.prog_xid[x0] : 0;
+      v1[u] = cnt >= 2 ? T.prog_xval[x0] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int cnt = (h[u] >> 8) & 15, x0 = (int)(h[u] >> 12);
+      double acc = cnt ? fma(sh.alpha[h[u] & 255], v0[u], 0.0) : 0.0;
+      if (cnt >= 2) acc = fma(sh.alpha[i1[u]], v1[u], acc);
+      for (int k = 1; k < cnt - 1; ++k)
+        acc = fma(sh.alpha[T.prog_xid[x0 + k]], T.prog_xval[x0 + k], acc);
+      if (ok[u] && col[u] >= n) acc = row[u] < T.ni ? sh.b[row[u]] : 0.0;
+      out[u] = ok[u] ? acc : 0.0;
+    }
+  }
+
+  // right-hand side: per-cell quadrature of the source (|det J| * values at
+  // the cell quadrature points, src[q][component]) against the test basis
+  __device__ double rhs(const double (*src)[4], int row) const {
     const int D = T.D, np = T.np, nq = T.nq, na = T.na, ni = T.ni;
     if (row >= ni) return 0.0;
     double s = 0.0;
     if (PHYS == kDarcy) {
       if (row < na) {  // v . f
         const int a = row / np, i = row - a * np;
-        for (int q = 0; q < T.nqc; ++q) s += T.Bp[q * np + i] * sh.src[q][a];
+        for (int q = 0; q < T.nqc; ++q) s += T.Bp[q * np + i] * src[q][a];
       } else {  // w g
         const int i = row - na;
-        for (int q = 0; q < T.nqc; ++q) s += T.Bq[q * nq + i] * sh.src[q][D];
+        for (int q = 0; q < T.nqc; ++q) s += T.Bq[q * nq + i] * src[q][D];
       }
       return s;
     }
     if (row < na) return 0.0;
     const int c = (row - na) / nq, j = row - na - c * nq;  // -omega . f
-    for (int q = 0; q < T.nqc; ++q) s += T.Bq[q * nq + j] * sh.src[q][c];
+    for (int q = 0; q < T.nqc; ++q) s += T.Bq[q * nq + j] * src[q][c];
     return -s;
   }
 };
@@ -2524,8 +2570,29 @@ bool condense_supported(int n) {
   return pick_bucket(n, b) || (n > 64 && n <= kLargeMaxN);
 }
 
+// Interior right-hand side b_i of every cell, one wavefront per cell and a
+// lane per row: FusedGen::rhs (the quadrature of the source against the test
+// basis) evaluated once and kept in the source scratch, so the generators read
+// one value per right-hand-side entry.
+template <int PHYS>
+__global__ void __launch_bounds__(256) rhs_kernel(FusedGen<PHYS> gen) {
+  constexpr int WPB = 4;
+  __shared__ double s_src[WPB][kMaxQP][4];
+  const Tables& T = gen.T;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int64_t cell = (int64_t)blockIdx.x * WPB + w; cell < gen.M.ncells;
+       cell += (int64_t)gridDim.x * WPB) {
+    double* sq = gen.M.srcq + cell * srcq_stride(T);
+    for (int t = lane; t < T.nqc * 4; t += 64) s_src[w][t >> 2][t & 3] = sq[t];
+    wave_sync_lds();
+    for (int r = lane; r < T.ni; r += 64) sq[T.nqc * 4 + r] = gen.rhs(s_src[w], r);
+    wave_sync_lds();
+  }
+}
+
 static hipError_t launch_prep(const Tables& T, const MeshDev& M, hipStream_t st) {
   if (M.ncells <= 0) return hipSuccess;
+  if (T.ni > kMaxRhsRows) return hipErrorInvalidValue;
   const int64_t blocks = (M.ncells + 127) / 128;
   const int grid = (int)(blocks < 8192 ? blocks : 8192);
   if (T.phys == kDarcy)
@@ -2538,6 +2605,16 @@ static hipError_t launch_prep(const Tables& T, const MeshDev& M, hipStream_t st)
       ClockScope clk_(st, "prep");
       hipLaunchKernelGGL(prep_kernel<kElasticity>, dim3(grid), dim3(128), 0, st, T, M);
     }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const int64_t wblocks = (M.ncells + 3) / 4;
+  const int wgrid = (int)(wblocks < 65536 ? wblocks : 65536);
+  ClockScope clk_(st, "prep_rhs");
+  if (T.phys == kDarcy)
+    hipLaunchKernelGGL(rhs_kernel<kDarcy>, dim3(wgrid), dim3(256), 0, st, FusedGen<kDarcy>{T, M});
+  else
+    hipLaunchKernelGGL(rhs_kernel<kElasticity>, dim3(wgrid), dim3(256), 0, st,
+                       FusedGen<kElasticity>{T, M});
   return hipGetLastError();
 }
 
@@ -2626,7 +2703,7 @@ __global__ void __launch_bounds__(256)
                          const double* __restrict__ lam_free, const double* __restrict__ lam_dir,
                          double* __restrict__ out) {
   constexpr int WPB = 4;
-  __shared__ FusedShared s_sh[WPB];
+  __shared__ CellCoefs s_sh[WPB];
   __shared__ double s_v[WPB][kMixedMaxNi + kRecoverMaxNb];  // [x0; lambda]
   __shared__ double s_r[WPB][kMixedMaxNi];
   const Tables& T = gen.T;
@@ -2636,8 +2713,8 @@ __global__ void __launch_bounds__(256)
   for (int64_t cell = (int64_t)blockIdx.x * WPB + w; cell < ncells;
        cell += (int64_t)gridDim.x * WPB) {
     for (int t = lane; t < kMaxAlpha; t += 64) s_sh[w].alpha[t] = gen.M.alpha[cell * kMaxAlpha + t];
-    for (int t = lane; t < nqc * 4; t += 64)
-      s_sh[w].src[t >> 2][t & 3] = gen.M.srcq[cell * (int64_t)nqc * 4 + t];
+    for (int t = lane; t < ni; t += 64)
+      s_sh[w].b[t] = gen.M.srcq[cell * srcq_stride(T) + nqc * 4 + t];
     for (int j = lane; j < nb; j += 64) {
       const int32_t d = cell_dofs[cell * nb + j];
       s_v[w][ni + j] = d >= 0 ? lam_free[d] : lam_dir[-(int64_t)d - 1];
@@ -2662,7 +2739,7 @@ __global__ void __launch_bounds__(256)
     for (int t = 0; t < KMAX; ++t) {
       const int r = lane + 64 * t;
       if (r >= ni) continue;
-      double res = gen.rhs(s_sh[w], r);
+      double res = s_sh[w].b[r];
       const int k1 = T.prog_rptr[r + 1];
       for (int k = T.prog_rptr[r]; k < k1; ++k)
         res = fma(-(s_sh[w].alpha[T.prog_rid[k]] * T.prog_rval[k]), s_v[w][T.prog_rcol[k]], res);
@@ -2730,10 +2807,10 @@ hipError_t launch_recover_mixed(const Tables& T, const MeshDev& M, const double*
 // One workgroup (NW wavefronts) per cell.  Both products run on
 // v_mfma_f64_16x16x4f64:
 //  * G1 is generated into LDS ([k][col], odd leading dimension), mixed over
-//    components by C^-1 in place, and read as B operands.  The A operands of
-//    (I (x) M^-1) are the same for every cell: the host stores them in
-//    fragment order (one coalesced 512-byte load per k-step), and blocks
-//    between different components are skipped (zmask).
+//    components by C^-1 in place (for a diagonal C, C^-1 scales the rows of
+//    Z instead), and read as B operands.  The A operands of (I (x) M^-1) come
+//    from M^-1 in LDS; blocks between different components are skipped
+//    (zmask).
 //  * Wavefront w owns column tiles w, w + NW, ... of Z and R.  Its Z tiles
 //    stay in registers: the accumulator layout of the f64 MFMA (row
 //    4v + lane/16 of the tile in register v) is the B-operand layout of
@@ -2751,6 +2828,8 @@ struct FluxPlan {
   int nr, nc, KS;     // reduced rows, columns (incl. rhs), k-steps of the flux block
   int ldg;            // leading dimension of G1 in LDS (odd)
   int ncols;          // columns of a row tile: 4 KS + 16 ceil(nc / 16)
+  int cdiag;          // C is diagonal (structurally): C^-1 scales the rows of Z
+  int minv_off;       // offset (doubles) of M^-1 in the dynamic LDS
   uint32_t zmask[8];  // bit s of zmask[t]: block (row tile t, k-step s) of I (x) M^-1
                       // is not identically zero
 };
@@ -2795,8 +2874,8 @@ __device__ inline int flux_cinv(int ncomp, const int8_t* cid, const double* alph
 }
 
 template <int PHYS, int CT, int KSM>
-__global__ void __launch_bounds__(64 * kFluxWaves)
-    flux_schur_kernel(FusedGen<PHYS> gen, const double* __restrict__ kfrag, int64_t first,
+__global__ void __launch_bounds__(64 * kFluxWaves, 3)  // two workgroups per CU
+    flux_schur_kernel(FusedGen<PHYS> gen, int64_t first,
                       int64_t count, double* __restrict__ R, FluxPlan fp,
                       double* __restrict__ blocks, int64_t bstride) {
   constexpr int NW = kFluxWaves, NT = 64 * NW, LT = kFluxTileLd;
@@ -2808,12 +2887,17 @@ __global__ void __launch_bounds__(64 * kFluxWaves)
   const int nk = 4 * KS;  // flux rows incl. zero padding
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int i16 = lane & 15, q4 = lane >> 4;
-  extern __shared__ double s_dyn[];  // G1 [nk][ldg], later row tiles [2][ncols][LT]
+  extern __shared__ double s_dyn[];  // G1 [nk][ldg], later row tiles [2][ncols][LT]; then M^-1
   __shared__ FusedShared s_gen;
   __shared__ double s_cinv[36];
+  __shared__ uint16_t s_split[16 * RTZ];  // flux index -> component << 8 | scalar index
   __shared__ double s_cw[72];
   __shared__ int s_bad;
   const int tsz = ncols * LT;
+  double* s_minv = s_dyn + fp.minv_off;  // [np][np]
+  for (int t = tid; t < np * np; t += NT) s_minv[t] = T.fb_minv[t];
+  for (int t = tid; t < 16 * RTZ; t += NT)
+    s_split[t] = t < na ? (uint16_t)(((t / np) << 8) | (t % np)) : (uint16_t)0xffff;
 
   for (int64_t lc = blockIdx.x; lc < count; lc += gridDim.x) {
     gen.begin(first + lc, s_gen);
@@ -2821,10 +2905,23 @@ __global__ void __launch_bounds__(64 * kFluxWaves)
     if (tid == 0) s_bad = flux_cinv(ncomp, T.fb_cid, s_gen.alpha, s_cw, s_cinv);
     double* Rc = R + lc * (int64_t)nc * nr;  // [nc][nr] column-major
     // ---- G1 = [A_qu A_qb b_q] into LDS ------------------------------------
-    for (int t = tid; t < nk * ldg; t += NT) {
-      const int c = t / nk, k = t - c * nk;
-      if (c >= ldg) continue;
-      s_dyn[k * ldg + c] = (k < na && c < nc) ? gen.entry(s_gen, k, na + c) : 0.0;
+    for (int t0 = tid; t0 < nk * ldg; t0 += 4 * NT) {
+      int rows[4], cols[4];
+      bool ok[4];
+      double v[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int t = t0 + u * NT, c = t / nk, k = t - c * nk;
+        rows[u] = k;
+        cols[u] = na + c;
+        ok[u] = t < nk * ldg && k < na && c < nc;
+      }
+      gen.template entries<4>(s_gen, rows, cols, ok, v);
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int t = t0 + u * NT, c = t / nk, k = t - c * nk;
+        if (t < nk * ldg) s_dyn[k * ldg + c] = v[u];
+      }
     }
     __syncthreads();
     // interior blocks kept for the recovery: [A_uu][A_qu][A_uq], column-major
@@ -2840,7 +2937,9 @@ __global__ void __launch_bounds__(64 * kFluxWaves)
       __syncthreads();
       continue;
     }
-    // ---- G1 <- (C^-1 (x) I) G1, one (scalar index, column) pair per thread --
+    // ---- G1 <- (C^-1 (x) I) G1, one (scalar index, column) pair per thread
+    //      (diagonal C: applied to the rows of Z instead) ------------------
+    if (!fp.cdiag)
     for (int t = tid; t < np * nc; t += NT) {
       const int i = t / nc, c = t - i * nc;
       double v[6];
@@ -2864,10 +2963,14 @@ __global__ void __launch_bounds__(64 * kFluxWaves)
       dbl4 acc[CT];
 #pragma unroll
       for (int ct = 0; ct < CT; ++ct) acc[ct] = dbl4{0.0, 0.0, 0.0, 0.0};
+      // A operand (I (x) M^-1)[16 rt + lane%16][4 s + lane/16] from M^-1 in LDS
+      const int si = s_split[16 * rt + i16];  // 0xffff past the flux rows
+      const double* mrow = s_minv + (si == 0xffff ? 0 : (si & 255) * np);
 #pragma unroll 2
       for (int s = 0; s < KS; ++s) {
         if (!((fp.zmask[rt] >> s) & 1u)) continue;  // wave-uniform
-        const double a = kfrag[((int64_t)rt * KS + s) * 64 + lane];
+        const int sk = s_split[4 * s + q4];
+        const double a = (si != 0xffff && (si >> 8) == (sk >> 8)) ? mrow[sk & 255] : 0.0;
         const double* brow = s_dyn + (4 * s + q4) * ldg + i16;
 #pragma unroll
         for (int ct = 0; ct < CT; ++ct)
@@ -2878,30 +2981,57 @@ __global__ void __launch_bounds__(64 * kFluxWaves)
 #pragma unroll
       for (int v = 0; v < 4; ++v)
         if (4 * rt + v < KSM) {
+          double sc = 1.0;
+          if (fp.cdiag) {
+            const int row = 16 * rt + q4 + 4 * v;
+            sc = row < na ? s_cinv[7 * (row / np)] : 0.0;
+          }
 #pragma unroll
-          for (int ct = 0; ct < CT; ++ct) zf[ct][4 * rt + v] = acc[ct][v];
+          for (int ct = 0; ct < CT; ++ct) zf[ct][4 * rt + v] = acc[ct][v] * sc;
         }
     }
     __syncthreads();  // G1 fully read: its region now holds the row tiles
 
     // ---- R = R0 - [A_uq; A_bq] Z, one 16-row tile of R at a time ----------
     const int RTR = (nr + 15) >> 4;
+    // R tiles are stored one iteration late, after the next tile's entry
+    // loads: the vector-memory counter orders loads behind earlier stores
+    dbl4 pend[CT];
+    int pend_rr = -1;
+    auto store_pending = [&]() {
+#pragma unroll
+      for (int ct = 0; ct < CT; ++ct) {
+        const int col = 16 * (w + NW * ct) + i16;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int row = 16 * pend_rr + q4 + 4 * v;
+          if (row < nr && col < nc) Rc[(int64_t)col * nr + row] = pend[ct][v];
+        }
+      }
+    };
 #pragma unroll 1
     for (int rr = 0; rr < RTR; ++rr) {
       double* g = s_dyn + (rr & 1) * tsz;  // [ncols][LT]: cols < nk: -A_(u,b)q, then R0
-      for (int t = tid; t < 16 * ncols; t += NT) {
-        const int col = t >> 4, i = t & 15, row = 16 * rr + i;
-        double v = 0.0;
-        if (row < nr) {
-          if (col < nk) {
-            if (col < na) v = -gen.entry(s_gen, na + row, col);
-          } else if (col - nk < nc) {
-            v = gen.entry(s_gen, na + row, na + col - nk);
-          }
+      for (int t0 = tid; t0 < 16 * ncols; t0 += 4 * NT) {
+        int rows[4], cols[4];
+        bool ok[4];
+        double v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int t = t0 + u * NT, col = t >> 4, row = 16 * rr + (t & 15);
+          rows[u] = na + row;
+          cols[u] = col < nk ? col : na + col - nk;
+          ok[u] = t < 16 * ncols && row < nr && (col < na || (col >= nk && col - nk < nc));
         }
-        g[col * LT + i] = v;
+        gen.template entries<4>(s_gen, rows, cols, ok, v);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int t = t0 + u * NT, col = t >> 4;
+          if (t < 16 * ncols) g[col * LT + (t & 15)] = col < nk ? -v[u] : v[u];
+        }
       }
       __syncthreads();
+      if (pend_rr >= 0) store_pending();
       if (16 * rr < nu)  // rows of the remaining interior unknowns: A_uq and A_uu
         for (int t = tid; t < 16 * (na + nu); t += NT) {
           const int col = t >> 4, i = t & 15, row = 16 * rr + i;
@@ -2914,6 +3044,7 @@ __global__ void __launch_bounds__(64 * kFluxWaves)
 #pragma unroll
       for (int ct = 0; ct < CT; ++ct) {
         const int c0 = 16 * (w + NW * ct);
+        pend[ct] = dbl4{0.0, 0.0, 0.0, 0.0};
         if (c0 >= nc) continue;  // wave-uniform
         const int col = c0 + i16;
         dbl4 acc;
@@ -2924,13 +3055,11 @@ __global__ void __launch_bounds__(64 * kFluxWaves)
           if (s < KS)
             acc = __builtin_amdgcn_mfma_f64_16x16x4f64(g[(4 * s + q4) * LT + i16], zf[ct][s], acc,
                                                        0, 0, 0);
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          const int row = 16 * rr + q4 + 4 * v;
-          if (row < nr && col < nc) Rc[(int64_t)col * nr + row] = acc[v];
-        }
+        pend[ct] = acc;
       }
+      pend_rr = rr;
     }
+    if (pend_rr >= 0) store_pending();
     __syncthreads();  // LDS reuse by the next cell
   }
 }
@@ -2971,14 +3100,14 @@ static bool flux_fits(int fb_nc, int na, int np, int ni, int nb, int* ct, int* k
   if (nu > 64 || na > 128 || nb > 192) return false;  // one wavefront per cell in the recovery
   const size_t ncp = 16 * (size_t)((nr + 1 + 15) / 16);
   const size_t g1 = (size_t)4 * KS * (ncp | 1), tiles = 2 * (4 * (size_t)KS + ncp) * kFluxTileLd;
-  if (std::max(g1, tiles) * sizeof(double) > 150 * 1024) return false;
+  if ((std::max(g1, tiles) + (size_t)np * np) * sizeof(double) > 150 * 1024) return false;
   if (ct) *ct = c;
   if (ksm) *ksm = k;
   return true;
 }
 
 static bool flux_plan(const Tables& T, FluxPlan& fp, int* ct, int* ksm) {
-  if (!T.fb_minv || !T.fb_kfrag || !flux_fits(T.fb_nc, T.na, T.np, T.ni, T.nb, ct, ksm))
+  if (!T.fb_minv || !flux_fits(T.fb_nc, T.na, T.np, T.ni, T.nb, ct, ksm))
     return false;
   const int na = T.na, np = T.np;
   fp.nr = T.ni - na + T.nb;
@@ -2987,6 +3116,11 @@ static bool flux_plan(const Tables& T, FluxPlan& fp, int* ct, int* ksm) {
   const int ncp = 16 * ((fp.nc + 15) / 16);
   fp.ldg = ncp | 1;
   fp.ncols = 4 * fp.KS + ncp;
+  fp.minv_off = (int)std::max((size_t)4 * fp.KS * fp.ldg, (size_t)2 * fp.ncols * kFluxTileLd);
+  fp.cdiag = 1;
+  for (int a = 0; a < T.fb_nc; ++a)
+    for (int b = 0; b < T.fb_nc; ++b)
+      if (a != b && T.fb_cid[6 * a + b] >= 0) fp.cdiag = 0;
   for (int t = 0; t < 8; ++t) {
     fp.zmask[t] = 0;
     for (int s = 0; s < fp.KS && s < 32; ++s) {
@@ -2999,9 +3133,8 @@ static bool flux_plan(const Tables& T, FluxPlan& fp, int* ct, int* ksm) {
   return true;
 }
 
-static size_t flux_lds(const FluxPlan& fp) {
-  const size_t g1 = (size_t)4 * fp.KS * fp.ldg, tiles = (size_t)2 * fp.ncols * kFluxTileLd;
-  return std::max(g1, tiles) * sizeof(double);
+static size_t flux_lds(const FluxPlan& fp, int np) {
+  return ((size_t)fp.minv_off + (size_t)np * np) * sizeof(double);
 }
 
 // Cells per flux chunk: the reduced systems of one chunk live in the workspace.
@@ -3032,7 +3165,7 @@ static hipError_t launch_flux_phys(const Tables& T, const MeshDev& M, const Cond
   const int64_t chunk = std::min<int64_t>(ncells, flux_chunk(nr));
   hipError_t e = workspace_reserve(ws, flux_workspace_bytes(nr, ncells));
   if (e != hipSuccess) return e;
-  const size_t lds = flux_lds(fp);
+  const size_t lds = flux_lds(fp, T.np);
   if (lds > 160 * 1024 - sizeof(FusedShared) - 1024) return hipErrorInvalidValue;
   FusedGen<PHYS> gen{T, M};
   // reduced factors and kept blocks in the per-cell slots of the class
@@ -3048,7 +3181,7 @@ static hipError_t launch_flux_phys(const Tables& T, const MeshDev& M, const Cond
 #define HDG_FLUX(CT_, KS_)                                                                   \
   if (ct == CT_ && ksm == KS_)                                                               \
     hipLaunchKernelGGL((flux_schur_kernel<PHYS, CT_, KS_>), dim3(grid), dim3(64 * kFluxWaves), \
-                       lds, st, gen, T.fb_kfrag, first, count, ws.p, fp,                      \
+                       lds, st, gen, first, count, ws.p, fp,                                  \
                        out.U + first * usz + boff, usz);
       HDG_FLUX(1, 8)
       HDG_FLUX(1, 16)
@@ -3134,7 +3267,7 @@ __global__ void __launch_bounds__(256)
                         const double* __restrict__ lam_dir, double* __restrict__ out) {
   constexpr int WPB = 4;
   constexpr int NQ = 64 * KQ;  // flux rows per wavefront, at most
-  __shared__ FusedShared s_sh[WPB];
+  __shared__ CellCoefs s_sh[WPB];
   __shared__ double s_lam[WPB][kFluxMaxNb];
   __shared__ double s_xq[WPB][NQ];  // x_q
   __shared__ double s_xu[WPB][64];  // x_u, then corrections d_u
@@ -3215,8 +3348,8 @@ __global__ void __launch_bounds__(256)
   for (int64_t cell = (int64_t)blockIdx.x * WPB + w; cell < ncells;
        cell += (int64_t)gridDim.x * WPB) {
     for (int t = lane; t < kMaxAlpha; t += 64) s_sh[w].alpha[t] = gen.M.alpha[cell * kMaxAlpha + t];
-    for (int t = lane; t < nqc * 4; t += 64)
-      s_sh[w].src[t >> 2][t & 3] = gen.M.srcq[cell * (int64_t)nqc * 4 + t];
+    for (int t = lane; t < ni; t += 64)
+      s_sh[w].b[t] = gen.M.srcq[cell * srcq_stride(T) + nqc * 4 + t];
     for (int j = lane; j < nb; j += 64) {
       const int32_t d = cell_dofs[cell * nb + j];
       lam[j] = d >= 0 ? lam_free[d] : lam_dir[-(int64_t)d - 1];
@@ -3242,7 +3375,7 @@ __global__ void __launch_bounds__(256)
     for (int t = 0; t < KQ + 1; ++t) {
       const int r = lane + 64 * t;
       if (r < ni) {
-        double v = gen.rhs(s_sh[w], r);
+        double v = s_sh[w].b[r];
         const int k1 = T.prog_rptr[r + 1];
         for (int k = T.prog_rbsplit[r]; k < k1; ++k)
           v = fma(-(s_sh[w].alpha[T.prog_rid[k]] * T.prog_rval[k]), lam[T.prog_rcol[k] - ni], v);
