@@ -267,7 +267,7 @@ constexpr int kMaxRhsRows = 256;  // interior rows of the fused generators, at m
 
 struct FusedShared {
   double alpha[kMaxAlpha];  // per-cell coefficients of the entry program
-  double b[kMaxRhsRows];    // right-hand side b_i of the interior rows (rhs_kernel)
+  double b[kMaxRhsRows];    // right-hand side b_i of the interior rows (cell_rhs_kernel)
 };
 using CellCoefs = FusedShared;
 
@@ -2575,7 +2575,7 @@ bool condense_supported(int n) {
 // basis) evaluated once and kept in the source scratch, so the generators read
 // one value per right-hand-side entry.
 template <int PHYS>
-__global__ void __launch_bounds__(256) rhs_kernel(FusedGen<PHYS> gen) {
+__global__ void __launch_bounds__(256) cell_rhs_kernel(FusedGen<PHYS> gen) {
   constexpr int WPB = 4;
   __shared__ double s_src[WPB][kMaxQP][4];
   const Tables& T = gen.T;
@@ -2611,9 +2611,10 @@ static hipError_t launch_prep(const Tables& T, const MeshDev& M, hipStream_t st)
   const int wgrid = (int)(wblocks < 65536 ? wblocks : 65536);
   ClockScope clk_(st, "prep_rhs");
   if (T.phys == kDarcy)
-    hipLaunchKernelGGL(rhs_kernel<kDarcy>, dim3(wgrid), dim3(256), 0, st, FusedGen<kDarcy>{T, M});
+    hipLaunchKernelGGL(cell_rhs_kernel<kDarcy>, dim3(wgrid), dim3(256), 0, st,
+                       FusedGen<kDarcy>{T, M});
   else
-    hipLaunchKernelGGL(rhs_kernel<kElasticity>, dim3(wgrid), dim3(256), 0, st,
+    hipLaunchKernelGGL(cell_rhs_kernel<kElasticity>, dim3(wgrid), dim3(256), 0, st,
                        FusedGen<kElasticity>{T, M});
   return hipGetLastError();
 }
