@@ -202,6 +202,7 @@ struct SysShape {
   int ni = 0, nb = 0;
   int flux_na = 0;
   bool flux = false;
+  bool fused = false;  // the fused flux-block kernel fits (family 9)
 };
 SysShape sys_shape(const Tables& T);
 SysShape sys_shape(const TablesHost& t);
@@ -215,6 +216,11 @@ size_t factor_slot(const SysShape& sh);
 // ni - n_flux interior unknowns is condensed by an elimination kernel.  The
 // resident factors are then those of the reduced system.
 constexpr int kFamilyFluxSchur = 8;
+// Family id of the fused flux-block condensation: the remaining interior
+// unknowns are eliminated in the same kernel (Gauss-Jordan of R_uu); the
+// rows of W = A_ii^-1 [A_ib b_i] of those unknowns are kept in Y (natural
+// order) with R_uu^-1 and the blocks A_uu, A_qu, A_uq in the factor slot.
+constexpr int kFamilyFluxFused = 9;
 // kernel families able to condense a class; returns their count
 int condense_variants(const SysShape& sh, int* fams, int max);
 int condense_default_family(const SysShape& sh);
@@ -223,6 +229,13 @@ int condense_grid_options(int fam, int* gm, int max);
 size_t condense_workspace_bytes(const SysShape& sh, int fam, int64_t ncells);
 hipError_t launch_condense_flux(const Tables& T, const MeshDev& M, const CondenseOut& out,
                                 hipStream_t st, Workspace& ws, int gridmul);
+hipError_t launch_condense_fused_flux(const Tables& T, const MeshDev& M, const CondenseOut& out,
+                                      hipStream_t st, int gridmul);
+hipError_t launch_recover_fused_flux(const Tables& T, const MeshDev& M, const double* U,
+                                     const double* Y, int64_t usz, int64_t ysz,
+                                     const int32_t* cell_dofs, const double* lam_free,
+                                     const double* lam_dir, double* out, int64_t nfree,
+                                     hipStream_t st);
 // Recovery after a flux-block condensation: the reduced back substitution
 // (U, Y of the reduced system) gives x_u, x_q = (C^-1 (x) M^-1)(b_q - A_qu x_u
 // - A_qb lambda); then FP64 refinement against the residual of the full

@@ -261,8 +261,11 @@ int hdg_cell_blocks(hdg_mesh* mesh, const double** S, const double** g,
  * mixed-precision condensation *U is NULL and Y holds W = A_ii^-1 [A_ib b_i]
  * in natural interior order.  After a flux-block condensation (family 8) U
  * and Y are the factors of the reduced system of the m = ni - n_flux
- * non-flux interior unknowns, at the start of each cell's slot.  Cell c's
- * slots start at c * u_stride and c * y_stride (hdg_mesh_factor_layout). */
+ * non-flux interior unknowns, at the start of each cell's slot.  After a
+ * fused flux-block condensation (family 9) *U is NULL and Y holds the rows
+ * n_flux..ni-1 of W = A_ii^-1 [A_ib b_i] (natural order; its flux rows are
+ * not written).  Cell c's slots start at c * u_stride and c * y_stride
+ * (hdg_mesh_factor_layout). */
 int hdg_local_factors(hdg_mesh* mesh, const double** U, const double** Y);
 /* Layout of the resident factors: rows m of the factored system (ni, or
  * ni - n_flux after a flux-block condensation), the kernel family of the last
@@ -328,8 +331,10 @@ typedef struct {
  * complement: where the flux block of A_ii is C (x) M (per-cell coefficient
  * matrix C, fixed scalar mass matrix M; both weak forms here), the flux
  * unknowns are eliminated with C^-1 (x) M^-1 in two FP64 MFMA products and
- * the reduced system is condensed by one of the families above (the default
- * where it applies).  With the environment variable HDG_AUTOTUNE_LOG set,
+ * the reduced system is condensed by one of the families above, 9 the same
+ * elimination with the remaining interior unknowns eliminated in the same
+ * kernel (Gauss-Jordan of the reduced interior block), keeping the rows of
+ * W = A_ii^-1 [A_ib b_i] of those unknowns (the default where it fits, else 8).  With the environment variable HDG_AUTOTUNE_LOG set,
  * the autotuner prints every candidate's time to stderr. */
 
 /* flags: HDG_BATCH_AUTOTUNE times every kernel family and grid size able to
