@@ -280,9 +280,11 @@ def load_traffic():
 
 def serve_main(gh, args):
     """Serving: a seeded Poisson stream of HDG solves drawn from the uneven-batch
-    distribution, served by the continuous-batching loop in mixed precision,
-    and by a per-request FP64 loop (one request per launch, one stream) as the
-    baseline.  Latency = arrival -> recovered free values on the host."""
+    distribution, served by the continuous-batching loop in FP64 (default
+    kernel families: the flux-block elimination for the large classes) and in
+    mixed precision, and by a per-request FP64 loop (one request per launch,
+    one stream) as the baseline.  Latency = arrival -> recovered free values on
+    the host."""
     from gridaphybrid import serving as sv
     reqs = sv.draw_requests(args.serve, args.rate, SEED)
 
@@ -291,7 +293,8 @@ def serve_main(gh, args):
                 for r in reqs]
 
     out = {}
-    for name, kw in (("continuous_batching_mixed", dict(precision=gh.MIXED, streams=2)),
+    for name, kw in (("continuous_batching_fp64", dict(precision=gh.FP64, streams=2)),
+                     ("continuous_batching_mixed", dict(precision=gh.MIXED, streams=2)),
                      ("per_request_fp64", dict(precision=gh.FP64, streams=1, batching=False))):
         srv = sv.Server(gh, hbm_budget=200 << 30, **kw)
         rs = fresh()
@@ -303,7 +306,9 @@ def serve_main(gh, args):
         summ["mean_round"] = float(np.mean(srv.rounds)) if srv.rounds else 0.0
         out[name] = summ
         srv.close()
-    m = out["continuous_batching_mixed"]
+    best = max(("continuous_batching_fp64", "continuous_batching_mixed"),
+               key=lambda n: out[n]["cells_per_s"] or 0.0)
+    m = out[best]
     line = {
         "metric": METRIC,
         "value": m["cells_per_s"],
@@ -315,11 +320,12 @@ def serve_main(gh, args):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32+f64",
+        "dtype": "f64" if best.endswith("fp64") else "f32+f64",
         "data": "synthetic (seeded Poisson arrivals, uneven-batch mesh distribution)",
         "config": {"workload": f"serving: {args.serve} HDG solve requests, Poisson "
                                f"{args.rate:g} req/s, half Darcy 2D/3D, half 3D elasticity, "
-                               f"k in 1..3; skeleton = manufactured-trace interpolant"},
+                               f"k in 1..3; skeleton = manufactured-trace interpolant; "
+                               f"value = {best}"},
         "p50_ms": m["p50_ms"], "p99_ms": m["p99_ms"],
         "serving": out,
     }
@@ -380,18 +386,20 @@ def batch_main(gh, args):
         for op in ops:
             op.close()
     tot = {k: sum(x["graph_ms"] + x["recover_ms"] for x in v) for k, v in res.items()}
+    best = min(tot, key=tot.get)
     line = {
         "metric": METRIC,
-        "value": cells / (tot["mixed"] * 1e-3),
+        "value": cells / (tot[best] * 1e-3),
         "unit": "cells/s",
         "n_gpus": 1, "steps": 1, "warmup": 1,
-        "ms_per_step": tot["mixed"],
+        "ms_per_step": tot[best],
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32+f64",
+        "dtype": "f64" if best == "fp64" else "f32+f64",
         "data": "synthetic (seeded uneven batch)",
         "config": {"workload": f"uneven batch: {args.nmesh} meshes, {cells} cells, "
                                f"{len(waves)} HBM-sized batch graph(s)"},
         "fp64_cells_per_s": cells / (tot["fp64"] * 1e-3),
+        "mixed_cells_per_s": cells / (tot["mixed"] * 1e-3),
         "waves": res, "classes": classes,
     }
     print(json.dumps(line))
