@@ -1093,17 +1093,10 @@ int hdg_recover(hdg_mesh* m, const double* lam_free, double* free_values) {
             "mixed recover kernel");
     return HDG_OK;
   }
-  if (m->used_family == kFamilyFluxFused) {
+  if (m->used_family == kFamilyFluxFused || m->used_family == kFamilyFluxSchur) {
     const CellLayout L(m->th);
-    HIP_TRY(ctx, launch_recover_fused_flux(m->tdev, m->meshdev(), m->pU, m->pY, (int64_t)L.U,
-                                           (int64_t)L.Y, m->d_cell_dofs.as<int32_t>(), lam_free,
-                                           m->d_lam_dir.as<double>(), free_values, m->nfree,
-                                           ctx->stream),
-            "fused flux recover kernels");
-    return HDG_OK;
-  }
-  if (m->used_family == kFamilyFluxSchur) {
-    HIP_TRY(ctx, launch_recover_flux(m->tdev, m->meshdev(), m->pU, m->pY,
+    HIP_TRY(ctx, launch_recover_flux(m->tdev, m->meshdev(), m->pU, m->pY, (int64_t)L.U,
+                                     (int64_t)L.Y, m->used_family == kFamilyFluxFused,
                                      m->d_cell_dofs.as<int32_t>(), lam_free,
                                      m->d_lam_dir.as<double>(), free_values, m->nfree,
                                      ctx->stream),

@@ -231,17 +231,14 @@ hipError_t launch_condense_flux(const Tables& T, const MeshDev& M, const Condens
                                 hipStream_t st, Workspace& ws, int gridmul);
 hipError_t launch_condense_fused_flux(const Tables& T, const MeshDev& M, const CondenseOut& out,
                                       hipStream_t st, int gridmul);
-hipError_t launch_recover_fused_flux(const Tables& T, const MeshDev& M, const double* U,
-                                     const double* Y, int64_t usz, int64_t ysz,
-                                     const int32_t* cell_dofs, const double* lam_free,
-                                     const double* lam_dir, double* out, int64_t nfree,
-                                     hipStream_t st);
-// Recovery after a flux-block condensation: the reduced back substitution
-// (U, Y of the reduced system) gives x_u, x_q = (C^-1 (x) M^-1)(b_q - A_qu x_u
-// - A_qb lambda); then FP64 refinement against the residual of the full
-// interior system (the reduced system is worse conditioned than A_ii).
+// Recovery after a flux-block condensation (families 8 and 9; fused selects
+// family 9's slot layout): x_u = w_u - W_u lambda from the kept rows W_u of
+// A_ii^-1 [A_ib b_i], x_q = (C^-1 (x) M^-1)(b_q - A_qu x_u - A_qb lambda); then
+// FP64 refinement against the residual of the full interior system (the
+// reduced system is worse conditioned than A_ii).
 hipError_t launch_recover_flux(const Tables& T, const MeshDev& M, const double* U,
-                               const double* Y, const int32_t* cell_dofs, const double* lam_free,
+                               const double* Y, int64_t usz, int64_t ysz, bool fused,
+                               const int32_t* cell_dofs, const double* lam_free,
                                const double* lam_dir, double* out, int64_t nfree,
                                hipStream_t st);
 constexpr int kFluxRefineSteps = 2;  // FP64 refinement steps of the flux recovery (at most)

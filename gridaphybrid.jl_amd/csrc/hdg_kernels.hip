@@ -3576,6 +3576,156 @@ hipError_t launch_condense_fused_flux(const Tables& T, const MeshDev& M, const C
                           : launch_fused_phys<kElasticity>(T, M, out, st, gridmul);
 }
 
+// Regions of the flux family's per-cell factor slot (doubles from the slot's
+// start): the reduced factors U (packed by columns), the multipliers L
+// [step][row] and the pivot order P, then the kept interior blocks A_uu
+// [u col][u row], A_qu [u col][flux row], A_uq [flux col][u row], then
+// R_uu^-1 and U^-1, both [col][row].
+struct FluxSlot {
+  int64_t auu, aqu, auq, ri, uinv, size;
+};
+
+static FluxSlot flux_slot(int na, int nu) {
+  FluxSlot o;
+  o.auu = (int64_t)nu * (nu + 1) / 2 + (int64_t)nu * nu + nu;
+  o.aqu = o.auu + (int64_t)nu * nu;
+  o.auq = o.aqu + (int64_t)na * nu;
+  o.ri = o.auq + (int64_t)na * nu;
+  o.uinv = o.ri + (int64_t)nu * nu;
+  o.size = o.uinv + (int64_t)nu * nu;
+  return o;
+}
+
+// R_uu^-1 = U^-1 L^-1 P and U^-1 from the reduced factors, one wavefront per
+// cell with a lane per column of the result held in registers (NUM >= nu).
+// U (column-major) and the multipliers (permuted to positions) are staged in
+// LDS zero-padded to NUM x NUM, with 1 / diag(U) (0 for a zero pivot), so
+// the unrolled substitutions test bounds once per column (each column's
+// operands load together) and read them as broadcasts.  With NUM = 32 one pass does both inverses: lanes
+// [0, nu) solve for the columns of R_uu^-1, lanes [32, 32 + nu) for those of
+// U^-1, and the columns go out through LDS as one contiguous copy; with
+// NUM = 64 each inverse takes a pass.  The recovery then needs only products
+// with kept matrices.  A zero pivot of U gives a zero row, as in the
+// substitution (the condensation reports such cells as singular).
+constexpr int kInvWaves = 2;
+
+template <int NUM>
+__global__ void __launch_bounds__(64 * kInvWaves)
+    flux_invert_kernel(double* __restrict__ slot, int64_t usz, int64_t ncells, int nu,
+                       int64_t ri_off) {
+  extern __shared__ double s_inv[];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int nup = nu * (nu + 1) / 2, nn = nu * nu;
+  double* sU = s_inv + (int64_t)w * (2 * NUM * NUM + 2 * NUM);  // U [col][row], padded
+  double* sL = sU + NUM * NUM;  // multipliers [step][position], padded
+  double* sP = sL + NUM * NUM;  // row at each position (-1 past nu)
+  double* sD = sP + NUM;        // 1 / U[c][c]
+  for (int64_t cell = (int64_t)blockIdx.x * kInvWaves + w; cell < ncells;
+       cell += (int64_t)gridDim.x * kInvWaves) {
+    double* Uc = slot + cell * usz;
+    const double* Lc = Uc + nup;
+    // multipliers as stored into sU's region first, then permuted into sL
+    for (int t = lane; t < nn; t += 64) sU[t] = Lc[t];
+    for (int t = lane; t < NUM; t += 64) sP[t] = t < nu ? Lc[nn + t] : -1.0;
+    wave_sync_lds();
+    for (int t = lane; t < NUM * NUM; t += 64) {
+      const int i = t / NUM, k = t - i * NUM;
+      sL[t] = (i < nu && k < nu) ? sU[i * nu + (int)sP[k]] : 0.0;
+    }
+    wave_sync_lds();
+    for (int t = lane; t < NUM * NUM; t += 64) {
+      const int c = t / NUM, k = t - c * NUM;
+      sU[t] = (c < nu && k <= c) ? Uc[c * (c + 1) / 2 + k] : 0.0;
+    }
+    for (int t = lane; t < NUM; t += 64) {
+      const double d = t < nu ? Uc[t * (t + 1) / 2 + t] : 0.0;
+      sD[t] = d != 0.0 ? 1.0 / d : 0.0;
+    }
+    wave_sync_lds();
+    // y <- U^-1 y
+    auto back_subst = [&](double (&y)[NUM]) {
+#pragma unroll
+      for (int c = NUM - 1; c >= 0; --c) {
+        if (c < nu) {  // one block per column: its operands load together
+          const double* col = sU + c * NUM;
+          y[c] *= sD[c];
+#pragma unroll
+          for (int k = 0; k < c; ++k) y[k] = fma(-col[k], y[c], y[k]);
+        }
+      }
+    };
+    // y <- L^-1 y (positions)
+    auto forward = [&](double (&y)[NUM]) {
+#pragma unroll
+      for (int i = 0; i < NUM - 1; ++i) {
+        if (i + 1 < nu) {
+          const double* li = sL + i * NUM;
+#pragma unroll
+          for (int k = i + 1; k < NUM; ++k) y[k] = fma(-li[k], y[i], y[k]);
+        }
+      }
+    };
+    double* out = Uc + ri_off;  // R_uu^-1 then U^-1, [col][row]
+    if (NUM == 32) {
+      const bool rinv = lane < 32;
+      const int j = lane & 31;
+      double y[NUM];
+#pragma unroll
+      for (int k = 0; k < NUM; ++k)
+        y[k] = rinv ? ((int)sP[k] == j ? 1.0 : 0.0) : (k == j ? 1.0 : 0.0);
+      if (rinv) forward(y);
+      back_subst(y);
+      wave_sync_lds();  // the staged factors are dead: their LDS takes the columns
+      if (j < nu) {
+        double* col = sU + (rinv ? 0 : nn) + j * nu;
+#pragma unroll
+        for (int k = 0; k < NUM; ++k)
+          if (k < nu) col[k] = y[k];
+      }
+      wave_sync_lds();
+      for (int t = lane; t < 2 * nn; t += 64) out[t] = sU[t];
+    } else {
+#pragma unroll 1
+      for (int pass = 0; pass < 2; ++pass) {
+        const int j = lane;
+        double y[NUM];
+#pragma unroll
+        for (int k = 0; k < NUM; ++k)
+          y[k] = pass == 0 ? ((int)sP[k] == j ? 1.0 : 0.0) : (k == j ? 1.0 : 0.0);
+        if (pass == 0) forward(y);
+        back_subst(y);
+        if (j < nu) {
+          double* col = out + (int64_t)pass * nn + (int64_t)j * nu;
+#pragma unroll
+          for (int k = 0; k < NUM; ++k)
+            if (k < nu) col[k] = y[k];
+        }
+      }
+    }
+    wave_sync_lds();
+  }
+}
+
+static hipError_t launch_flux_invert(double* slot, int64_t usz, int64_t ncells, int na, int nu,
+                                     hipStream_t st) {
+  if (ncells <= 0) return hipSuccess;
+  const FluxSlot fs = flux_slot(na, nu);
+  const int num = nu <= 32 ? 32 : 64;
+  const size_t lds = kInvWaves * (size_t)(2 * num * num + 2 * num) * sizeof(double);
+  const int64_t blocks = (ncells + kInvWaves - 1) / kInvWaves;
+  const int grid = (int)(blocks < 65536 ? blocks : 65536);
+  ClockScope clk_(st, "flux_invert");
+  if (nu <= 32)
+    hipLaunchKernelGGL((flux_invert_kernel<32>), dim3(grid), dim3(64 * kInvWaves), lds, st,
+                       slot, usz, ncells, nu, fs.ri);
+  else if (nu <= 64)
+    hipLaunchKernelGGL((flux_invert_kernel<64>), dim3(grid), dim3(64 * kInvWaves), lds, st,
+                       slot, usz, ncells, nu, fs.ri);
+  else
+    return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
 // Cells per flux chunk: the reduced systems of one chunk live in the workspace.
 static int64_t flux_chunk(int nr) {
   const int64_t per = (int64_t)nr * (nr + 1) * (int64_t)sizeof(double);
@@ -3646,7 +3796,7 @@ static hipError_t launch_flux_phys(const Tables& T, const MeshDev& M, const Cond
     e = launch_condense<ReducedGen, false>(rg, o, count, nu, nb, st, none, rfam, gridmul);
     if (e != hipSuccess) return e;
   }
-  return hipSuccess;
+  return launch_flux_invert(out.U, usz, ncells, T.na, nu, st);
 }
 
 hipError_t launch_condense_flux(const Tables& T, const MeshDev& M, const CondenseOut& out,
@@ -3659,7 +3809,7 @@ size_t factor_slot(const SysShape& sh) {
   const size_t dense = (size_t)sh.ni * (sh.ni + 1) / 2;
   if (!sh.flux) return dense;
   const size_t m = (size_t)(sh.ni - sh.flux_na), na = (size_t)sh.flux_na;
-  size_t s = std::max(dense, m * (m + 1) / 2 + 2 * m * m + m + 2 * na * m);
+  size_t s = std::max(dense, (size_t)flux_slot((int)na, (int)m).size);
   if (sh.fused) s = std::max(s, fused_slot(sh.flux_na, (int)m));
   return s;
 }
@@ -3685,347 +3835,67 @@ SysShape sys_shape(const TablesHost& t) {
   return sh;
 }
 
-// Flux recovery.  One wavefront per cell, with the reduced factors and the
-// interior blocks A_uu, A_qu, A_uq the condensation kept in the cell's slot:
+// Flux recovery, one wavefront per cell, for both flux families.  The cell's
+// slot holds R_uu^-1, the interior blocks A_uu, A_qu, A_uq and the rows W_u of
+// W = A_ii^-1 [A_ib b_i] of the remaining interior unknowns (RecSlot gives
+// where):
 //   c   = b_i - A_ib lambda                 (boundary columns of the program)
-//   x_u = U^-1 (y - Y lambda)               (reduced factors)
+//   x_u = w_u - W_u lambda                  (family 9 keeps the rows W_u), or
+//   x_u = U^-1 (y - Y lambda)               (family 8: Y = L^-1 P [R_ub r_u] of
+//                                            the reduced system, U^-1 kept)
 //   x_q = A_qq^-1 (c_q - A_qu x_u)          (A_qq^-1 = C^-1 (x) M^-1)
 // then up to kFluxRefineSteps FP64 refinement steps against the residual of
 // the full interior system, r = c - A_ii x, with the structured solve as the
 // approximate inverse:
-//   z = A_qq^-1 r_q,  d_u = R_uu^-1 (r_u - A_uq z),  d_q = z - A_qq^-1 A_qu d_u
-// (R_uu^-1 through the kept multipliers, pivot order and U).  The reduced
-// system R_uu is worse conditioned than A_ii when the flux block is small
-// (large kappa): the first x_u can be off by cond(R_uu) eps in the near-null
-// mode of A_uq A_qq^-1 A_qu, and the refinement brings it to the accuracy of
-// a direct FP64 solve of A_ii.  It stops once a correction is below 2^-48 of
+//   z = A_qq^-1 r_q,  d_u = R_uu^-1 (r_u - A_uq z),  d_q = z - A_qq^-1 A_qu d_u.
+// The reduced system R_uu is worse conditioned than A_ii when the flux block
+// is small (large kappa): the first x_u can be off by cond(R_uu) eps in the
+// near-null mode of A_uq A_qq^-1 A_qu, and the refinement brings it to the
+// accuracy of a direct FP64 solve of A_ii.  Each step reads every kept block
+// once: A_qu x_u is carried from step to step (A_qu x_u + A_qu d_u), and
+// r_u - A_uq z is formed as c_u - A_uu x_u - A_uq (x_q + z), whose rounding is
+// that of the residual itself.  It stops once a correction is below 2^-48 of
 // the solution.
-constexpr int kFluxMaxNb = 192;
-constexpr int kFusedMaxNb = 128;  // boundary values held in LDS by the flux recoveries
+constexpr int kFusedMaxNb = 128;  // boundary values held in LDS by the flux recovery
+constexpr int kRecWaves = 4;      // cells (wavefronts) per workgroup of the flux recovery
+
+// doubles of LDS per wavefront of the flux recovery
+__host__ __device__ inline int rec_wave_lds(int na, int nu, int nb) {
+  return kMaxAlpha + 72 + nb + 4 * na + (na > 72 ? na : 72) + 2 * nu;
+}
+
+struct RecSlot {
+  const double* slot;  // per-cell slots, stride usz
+  int64_t usz;
+  int64_t ri, auu, aqu, auq;  // R_uu^-1 [col][row], A_uu, A_qu, A_uq (offsets in the slot)
+  int64_t uinv;               // U^-1 [col][row], or -1: x_u = w - W lambda directly
+  const double* w;     // W_u or Y: w + cell * wsz + column * wld + u row
+  int64_t wsz;
+  int wld;
+};
 
 template <int PHYS, int KQ>
 __global__ void __launch_bounds__(256)
-    recover_flux_kernel(FusedGen<PHYS> gen, const double* __restrict__ U,
-                        const double* __restrict__ Y, int64_t usz, int64_t ysz,
-                        const int32_t* __restrict__ cell_dofs, const double* __restrict__ lam_free,
-                        const double* __restrict__ lam_dir, double* __restrict__ out) {
-  constexpr int WPB = 4;
-  constexpr int NQ = 64 * KQ;  // flux rows per wavefront, at most
-  __shared__ CellCoefs s_sh[WPB];
-  __shared__ double s_lam[WPB][kFusedMaxNb];
-  __shared__ double s_xq[WPB][NQ];  // x_q
-  __shared__ double s_xu[WPB][64];  // x_u, then corrections d_u
-  __shared__ double s_cq[WPB][NQ];  // c_q
-  __shared__ double s_q[WPB][NQ];   // flux work vector
-  __shared__ double s_z[WPB][NQ];
-  __shared__ double s_u[WPB][64];   // reduced work vector
-  __shared__ double s_c6[WPB][2][36];  // C and C^-1
-  __shared__ double s_cw[WPB][72];
-  extern __shared__ double s_m[];  // M^-1 [np][np], then M
+    recover_flux_kernel(FusedGen<PHYS> gen, RecSlot rs, const int32_t* __restrict__ cell_dofs,
+                        const double* __restrict__ lam_free, const double* __restrict__ lam_dir,
+                        double* __restrict__ out) {
+  constexpr int WPB = kRecWaves;
+  extern __shared__ double s_m[];  // M^-1 [np][np], M, then the wavefronts' vectors
   const Tables& T = gen.T;
   const int ni = T.ni, nb = T.nb, na = T.na, np = T.np, nu = ni - na, nqc = T.nqc;
   const int ncomp = T.fb_nc;
   const int64_t ncells = gen.M.ncells;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  double* lam = s_lam[w];
-  double* xq = s_xq[w];
-  double* xuv = s_xu[w];
-  double* cq = s_cq[w];
-  double* qv = s_q[w];
-  double* zv = s_z[w];
-  double* uv = s_u[w];
-  for (int t = threadIdx.x; t < np * np; t += 64 * WPB) {
-    s_m[t] = T.fb_minv[t];
-    s_m[np * np + t] = T.Mpp[t];
-  }
-  __syncthreads();
-
-  // v <- (Cm (x) Mm) v over the flux rows (v in LDS): Mm = M^-1 with Cm = C^-1
-  // for a solve, Mm = M with Cm = C for a product
-  auto flux_apply = [&](double* v, const double* Mm, const double* Cm) {
-    double y[KQ];
-#pragma unroll
-    for (int t = 0; t < KQ; ++t) {
-      const int r = lane + 64 * t;
-      double acc = 0.0;
-      if (r < na) {
-        const int a = r / np, i = r - a * np;
-        const double* mr = Mm + (int64_t)i * np;
-        const double* vr = v + a * np;
-        for (int j = 0; j < np; ++j) acc = fma(mr[j], vr[j], acc);
-      }
-      y[t] = acc;
-    }
-    wave_sync_lds();
-#pragma unroll
-    for (int t = 0; t < KQ; ++t) {
-      const int r = lane + 64 * t;
-      if (r < na) v[r] = y[t];
-    }
-    wave_sync_lds();
-#pragma unroll
-    for (int t = 0; t < KQ; ++t) {
-      const int r = lane + 64 * t;
-      double acc = 0.0;
-      if (r < na) {
-        const int a = r / np, i = r - a * np;
-        for (int b = 0; b < ncomp; ++b) acc = fma(Cm[6 * a + b], v[b * np + i], acc);
-      }
-      y[t] = acc;
-    }
-    wave_sync_lds();
-#pragma unroll
-    for (int t = 0; t < KQ; ++t) {
-      const int r = lane + 64 * t;
-      if (r < na) v[r] = y[t];
-    }
-    wave_sync_lds();
-  };
-  // U x = y over the reduced unknowns, lane k holding y_k / x_k (k < nu <= 64)
-  auto back_subst = [&](const double* Uc, double x) {
-#pragma unroll 1
-    for (int c = nu - 1; c >= 0; --c) {
-      const int64_t col0 = (int64_t)c * (c + 1) / 2;
-      const double ucc = Uc[col0 + c];
-      const double xc0 = __shfl(x, c, 64);
-      const double xc = ucc != 0.0 ? xc0 / ucc : 0.0;
-      if (lane < c) x = fma(-Uc[col0 + lane], xc, x);
-      if (lane == c) x = xc;
-    }
-    return x;
-  };
-
-  for (int64_t cell = (int64_t)blockIdx.x * WPB + w; cell < ncells;
-       cell += (int64_t)gridDim.x * WPB) {
-    for (int t = lane; t < kMaxAlpha; t += 64) s_sh[w].alpha[t] = gen.M.alpha[cell * kMaxAlpha + t];
-    for (int t = lane; t < ni; t += 64)
-      s_sh[w].b[t] = gen.M.srcq[cell * srcq_stride(T) + nqc * 4 + t];
-    for (int j = lane; j < nb; j += 64) {
-      const int32_t d = cell_dofs[cell * nb + j];
-      lam[j] = d >= 0 ? lam_free[d] : lam_dir[-(int64_t)d - 1];
-    }
-    wave_sync_lds();
-    if (lane < 36) {
-      const int a = lane / 6, b = lane - 6 * (lane / 6);
-      const int id = T.fb_cid[lane];
-      s_c6[w][0][lane] = (a < ncomp && b < ncomp && id >= 0) ? s_sh[w].alpha[id] : 0.0;
-    }
-    if (lane == 0 && flux_cinv(ncomp, T.fb_cid, s_sh[w].alpha, s_cw[w], s_c6[w][1]))
-      for (int t = 0; t < 36; ++t) s_c6[w][1][t] = 0.0;
-    const double* Uc = U + cell * usz;
-    const double* Lc = Uc + (int64_t)nu * (nu + 1) / 2;  // multipliers [step][row]
-    const double* Pc = Lc + (int64_t)nu * nu;             // row at each position
-    const double* Auu = Pc + nu;                          // [u col][u row]
-    const double* Aqu = Auu + (int64_t)nu * nu;           // [u col][flux row]
-    const double* Auq = Aqu + (int64_t)na * nu;           // [flux col][u row]
-    const double* Yc = Y + cell * ysz;
-    // ---- c = b_i - A_ib lambda ------------------------------------------------
-    double cu = 0.0;
-#pragma unroll
-    for (int t = 0; t < KQ + 1; ++t) {
-      const int r = lane + 64 * t;
-      if (r < ni) {
-        double v = s_sh[w].b[r];
-        const int k1 = T.prog_rptr[r + 1];
-        for (int k = T.prog_rbsplit[r]; k < k1; ++k)
-          v = fma(-(s_sh[w].alpha[T.prog_rid[k]] * T.prog_rval[k]), lam[T.prog_rcol[k] - ni], v);
-        if (r < na)
-          cq[r] = v;
-        else
-          uv[r - na] = v;
-      }
-    }
-    wave_sync_lds();
-    if (lane < nu) cu = uv[lane];
-    // ---- x_u = U^-1 (y - Y lambda) --------------------------------------------
-    double xu = 0.0;
-    if (lane < nu) {
-      xu = Yc[(int64_t)nb * nu + lane];
-      for (int j = 0; j < nb; ++j) xu = fma(-Yc[(int64_t)j * nu + lane], lam[j], xu);
-    }
-    xu = back_subst(Uc, xu);
-    if (lane < nu) xuv[lane] = xu;
-    wave_sync_lds();
-    // ---- x_q = A_qq^-1 (c_q - A_qu x_u) ----------------------------------------
-#pragma unroll
-    for (int t = 0; t < KQ; ++t) {
-      const int r = lane + 64 * t;
-      if (r < na) {
-        double v = cq[r];
-        for (int c = 0; c < nu; ++c) v = fma(-Aqu[(int64_t)c * na + r], xuv[c], v);
-        qv[r] = v;
-      }
-    }
-    wave_sync_lds();
-    flux_apply(qv, s_m, s_c6[w][1]);
-#pragma unroll
-    for (int t = 0; t < KQ; ++t) {
-      const int r = lane + 64 * t;
-      if (r < na) xq[r] = qv[r];
-    }
-    wave_sync_lds();
-    const int at = lane < nu ? (int)Pc[lane] : 0;
-    // ---- FP64 refinement against the full interior system ---------------------
-#pragma unroll 1
-    for (int it = 0; it < kFluxRefineSteps; ++it) {
-      // r_q = c_q - (C (x) M) x_q - A_qu x_u  (into zv);  r_u = c_u - A_uq x_q - A_uu x_u
-#pragma unroll
-      for (int t = 0; t < KQ; ++t) {
-        const int r = lane + 64 * t;
-        if (r < na) zv[r] = xq[r];
-      }
-      wave_sync_lds();
-      flux_apply(zv, s_m + np * np, s_c6[w][0]);
-#pragma unroll
-      for (int t = 0; t < KQ; ++t) {
-        const int r = lane + 64 * t;
-        if (r < na) {
-          double v = cq[r] - zv[r];
-          for (int c = 0; c < nu; ++c) v = fma(-Aqu[(int64_t)c * na + r], xuv[c], v);
-          zv[r] = v;
-        }
-      }
-      double ru = 0.0;
-      if (lane < nu) {
-        ru = cu;
-        for (int j = 0; j < na; ++j) ru = fma(-Auq[(int64_t)j * nu + lane], xq[j], ru);
-        for (int c = 0; c < nu; ++c) ru = fma(-Auu[(int64_t)c * nu + lane], xuv[c], ru);
-      }
-      wave_sync_lds();
-      flux_apply(zv, s_m, s_c6[w][1]);  // z = A_qq^-1 r_q
-      // t = r_u - A_uq z, in LAPACK order, then d_u = U^-1 L^-1 P t
-      if (lane < nu) {
-        for (int j = 0; j < na; ++j) ru = fma(-Auq[(int64_t)j * nu + lane], zv[j], ru);
-        uv[lane] = ru;
-      }
-      wave_sync_lds();
-      double y = lane < nu ? uv[at] : 0.0;
-#pragma unroll 1
-      for (int i = 0; i < nu; ++i) {
-        const double yi = __shfl(y, i, 64);
-        if (lane > i && lane < nu) y = fma(-Lc[(int64_t)i * nu + at], yi, y);
-      }
-      const double du = back_subst(Uc, y);
-      if (lane < nu) uv[lane] = du;
-      wave_sync_lds();
-      // d_q = z - A_qq^-1 (A_qu d_u)
-#pragma unroll
-      for (int t = 0; t < KQ; ++t) {
-        const int r = lane + 64 * t;
-        if (r < na) {
-          double v = 0.0;
-          for (int c = 0; c < nu; ++c) v = fma(Aqu[(int64_t)c * na + r], uv[c], v);
-          qv[r] = v;
-        }
-      }
-      wave_sync_lds();
-      flux_apply(qv, s_m, s_c6[w][1]);
-      double dmax = lane < nu ? fabs(du) : 0.0, xmax = 0.0;
-#pragma unroll
-      for (int t = 0; t < KQ; ++t) {
-        const int r = lane + 64 * t;
-        if (r < na) {
-          const double d = zv[r] - qv[r];
-          const double x = xq[r] + d;
-          xq[r] = x;
-          dmax = fmax(dmax, fabs(d));
-          xmax = fmax(xmax, fabs(x));
-        }
-      }
-      if (lane < nu) {
-        const double x = xuv[lane] + du;
-        xuv[lane] = x;
-        xmax = fmax(xmax, fabs(x));
-      }
-      wave_sync_lds();
-      dmax = __builtin_bit_cast(double, wave_max_i64(__builtin_bit_cast(int64_t, dmax)));
-      xmax = __builtin_bit_cast(double, wave_max_i64(__builtin_bit_cast(int64_t, xmax)));
-      if (dmax <= 0x1p-48 * xmax) break;  // wave-uniform
-    }
-#pragma unroll
-    for (int t = 0; t < KQ; ++t) {
-      const int r = lane + 64 * t;
-      if (r < na) out[cell * na + r] = xq[r];
-    }
-    if (lane < nu) out[ncells * na + cell * nu + lane] = xuv[lane];
-    wave_sync_lds();
-  }
-}
-
-hipError_t launch_recover_flux(const Tables& T, const MeshDev& M, const double* U,
-                               const double* Y, const int32_t* cell_dofs, const double* lam_free,
-                               const double* lam_dir, double* out, int64_t nfree,
-                               hipStream_t st) {
-  const int na = T.na, nu = T.ni - na;
-  if (na > kMixedMaxNi || nu > 64 || T.nb > kFusedMaxNb) return hipErrorInvalidValue;
-  const int64_t usz = (int64_t)factor_slot(sys_shape(T)), ysz = (int64_t)(T.nb + 1) * T.ni;
-  if (M.ncells > 0) {
-    const int64_t blocks = (M.ncells + 3) / 4;
-    const int grid = (int)(blocks < 65536 ? blocks : 65536);
-    const size_t lds = 2 * (size_t)T.np * T.np * sizeof(double);
-    ClockScope clk_(st, "recover_flux");
-#define HDG_RF(P_, K_)                                                                      \
-  hipLaunchKernelGGL((recover_flux_kernel<P_, K_>), dim3(grid), dim3(256), lds, st,         \
-                     FusedGen<P_>{T, M}, U, Y, usz, ysz, cell_dofs, lam_free, lam_dir, out)
-    if (T.phys == kDarcy) {
-      if (na <= 64) HDG_RF(kDarcy, 1); else HDG_RF(kDarcy, 2);
-    } else {
-      if (na <= 64) HDG_RF(kElasticity, 1); else HDG_RF(kElasticity, 2);
-    }
-#undef HDG_RF
-  }
-  if (nfree > 0) {
-    ClockScope clk_(st, "recover_copy");
-    hipLaunchKernelGGL(copy_kernel, dim3(1024), dim3(256), 0, st, lam_free,
-                       out + M.ncells * T.ni, nfree);
-  }
-  return hipGetLastError();
-}
-
-// Recovery after the fused flux-block condensation, one wavefront per cell:
-//   x_u = w_u - W_u lambda                 (rows of the remaining interior
-//                                           unknowns of W = A_ii^-1 [A_ib b_i])
-//   x_q = A_qq^-1 (c_q - A_qu x_u),  c = b_i - A_ib lambda
-// then up to kFluxRefineSteps FP64 refinement steps against the residual of
-// the full interior system, r = b_i - A_ib lambda - A_ii x, with the
-// structured solve as the approximate inverse:
-//   z = A_qq^-1 r_q,  d_u = R_uu^-1 (r_u - A_uq z),  d_q = z - A_qq^-1 A_qu d_u
-// (R_uu^-1, A_uu, A_qu, A_uq kept in the cell's slot; A_qq = C (x) M).  The
-// first x_u carries the cancellation of w_u - W_u lambda and, in the
-// near-null mode of A_uq A_qq^-1 A_qu, cond(R_uu) eps; the refinement brings
-// it to the accuracy of a direct FP64 solve of A_ii.
-template <int PHYS, int KQ>
-__global__ void __launch_bounds__(256)
-    recover_fused_kernel(FusedGen<PHYS> gen, const double* __restrict__ slot,
-                         const double* __restrict__ W, int64_t usz, int64_t ysz,
-                         const int32_t* __restrict__ cell_dofs,
-                         const double* __restrict__ lam_free, const double* __restrict__ lam_dir,
-                         double* __restrict__ out) {
-  constexpr int WPB = 4;
-  constexpr int NQ = 64 * KQ;
-  __shared__ CellCoefs s_sh[WPB];
-  __shared__ double s_lam[WPB][kFusedMaxNb];
-  __shared__ double s_xq[WPB][NQ];
-  __shared__ double s_xu[WPB][64];
-  __shared__ double s_cq[WPB][NQ];
-  __shared__ double s_q[WPB][NQ];
-  __shared__ double s_z[WPB][NQ];
-  __shared__ double s_u[WPB][64];
-  __shared__ double s_c6[WPB][2][36];
-  __shared__ double s_cw[WPB][72];
-  extern __shared__ double s_m[];  // M^-1 [np][np], then M
-  const Tables& T = gen.T;
-  const int ni = T.ni, nb = T.nb, na = T.na, np = T.np, nu = ni - na, nqc = T.nqc;
-  const int ncomp = T.fb_nc;
-  const int64_t ncells = gen.M.ncells;
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  double* lam = s_lam[w];
-  double* xq = s_xq[w];
-  double* xuv = s_xu[w];
-  double* cq = s_cq[w];
-  double* qv = s_q[w];
-  double* zv = s_z[w];
-  double* uv = s_u[w];
+  double* alpha = s_m + 2 * np * np + (int64_t)w * rec_wave_lds(na, nu, nb);  // [kMaxAlpha]
+  double* c6 = alpha + kMaxAlpha;  // C, then C^-1 [2][36]
+  double* lam = c6 + 72;           // [nb]
+  double* xq = lam + nb;           // x_q [na]
+  double* cq = xq + na;            // c_q [na]
+  double* aq = cq + na;            // A_qu x_u [na]
+  double* qv = aq + na;            // flux work vector [na]
+  double* zv = qv + na;            // [max(na, 72)]; flux_cinv's scratch at a cell's start
+  double* xuv = zv + (na > 72 ? na : 72);  // x_u [nu]
+  double* uv = xuv + nu;           // reduced work vector [nu]
   for (int t = threadIdx.x; t < np * np; t += 64 * WPB) {
     s_m[t] = T.fb_minv[t];
     s_m[np * np + t] = T.Mpp[t];
@@ -4034,7 +3904,8 @@ __global__ void __launch_bounds__(256)
   const double* Minv = s_m;
   const double* Mm = s_m + np * np;
 
-  // v <- (Cm (x) Mt) v over the flux rows (v in LDS)
+  // v <- (Cm (x) Mt) v over the flux rows (v in LDS): Mt = M^-1 with Cm = C^-1
+  // for a solve, Mt = M with Cm = C for a product
   auto flux_apply = [&](double* v, const double* Mt, const double* Cm) {
     double y[KQ];
 #pragma unroll
@@ -4074,12 +3945,22 @@ __global__ void __launch_bounds__(256)
     }
     wave_sync_lds();
   };
+  // qv = A_qu v (v over the u unknowns, in LDS)
+  auto aqu_apply = [&](const double* Aqu, const double* v) {
+#pragma unroll
+    for (int t = 0; t < KQ; ++t) {
+      const int r = lane + 64 * t;
+      if (r < na) {
+        double acc = 0.0;
+        for (int c = 0; c < nu; ++c) acc = fma(Aqu[(int64_t)c * na + r], v[c], acc);
+        qv[r] = acc;
+      }
+    }
+  };
 
   for (int64_t cell = (int64_t)blockIdx.x * WPB + w; cell < ncells;
        cell += (int64_t)gridDim.x * WPB) {
-    for (int t = lane; t < kMaxAlpha; t += 64) s_sh[w].alpha[t] = gen.M.alpha[cell * kMaxAlpha + t];
-    for (int t = lane; t < ni; t += 64)
-      s_sh[w].b[t] = gen.M.srcq[cell * srcq_stride(T) + nqc * 4 + t];
+    for (int t = lane; t < kMaxAlpha; t += 64) alpha[t] = gen.M.alpha[cell * kMaxAlpha + t];
     for (int j = lane; j < nb; j += 64) {
       const int32_t d = cell_dofs[cell * nb + j];
       lam[j] = d >= 0 ? lam_free[d] : lam_dir[-(int64_t)d - 1];
@@ -4088,86 +3969,101 @@ __global__ void __launch_bounds__(256)
     if (lane < 36) {
       const int a = lane / 6, b = lane - 6 * (lane / 6);
       const int id = T.fb_cid[lane];
-      s_c6[w][0][lane] = (a < ncomp && b < ncomp && id >= 0) ? s_sh[w].alpha[id] : 0.0;
+      c6[lane] = (a < ncomp && b < ncomp && id >= 0) ? alpha[id] : 0.0;
     }
-    if (lane == 0 && flux_cinv(ncomp, T.fb_cid, s_sh[w].alpha, s_cw[w], s_c6[w][1]))
-      for (int t = 0; t < 36; ++t) s_c6[w][1][t] = 0.0;
-    const double* Ri = slot + cell * usz;                // R_uu^-1 [col][row]
-    const double* Auu = Ri + (int64_t)nu * nu;           // [u col][u row]
-    const double* Aqu = Auu + (int64_t)nu * nu;          // [u col][flux row]
-    const double* Auq = Aqu + (int64_t)na * nu;          // [flux col][u row]
-    const double* Wc = W + cell * ysz;                   // [nb+1][ni]
-    // ---- c = b_i - A_ib lambda and x = w - W lambda ---------------------------------
+    if (lane == 0 && flux_cinv(ncomp, T.fb_cid, alpha, zv, c6 + 36))
+      for (int t = 0; t < 36; ++t) c6[36 + t] = 0.0;
+    wave_sync_lds();
+    const double* base = rs.slot + cell * rs.usz;
+    const double* Ri = base + rs.ri;
+    const double* Auu = base + rs.auu;
+    const double* Aqu = base + rs.aqu;
+    const double* Auq = base + rs.auq;
+    const double* Wc = rs.w + cell * rs.wsz;
+    const double* bi = gen.M.srcq + cell * srcq_stride(T) + nqc * 4;
+    // ---- c = b_i - A_ib lambda and x_u = w_u - W_u lambda ------------------------
     double cu = 0.0;
 #pragma unroll
     for (int t = 0; t < KQ + 1; ++t) {
       const int r = lane + 64 * t;
       if (r < ni) {
-        double v = s_sh[w].b[r];
+        double v = bi[r];
         const int k1 = T.prog_rptr[r + 1];
         for (int k = T.prog_rbsplit[r]; k < k1; ++k)
-          v = fma(-(s_sh[w].alpha[T.prog_rid[k]] * T.prog_rval[k]), lam[T.prog_rcol[k] - ni], v);
+          v = fma(-(alpha[T.prog_rid[k]] * T.prog_rval[k]), lam[T.prog_rcol[k] - ni], v);
         if (r < na) {
           cq[r] = v;
         } else {
-          double x = Wc[(int64_t)nb * ni + r];
-          for (int j = 0; j < nb; ++j) x = fma(-Wc[(int64_t)j * ni + r], lam[j], x);
-          uv[r - na] = v;
-          xuv[r - na] = x;
+          const int u = r - na;
+          double x = Wc[(int64_t)nb * rs.wld + u];
+          for (int j = 0; j < nb; ++j) x = fma(-Wc[(int64_t)j * rs.wld + u], lam[j], x);
+          uv[u] = v;
+          xuv[u] = x;
         }
       }
     }
     wave_sync_lds();
     if (lane < nu) cu = uv[lane];
-    // x_q = A_qq^-1 (c_q - A_qu x_u)
+    if (rs.uinv >= 0) {  // x_u = U^-1 (y - Y lambda)
+      const double* Ui = base + rs.uinv;
+      double x = 0.0;
+      if (lane < nu)
+        for (int c = 0; c < nu; ++c) x = fma(Ui[(int64_t)c * nu + lane], xuv[c], x);
+      wave_sync_lds();
+      if (lane < nu) xuv[lane] = x;
+      wave_sync_lds();
+    }
+    // ---- x_q = A_qq^-1 (c_q - A_qu x_u) -------------------------------------------
+    aqu_apply(Aqu, xuv);
 #pragma unroll
     for (int t = 0; t < KQ; ++t) {
       const int r = lane + 64 * t;
       if (r < na) {
-        double v = cq[r];
-        for (int c = 0; c < nu; ++c) v = fma(-Aqu[(int64_t)c * na + r], xuv[c], v);
-        qv[r] = v;
+        aq[r] = qv[r];
+        qv[r] = cq[r] - qv[r];
       }
     }
     wave_sync_lds();
-    flux_apply(qv, Minv, s_c6[w][1]);
+    flux_apply(qv, Minv, c6 + 36);
 #pragma unroll
     for (int t = 0; t < KQ; ++t) {
       const int r = lane + 64 * t;
       if (r < na) xq[r] = qv[r];
     }
     wave_sync_lds();
-    // ---- FP64 refinement against the full interior system ----------------------------
+    // ---- FP64 refinement against the full interior system ---------------------
 #pragma unroll 1
     for (int it = 0; it < kFluxRefineSteps; ++it) {
-      // r_q = c_q - (C (x) M) x_q - A_qu x_u  (into zv);  r_u = c_u - A_uq x_q - A_uu x_u
+      // r_q = c_q - (C (x) M) x_q - A_qu x_u  (into zv)
 #pragma unroll
       for (int t = 0; t < KQ; ++t) {
         const int r = lane + 64 * t;
         if (r < na) zv[r] = xq[r];
       }
       wave_sync_lds();
-      flux_apply(zv, Mm, s_c6[w][0]);
+      flux_apply(zv, Mm, c6);
 #pragma unroll
       for (int t = 0; t < KQ; ++t) {
         const int r = lane + 64 * t;
-        if (r < na) {
-          double v = cq[r] - zv[r];
-          for (int c = 0; c < nu; ++c) v = fma(-Aqu[(int64_t)c * na + r], xuv[c], v);
-          zv[r] = v;
-        }
+        if (r < na) zv[r] = (cq[r] - zv[r]) - aq[r];
       }
       double ru = 0.0;
       if (lane < nu) {
         ru = cu;
-        for (int j = 0; j < na; ++j) ru = fma(-Auq[(int64_t)j * nu + lane], xq[j], ru);
         for (int c = 0; c < nu; ++c) ru = fma(-Auu[(int64_t)c * nu + lane], xuv[c], ru);
       }
       wave_sync_lds();
-      flux_apply(zv, Minv, s_c6[w][1]);  // z = A_qq^-1 r_q
+      flux_apply(zv, Minv, c6 + 36);  // z = A_qq^-1 r_q
+#pragma unroll
+      for (int t = 0; t < KQ; ++t) {
+        const int r = lane + 64 * t;
+        if (r < na) qv[r] = xq[r] + zv[r];
+      }
+      wave_sync_lds();
+      // t = r_u - A_uq z = c_u - A_uu x_u - A_uq (x_q + z);  d_u = R_uu^-1 t
       if (lane < nu) {
-        for (int j = 0; j < na; ++j) ru = fma(-Auq[(int64_t)j * nu + lane], zv[j], ru);
-        uv[lane] = ru;  // t = r_u - A_uq z
+        for (int j = 0; j < na; ++j) ru = fma(-Auq[(int64_t)j * nu + lane], qv[j], ru);
+        uv[lane] = ru;
       }
       wave_sync_lds();
       double du = 0.0;
@@ -4176,18 +4072,15 @@ __global__ void __launch_bounds__(256)
       wave_sync_lds();
       if (lane < nu) uv[lane] = du;
       wave_sync_lds();
-      // d_q = z - A_qq^-1 (A_qu d_u)
+      // d_q = z - A_qq^-1 (A_qu d_u);  A_qu x_u advances by A_qu d_u
+      aqu_apply(Aqu, uv);
 #pragma unroll
       for (int t = 0; t < KQ; ++t) {
         const int r = lane + 64 * t;
-        if (r < na) {
-          double v = 0.0;
-          for (int c = 0; c < nu; ++c) v = fma(Aqu[(int64_t)c * na + r], uv[c], v);
-          qv[r] = v;
-        }
+        if (r < na) aq[r] += qv[r];
       }
       wave_sync_lds();
-      flux_apply(qv, Minv, s_c6[w][1]);
+      flux_apply(qv, Minv, c6 + 36);
       double dmax = lane < nu ? fabs(du) : 0.0, xmax = 0.0;
 #pragma unroll
       for (int t = 0; t < KQ; ++t) {
@@ -4220,22 +4113,47 @@ __global__ void __launch_bounds__(256)
   }
 }
 
-hipError_t launch_recover_fused_flux(const Tables& T, const MeshDev& M, const double* U,
-                                     const double* Y, int64_t usz, int64_t ysz,
-                                     const int32_t* cell_dofs, const double* lam_free,
-                                     const double* lam_dir, double* out, int64_t nfree,
-                                     hipStream_t st) {
-  const int na = T.na, nu = T.ni - na;
-  if (na > kMixedMaxNi || nu > 64 || T.nb > kFusedMaxNb || T.np > kFluxMaxNp)
+hipError_t launch_recover_flux(const Tables& T, const MeshDev& M, const double* U,
+                               const double* Y, int64_t usz, int64_t ysz, bool fused,
+                               const int32_t* cell_dofs, const double* lam_free,
+                               const double* lam_dir, double* out, int64_t nfree,
+                               hipStream_t st) {
+  const int na = T.na, nu = T.ni - na, nb = T.nb;
+  if (na > kMixedMaxNi || nu > 64 || nb > kFusedMaxNb || T.np > kFluxMaxNp)
     return hipErrorInvalidValue;
+  RecSlot rs;
+  rs.slot = U;
+  rs.usz = usz;
+  if (fused) {  // [R_uu^-1][A_uu][A_qu][A_uq]; W_u rows in the interior rows of Y
+    rs.ri = 0;
+    rs.auu = (int64_t)nu * nu;
+    rs.aqu = 2 * (int64_t)nu * nu;
+    rs.auq = rs.aqu + (int64_t)na * nu;
+    rs.uinv = -1;
+    rs.w = Y + na;
+    rs.wsz = ysz;
+    rs.wld = T.ni;
+  } else {
+    const FluxSlot fs = flux_slot(na, nu);
+    if (usz < fs.size || ysz < (int64_t)(nb + 1) * nu) return hipErrorInvalidValue;
+    rs.ri = fs.ri;
+    rs.auu = fs.auu;
+    rs.aqu = fs.aqu;
+    rs.auq = fs.auq;
+    rs.uinv = fs.uinv;
+    rs.w = Y;
+    rs.wsz = ysz;
+    rs.wld = nu;
+  }
   if (M.ncells > 0) {
-    const int64_t blocks = (M.ncells + 3) / 4;
+    const int64_t blocks = (M.ncells + kRecWaves - 1) / kRecWaves;
     const int grid = (int)(blocks < 65536 ? blocks : 65536);
-    const size_t lds = 2 * (size_t)T.np * T.np * sizeof(double);
-    ClockScope clk_(st, "recover_fused");
+    const size_t lds =
+        (2 * (size_t)T.np * T.np + kRecWaves * (size_t)rec_wave_lds(na, nu, nb)) * sizeof(double);
+    ClockScope clk_(st, "recover_flux");
 #define HDG_RF(P_, K_)                                                                      \
-  hipLaunchKernelGGL((recover_fused_kernel<P_, K_>), dim3(grid), dim3(256), lds, st,        \
-                     FusedGen<P_>{T, M}, U, Y, usz, ysz, cell_dofs, lam_free, lam_dir, out)
+  hipLaunchKernelGGL((recover_flux_kernel<P_, K_>), dim3(grid), dim3(64 * kRecWaves), lds, st, \
+                     FusedGen<P_>{T, M}, rs, cell_dofs, lam_free, lam_dir, out)
     if (T.phys == kDarcy) {
       if (na <= 64) HDG_RF(kDarcy, 1); else HDG_RF(kDarcy, 2);
     } else {

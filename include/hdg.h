@@ -261,7 +261,9 @@ int hdg_cell_blocks(hdg_mesh* mesh, const double** S, const double** g,
  * mixed-precision condensation *U is NULL and Y holds W = A_ii^-1 [A_ib b_i]
  * in natural interior order.  After a flux-block condensation (family 8) U
  * and Y are the factors of the reduced system of the m = ni - n_flux
- * non-flux interior unknowns, at the start of each cell's slot.  After a
+ * non-flux interior unknowns, at the start of each cell's slot (the rest of
+ * the slot holds the recovery's kept blocks and the inverses of the reduced
+ * factors; not part of this interface).  After a
  * fused flux-block condensation (family 9) *U is NULL and Y holds the rows
  * n_flux..ni-1 of W = A_ii^-1 [A_ib b_i] (natural order; its flux rows are
  * not written).  Cell c's slots start at c * u_stride and c * y_stride
