@@ -3095,6 +3095,129 @@ static int reduced_family(int ni, int nb) {
   return -1;
 }
 
+// Row-per-lane elimination of the reduced system [R_uu R_ub r_u; R_bu R_bb
+// r_b] (NU interior + NB boundary rows, column-major in the workspace, as
+// ReducedGen reads it) for the sizes of the flux family's 3D classes.  One
+// workgroup of two wavefronts per cell; lane r of wavefront w holds row
+// 64 w + r in registers, the unrolled steps giving every register index as a
+// constant.  Two passes keep the registers to one part of the row:
+//  1. the NU interior columns: all interior rows sit in wavefront 0, so the
+//     pivot search (largest magnitude, smallest LAPACK position on ties) is a
+//     DPP reduction and the LAPACK position bookkeeping lives in a register;
+//     every row's multiplier and the pivot row of each step go to LDS;
+//  2. the NB + 1 trailing columns, in chunks of CW: the same rank-1 updates
+//     replayed with the recorded multipliers, the pivot lane publishing its
+//     chunk in LDS each step.
+// One barrier per step separates the pivot row's publication from the
+// updates.  Every entry sees the same operations in the same order as in the
+// grid kernels.  Outputs as condense_kernel: S, g, U, Y = L^-1 P [R_ub r_u],
+// the multipliers L [step][row], the pivot order P and the singular-cell info.
+template <int NU, int NB>
+__global__ void __launch_bounds__(128)
+    reduced_rowlu_kernel(const double* __restrict__ R, CondenseOut out, int64_t ncells) {
+  constexpr int NR = NU + NB, NC = NR + 1, NT = NB + 1;
+  constexpr int CW = 28;  // trailing columns per pass
+  static_assert(NU <= 64 && NR <= 128, "interior rows in one wavefront, two wavefronts");
+  __shared__ double s_l[NU][NR];  // multiplier of each row at each step
+  __shared__ double s_prow[2][CW > NU ? CW : NU];
+  __shared__ double s_rinv[2];
+  __shared__ int s_piv[NU];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int row = 64 * w + lane;
+  const int64_t us = out.u_stride, ys = out.y_stride;
+  for (int64_t cell = blockIdx.x; cell < ncells; cell += gridDim.x) {
+    const double* Rc = R + cell * (int64_t)NC * NR;
+    int pos = row;  // LAPACK position of this interior row
+    int info = 0;
+    // ---- pass 1: the interior columns, with pivoting -------------------------
+    {
+      double a[NU];
+#pragma unroll
+      for (int c = 0; c < NU; ++c) a[c] = row < NR ? Rc[(int64_t)c * NR + row] : 0.0;
+#pragma unroll
+      for (int j = 0; j < NU; ++j) {
+        const int buf = j & 1;
+        double l = 0.0;
+        if (w == 0) {  // wave-uniform
+          const bool cand = lane < NU && pos >= j;
+          const int64_t key = cand ? dbits(fabs(a[j])) : (int64_t)-1;
+          const int64_t m = wave_max_i64(key);
+          const int ppos = wave_min_i32(key == m && cand ? pos : INT_MAX);
+          const int p = __builtin_ffsll(__ballot(cand && pos == ppos)) - 1;
+          const int q = __builtin_ffsll(__ballot(lane < NU && pos == j)) - 1;
+          const double pv = wave_readlane_f64(a[j], p);
+          const double rinv = pv != 0.0 ? 1.0 / pv : 0.0;
+          if (pv == 0.0 && info == 0) info = j + 1;
+          if (lane == q) pos = ppos;  // LAPACK interchange of positions j and ppos
+          if (lane == p) {
+            pos = j;
+#pragma unroll
+            for (int c = j + 1; c < NU; ++c) s_prow[buf][c] = a[c];
+          }
+          if (lane == 0) {
+            s_rinv[buf] = rinv;
+            s_piv[j] = p;
+          }
+          const bool use = lane < NU ? pos > j : row < NR;
+          l = use ? a[j] * rinv : 0.0;
+          if (lane < NU && use && out.L) out.L[cell * us + (int64_t)j * NU + lane] = l;
+        }
+        __syncthreads();
+        if (w != 0) l = row < NR ? a[j] * s_rinv[buf] : 0.0;
+        if (row < NR) s_l[j][row] = l;
+#pragma unroll
+        for (int c = j + 1; c < NU; ++c) a[c] = fma(-l, s_prow[buf][c], a[c]);
+      }
+      if (w == 0 && lane < NU && out.U) {  // U, packed by columns in LAPACK row order
+#pragma unroll
+        for (int c = 0; c < NU; ++c)
+          if (c >= pos) out.U[cell * us + c * (c + 1) / 2 + pos] = a[c];
+      }
+    }
+    __syncthreads();  // the last step's pivot row is read before pass 2 reuses it
+    // ---- pass 2: the trailing columns, replaying the updates ---------------
+#pragma unroll
+    for (int c0 = 0; c0 < NT; c0 += CW) {
+      double b[CW];
+#pragma unroll
+      for (int c = 0; c < CW; ++c)
+        b[c] = (row < NR && c0 + c < NT) ? Rc[(int64_t)(NU + c0 + c) * NR + row] : 0.0;
+#pragma unroll 1
+      for (int j = 0; j < NU; ++j) {
+        const int buf = j & 1;
+        if (row == s_piv[j]) {
+#pragma unroll
+          for (int c = 0; c < CW; ++c) s_prow[buf][c] = b[c];
+        }
+        __syncthreads();
+        const double l = row < NR ? s_l[j][row] : 0.0;
+#pragma unroll
+        for (int c = 0; c < CW; ++c) b[c] = fma(-l, s_prow[buf][c], b[c]);
+      }
+      if (w == 0 && lane < NU && out.Y) {
+#pragma unroll
+        for (int c = 0; c < CW; ++c)
+          if (c0 + c < NT) out.Y[cell * ys + (int64_t)(c0 + c) * NU + pos] = b[c];
+      }
+      if (row >= NU && row < NR) {
+        const int rb = row - NU;
+#pragma unroll
+        for (int c = 0; c < CW; ++c) {
+          const int col = c0 + c;
+          if (col < NB) {
+            if (out.S) out.S[(cell * NB + col) * NB + rb] = b[c];
+          } else if (col == NB && out.g) {
+            out.g[cell * NB + rb] = b[c];
+          }
+        }
+      }
+      __syncthreads();  // the pivot-row buffers are reused by the next chunk
+    }
+    if (w == 0 && lane < NU && out.P) out.P[cell * us + pos] = (double)lane;
+    if (threadIdx.x == 0 && out.info) out.info[cell] = info;
+  }
+}
+
 // Register tiling of the flux kernel for a size: column tiles per wavefront
 // and the k-step bound of the unrolled loops.
 static bool flux_fits(int fb_nc, int na, int np, int ni, int nb, int* ct, int* ksm) {
@@ -3726,6 +3849,26 @@ static hipError_t launch_flux_invert(double* slot, int64_t usz, int64_t ncells, 
   return hipGetLastError();
 }
 
+// Elimination of a chunk of reduced systems: the row-per-lane kernel for the
+// 3D flux classes, else the grid family of the size.
+static hipError_t launch_reduced(const double* R, int nr, const CondenseOut& o, int64_t count,
+                                 int nu, int nb, hipStream_t st, int rfam, int gridmul) {
+  const int grid = (int)std::min<int64_t>(count, 256 * (gridmul > 0 ? gridmul : 8));
+#define HDG_ROWLU(NU_, NB_)                                                                  \
+  if (nu == NU_ && nb == NB_) {                                                              \
+    ClockScope clk_(st, "reduced_rowlu");                                                    \
+    hipLaunchKernelGGL((reduced_rowlu_kernel<NU_, NB_>), dim3(grid), dim3(128), 0, st, R, o, \
+                       count);                                                               \
+    return hipGetLastError();                                                                \
+  }
+  HDG_ROWLU(27, 54)
+  HDG_ROWLU(18, 54)
+#undef HDG_ROWLU
+  ReducedGen rg{R, nr};
+  Workspace none;
+  return launch_condense<ReducedGen, false>(rg, o, count, nu, nb, st, none, rfam, gridmul);
+}
+
 // Cells per flux chunk: the reduced systems of one chunk live in the workspace.
 static int64_t flux_chunk(int nr) {
   const int64_t per = (int64_t)nr * (nr + 1) * (int64_t)sizeof(double);
@@ -3791,9 +3934,7 @@ static hipError_t launch_flux_phys(const Tables& T, const MeshDev& M, const Cond
     o.L = o.U ? o.U + (int64_t)nu * (nu + 1) / 2 : nullptr;
     o.P = o.L ? o.L + (int64_t)nu * nu : nullptr;
     o.info = out.info ? out.info + first : nullptr;
-    ReducedGen rg{ws.p, nr};
-    Workspace none;
-    e = launch_condense<ReducedGen, false>(rg, o, count, nu, nb, st, none, rfam, gridmul);
+    e = launch_reduced(ws.p, nr, o, count, nu, nb, st, rfam, gridmul);
     if (e != hipSuccess) return e;
   }
   return launch_flux_invert(out.U, usz, ncells, T.na, nu, st);
