@@ -659,6 +659,7 @@ int hdg_mesh_create(hdg_ctx* ctx, const hdg_problem* prob, const hdg_mesh_desc* 
     T.prog_rid = reinterpret_cast<const uint8_t*>(base + off_ri);
     T.prog_rval = reinterpret_cast<const double*>(base + off_rv);
     T.fb_nc = th.fb_nc;
+    T.fb_sym = th.fb_sym;
     std::memcpy(T.fb_cid, th.fb_cid, sizeof(T.fb_cid));
     if (th.fb_minv.empty()) T.fb_minv = nullptr;
   }

@@ -14,6 +14,7 @@
 // fastest).  space Q: tensor products of 1D Lagrange polynomials; space P:
 // total degree <= k with the lattice nodes of that order (nodal basis from a
 // Vandermonde solve in centred monomials).
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <utility>
@@ -24,6 +25,7 @@ namespace hdg {
 void build_program(TablesHost& t);
 int inline_program(TablesHost& t, std::string& err);
 void detect_flux_block(TablesHost& t);
+void detect_flux_sym(TablesHost& t);
 namespace {
 
 // Gauss-Legendre nodes/weights on [0,1]: Newton on P_n from the asymptotic
@@ -406,6 +408,7 @@ int build_tables(int phys, int D, int space, int kp, int kq, int km, int quad_de
     for (int a = 0; a < D - 1; ++a) t.trace_nodes[(size_t)i * (D - 1) + a] = bm.node(i, a);
   build_program(t);
   detect_flux_block(t);
+  detect_flux_sym(t);
   return inline_program(t, err);
 }
 
@@ -452,8 +455,6 @@ void build_program(TablesHost& t) {
   Prog P{t.prog_off, t.prog_id, t.prog_val};
   auto Mpp = [&](int i, int j) { return t.Mpp[(size_t)i * np + j]; };
   auto Gpq = [&](int e, int i, int j) { return t.Gpq[((size_t)e * np + i) * nq + j]; };
-  auto Gqp = [&](int e, int i, int j) { return t.Gqp[((size_t)e * nq + i) * np + j]; };
-  auto Fqp = [&](int F, int i, int j) { return t.Fqp[((size_t)F * nq + i) * np + j]; };
   auto Fqq = [&](int F, int i, int j) { return t.Fqq[((size_t)F * nq + i) * nq + j]; };
   auto Fpm = [&](int F, int i, int m) { return t.Fpm[((size_t)F * np + i) * nm + m]; };
   auto Fqm = [&](int F, int i, int m) { return t.Fqm[((size_t)F * nq + i) * nm + m]; };
@@ -482,10 +483,13 @@ void build_program(TablesHost& t) {
         } else if (row < ni) {
           const int i = row - na;
           if (col < na) {
+            // -(grad w, q) + <w, q.n> (test/DarcyHDGTests.jl:125-129) integrated
+            // by parts on the affine cell: (w, div q), i.e. minus the transpose
+            // of the (div v, u) block.  With the exact quadrature of the tables
+            // the two forms agree to rounding, and the flux coupling is then
+            // exactly transpose-symmetric in the program (detect_flux_sym).
             const int b = col / np, j = col % np;
-            for (int e = 0; e < D; ++e) P.add(kJ + e * D + b, Gqp(e, i, j));
-            for (int F = 0; F < nF; ++F)
-              if (qm(i, F) && pm(j, F)) P.add(kAn + 3 * F + b, Fqp(F, i, j));
+            for (int e = 0; e < D; ++e) P.add(kJ + e * D + b, -Gpq(e, j, i));
           } else if (col < ni) {
             const int j = col - na;
             for (int F = 0; F < nF; ++F)
@@ -732,4 +736,51 @@ void detect_flux_block(TablesHost& t) {
   for (int k = 0; k < 36; ++k) t.fb_cid[k] = (int8_t)(cid[k] >= 0 ? cid[k] : -1);
   t.fb_nc = nc;
 }
+
+// Transpose symmetry of the flux coupling: for every flux unknown k and
+// every other unknown r, entry (r, k) equals s entry (k, r), coefficient by
+// coefficient (the values of each coefficient index summed over the entry's
+// terms agree to a relative 1e-14), with one sign s_u for the remaining
+// interior rows and one s_b for the boundary rows.  Then [A_uq; A_bq] can be
+// read from the generated [A_qu A_qb] instead of being generated again.
+void detect_flux_sym(TablesHost& t) {
+  t.fb_sym = 0;
+  if (t.fb_nc <= 0) return;
+  const int na = t.na, n = t.n, ni = t.ni;
+  std::vector<double> va(256, 0.0), vb(256, 0.0);
+  std::vector<int> ids;
+  int su = 0, sb = 0;
+  for (int r = na; r < n; ++r)
+    for (int k = 0; k < na; ++k) {
+      const size_t e1 = (size_t)k * n + r, e2 = (size_t)r * n + k;  // (r, k) and (k, r)
+      ids.clear();
+      for (int q = t.prog_off[e1]; q < t.prog_off[e1 + 1]; ++q) {
+        va[t.prog_id[q]] += t.prog_val[q];
+        ids.push_back(t.prog_id[q]);
+      }
+      for (int q = t.prog_off[e2]; q < t.prog_off[e2 + 1]; ++q) {
+        vb[t.prog_id[q]] += t.prog_val[q];
+        ids.push_back(t.prog_id[q]);
+      }
+      int& s = r < ni ? su : sb;
+      bool ok = true;
+      for (int id : ids) {
+        const double a = va[id], b = vb[id];
+        if (a == 0.0 && b == 0.0) continue;
+        if (a == 0.0 || b == 0.0) {
+          ok = false;
+          break;
+        }
+        if (s == 0) s = (a > 0.0) == (b > 0.0) ? 1 : -1;
+        if (std::fabs(a - s * b) > 1e-14 * std::max(std::fabs(a), std::fabs(b))) {
+          ok = false;
+          break;
+        }
+      }
+      for (int id : ids) va[id] = vb[id] = 0.0;
+      if (!ok) return;
+    }
+  t.fb_sym = 1 | (su < 0 ? 2 : 0) | (sb < 0 ? 4 : 0);
+}
+
 }  // namespace hdg

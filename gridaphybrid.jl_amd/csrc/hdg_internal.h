@@ -65,6 +65,10 @@ struct TablesHost {
   int fb_nc = 0;
   int8_t fb_cid[36] = {};
   std::vector<double> fb_minv;
+  // Transpose symmetry of the flux coupling (0: none): bit 0 set when
+  // A_uq = s_u A_qu^T and A_bq = s_b A_qb^T entry by entry, bit 1: s_u = -1,
+  // bit 2: s_b = -1.
+  int fb_sym = 0;
 };
 
 // Device view of the tables (pointers into one device allocation).
@@ -102,6 +106,7 @@ struct Tables {
   int fb_nc;              // flux-block components (0: no Kronecker flux block)
   int8_t fb_cid[36];      // coefficient id of C[a][b] (< 0: zero)
   const double* fb_minv;  // [np][np] inverse of the scalar flux mass matrix
+  int fb_sym;             // transpose symmetry of the flux coupling (TablesHost::fb_sym)
 };
 
 // Device view of a mesh for the fused element generator.

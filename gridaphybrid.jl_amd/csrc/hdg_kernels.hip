@@ -2837,6 +2837,7 @@ struct FluxPlan {
   int ldg;            // leading dimension of G1 in LDS (odd)
   int ncols;          // columns of a row tile: 4 KS + 16 ceil(nc / 16)
   int cdiag;          // C is diagonal (structurally): C^-1 scales the rows of Z
+  int sym;            // C diagonal and [A_uq; A_bq] = diag(s_u, s_b) [A_qu A_qb]^T
   int minv_off;       // offset (doubles) of M^-1 in the dynamic LDS
   uint32_t zmask[8];  // bit s of zmask[t]: block (row tile t, k-step s) of I (x) M^-1
                       // is not identically zero
@@ -2881,7 +2882,7 @@ __device__ inline int flux_cinv(int ncomp, const int8_t* cid, const double* alph
   return 0;
 }
 
-template <int PHYS, int CT, int KSM>
+template <int PHYS, int CT, int KSM, bool SYM>
 __global__ void __launch_bounds__(64 * kFluxWaves, 3)  // two workgroups per CU
     flux_schur_kernel(FusedGen<PHYS> gen, int64_t first,
                       int64_t count, double* __restrict__ R, FluxPlan fp,
@@ -2997,6 +2998,88 @@ __global__ void __launch_bounds__(64 * kFluxWaves, 3)  // two workgroups per CU
 #pragma unroll
           for (int ct = 0; ct < CT; ++ct) zf[ct][4 * rt + v] = acc[ct][v] * sc;
         }
+    }
+    if (SYM) {
+      // ---- R = R0 - [A_uq; A_bq] Z with [A_uq; A_bq] = diag(s_u, s_b) G^T,
+      //      G = [A_qu A_qb] the first nr columns of G1 (left unmixed: C is
+      //      diagonal).  The A operand is read from G1 and R0 is generated
+      //      straight into the accumulators: each wavefront runs its column
+      //      tiles on its own, without LDS tiles or barriers ---------------------
+      const double su = (T.fb_sym & 2) ? -1.0 : 1.0, sb = (T.fb_sym & 4) ? -1.0 : 1.0;
+      for (int t = tid; t < na * nu; t += NT) {  // A_uq [flux column][u row]
+        const int j = t / nu, r = t - j * nu;
+        Bk[nu * nu + na * nu + t] = su * s_dyn[j * ldg + r];
+      }
+      const int RTR = (nr + 15) >> 4;
+      dbl4 pend[CT];
+      int pend_rr = -1;
+      auto store_pending_sym = [&]() {
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct) {
+          const int col = 16 * (w + NW * ct) + i16;
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int row = 16 * pend_rr + q4 + 4 * v;
+            if (row < nr && col < nc) Rc[(int64_t)col * nr + row] = pend[ct][v];
+          }
+        }
+      };
+#pragma unroll 1
+      for (int rr = 0; rr < RTR; ++rr) {
+        const int arow = 16 * rr + i16;  // this lane's row of the A operand (G^T)
+        double r0[CT][4];
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct) {
+          const int col = 16 * (w + NW * ct) + i16;
+          int rows[4], cols[4];
+          bool ok[4];
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int row = 16 * rr + q4 + 4 * v;
+            rows[v] = na + row;
+            cols[v] = na + col;
+            ok[v] = row < nr && col < nc;
+          }
+          gen.template entries<4>(s_gen, rows, cols, ok, r0[ct]);
+        }
+        // the previous tile's stores go out after this tile's entry loads: the
+        // vector-memory counter orders loads behind earlier stores
+        if (pend_rr >= 0) store_pending_sym();
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct) {
+          const int c0 = 16 * (w + NW * ct);
+          pend[ct] = dbl4{0.0, 0.0, 0.0, 0.0};
+          if (c0 >= nc) continue;  // wave-uniform
+          const int col = c0 + i16;
+          // row r of R = R0 - s_r (G^T Z): accumulate -s_r R0 + G^T Z, then
+          // scale by -s_r (s_r = +-1, so both steps are exact)
+          dbl4 acc;
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int row = 16 * rr + q4 + 4 * v;
+            acc[v] = row < nu ? -su * r0[ct][v] : -sb * r0[ct][v];
+            if (row < nu && col < nu) Bk[(int64_t)col * nu + row] = r0[ct][v];  // A_uu
+          }
+#pragma unroll
+          for (int s = 0; s < KSM; ++s) {
+            if (s < KS)
+              acc = __builtin_amdgcn_mfma_f64_16x16x4f64(s_dyn[(4 * s + q4) * ldg + arow],
+                                                         zf[ct][s], acc, 0, 0, 0);
+            // A operands load 8 k-steps ahead at most (all at once would spill)
+            if ((s & 7) == 7) __builtin_amdgcn_sched_barrier(0);
+          }
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int row = 16 * rr + q4 + 4 * v;
+            acc[v] = row < nu ? -su * acc[v] : -sb * acc[v];
+          }
+          pend[ct] = acc;
+        }
+        pend_rr = rr;
+      }
+      if (pend_rr >= 0) store_pending_sym();
+      __syncthreads();  // LDS reuse by the next cell
+      continue;
     }
     __syncthreads();  // G1 fully read: its region now holds the row tiles
 
@@ -3252,6 +3335,7 @@ static bool flux_plan(const Tables& T, FluxPlan& fp, int* ct, int* ksm) {
   for (int a = 0; a < T.fb_nc; ++a)
     for (int b = 0; b < T.fb_nc; ++b)
       if (a != b && T.fb_cid[6 * a + b] >= 0) fp.cdiag = 0;
+  fp.sym = fp.cdiag && (T.fb_sym & 1);
   for (int t = 0; t < 8; ++t) {
     fp.zmask[t] = 0;
     for (int s = 0; s < fp.KS && s < 32; ++s) {
@@ -3911,10 +3995,16 @@ static hipError_t launch_flux_phys(const Tables& T, const MeshDev& M, const Cond
     {
       ClockScope clk_(st, "flux_schur");
 #define HDG_FLUX(CT_, KS_)                                                                   \
-  if (ct == CT_ && ksm == KS_)                                                               \
-    hipLaunchKernelGGL((flux_schur_kernel<PHYS, CT_, KS_>), dim3(grid), dim3(64 * kFluxWaves), \
-                       lds, st, gen, first, count, ws.p, fp,                                  \
-                       out.U + first * usz + boff, usz);
+  if (ct == CT_ && ksm == KS_) {                                                             \
+    if (PHYS == kDarcy && fp.sym)                                                            \
+      hipLaunchKernelGGL((flux_schur_kernel<PHYS, CT_, KS_, PHYS == kDarcy>), dim3(grid),    \
+                         dim3(64 * kFluxWaves), lds, st, gen, first, count, ws.p, fp,         \
+                         out.U + first * usz + boff, usz);                                    \
+    else                                                                                     \
+      hipLaunchKernelGGL((flux_schur_kernel<PHYS, CT_, KS_, false>), dim3(grid),             \
+                         dim3(64 * kFluxWaves), lds, st, gen, first, count, ws.p, fp,         \
+                         out.U + first * usz + boff, usz);                                    \
+  }
       HDG_FLUX(1, 8)
       HDG_FLUX(1, 16)
       HDG_FLUX(1, 24)
