@@ -2913,24 +2913,30 @@ __global__ void __launch_bounds__(64 * kFluxWaves, 3)  // two workgroups per CU
     __syncthreads();
     if (tid == 0) s_bad = flux_cinv(ncomp, T.fb_cid, s_gen.alpha, s_cw, s_cinv);
     double* Rc = R + lc * (int64_t)nc * nr;  // [nc][nr] column-major
-    // ---- G1 = [A_qu A_qb b_q] into LDS ------------------------------------
-    for (int t0 = tid; t0 < nk * ldg; t0 += 4 * NT) {
+    // ---- G1 = [A_qu A_qb b_q] into LDS (entries column by column), and zeros
+    //      in its padding rows [na, nk) and columns [nc, ldg) ------------------
+    for (int t0 = tid; t0 < na * nc; t0 += 4 * NT) {
       int rows[4], cols[4];
       bool ok[4];
       double v[4];
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
-        const int t = t0 + u * NT, c = t / nk, k = t - c * nk;
+        const int t = t0 + u * NT, c = t / na, k = t - c * na;
         rows[u] = k;
         cols[u] = na + c;
-        ok[u] = t < nk * ldg && k < na && c < nc;
+        ok[u] = t < na * nc;
       }
       gen.template entries<4>(s_gen, rows, cols, ok, v);
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
-        const int t = t0 + u * NT, c = t / nk, k = t - c * nk;
-        if (t < nk * ldg) s_dyn[k * ldg + c] = v[u];
+        const int t = t0 + u * NT, c = t / na, k = t - c * na;
+        if (t < na * nc) s_dyn[k * ldg + c] = v[u];
       }
+    }
+    for (int t = tid; t < (nk - na) * ldg; t += NT) s_dyn[na * ldg + t] = 0.0;
+    for (int t = tid; t < na * (ldg - nc); t += NT) {
+      const int k = t / (ldg - nc);
+      s_dyn[k * ldg + nc + (t - k * (ldg - nc))] = 0.0;
     }
     __syncthreads();
     // interior blocks kept for the recovery: [A_uu][A_qu][A_uq], column-major
@@ -3042,9 +3048,6 @@ __global__ void __launch_bounds__(64 * kFluxWaves, 3)  // two workgroups per CU
           }
           gen.template entries<4>(s_gen, rows, cols, ok, r0[ct]);
         }
-        // the previous tile's stores go out after this tile's entry loads: the
-        // vector-memory counter orders loads behind earlier stores
-        if (pend_rr >= 0) store_pending_sym();
 #pragma unroll
         for (int ct = 0; ct < CT; ++ct) {
           const int c0 = 16 * (w + NW * ct);
@@ -3065,8 +3068,6 @@ __global__ void __launch_bounds__(64 * kFluxWaves, 3)  // two workgroups per CU
             if (s < KS)
               acc = __builtin_amdgcn_mfma_f64_16x16x4f64(s_dyn[(4 * s + q4) * ldg + arow],
                                                          zf[ct][s], acc, 0, 0, 0);
-            // A operands load 8 k-steps ahead at most (all at once would spill)
-            if ((s & 7) == 7) __builtin_amdgcn_sched_barrier(0);
           }
 #pragma unroll
           for (int v = 0; v < 4; ++v) {
@@ -3076,8 +3077,8 @@ __global__ void __launch_bounds__(64 * kFluxWaves, 3)  // two workgroups per CU
           pend[ct] = acc;
         }
         pend_rr = rr;
+        store_pending_sym();
       }
-      if (pend_rr >= 0) store_pending_sym();
       __syncthreads();  // LDS reuse by the next cell
       continue;
     }
