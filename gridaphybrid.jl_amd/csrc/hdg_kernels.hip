@@ -3017,19 +3017,6 @@ __global__ void __launch_bounds__(64 * kFluxWaves, 3)  // two workgroups per CU
         Bk[nu * nu + na * nu + t] = su * s_dyn[j * ldg + r];
       }
       const int RTR = (nr + 15) >> 4;
-      dbl4 pend[CT];
-      int pend_rr = -1;
-      auto store_pending_sym = [&]() {
-#pragma unroll
-        for (int ct = 0; ct < CT; ++ct) {
-          const int col = 16 * (w + NW * ct) + i16;
-#pragma unroll
-          for (int v = 0; v < 4; ++v) {
-            const int row = 16 * pend_rr + q4 + 4 * v;
-            if (row < nr && col < nc) Rc[(int64_t)col * nr + row] = pend[ct][v];
-          }
-        }
-      };
 #pragma unroll 1
       for (int rr = 0; rr < RTR; ++rr) {
         const int arow = 16 * rr + i16;  // this lane's row of the A operand (G^T)
@@ -3051,7 +3038,6 @@ __global__ void __launch_bounds__(64 * kFluxWaves, 3)  // two workgroups per CU
 #pragma unroll
         for (int ct = 0; ct < CT; ++ct) {
           const int c0 = 16 * (w + NW * ct);
-          pend[ct] = dbl4{0.0, 0.0, 0.0, 0.0};
           if (c0 >= nc) continue;  // wave-uniform
           const int col = c0 + i16;
           // row r of R = R0 - s_r (G^T Z): accumulate -s_r R0 + G^T Z, then
@@ -3072,12 +3058,10 @@ __global__ void __launch_bounds__(64 * kFluxWaves, 3)  // two workgroups per CU
 #pragma unroll
           for (int v = 0; v < 4; ++v) {
             const int row = 16 * rr + q4 + 4 * v;
-            acc[v] = row < nu ? -su * acc[v] : -sb * acc[v];
+            if (row < nr && col < nc)
+              Rc[(int64_t)col * nr + row] = row < nu ? -su * acc[v] : -sb * acc[v];
           }
-          pend[ct] = acc;
         }
-        pend_rr = rr;
-        store_pending_sym();
       }
       __syncthreads();  // LDS reuse by the next cell
       continue;
@@ -3308,7 +3292,9 @@ static bool flux_fits(int fb_nc, int na, int np, int ni, int nb, int* ct, int* k
   if (fb_nc <= 0 || np > kFluxMaxNp || na <= 0 || na >= ni) return false;
   const int nr = ni - na + nb, KS = (na + 3) / 4;
   const int c = ((nr + 1 + 15) / 16 + kFluxWaves - 1) / kFluxWaves;
-  const int k = KS <= 8 ? 8 : KS <= 16 ? 16 : KS <= 24 ? 24 : 0;
+  // k-step bounds of the instantiations (21: the 3D k=2 flux block exactly,
+  // which keeps Z's registers within three wavefronts per SIMD)
+  const int k = KS <= 8 ? 8 : KS <= 16 ? 16 : KS <= 21 ? 21 : KS <= 24 ? 24 : 0;
   if (k == 0 || c > 2 || c * k > 32) return false;
   if (reduced_family(ni - na, nb) < 0) return false;
   const int nu = ni - na;
@@ -3706,6 +3692,7 @@ static bool fused_fits(int fb_nc, int na, int np, int ni, int nb, int* ct, int* 
                        FusedPlan* fp) {
   int c = 0, k = 0;
   if (!flux_fits(fb_nc, na, np, ni, nb, &c, &k)) return false;
+  if (k == 21) k = 24;  // the fused kernel's instantiations
   const int nu = ni - na, nc = nu + nb + 1, KS = (na + 3) / 4, KU = (nu + 3) / 4;
   const int ncp = 16 * ((nc + 15) / 16), ldg = ncp | 1, ldr = nc + nu;
   if (nu > 64 || ldr > 128) return false;  // Gauss-Jordan: a lane per row, two columns per lane
@@ -4008,6 +3995,7 @@ static hipError_t launch_flux_phys(const Tables& T, const MeshDev& M, const Cond
   }
       HDG_FLUX(1, 8)
       HDG_FLUX(1, 16)
+      HDG_FLUX(1, 21)
       HDG_FLUX(1, 24)
       HDG_FLUX(2, 8)
       HDG_FLUX(2, 16)
