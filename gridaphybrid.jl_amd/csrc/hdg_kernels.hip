@@ -1861,23 +1861,45 @@ __global__ void symbolic_kernel(PairTable P, int32_t* __restrict__ rowval) {
 
 __global__ void assemble_kernel(PairTable P, const double* __restrict__ S,
                                 double* __restrict__ nzval) {
+  constexpr int kMaxPairs = 32;
+  __shared__ int64_t s_off[2 * kMaxPairs];  // S offset of each contributing block, -1 if none
   const int ntf = P.ntf, nb = P.nb;
+  const float inv_ntf = 1.0f / (float)ntf;
   for (int64_t G = blockIdx.x; G < P.nff; G += gridDim.x) {
     const int p0 = P.pptr[G], np = P.pptr[G + 1] - p0;
     const int per_col = np * ntf;
     const int64_t base = P.fptr[G];
+    // S offset of block (G, F) in each contributing cell: row block of F,
+    // column block of G (the column within it and the row are added below)
+    auto block_off = [&](int k2) -> int64_t {
+      const int32_t c = P.pcell[2 * p0 + k2];
+      if (c < 0) return -1;
+      const uint8_t lp = P.plpos[2 * p0 + k2];
+      const int lg = lp & 15, lf = lp >> 4;
+      return ((int64_t)c * nb + lg * ntf) * nb + lf * ntf;
+    };
+    const bool staged = np <= kMaxPairs;
+    __syncthreads();  // the previous facet's offsets are consumed
+    if (staged)
+      for (int k2 = threadIdx.x; k2 < 2 * np; k2 += blockDim.x) s_off[k2] = block_off(k2);
+    __syncthreads();
+    const float inv_pc = 1.0f / (float)per_col;
     for (int t = threadIdx.x; t < per_col * ntf; t += blockDim.x) {
-      const int mc = t / per_col, rem = t - mc * per_col;
-      const int pr = rem / ntf, m = rem - pr * ntf;
+      // t = (mc * np + pr) * ntf + m; quotients from float reciprocals, then
+      // corrected to the exact integer quotient
+      int mc = (int)((float)t * inv_pc);
+      mc += (mc + 1) * per_col <= t;
+      mc -= mc * per_col > t;
+      const int rem = t - mc * per_col;
+      int pr = (int)((float)rem * inv_ntf);
+      pr += (pr + 1) * ntf <= rem;
+      pr -= pr * ntf > rem;
+      const int m = rem - pr * ntf;
       double v = 0.0;
 #pragma unroll
       for (int k = 0; k < 2; ++k) {
-        const int32_t c = P.pcell[2 * (p0 + pr) + k];
-        if (c >= 0) {
-          const uint8_t lp = P.plpos[2 * (p0 + pr) + k];
-          const int lg = lp & 15, lf = lp >> 4;
-          v += S[((int64_t)c * nb + (lg * ntf + mc)) * nb + lf * ntf + m];
-        }
+        const int64_t off = staged ? s_off[2 * pr + k] : block_off(2 * pr + k);
+        if (off >= 0) v += S[off + (int64_t)mc * nb + m];
       }
       nzval[base + t] = v;
     }
