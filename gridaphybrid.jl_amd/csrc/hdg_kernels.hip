@@ -3947,7 +3947,7 @@ static hipError_t launch_flux_invert(double* slot, int64_t usz, int64_t ncells, 
 // 3D flux classes, else the grid family of the size.
 static hipError_t launch_reduced(const double* R, int nr, const CondenseOut& o, int64_t count,
                                  int nu, int nb, hipStream_t st, int rfam, int gridmul) {
-  const int grid = (int)std::min<int64_t>(count, 256 * (gridmul > 0 ? gridmul : 8));
+  const int grid = (int)std::min<int64_t>(count, 256 * (gridmul > 0 ? gridmul : 16));
 #define HDG_ROWLU(NU_, NB_)                                                                  \
   if (nu == NU_ && nb == NB_) {                                                              \
     ClockScope clk_(st, "reduced_rowlu");                                                    \
@@ -4194,6 +4194,7 @@ __global__ void __launch_bounds__(256)
       const int r = lane + 64 * t;
       if (r < na) {
         double acc = 0.0;
+#pragma unroll 4
         for (int c = 0; c < nu; ++c) acc = fma(Aqu[(int64_t)c * na + r], v[c], acc);
         qv[r] = acc;
       }
@@ -4238,6 +4239,7 @@ __global__ void __launch_bounds__(256)
         } else {
           const int u = r - na;
           double x = Wc[(int64_t)nb * rs.wld + u];
+#pragma unroll 4
           for (int j = 0; j < nb; ++j) x = fma(-Wc[(int64_t)j * rs.wld + u], lam[j], x);
           uv[u] = v;
           xuv[u] = x;
@@ -4250,6 +4252,7 @@ __global__ void __launch_bounds__(256)
       const double* Ui = base + rs.uinv;
       double x = 0.0;
       if (lane < nu)
+#pragma unroll 4
         for (int c = 0; c < nu; ++c) x = fma(Ui[(int64_t)c * nu + lane], xuv[c], x);
       wave_sync_lds();
       if (lane < nu) xuv[lane] = x;
@@ -4292,6 +4295,7 @@ __global__ void __launch_bounds__(256)
       double ru = 0.0;
       if (lane < nu) {
         ru = cu;
+#pragma unroll 4
         for (int c = 0; c < nu; ++c) ru = fma(-Auu[(int64_t)c * nu + lane], xuv[c], ru);
       }
       wave_sync_lds();
@@ -4304,12 +4308,14 @@ __global__ void __launch_bounds__(256)
       wave_sync_lds();
       // t = r_u - A_uq z = c_u - A_uu x_u - A_uq (x_q + z);  d_u = R_uu^-1 t
       if (lane < nu) {
+#pragma unroll 4
         for (int j = 0; j < na; ++j) ru = fma(-Auq[(int64_t)j * nu + lane], qv[j], ru);
         uv[lane] = ru;
       }
       wave_sync_lds();
       double du = 0.0;
       if (lane < nu)
+#pragma unroll 4
         for (int j = 0; j < nu; ++j) du = fma(Ri[(int64_t)j * nu + lane], uv[j], du);
       wave_sync_lds();
       if (lane < nu) uv[lane] = du;
