@@ -347,15 +347,18 @@ struct FusedGen {
     if (PHYS == kDarcy) {
       if (row < na) {  // v . f
         const int a = row / np, i = row - a * np;
+#pragma unroll 4
         for (int q = 0; q < T.nqc; ++q) s += T.Bp[q * np + i] * src[q][a];
       } else {  // w g
         const int i = row - na;
+#pragma unroll 4
         for (int q = 0; q < T.nqc; ++q) s += T.Bq[q * nq + i] * src[q][D];
       }
       return s;
     }
     if (row < na) return 0.0;
     const int c = (row - na) / nq, j = row - na - c * nq;  // -omega . f
+#pragma unroll 4
     for (int q = 0; q < T.nqc; ++q) s += T.Bq[q * nq + j] * src[q][c];
     return -s;
   }
