@@ -3827,7 +3827,7 @@ static FluxSlot flux_slot(int na, int nu) {
 // NUM = 64 each inverse takes a pass.  The recovery then needs only products
 // with kept matrices.  A zero pivot of U gives a zero row, as in the
 // substitution (the condensation reports such cells as singular).
-constexpr int kInvWaves = 2;
+constexpr int kInvWaves = 1;
 
 template <int NUM>
 __global__ void __launch_bounds__(64 * kInvWaves)

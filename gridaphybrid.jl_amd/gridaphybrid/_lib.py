@@ -131,6 +131,14 @@ def lib():
         if not os.path.exists(LIB_PATH):
             raise HDGError(HDG_ERR_NO_DEVICE,
                            f"{LIB_PATH} not built (run __graft_entry__.build())")
+        # torch ships its own HIP runtime under another file name: loaded
+        # after libhdg it would be a second runtime in the process, and the
+        # first device query of libhdg's fails.  Loaded first, libhdg binds to
+        # it.  (A host without torch, such as a Julia ccall, is not affected.)
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(LIB_PATH)
         vp, i64, i32, dbl = C.c_void_p, C.c_int64, C.c_int, C.c_double
         L.hdg_last_error.restype = C.c_char_p
