@@ -1222,11 +1222,14 @@ __global__ void __launch_bounds__(256)
         if (BACKSUB) M[(int64_t)col * ld + s_piv[jj]] = v;  // U row kept for back substitution
       }
       for (int c = jt + tid; c < ncol; c += NT) {
-        double u[KB];
+        double u[KB], mv[KB];
+#pragma unroll
+        for (int jj = 0; jj < KB; ++jj)  // the pivot rows' entries, all loads in flight at once
+          mv[jj] = jj < kb ? M[(int64_t)c * ld + s_piv[jj]] : 0.0;
 #pragma unroll
         for (int jj = 0; jj < KB; ++jj) {
           if (jj < kb) {
-            double v = M[(int64_t)c * ld + s_piv[jj]];
+            double v = mv[jj];
             for (int ii = 0; ii < jj; ++ii) v = fma(-s_P[ii][s_piv[jj]], u[ii], v);
             u[jj] = v;
             s_U[jj][c - jt] = v;
@@ -1257,6 +1260,15 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
         for (int i = 0; i < 4; ++i) rowok[i] = tr4 + i < n && s_act[tr4 + i];
         if (!(rowok[0] | rowok[1] | rowok[2] | rowok[3])) continue;
+        // the tile's entries load first, in flight during the rank-kb products
+        double mt[4][4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int c = jt + tc4 + q;
+#pragma unroll
+          for (int i = 0; i < 4; ++i)
+            mt[i][q] = (c < ncol && rowok[i]) ? M[(int64_t)c * ld + tr4 + i] : 0.0;
+        }
         for (int jj = 0; jj < kb; ++jj) {
           double l[4], u[4];
 #pragma unroll
@@ -1274,7 +1286,7 @@ __global__ void __launch_bounds__(256)
           if (c >= ncol) continue;
 #pragma unroll
           for (int i = 0; i < 4; ++i)
-            if (rowok[i]) M[(int64_t)c * ld + tr4 + i] -= acc[i][q];
+            if (rowok[i]) M[(int64_t)c * ld + tr4 + i] = mt[i][q] - acc[i][q];
         }
       }
       __syncthreads();
