@@ -3831,7 +3831,7 @@ static FluxSlot flux_slot(int na, int nu) {
 // R_uu^-1 = U^-1 L^-1 P and U^-1 from the reduced factors, one wavefront per
 // cell with a lane per column of the result held in registers (NUM >= nu).
 // U (column-major) and the multipliers (permuted to positions) are staged in
-// LDS zero-padded to NUM x NUM, with 1 / diag(U) (0 for a zero pivot), so
+// LDS as nu rows zero-padded to NUM, with 1 / diag(U) (0 for a zero pivot), so
 // the unrolled substitutions test bounds once per column (each column's
 // operands load together) and read them as broadcasts.  With NUM = 32 one pass does both inverses: lanes
 // [0, nu) solve for the columns of R_uu^-1, lanes [32, 32 + nu) for those of
@@ -3848,9 +3848,9 @@ __global__ void __launch_bounds__(64 * kInvWaves)
   extern __shared__ double s_inv[];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int nup = nu * (nu + 1) / 2, nn = nu * nu;
-  double* sU = s_inv + (int64_t)w * (2 * NUM * NUM + 2 * NUM);  // U [col][row], padded
-  double* sL = sU + NUM * NUM;  // multipliers [step][position], padded
-  double* sP = sL + NUM * NUM;  // row at each position (-1 past nu)
+  double* sU = s_inv + (int64_t)w * (2 * nu * NUM + 2 * NUM);  // U [col][row], rows padded
+  double* sL = sU + nu * NUM;  // multipliers [step][position], positions padded
+  double* sP = sL + nu * NUM;  // row at each position (-1 past nu)
   double* sD = sP + NUM;        // 1 / U[c][c]
   for (int64_t cell = (int64_t)blockIdx.x * kInvWaves + w; cell < ncells;
        cell += (int64_t)gridDim.x * kInvWaves) {
@@ -3860,14 +3860,14 @@ __global__ void __launch_bounds__(64 * kInvWaves)
     for (int t = lane; t < nn; t += 64) sU[t] = Lc[t];
     for (int t = lane; t < NUM; t += 64) sP[t] = t < nu ? Lc[nn + t] : -1.0;
     wave_sync_lds();
-    for (int t = lane; t < NUM * NUM; t += 64) {
+    for (int t = lane; t < nu * NUM; t += 64) {
       const int i = t / NUM, k = t - i * NUM;
-      sL[t] = (i < nu && k < nu) ? sU[i * nu + (int)sP[k]] : 0.0;
+      sL[t] = k < nu ? sU[i * nu + (int)sP[k]] : 0.0;
     }
     wave_sync_lds();
-    for (int t = lane; t < NUM * NUM; t += 64) {
+    for (int t = lane; t < nu * NUM; t += 64) {
       const int c = t / NUM, k = t - c * NUM;
-      sU[t] = (c < nu && k <= c) ? Uc[c * (c + 1) / 2 + k] : 0.0;
+      sU[t] = k <= c ? Uc[c * (c + 1) / 2 + k] : 0.0;
     }
     for (int t = lane; t < NUM; t += 64) {
       const double d = t < nu ? Uc[t * (t + 1) / 2 + t] : 0.0;
@@ -3943,7 +3943,7 @@ static hipError_t launch_flux_invert(double* slot, int64_t usz, int64_t ncells, 
   if (ncells <= 0) return hipSuccess;
   const FluxSlot fs = flux_slot(na, nu);
   const int num = nu <= 32 ? 32 : 64;
-  const size_t lds = kInvWaves * (size_t)(2 * num * num + 2 * num) * sizeof(double);
+  const size_t lds = kInvWaves * (size_t)(2 * nu * num + 2 * num) * sizeof(double);
   const int64_t blocks = (ncells + kInvWaves - 1) / kInvWaves;
   const int grid = (int)(blocks < 65536 ? blocks : 65536);
   ClockScope clk_(st, "flux_invert");
