@@ -403,12 +403,24 @@ struct GenCall {
                                  int64_t cell) {
     return gen.entry(sh, row, col, cell);
   }
+  // four entries (ok[u] false gives 0)
+  __device__ static void entries4(const G& gen, const typename G::Shared& sh, const int* row,
+                                  const int* col, const bool* ok, int64_t cell, double* v) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) v[u] = ok[u] ? gen.entry(sh, row[u], col[u], cell) : 0.0;
+  }
 };
 template <int PHYS>
 struct GenCall<FusedGen<PHYS>> {
   __device__ static double entry(const FusedGen<PHYS>& gen, const FusedShared& sh, int row,
                                  int col, int64_t) {
     return gen.entry(sh, row, col);
+  }
+  // the batched form: every load of the four entries in flight at once
+  __device__ static void entries4(const FusedGen<PHYS>& gen, const FusedShared& sh,
+                                  const int* row, const int* col, const bool* ok, int64_t,
+                                  double* v) {
+    gen.template entries<4>(sh, row, col, ok, v);
   }
 };
 
@@ -1146,10 +1158,23 @@ __global__ void __launch_bounds__(256)
     }
     if (tid == 0) s_info = 0;
     __syncthreads();
-    // generate into the workspace (coalesced, column-major)
-    for (int64_t t = tid; t < (int64_t)n * ncol; t += NT) {
-      const int col = (int)(t / n), row = (int)(t - (int64_t)col * n);
-      M[(int64_t)col * ld + row] = GenCall<Gen>::entry(gen, s_gen, row, col, cell);
+    // generate into the workspace (coalesced, column-major), four entries per
+    // thread at a time
+    for (int64_t t0 = tid; t0 < (int64_t)n * ncol; t0 += 4 * NT) {
+      int rows[4], cols[4];
+      bool ok[4];
+      double v[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int64_t t = t0 + u * NT;
+        ok[u] = t < (int64_t)n * ncol;
+        cols[u] = ok[u] ? (int)(t / n) : 0;
+        rows[u] = ok[u] ? (int)(t - (int64_t)cols[u] * n) : 0;
+      }
+      GenCall<Gen>::entries4(gen, s_gen, rows, cols, ok, cell, v);
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (ok[u]) M[(int64_t)cols[u] * ld + rows[u]] = v[u];
     }
     __syncthreads();
 
