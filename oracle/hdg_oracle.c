@@ -1343,3 +1343,75 @@ int orc_l2_errors(const orc_problem* p, int64_t ncells, const double* x0, const 
   space_free(&smu);
   return 0;
 }
+
+/* ------------------------------------------------------------------------ */
+/* CPU-baseline variants                                                      */
+/* ------------------------------------------------------------------------ */
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+int orc_set_threads(int n) {
+#ifdef _OPENMP
+  int prev = omp_get_max_threads();
+  if (n > 0) omp_set_num_threads(n);
+  return prev;
+#else
+  (void)n;
+  return 1;
+#endif
+}
+
+/* Condensation with the factors kept for the recovery (the alternative to
+ * the reference's re-factorisation at src/BackwardStaticCondensationMap.jl:95):
+ * per cell getrf, W = A11^-1 A12, w = A11^-1 b1 (getrs, :183 and :189), S and
+ * g as in orc_condense, then x_i = w - W x_b (one gemv) for the given trace
+ * values xb.  Outputs as orc_condense and orc_recover. */
+int orc_condense_recover_stored(int ni, int nb, int64_t ncells, const double* A, const double* b,
+                                const double* xb, double* S, double* g, double* xi,
+                                int32_t* info) {
+  int n = ni + nb;
+  int bad = 0;
+#pragma omp parallel reduction(+ : bad)
+  {
+    double* A11 = (double*)malloc(sizeof(double) * ni * ni);
+    double* W = (double*)malloc(sizeof(double) * ni * nb);
+    double* A21 = (double*)malloc(sizeof(double) * nb * ni);
+    int* ipiv = (int*)malloc(sizeof(int) * ni);
+#pragma omp for schedule(static)
+    for (int64_t c = 0; c < ncells; c++) {
+      const double* Ac = A + (size_t)c * n * n;
+      const double* bc = b + (size_t)c * n;
+      double* Sc = S + (size_t)c * nb * nb;
+      double* gc = g + (size_t)c * nb;
+      double* x = xi + (size_t)c * ni;
+      for (int j = 0; j < ni; j++)
+        for (int i = 0; i < ni; i++) A11[i + (size_t)j * ni] = Ac[i + (size_t)j * n];
+      for (int j = 0; j < nb; j++)
+        for (int i = 0; i < ni; i++) W[i + (size_t)j * ni] = Ac[i + (size_t)(ni + j) * n];
+      for (int j = 0; j < ni; j++)
+        for (int i = 0; i < nb; i++) A21[i + (size_t)j * nb] = Ac[ni + i + (size_t)j * n];
+      for (int j = 0; j < nb; j++)
+        for (int i = 0; i < nb; i++) Sc[i + (size_t)j * nb] = Ac[ni + i + (size_t)(ni + j) * n];
+      for (int i = 0; i < ni; i++) x[i] = bc[i];
+      for (int i = 0; i < nb; i++) gc[i] = bc[ni + i];
+      int inf = lu_getf2(ni, A11, ni, ipiv);
+      info[c] = inf;
+      if (inf) {
+        bad++;
+        continue;
+      }
+      lu_getrs(ni, nb, A11, ni, ipiv, W, ni);
+      mm_acc(nb, nb, ni, -1.0, A21, nb, W, ni, Sc, nb);
+      lu_getrs(ni, 1, A11, ni, ipiv, x, ni);
+      mm_acc(nb, 1, ni, -1.0, A21, nb, x, ni, gc, nb);
+      /* recovery from the kept W and w: x = w - W xb */
+      mm_acc(ni, 1, nb, -1.0, W, ni, xb + (size_t)c * nb, nb, x, ni);
+    }
+    free(A11);
+    free(W);
+    free(A21);
+    free(ipiv);
+  }
+  return bad;
+}

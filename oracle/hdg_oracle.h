@@ -132,6 +132,17 @@ int orc_l2_errors(const orc_problem* p, int64_t ncells, const double* x0, const 
  * right-hand sides, in place.  Returns info. */
 int orc_dense_solve(int n, int nrhs, double* A, double* B);
 
+/* OpenMP threads of the following calls (n <= 0 keeps the current count);
+ * returns the previous count. */
+int orc_set_threads(int n);
+
+/* CPU-baseline variant that keeps the factors instead of re-factorising in
+ * the recovery: per cell getrf, W = A11^-1 A12, w = A11^-1 b1, S, g as in
+ * orc_condense, then x_i = w - W xb. */
+int orc_condense_recover_stored(int ni, int nb, int64_t ncells, const double* A, const double* b,
+                                const double* xb, double* S, double* g, double* xi,
+                                int32_t* info);
+
 #ifdef __cplusplus
 }
 #endif

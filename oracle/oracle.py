@@ -353,3 +353,105 @@ def monolithic_solve(sol: Solution):
     K = sp.csc_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
                       shape=(N, N))
     return spla.spsolve(K, rhs)
+
+
+# ----------------------------------------------------------------------------
+# Self-variation: how far the oracle's own FP64 answer moves under changes that
+# are exact in real arithmetic.  Two independent FP64 implementations of the
+# same cell systems cannot agree more closely than this.
+# ----------------------------------------------------------------------------
+def darcy_ibp_systems(A, n_flux, ni):
+    """The same Darcy cell systems with the u-q block in integrated-by-parts
+    form.  test/DarcyHDGTests.jl:125-129 writes the potential equation's flux
+    terms as -(grad w, q) + <w, q.n>; on an affine cell that equals (w, div q),
+    the negated transpose of the q-u block -(div v, p) (both integrated exactly
+    by the degree 2(k+1) rules).  The two differ only by rounding."""
+    A2 = np.array(A, copy=True)
+    A2[:, n_flux:ni, :n_flux] = -np.transpose(A[:, :n_flux, n_flux:ni], (0, 2, 1))
+    return A2
+
+
+def interior_reorder(ni, n_flux, nb):
+    """Permutation of the cell unknowns that puts the potential block before
+    the flux block (boundary unknowns unchanged): getrf then pivots in another
+    order."""
+    return np.concatenate([np.arange(n_flux, ni), np.arange(n_flux), np.arange(ni, ni + nb)])
+
+
+def solve_reordered(A, b, xb, ni, nb, perm):
+    """condense + recover with the unknowns permuted by perm, results mapped
+    back to the natural order."""
+    Ap = A[:, perm][:, :, perm]
+    bp = b[:, perm]
+    S, g, info = condense(Ap, bp, ni, nb)
+    xp, info2 = recover(Ap, bp, xb, ni, nb)
+    xi = np.empty_like(xp)
+    xi[:, perm[:ni]] = xp
+    return S, g, xi, info | info2
+
+
+def refined_interior(A, b, xb, ni):
+    """x_i = A_ii^-1 (b_i - A_ib x_b) of the given FP64 systems, refined with
+    extended-precision residuals until it is the correctly rounded solution up
+    to a few ulps: the reference point that separates a solver's own rounding
+    from the sensitivity of the cell systems."""
+    L = np.longdouble
+    Aii = np.ascontiguousarray(A[:, :ni, :ni])
+    rhsL = b[:, :ni].astype(L) - np.einsum("cij,cj->ci", A[:, :ni, ni:].astype(L), xb.astype(L))
+    x = np.linalg.solve(Aii, rhsL.astype(np.float64)[..., None])[..., 0]
+    AL = Aii.astype(L)
+    for _ in range(3):
+        r = rhsL - np.einsum("cij,cj->ci", AL, x.astype(L))
+        d = np.linalg.solve(Aii, r.astype(np.float64)[..., None])[..., 0]
+        x = (x.astype(L) + d.astype(L)).astype(np.float64)
+    return x
+
+
+def rel_err(a, b):
+    """max |a - b| / max |b| (0 for empty arrays)."""
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    if b.size == 0:
+        return 0.0
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def darcy_self_variation(A, b, xb, ni, nb, n_flux):
+    """The oracle's answers on Darcy cell systems (S, g, q, u) and how far they
+    move (relative max norm, per field) under (1) the integrated-by-parts u-q
+    block and (2) the reordered interior unknowns.  Returns (base, variation)
+    with base = dict(S, g, xi) and variation = dict(S, g, q, u)."""
+    S0, g0, info = condense(A, b, ni, nb)
+    x0, info2 = recover(A, b, xb, ni, nb)
+    assert not (info.any() or info2.any()), "singular A_ii"
+    base = dict(S=S0, g=g0, xi=x0)
+    A2 = darcy_ibp_systems(A, n_flux, ni)
+    S1, g1, _ = condense(A2, b, ni, nb)
+    x1, _ = recover(A2, b, xb, ni, nb)
+    S2, g2, x2, _ = solve_reordered(A, b, xb, ni, nb, interior_reorder(ni, n_flux, nb))
+    var = {}
+    for key, ref, alts in (("S", S0, (S1, S2)), ("g", g0, (g1, g2)),
+                           ("q", x0[:, :n_flux], (x1[:, :n_flux], x2[:, :n_flux])),
+                           ("u", x0[:, n_flux:], (x1[:, n_flux:], x2[:, n_flux:]))):
+        var[key] = max(rel_err(a, ref) for a in alts)
+    return base, var
+
+
+def set_threads(n):
+    """OpenMP threads of the oracle's following calls; returns the previous count."""
+    return lib().orc_set_threads(int(n))
+
+
+def condense_recover_stored(A, b, xb, ni, nb):
+    """Condensation keeping W = A_ii^-1 A_ib and w = A_ii^-1 b_i, then x_i = w - W x_b
+    (no re-factorisation in the recovery)."""
+    nc = A.shape[0]
+    S = np.zeros((nc, nb, nb))
+    g = np.zeros((nc, nb))
+    xi = np.zeros((nc, ni))
+    info = np.zeros(nc, np.int32)
+    lib().orc_condense_recover_stored(ni, nb, C.c_int64(nc), _p(to_colmajor(A)),
+                                      _p(np.ascontiguousarray(b)),
+                                      _p(np.ascontiguousarray(xb, np.float64)), _p(S), _p(g),
+                                      _p(xi), _p(info))
+    return S.transpose(0, 2, 1), g, xi, info

@@ -386,6 +386,17 @@ def main():
     cases["darcy_q1_3d"] = dict(A=A, b=b, D=3, k=(1, 1, 1), space=0, x0=[0.5, 0.0, 0.25],
                                 h=[0.25, 0.5, 0.125], kappa=1.25, tau=1.0,
                                 signs=[1, -1, 1, 1, -1, 1])
+    # the 3D Darcy workload's own space (Q2/Q2/Q2, 162 x 162) and the cubic
+    # 2D space, on anisotropic box cells with non-unit kappa and tau
+    A, b = darcy_cell_exact(3, 2, 2, 2, False, h_3, Fr(3, 4), Fr(2), (-1, 1, 1, -1, 1, -1),
+                            kat_sources(3, x0_3, h_3))
+    cases["darcy_q2_3d"] = dict(A=A, b=b, D=3, k=(2, 2, 2), space=0, x0=[0.5, 0.0, 0.25],
+                                h=[0.25, 0.5, 0.125], kappa=0.75, tau=2.0,
+                                signs=[-1, 1, 1, -1, 1, -1])
+    A, b = darcy_cell_exact(2, 3, 3, 3, False, h_2, Fr(5, 2), Fr(1, 2), (1, 1, -1, -1),
+                            kat_sources(2, x0_2, h_2))
+    cases["darcy_q3_2d"] = dict(A=A, b=b, D=2, k=(3, 3, 3), space=0, x0=[0.25, 0.5],
+                                h=[0.5, 0.25], kappa=2.5, tau=0.5, signs=[1, 1, -1, -1])
     for name, c in cases.items():
         np.savez(os.path.join(HERE, f"{name}.npz"), A=c["A"], b=c["b"], D=c["D"],
                  k=np.array(c["k"]), space=c["space"], x0=np.array(c["x0"]),
@@ -395,7 +406,9 @@ def main():
     # --- elasticity cells (zero load) -----------------------------------
     for D, k, h, nu, tau, name in ((2, 1, (Fr(1, 2), Fr(1, 4)), Fr(3, 10), Fr(4), "elas_p1_2d"),
                                    (3, 1, (Fr(1, 4), Fr(1, 2), Fr(1, 8)), Fr(1, 4), Fr(2),
-                                    "elas_p1_3d")):
+                                    "elas_p1_3d"),
+                                   (3, 2, (Fr(1, 4), Fr(1, 2), Fr(1, 8)), Fr(1, 3), Fr(3),
+                                    "elas_p2_3d")):
         A = elasticity_cell_exact(D, k, h, Fr(1), nu, tau)
         np.savez(os.path.join(HERE, f"{name}.npz"), A=A, D=D, k=k,
                  h=np.array([float(v) for v in h]), nu=float(nu), tau=float(tau))

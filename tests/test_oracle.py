@@ -34,7 +34,8 @@ def one_box_cell(D, x0, h, coef, signs):
     return mesh
 
 
-@pytest.mark.parametrize("name", ["darcy_q1_2d", "darcy_q2_2d", "darcy_p1p0p1_2d", "darcy_q1_3d"])
+@pytest.mark.parametrize("name", ["darcy_q1_2d", "darcy_q2_2d", "darcy_p1p0p1_2d", "darcy_q1_3d",
+                                  "darcy_q2_3d", "darcy_q3_2d"])
 def test_darcy_cell_system_exact(name):
     g = np.load(os.path.join(GOLD, f"{name}.npz"))
     D = int(g["D"])
@@ -48,7 +49,7 @@ def test_darcy_cell_system_exact(name):
     assert rel(b[0], g["b"]) < 1e-13
 
 
-@pytest.mark.parametrize("name", ["elas_p1_2d", "elas_p1_3d"])
+@pytest.mark.parametrize("name", ["elas_p1_2d", "elas_p1_3d", "elas_p2_3d"])
 def test_elasticity_cell_system_exact(name):
     g = np.load(os.path.join(GOLD, f"{name}.npz"))
     D, k = int(g["D"]), int(g["k"])
@@ -248,3 +249,43 @@ def test_elasticity_convergence():
         errs.append(O.l2_errors(p, sol.mesh, sol.coef, sol.xi))
     r = np.log2(np.array(errs[0]) / np.array(errs[1]))
     assert np.all(r > 0.8)
+
+
+@pytest.mark.parametrize("name", ["darcy_q1_2d", "darcy_q2_2d", "darcy_q1_3d", "darcy_q2_3d",
+                                  "darcy_q3_2d"])
+def test_darcy_uq_block_integrates_by_parts(name):
+    """On the exact cell systems the potential equation's flux block
+    -(grad w, q) + <w, q.n> (test/DarcyHDGTests.jl:125-129) equals the negated
+    transpose of the flux equation's potential block -(div v, p): the identity
+    behind the integrated-by-parts u-q block and the oracle's self-variation
+    variant (oracle.darcy_ibp_systems)."""
+    g = np.load(os.path.join(GOLD, f"{name}.npz"))
+    D = int(g["D"])
+    kf, kp, kt = (int(v) for v in g["k"])
+    prob = O.Problem(O.PHYS_DARCY, D, int(g["space"]), kf, kp, kt, 2 * (kf + 1), float(g["tau"]),
+                     1.0, 0.3, O.MMS_KAT)
+    s = prob.sizes()
+    na, ni = s["n_flux"], s["ni"]
+    A = g["A"]
+    assert rel(A[na:ni, :na], -A[:na, na:ni].T) < 1e-15
+    A2 = O.darcy_ibp_systems(A[None], na, ni)[0]
+    assert rel(A2, A) < 1e-15
+
+
+def test_condense_recover_stored_vs_lapack():
+    """The CPU-baseline variant that keeps W = A_ii^-1 A_ib and w = A_ii^-1 b_i
+    (x_i = w - W x_b) against dgetrf/dgetrs."""
+    g = np.load(os.path.join(GOLD, "condense_lapack.npz"))
+    S, gg, xi, info = O.condense_recover_stored(g["A"], g["b"], g["x"], 6, 3)
+    assert not info.any()
+    assert rel(S, g["S"]) < 1e-13 and rel(gg, g["g"]) < 1e-13 and rel(xi, g["xi"]) < 1e-13
+
+
+def test_self_variation_reorder_is_exact_permutation():
+    """solve_reordered maps its results back: on a well-conditioned system it
+    agrees with the unpermuted solve to rounding."""
+    g = np.load(os.path.join(GOLD, "condense_lapack.npz"))
+    perm = np.r_[3:6, 0:3, 6:9]
+    S, gg, xi, info = O.solve_reordered(g["A"], g["b"], g["x"], 6, 3, perm)
+    assert not info.any()
+    assert rel(S, g["S"]) < 1e-13 and rel(gg, g["g"]) < 1e-13 and rel(xi, g["xi"]) < 1e-13
