@@ -78,6 +78,11 @@ def parse():
     ap.add_argument("--batch", action="store_true",
                     help="uneven-batch run: 256 seeded meshes in HBM-sized batch graphs")
     ap.add_argument("--nmesh", type=int, default=256)
+    ap.add_argument("--only", choices=("fp64", "fp64_dense", "mixed"), default=None,
+                    help="run only this path of the 3D case (no oracle, no 2D line): the "
+                         "command the rocprofv3 summaries are taken from")
+    ap.add_argument("--physics", choices=("darcy", "elasticity"), default="darcy",
+                    help="physics of the --only case")
     ap.add_argument("--copy-gib", type=float, default=0.0,
                     help="only run one plain device copy of this many GiB (the reference "
                          "kernel that scales the FETCH_SIZE / WRITE_SIZE counters)")
@@ -87,16 +92,22 @@ def parse():
 class Case:
     """One mesh + operator, with device buffers for the recovery input/output."""
 
-    def __init__(self, gh, ctx, D, n, k):
+    def __init__(self, gh, ctx, D, n, k, physics="darcy"):
         import torch
         from gridaphybrid import _lib
         self.gh, self.ctx, self.D, self.n, self.k = gh, ctx, D, n, k
         nc = n ** D
         rng = np.random.default_rng(SEED)
-        self.kappa = np.exp(rng.standard_normal(nc))
         model = gh.CartesianDiscreteModel(tuple([0.0, 1.0] * D), tuple([n] * D))
-        X = gh.HDGSpaces.darcy_baseline(model, k)
-        wf = gh.DarcyHDG(tau=1.0, kappa=self.kappa, source=_lib.SOURCE_DARCY_SINPROD)
+        if physics == "darcy":
+            self.kappa = np.exp(rng.standard_normal(nc))
+            X = gh.HDGSpaces.darcy_baseline(model, k)
+            wf = gh.DarcyHDG(tau=1.0, kappa=self.kappa, source=_lib.SOURCE_DARCY_SINPROD)
+        else:   # 3D linear elasticity HDG, nu per cell in [0.2, 0.4], tau = n
+            self.kappa = rng.uniform(0.2, 0.4, nc)
+            X = gh.HDGSpaces.elasticity(model, k)
+            wf = gh.ElasticityHDG(tau=float(n), nu=self.kappa, E=1.0,
+                                  source=_lib.SOURCE_ELAS_SIN)
         t0 = time.perf_counter()
         self.op = gh.HybridAffineFEOperator(wf, X, X, [1, 2], [3], ctx=ctx, assemble=False)
         self.setup_s = time.perf_counter() - t0
@@ -173,8 +184,14 @@ class Case:
 class OracleSample:
     """The oracle (C, OpenMP) on a bounded sample of the workload: per-cell
     integration + condensation + recovery (re-factorising A_ii as the
-    reference does), on the host cores.  Timed once; then used to check the
-    GPU results of each precision on the same cells."""
+    reference does), on the host cores.  Timed on all host threads, on one
+    thread (a smaller sample) and, as a second variant, keeping the factors
+    instead of re-factorising.  Then used to check the GPU results of each
+    precision on the same cells, against a bound derived from the oracle's own
+    movement under rounding-level changes of the same cell systems."""
+
+    CHUNK = 4096       # cells per oracle call (bounds the host memory of A)
+    NVAR = 8192        # cells of the self-variation subsample
 
     def __init__(self, case, seconds):
         from oracle import oracle as O
@@ -186,7 +203,7 @@ class OracleSample:
         self.ni, self.nb, self.na = s["ni"], s["nb"], s["n_flux"]
         self.cd, nfree, ndir, fd = O.trace_dofs(self.mesh, s["n_tf"])
         self.lam_dir = O.dirichlet_values(self.prob, self.mesh, fd, ndir)
-        self.threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        self.threads = O.set_threads(0)
         cal = np.arange(min(256, self.mesh.ncells))
         t0 = time.perf_counter()
         self._run(cal)
@@ -197,14 +214,41 @@ class OracleSample:
         self.S, self.g, self.xi = self._run(self.cells)
         self.seconds = time.perf_counter() - t0
         self.ns = ns
+        # the same work keeping the factors (no re-factorisation in the recovery)
+        t0 = time.perf_counter()
+        self._run(self.cells, stored=True)
+        self.stored_seconds = time.perf_counter() - t0
+        # one thread, on a sample sized for about a quarter of the time budget
+        ns1 = int(min(ns, max(64, ns / max(self.threads, 1) / 4)))
+        O.set_threads(1)
+        try:
+            t0 = time.perf_counter()
+            self._run(self.cells[:ns1])
+            self.one_seconds = time.perf_counter() - t0
+        finally:
+            O.set_threads(self.threads)
+        self.ns1 = ns1
+        # self-variation of the oracle on the first NVAR cells of the sample
+        nv = min(ns, self.NVAR)
+        A, b = O.cell_systems(self.prob, self.mesh, case.kappa, cells=self.cells[:nv])
+        xb = O.gather_lambda(self.cd[:nv], case.lam_host, self.lam_dir)
+        _, self.var = O.darcy_self_variation(A, b, xb, self.ni, self.nb, self.na)
+        self.nvar = nv
 
-    def _run(self, cells):
+    def _run(self, cells, stored=False):
         O = self.O
-        A, b = O.cell_systems(self.prob, self.mesh, self.case.kappa, cells=cells)
-        S, g, _ = O.condense(A, b, self.ni, self.nb)
-        xb = O.gather_lambda(self.cd[cells], self.case.lam_host, self.lam_dir)
-        xi, _ = O.recover(A, b, xb, self.ni, self.nb)
-        return S, g, xi
+        out = []
+        for c0 in range(0, len(cells), self.CHUNK):
+            cc = cells[c0:c0 + self.CHUNK]
+            A, b = O.cell_systems(self.prob, self.mesh, self.case.kappa, cells=cc)
+            xb = O.gather_lambda(self.cd[cc], self.case.lam_host, self.lam_dir)
+            if stored:
+                S, g, xi, _ = O.condense_recover_stored(A, b, xb, self.ni, self.nb)
+            else:
+                S, g, _ = O.condense(A, b, self.ni, self.nb)
+                xi, _ = O.recover(A, b, xb, self.ni, self.nb)
+            out.append((S, g, xi))
+        return tuple(np.concatenate([o[i] for o in out]) for i in range(3))
 
     def baseline(self):
         D, n, k = self.case.D, self.case.n, self.case.k
@@ -212,12 +256,18 @@ class OracleSample:
                     sample=f"first {self.ns} cells of the {n}^{D} mesh (k={k}): oracle cell "
                            f"integration + StaticCondensationMap + BackwardStaticCondensationMap "
                            f"(re-factorising A_ii as the reference does), OpenMP over cells, "
-                           f"FP64; {self.seconds:.1f} s")
+                           f"FP64; {self.seconds:.1f} s",
+                    one_core=dict(value=self.ns1 / self.one_seconds, cores=1,
+                                  sample=f"first {self.ns1} cells, {self.one_seconds:.1f} s"),
+                    stored_factors=dict(value=self.ns / self.stored_seconds, cores=self.threads,
+                                        sample=f"first {self.ns} cells, factors kept for the "
+                                               f"recovery, {self.stored_seconds:.1f} s"))
 
-    def errors(self):
+    def errors(self, precision):
         """GPU results of the last step against the oracle: on the oracle's own
-        integration, and on the GPU's cell systems (which isolates the
-        condensation arithmetic from cond(A_ii), up to ~1e7 at h = 1/64)."""
+        integration (whole sample, and the self-variation subsample with its
+        bound), and on the GPU's cell systems (which isolates the condensation
+        arithmetic from cond(A_ii), up to ~5e6 at h = 1/64)."""
         import torch
         from gridaphybrid import _lib
         O, case = self.O, self.case
@@ -230,16 +280,22 @@ class OracleSample:
         gg = np.empty((ns, nb))
         L.hdg_memcpy_d2h(case.ctx.h, Sg.ctypes.data, Sd, Sg.nbytes)
         L.hdg_memcpy_d2h(case.ctx.h, gg.ctypes.data, gd, gg.nbytes)
+        Sg = Sg.transpose(0, 2, 1)
         fv = case.out.cpu().numpy()
         nc = self.mesh.ncells
         qg = fv[: ns * na].reshape(ns, na)
         ug = fv[nc * na: nc * na + ns * (ni - na)].reshape(ns, ni - na)
+        relerr = O.rel_err
 
-        def relerr(a, b):
-            return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+        def fields(sl, S, g, xi):
+            return dict(S=relerr(Sg[sl], S), g=relerr(gg[sl], g), q=relerr(qg[sl], xi[:, :na]),
+                        u=relerr(ug[sl], xi[:, na:]))
 
-        err = dict(S=relerr(Sg.transpose(0, 2, 1), self.S), g=relerr(gg, self.g),
-                   q=relerr(qg, self.xi[:, :na]), u=relerr(ug, self.xi[:, na:]), cells=ns)
+        err = fields(slice(0, ns), self.S, self.g, self.xi)
+        err["cells"] = ns
+        nv = self.nvar
+        sub = fields(slice(0, nv), self.S[:nv], self.g[:nv], self.xi[:nv])
+        bound = {k: max(2.0 * v, 1e-12) for k, v in self.var.items()}
         nsame = min(ns, 2048)
         nt = ni + nb
         Ad = torch.empty((nsame, nt, nt), dtype=torch.float64, device="cuda")
@@ -250,12 +306,16 @@ class OracleSample:
         Ag = Ad.transpose(1, 2).cpu().numpy()
         bg = bd.cpu().numpy()
         S2, g2, _ = O.condense(Ag, bg, ni, nb)
-        xi2, _ = O.recover(Ag, bg, O.gather_lambda(self.cd[:nsame], case.lam_host, self.lam_dir),
-                           ni, nb)
-        same = dict(S=relerr(Sg[:nsame].transpose(0, 2, 1), S2), g=relerr(gg[:nsame], g2),
-                    q=relerr(qg[:nsame], xi2[:, :na]), u=relerr(ug[:nsame], xi2[:, na:]),
-                    cells=nsame)
-        return dict(same_cell_systems=same, oracle_integration=err)
+        xb2 = O.gather_lambda(self.cd[:nsame], case.lam_host, self.lam_dir)
+        xi2, _ = O.recover(Ag, bg, xb2, ni, nb)
+        same = fields(slice(0, nsame), S2, g2, xi2)
+        same["cells"] = nsame
+        # the plain FP64 bar is 1e-12 on identical systems, the mixed one 1e-10
+        bar = 1e-12 if precision == "fp64" else 1e-10
+        ok = all(sub[k] <= bound[k] for k in bound) and same["S"] <= bar and same["g"] <= bar
+        return dict(same_cell_systems=same, oracle_integration=err,
+                    self_variation=dict(cells=nv, oracle=self.var, bound=bound, gpu=sub),
+                    within_bound=bool(ok))
 
 
 def refine_flops(ni, nb, sweeps):
@@ -405,6 +465,22 @@ def batch_main(gh, args):
     print(json.dumps(line))
 
 
+def only_main(gh, args):
+    """One path of one 3D case, timed like the main run (for profiling)."""
+    ctx = gh.Context(0)
+    case = Case(gh, ctx, 3, args.n3d, args.order, physics=args.physics)
+    fam = DENSE_FAMILY if args.only == "fp64_dense" else None
+    prec = gh.MIXED if args.only == "mixed" else gh.FP64
+    r = case.run(prec, args.steps, args.warmup, fam, args.only)
+    info = case.info
+    print(json.dumps({"only": args.only, "physics": args.physics, "n3d": args.n3d,
+                      "order": args.order, "cells": info.ncells, "ni": info.ni, "nb": info.nb,
+                      "cells_per_s": r["cells_per_s"], "ms_per_step": r["ms_per_step"],
+                      "family": r["family"], "kernels_ms_per_launch": r["kernels_ms_per_launch"],
+                      "kernels_ms_per_step": r["kernels_ms_per_step"]}))
+    case.op.close()
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -426,6 +502,8 @@ def main():
         return serve_main(gh, args)
     if args.batch:
         return batch_main(gh, args)
+    if args.only:
+        return only_main(gh, args)
     ctx = gh.Context(0)
     case = Case(gh, ctx, 3, args.n3d, args.order)
     sample = None if args.no_cpu else OracleSample(case, args.cpu_seconds)
@@ -438,17 +516,20 @@ def main():
         r = case.run(prec, args.steps, args.warmup, fam, name)
         res[name] = r
         if sample is not None:
-            errs[name] = sample.errors()
+            errs[name] = sample.errors("mixed" if prec == gh.MIXED else "fp64")
     info = case.info
     ni, nb, nc = info.ni, info.nb, info.ncells
-    # value: the fastest path whose sampled error against the FP64 oracle
-    # stays within 1e-10 for every field (all paths when no sample was taken)
+    # value: the fastest path whose results against the FP64 oracle stay
+    # within the bound derived from the oracle's own self-variation on the
+    # oracle's integration, and within 1e-12 (FP64) / 1e-10 (mixed) on S and g
+    # of identical cell systems (all paths when no sample was taken)
     def ok(name):
         e = errs.get(name)
-        if not e:
-            return True
-        return all(v <= 1e-10 for k, v in e["same_cell_systems"].items() if k != "cells")
-    best = max((n for n in res if ok(n)), key=lambda n: res[n]["cells_per_s"])
+        return True if not e else e["within_bound"]
+    passing = [n for n in res if ok(n)]
+    best = max(passing or res, key=lambda n: res[n]["cells_per_s"])
+    if not passing:
+        print("bench.py: no path is within the parity bound of the oracle", file=sys.stderr)
     top = res[best]
     mixed, fp64, dense = res["mixed"], res["fp64"], res["fp64_dense"]
     # dominant kernel of the reported path and its roofline (VALU FP64 and

@@ -29,6 +29,7 @@ struct hdg_ctx {
   int32_t* d_perm = nullptr;   // field permutation of the map-level API
   int perm_cap = 0;
   Workspace ws;                // large-system kernel workspace of this context
+  hipError_t scratch_status = hipErrorNotInitialized;  // scratch_init on this device
 };
 
 namespace {
@@ -91,6 +92,10 @@ struct hdg_mesh {
   std::vector<int32_t> facet_cell;  // [nfacets][2]
   std::vector<int8_t> facet_lpos;   // [nfacets][2]
   std::vector<int8_t> facet_dir;
+  // facet orientation: code per (cell, local facet) and, per code, the
+  // cell-local trace node -> the facet's own node (empty when unoriented)
+  std::vector<int8_t> fperm;
+  std::vector<int> nperm;
   // device
   DevBuf tables, d_x0, d_jac, d_sign, d_coef, d_cell_dofs, d_lam_dir;
   DevBuf d_S, d_g, d_U, d_Y, d_info, d_alpha, d_srcq, d_iters, d_Xinv;
@@ -111,6 +116,7 @@ struct hdg_mesh {
   int family = -1, gridmul = 0;
   int used_family = -1;
   DevBuf d_fptr, d_pptr, d_pfacet, d_pcell, d_plpos, d_gcell, d_glpos;
+  DevBuf d_pperm, d_gperm, d_iperm;  // allocated for oriented meshes only
   DevBuf d_colptr, d_rowval, d_nzval, d_rhs;
   const double* src_qp = nullptr;
   bool condensed = false;
@@ -139,6 +145,9 @@ struct hdg_mesh {
     P.plpos = d_plpos.as<uint8_t>();
     P.gcell = d_gcell.as<int32_t>();
     P.glpos = d_glpos.as<uint8_t>();
+    P.pperm = d_pperm.as<uint8_t>();
+    P.gperm = d_gperm.as<uint8_t>();
+    P.iperm = d_iperm.as<uint8_t>();
     P.nff = nff;
     P.ntf = th.ntf;
     P.nb = th.nb;
@@ -210,6 +219,9 @@ int hdg_ctx_create(int device, hdg_ctx** out) {
   if (hipSetDevice(device) != hipSuccess) return HDG_ERR_HIP;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return HDG_ERR_HIP;
   if (hipMalloc(&c->d_count, sizeof(int64_t)) != hipSuccess) return HDG_ERR_ALLOC;
+  // the allocation-free entry point's scratch on this device (its failure only
+  // disables that entry point, which then reports it)
+  c->scratch_status = scratch_init(device);
   *out = c.release();
   return HDG_OK;
 }
@@ -372,6 +384,79 @@ int hdg_static_condensation(hdg_ctx* ctx, const hdg_fields* fields, int64_t ncel
   return HDG_OK;
 }
 
+int hdg_static_condensation_noalloc(hdg_ctx* ctx, const hdg_fields* fields, int64_t ncells,
+                                    const double* A, const double* b, double* S, double* g,
+                                    int32_t* info) {
+  if (!ctx) return HDG_ERR_ARG;
+  if (ncells < 0) return fail(ctx, HDG_ERR_ARG, "ncells < 0");
+  // _check_preconditions (src/StaticCondensationMap.jl:16-34) on fixed arrays
+  if (!fields || fields->nfields <= 0 || !fields->field_sizes)
+    return fail(ctx, HDG_ERR_ARG, "fields: empty");
+  if (fields->n_interior < 0 || fields->n_boundary < 0 ||
+      fields->n_interior + fields->n_boundary != fields->nfields ||
+      fields->nfields > kNoAllocMaxN)
+    return fail(ctx, HDG_ERR_ARG, "fields: interior and boundary fields must partition all fields");
+  int off[kNoAllocMaxN + 1];
+  bool touched[kNoAllocMaxN];
+  off[0] = 0;
+  for (int i = 0; i < fields->nfields; ++i) {
+    if (fields->field_sizes[i] < 0) return fail(ctx, HDG_ERR_ARG, "fields: negative block size");
+    if (off[i] + fields->field_sizes[i] > kNoAllocMaxN)
+      return fail(ctx, HDG_ERR_UNSUPPORTED,
+                  "cell system larger than the allocation-free maximum of " +
+                      std::to_string(kNoAllocMaxN) + " rows");
+    off[i + 1] = off[i] + fields->field_sizes[i];
+    touched[i] = false;
+  }
+  int32_t perm[kNoAllocMaxN];
+  int np = 0, ni = 0, nb = 0;
+  for (int pass = 0; pass < 2; ++pass) {
+    const int cnt = pass == 0 ? fields->n_interior : fields->n_boundary;
+    const int32_t* lst = pass == 0 ? fields->interior_fields : fields->boundary_fields;
+    for (int k = 0; k < cnt; ++k) {
+      const int fld = lst[k];
+      if (fld < 0 || fld >= fields->nfields) return fail(ctx, HDG_ERR_ARG, "fields: index out of range");
+      if (touched[fld]) return fail(ctx, HDG_ERR_ARG, "fields: repeated field");
+      touched[fld] = true;
+      for (int i = off[fld]; i < off[fld + 1]; ++i) perm[np++] = i;
+      (pass == 0 ? ni : nb) += fields->field_sizes[fld];
+    }
+  }
+  if (ncells == 0) return HDG_OK;
+  if (!A || !b || !S || !g || !info) return fail(ctx, HDG_ERR_ARG, "null buffer");
+  if (ni == 0) return fail(ctx, HDG_ERR_ARG, "no interior dofs");
+  const int fam = noalloc_family(ni, nb);
+  if (fam < 0)
+    return fail(ctx, HDG_ERR_UNSUPPORTED, "local system (" + std::to_string(ni) + ", " +
+                                              std::to_string(nb) +
+                                              ") needs a workspace; use hdg_static_condensation");
+  if (ctx->scratch_status != hipSuccess)
+    return fail(ctx, HDG_ERR_STATE, std::string("device scratch unavailable: ") +
+                                        hipGetErrorString(ctx->scratch_status));
+  HIP_TRY(ctx, hipSetDevice(ctx->device), "hipSetDevice");
+  char* scr = static_cast<char*>(scratch_address(ctx->device));
+  if (!scr) return fail(ctx, HDG_ERR_STATE, "device scratch not resolved");
+  int32_t* d_perm = reinterpret_cast<int32_t*>(scr + kScratchPermOffset);
+  int64_t* d_count = reinterpret_cast<int64_t*>(scr + kScratchCountOffset);
+  HIP_TRY(ctx, hipMemcpyAsync(d_perm, perm, (size_t)np * sizeof(int32_t), hipMemcpyHostToDevice,
+                              ctx->stream),
+          "upload perm");
+  DenseIn in{A, b, d_perm, ni + nb, ni, nb};
+  CondenseOut out;
+  out.S = S;
+  out.g = g;
+  out.info = info;
+  HIP_TRY(ctx, launch_condense_dense_family(in, ncells, out, ctx->stream, fam), "condense kernel");
+  // the counter is cleared by a fill kernel inside launch_count_nonzero
+  HIP_TRY(ctx, launch_count_nonzero(info, ncells, d_count, ctx->stream), "count kernel");
+  int64_t bad = 0;
+  HIP_TRY(ctx, hipMemcpyAsync(&bad, d_count, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream),
+          "count copy");
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream), "sync");  // perm[] and bad outlive the copies
+  if (bad) return fail(ctx, HDG_SINGULAR, std::to_string(bad) + " cell(s) with singular A_ii");
+  return HDG_OK;
+}
+
 int hdg_backward_static_condensation(hdg_ctx* ctx, const hdg_fields* fields, int64_t ncells,
                                      const double* A, const double* b, const double* x,
                                      double* xi, int32_t* info) {
@@ -427,7 +512,7 @@ int hdg_mesh_create(hdg_ctx* ctx, const hdg_problem* prob, const hdg_mesh_desc* 
   m->nfacets = nf;
   std::string err;
   if (build_tables(prob->physics, D, prob->space, prob->k_flux, prob->k_pot, prob->k_trace,
-                   prob->quad_degree, m->th, err))
+                   prob->quad_degree, prob->projection, m->th, err))
     return fail(ctx, HDG_ERR_ARG, err);
   TablesHost& th = m->th;
   if (th.nqc > 128) return fail(ctx, HDG_ERR_UNSUPPORTED, "too many cell quadrature points");
@@ -457,6 +542,44 @@ int hdg_mesh_create(hdg_ctx* ctx, const hdg_problem* prob, const hdg_mesh_desc* 
     }
   m->facet_dir.assign(desc->facet_dirichlet, desc->facet_dirichlet + nf);
 
+  // ---- facet orientation (pindex): per code, the trace node permutation ----
+  const int ncode = D == 2 ? 2 : 8, nm = th.nm, ncm = th.ncm;
+  bool oriented = false;
+  if (desc->facet_perm) {
+    for (int64_t t = 0; t < nc * nF; ++t) {
+      const int8_t p = desc->facet_perm[t];
+      if (p < 0 || p >= ncode) return fail(ctx, HDG_ERR_ARG, "facet_perm code out of range");
+      oriented |= p != 0;
+    }
+  }
+  if (oriented) {
+    m->fperm.assign(desc->facet_perm, desc->facet_perm + nc * nF);
+    m->nperm.assign((size_t)ncode * nm, -1);
+    for (int p = 0; p < ncode; ++p)
+      for (int i = 0; i < nm; ++i) {
+        const double* y = &th.trace_nodes[(size_t)i * (D - 1)];
+        double yo[2] = {0.0, 0.0};
+        if (D == 2) {
+          yo[0] = (p & 1) ? 1.0 - y[0] : y[0];
+        } else {
+          const double a = (p & 1) ? y[1] : y[0], b = (p & 1) ? y[0] : y[1];
+          yo[0] = (p & 2) ? 1.0 - a : a;
+          yo[1] = (p & 4) ? 1.0 - b : b;
+        }
+        for (int j = 0; j < nm; ++j) {
+          double d = 0.0;
+          for (int a = 0; a < D - 1; ++a) d += std::fabs(th.trace_nodes[(size_t)j * (D - 1) + a] - yo[a]);
+          if (d < 1e-12) m->nperm[(size_t)p * nm + i] = j;
+        }
+      }
+    for (int64_t t = 0; t < nc * nF; ++t)
+      for (int i = 0; i < nm; ++i)
+        if (m->nperm[(size_t)m->fperm[t] * nm + i] < 0)
+          return fail(ctx, HDG_ERR_UNSUPPORTED,
+                      "facet_perm code " + std::to_string(m->fperm[t]) +
+                          " does not map the trace nodes onto themselves");
+  }
+
   // ---- trace dof ids: free facets consecutive in facet order, Dirichlet
   //      facets numbered -1,-2,... (Gridap L2 facet space) ----
   std::vector<int64_t> first((size_t)nf);
@@ -480,8 +603,14 @@ int hdg_mesh_create(hdg_ctx* ctx, const hdg_problem* prob, const hdg_mesh_desc* 
   for (int64_t c = 0; c < nc; ++c)
     for (int lf = 0; lf < nF; ++lf) {
       const int64_t f = desc->cell_facets[c * nF + lf];
-      for (int k = 0; k < ntf; ++k)
-        cell_dofs[(c * nF + lf) * ntf + k] = (int32_t)(first[f] >= 0 ? first[f] + k : first[f] - k);
+      const int* np = oriented ? &m->nperm[(size_t)m->fperm[c * nF + lf] * nm] : nullptr;
+      for (int k = 0; k < ntf; ++k) {
+        // trace dof k = comp * nm + node of the cell's local facet is the
+        // facet's own dof comp * nm + perm(node)
+        const int own = np ? (k / nm) * nm + np[k % nm] : k;
+        cell_dofs[(c * nF + lf) * ntf + k] =
+            (int32_t)(first[f] >= 0 ? first[f] + own : first[f] - own);
+      }
     }
 
   // ---- facet-pair structure of the skeleton matrix ----
@@ -545,6 +674,27 @@ int hdg_mesh_create(hdg_ctx* ctx, const hdg_problem* prob, const hdg_mesh_desc* 
     fptr[G + 1] = fptr[G] + (int64_t)cnt * ntf * ntf;
   }
   m->nnz = fptr[nff];
+  // orientation codes of the contributing (cell, local facet) pairs and the
+  // inverse dof permutation per code
+  std::vector<uint8_t> pperm, gperm, iperm;
+  if (oriented) {
+    pperm.resize(pcell.size());
+    for (size_t t = 0; t < pcell.size(); ++t) {
+      const int32_t c = pcell[t];
+      const uint8_t lp = plpos[t];
+      pperm[t] = c < 0 ? 0
+                       : (uint8_t)(m->fperm[(int64_t)c * nF + (lp & 15)] |
+                                   (m->fperm[(int64_t)c * nF + (lp >> 4)] << 4));
+    }
+    gperm.resize(gcell.size());
+    for (size_t t = 0; t < gcell.size(); ++t)
+      gperm[t] = gcell[t] < 0 ? 0 : (uint8_t)m->fperm[(int64_t)gcell[t] * nF + glpos[t]];
+    iperm.resize((size_t)ncode * ntf);
+    for (int p = 0; p < ncode; ++p)
+      for (int i = 0; i < nm; ++i)
+        for (int comp = 0; comp < ncm; ++comp)
+          iperm[(size_t)p * ntf + comp * nm + m->nperm[(size_t)p * nm + i]] = (uint8_t)(comp * nm + i);
+  }
   std::vector<int64_t> colptr((size_t)m->nfree + 1, 0);
   for (int64_t G = 0; G < nff; ++G) {
     const int64_t np = pptr[G + 1] - pptr[G];
@@ -688,6 +838,11 @@ int hdg_mesh_create(hdg_ctx* ctx, const hdg_problem* prob, const hdg_mesh_desc* 
   HIP_TRY(ctx, upload(m->d_plpos, plpos, st), "upload plpos");
   HIP_TRY(ctx, upload(m->d_gcell, gcell, st), "upload gcell");
   HIP_TRY(ctx, upload(m->d_glpos, glpos, st), "upload glpos");
+  if (oriented) {
+    HIP_TRY(ctx, upload(m->d_pperm, pperm, st), "upload pperm");
+    HIP_TRY(ctx, upload(m->d_gperm, gperm, st), "upload gperm");
+    HIP_TRY(ctx, upload(m->d_iperm, iperm, st), "upload iperm");
+  }
   HIP_TRY(ctx, upload(m->d_colptr, colptr, st), "upload colptr");
   std::vector<double> zeros((size_t)std::max<int64_t>(m->ndir, 1), 0.0);
   HIP_TRY(ctx, upload(m->d_lam_dir, zeros, st), "upload lam_dir");
@@ -710,7 +865,7 @@ int hdg_footprint_cartesian(const hdg_problem* prob, const int64_t* cells, int p
   TablesHost th;
   std::string err;
   if (build_tables(prob->physics, D, prob->space, prob->k_flux, prob->k_pot, prob->k_trace,
-                   prob->quad_degree, th, err))
+                   prob->quad_degree, prob->projection, th, err))
     return HDG_ERR_ARG;
   int64_t nc = 1, nfac = 0, nbnd = 0;
   for (int a = 0; a < D; ++a) {
@@ -801,6 +956,7 @@ int hdg_mesh_create_cartesian(hdg_ctx* ctx, const hdg_problem* prob, const int64
   desc.cell_facets = cf.data();
   desc.facet_dirichlet = bnd.data();
   desc.coef = coef;
+  desc.facet_perm = nullptr;  // every cell's local facet frames agree
   return hdg_mesh_create(ctx, prob, &desc, out);
 }
 
@@ -878,7 +1034,15 @@ std::vector<double> trace_values(const hdg_mesh* m, int8_t want, int64_t count) 
     const int64_t c = m->facet_cell[2 * f];
     const int lf = m->facet_lpos[2 * f];
     const int fix = D - 1 - lf / 2;
-    for (int i = 0; i < nm; ++i) {
+    // own node o is node i of the owner's local facet with perm(i) = o
+    std::vector<int> inv(nm);
+    for (int i = 0; i < nm; ++i) inv[i] = i;
+    if (!m->fperm.empty()) {
+      const int* np = &m->nperm[(size_t)m->fperm[c * 2 * D + lf] * nm];
+      for (int i = 0; i < nm; ++i) inv[np[i]] = i;
+    }
+    for (int o = 0; o < nm; ++o) {
+      const int i = inv[o];
       double xi[3] = {0, 0, 0}, x[3] = {0, 0, 0};
       int j = 0;
       for (int a = 0; a < D; ++a)
@@ -889,7 +1053,7 @@ std::vector<double> trace_values(const hdg_mesh* m, int8_t want, int64_t count) 
       }
       double u[3];
       mms_trace(m->prob.source, D, x, m->prob.young, 0.0, u);
-      for (int comp = 0; comp < ncm; ++comp) vals[k + comp * nm + i] = u[comp];
+      for (int comp = 0; comp < ncm; ++comp) vals[k + comp * nm + o] = u[comp];
     }
     k += (int64_t)ncm * nm;
   }
@@ -910,6 +1074,16 @@ int hdg_mesh_interpolate_trace(hdg_mesh* m, double* lam_free) {
   if (m->nfree > 0 && !lam_free) return fail(m->ctx, HDG_ERR_ARG, "null output");
   std::vector<double> vals = trace_values(m, 0, m->nfree);
   if (m->nfree > 0) std::memcpy(lam_free, vals.data(), (size_t)m->nfree * sizeof(double));
+  return HDG_OK;
+}
+
+int hdg_mesh_cell_quadrature(const hdg_mesh* m, double* points, double* weights) {
+  if (!m || !points || !weights) return HDG_ERR_ARG;
+  const int D = m->D;
+  for (int q = 0; q < m->th.nqc; ++q) {
+    for (int a = 0; a < D; ++a) points[q * D + a] = m->th.xq[3 * q + a];
+    weights[q] = m->th.wq[q];
+  }
   return HDG_OK;
 }
 
@@ -1185,7 +1359,8 @@ namespace {
 bool same_class(const hdg_problem& a, const hdg_problem& b) {
   return a.physics == b.physics && a.dim == b.dim && a.space == b.space &&
          a.k_flux == b.k_flux && a.k_pot == b.k_pot && a.k_trace == b.k_trace &&
-         a.quad_degree == b.quad_degree;
+         a.quad_degree == b.quad_degree &&
+         (a.physics == HDG_DARCY || (a.projection != 0) == (b.projection != 0));
 }
 
 bool sources_ready(hdg_batch* b) {

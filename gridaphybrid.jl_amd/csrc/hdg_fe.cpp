@@ -284,7 +284,7 @@ void facet_point(int D, int f, const double* s, double* xi) {
 
 }  // namespace
 
-int build_tables(int phys, int D, int space, int kp, int kq, int km, int quad_degree,
+int build_tables(int phys, int D, int space, int kp, int kq, int km, int quad_degree, int proj,
                  TablesHost& t, std::string& err) {
   if (D != 2 && D != 3) {
     err = "dimension must be 2 or 3";
@@ -334,6 +334,7 @@ int build_tables(int phys, int D, int space, int kp, int kq, int km, int quad_de
   t.Bp.assign((size_t)nqc * np, 0.0);
   t.Bq.assign((size_t)nqc * nq, 0.0);
   t.xq.assign((size_t)nqc * 3, 0.0);
+  t.wq.assign((size_t)nqc, 0.0);
   std::vector<double> vp(np), gp(3 * np), vq(nq), gq(3 * nq), vm(nm);
   for (int q = 0; q < nqc; ++q) {
     double xi[3] = {0, 0, 0}, w = 1.0;
@@ -344,6 +345,7 @@ int build_tables(int phys, int D, int space, int kp, int kq, int km, int quad_de
       r /= m1;
     }
     for (int a = 0; a < 3; ++a) t.xq[3 * q + a] = xi[a];
+    t.wq[q] = w;
     bp.eval(xi, vp.data(), gp.data());
     bq.eval(xi, vq.data(), gq.data());
     for (int i = 0; i < np; ++i) {
@@ -395,6 +397,32 @@ int build_tables(int phys, int D, int space, int kp, int kq, int km, int quad_de
         for (int i = 0; i < nm; ++i)
           for (int j = 0; j < nm; ++j) t.Fmm[(size_t)i * nm + j] += vm[i] * vm[j] * w;
     }
+  if (proj && phys != kDarcy) {
+    // bulk-to-skeleton L2 projection P_m (src/GridapAPIExtensions.jl:453-500:
+    // dofs = A \ B with A = (mu, mu')_F, B = (mu, u)_F): the displacement's
+    // facet trace is replaced by its projection onto the trace space, so
+    // tau (w, P_m u)_F = tau (P_m w, P_m u)_F = tau B^T A^-1 B.  The other tau
+    // terms are unchanged: (mu, P_m u)_F = (mu, u)_F for mu in the trace space.
+    std::vector<double> Acopy = t.Fmm, I((size_t)nm * nm, 0.0);
+    for (int i = 0; i < nm; ++i) I[(size_t)i * nm + i] = 1.0;
+    if (!solve_dense(nm, Acopy, I, nm)) {
+      err = "singular trace mass matrix";
+      return -1;
+    }
+    // I now holds Fmm^-1 (row-major)
+    for (int f = 0; f < nF; ++f)
+      for (int i = 0; i < nq; ++i)
+        for (int j = 0; j < nq; ++j) {
+          double v = 0.0;
+          for (int m = 0; m < nm; ++m) {
+            double w = 0.0;
+            for (int m2 = 0; m2 < nm; ++m2)
+              w += I[(size_t)m * nm + m2] * t.Fqm[((size_t)f * nq + j) * nm + m2];
+            v += t.Fqm[((size_t)f * nq + i) * nm + m] * w;
+          }
+          t.Fqq[((size_t)f * nq + i) * nq + j] = v;
+        }
+  }
   // support masks: bit f set when the trace of the function on facet f is not
   // identically zero (its facet L2 norm is positive)
   for (int f = 0; f < nF; ++f) {

@@ -37,6 +37,7 @@ struct TablesHost {
   std::vector<double> Bp;         // [nqc][np] p_i(xi_q) * w_q
   std::vector<double> Bq;         // [nqc][nq]
   std::vector<double> xq;         // [nqc][3]  reference quadrature points
+  std::vector<double> wq;         // [nqc]     their weights
   std::vector<double> trace_nodes;  // [nm][D-1] trace nodes on the reference facet
   // entry program (column-major entries of the n x n matrix)
   int nalpha = 0;
@@ -164,6 +165,10 @@ struct PairTable {
   const uint8_t* plpos;    // [npairs][2] (lpos_G | lpos_F << 4)
   const int32_t* gcell;    // [nfree_facets][2] cells around G (-1 if absent)
   const uint8_t* glpos;    // [nfree_facets][2] local position of G in them
+  // facet orientation (nullptr when every (cell, local facet) has code 0):
+  const uint8_t* pperm;    // [npairs][2] codes (of G | of F << 4) in the contributing cell
+  const uint8_t* gperm;    // [nfree_facets][2] code of G in the cells around it
+  const uint8_t* iperm;    // [codes][ntf] own trace dof -> cell-local trace dof
   int64_t nff;             // free facets
   int ntf, nb;
 };
@@ -271,6 +276,20 @@ hipError_t launch_assemble(const PairTable& P, const double* S, const double* g,
                            double* rhs, hipStream_t st);
 hipError_t launch_cell_systems(const Tables& T, const MeshDev& M, int64_t first, int64_t count,
                                double* A, double* b, hipStream_t st);
+// Allocation-free condensation (hdg_static_condensation_noalloc): a fixed
+// device scratch array per device holds the field permutation and the
+// singular-cell counter.  scratch_init (the device current) resolves its
+// address once per device; scratch_address returns it (nullptr before).
+constexpr int kNoAllocMaxN = 175;            // rows of the largest cell system
+constexpr size_t kScratchPermOffset = 0;     // int32 [kNoAllocMaxN]
+constexpr size_t kScratchCountOffset = 1024; // uint64 singular-cell counter
+constexpr size_t kScratchBytes = 1280;
+hipError_t scratch_init(int device);
+void* scratch_address(int device);
+// the kernel family that condenses (ni, nb) without a workspace, or -1
+int noalloc_family(int ni, int nb);
+hipError_t launch_condense_dense_family(const DenseIn& in, int64_t ncells, const CondenseOut& out,
+                                        hipStream_t st, int family);
 hipError_t launch_count_nonzero(const int32_t* info, int64_t n, int64_t* count_dev,
                                 hipStream_t st);
 hipError_t launch_count_negative(const int32_t* v, int64_t n, int64_t* count_dev,
@@ -286,7 +305,9 @@ void clock_start();
 int clock_stop(std::vector<KernelTime>& out);  // returns the number of failed timings
 
 // Host helpers (hdg_fe.cpp)
-int build_tables(int phys, int D, int space, int kp, int kq, int km, int quad_degree,
+// proj (elasticity): the tau (w, P_m u) facet block is tau (P_m w, P_m u), i.e.
+// Fqq[F] = Fqm[F] Fmm^-1 Fqm[F]^T (every facet integral scales with the same area)
+int build_tables(int phys, int D, int space, int kp, int kq, int km, int quad_degree, int proj,
                  TablesHost& t, std::string& err);
 
 }  // namespace hdg

@@ -16,6 +16,7 @@
 #include <stdint.h>
 
 #include <algorithm>
+#include <atomic>
 #include <climits>
 
 #include "hdg_internal.h"
@@ -1936,10 +1937,23 @@ __global__ void assemble_kernel(PairTable P, const double* __restrict__ S,
       pr -= pr * ntf > rem;
       const int m = rem - pr * ntf;
       double v = 0.0;
+      if (!P.iperm) {
 #pragma unroll
-      for (int k = 0; k < 2; ++k) {
-        const int64_t off = staged ? s_off[2 * pr + k] : block_off(2 * pr + k);
-        if (off >= 0) v += S[off + (int64_t)mc * nb + m];
+        for (int k = 0; k < 2; ++k) {
+          const int64_t off = staged ? s_off[2 * pr + k] : block_off(2 * pr + k);
+          if (off >= 0) v += S[off + (int64_t)mc * nb + m];
+        }
+      } else {
+        // oriented facets: the facets' own trace dofs (mc of G, m of F) sit at
+        // permuted positions of each cell's local facet blocks
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+          const int64_t off = staged ? s_off[2 * pr + k] : block_off(2 * pr + k);
+          if (off < 0) continue;
+          const uint8_t code = P.pperm[2 * (p0 + pr) + k];
+          const int col = P.iperm[(code & 15) * ntf + mc], row = P.iperm[(code >> 4) * ntf + m];
+          v += S[off + (int64_t)col * nb + row];
+        }
       }
       nzval[base + t] = v;
     }
@@ -1961,7 +1975,8 @@ __global__ void rhs_kernel(PairTable P, const double* __restrict__ S, const doub
     for (int k = 0; k < 2; ++k) {
       const int32_t c = P.gcell[2 * G + k];
       if (c < 0) continue;
-      const int i = P.glpos[2 * G + k] * ntf + m;
+      const int mi = P.iperm ? P.iperm[P.gperm[2 * G + k] * ntf + m] : m;
+      const int i = P.glpos[2 * G + k] * ntf + mi;
       double v = g[(int64_t)c * nb + i];
       const int32_t* d = cell_dofs + (int64_t)c * nb;
       for (int j = 0; j < nb; ++j) {
@@ -2715,6 +2730,51 @@ hipError_t launch_condense_dense(const DenseIn& in, int64_t ncells, const Conden
                                  hipStream_t st, Workspace& ws) {
   DenseGen gen{in};
   return launch_condense<DenseGen, false>(gen, out, ncells, in.ni, in.nb, st, ws);
+}
+
+// ---------------------------------------------------------------------------
+// Fixed device scratch of the allocation-free entry point.  Every call on a
+// device shares that device's copy; it keeps the last call's contents.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+namespace {
+constexpr int kMaxScratchDevices = 64;
+std::atomic<void*> g_scratch[kMaxScratchDevices];
+}  // namespace
+
+hipError_t scratch_init(int device) {
+  if (device < 0 || device >= kMaxScratchDevices) return hipErrorInvalidDevice;
+  if (g_scratch[device].load()) return hipSuccess;
+  void* p = nullptr;
+  // the array's name is a host-side stand-in: its device address is resolved
+  // here, with the device current, and used everywhere else
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return e;
+  g_scratch[device].store(p);
+  return hipSuccess;
+}
+
+void* scratch_address(int device) {
+  if (device < 0 || device >= kMaxScratchDevices) return nullptr;
+  return g_scratch[device].load();
+}
+
+int noalloc_family(int ni, int nb) {
+  const int n = ni + nb;
+  if (n > kNoAllocMaxN) return -1;
+  const int fam = default_family(ni, nb);
+  if (fam != kFamLarge && fam != kFamPanel) return fam;  // no workspace
+  if (family_ok(kFamGrid16, ni, nb)) return kFamGrid16;
+  if (family_ok(kFamGrid32, ni, nb)) return kFamGrid32;
+  return -1;
+}
+
+hipError_t launch_condense_dense_family(const DenseIn& in, int64_t ncells, const CondenseOut& out,
+                                        hipStream_t st, int family) {
+  if (family == kFamLarge || family == kFamPanel) return hipErrorInvalidValue;
+  DenseGen gen{in};
+  Workspace none;  // the families above are the only ones that use it
+  return launch_condense<DenseGen, false>(gen, out, ncells, in.ni, in.nb, st, none, family);
 }
 
 hipError_t launch_backward_dense(const DenseIn& in, const double* x, int64_t ncells, double* xi,

@@ -29,8 +29,9 @@ from ._lib import (DARCY, ELASTICITY, FP64, MIXED, SPACE_P, SPACE_Q, HDGError,  
 
 __all__ = [
     "Context", "StaticCondensationMap", "BackwardStaticCondensationMap",
-    "CartesianDiscreteModel", "HDGSpaces", "DarcyHDG", "ElasticityHDG",
+    "CartesianDiscreteModel", "AffineDiscreteModel", "HDGSpaces", "DarcyHDG", "ElasticityHDG",
     "HybridAffineFEOperator", "solve", "HDGError", "SingularError", "FP64", "MIXED",
+    "HybridFEOperator", "NLSolver", "FESolver", "FEFunction", "solve_newton",
 ]
 
 _torch = None
@@ -232,6 +233,31 @@ class CartesianDiscreteModel:
 
 
 @dataclass
+class AffineDiscreteModel:
+    """A conforming mesh of affine cells (parallelograms / parallelepipeds)
+    given cell by cell: origin x0 (ncells, D), column-major Jacobians J
+    (ncells, D, D) as J[c][:, e] = the image of reference direction e, facet
+    ids per cell in Gridap's local facet order (ncells, 2D), and the
+    orientation code of each (cell, local facet) (the reference's pindex, see
+    hdg_mesh_desc).  Boundary facets (one cell) carry the Dirichlet trace
+    unless facet_dirichlet says otherwise."""
+    x0: np.ndarray
+    J: np.ndarray
+    cell_facets: np.ndarray
+    nfacets: int
+    facet_perm: np.ndarray = None
+    facet_dirichlet: np.ndarray = None
+
+    @property
+    def D(self):
+        return int(np.asarray(self.x0).shape[1])
+
+    @property
+    def num_cells(self):
+        return int(np.asarray(self.x0).shape[0])
+
+
+@dataclass
 class HDGSpaces:
     """The trial/test MultiFieldFESpace [flux, potential, trace] (Darcy) or
     [stress, displacement, trace] (elasticity) on a model."""
@@ -272,14 +298,17 @@ class DarcyHDG:
 
 @dataclass
 class ElasticityHDG:
-    """Elasticity HDG weak form of test/LinearElasticityHDGTests.jl:140-150
-    (no bulk-to-skeleton projection).  nu may vary per cell."""
+    """Elasticity HDG weak form of test/LinearElasticityHDGTests.jl:140-150.
+    nu may vary per cell.  bulk_to_skeleton_projection: the displacement's
+    facet traces in the tau terms are L2-projected onto the trace space (P_m,
+    test/P_m.jl:4-23), as the reference's bulk_to_skeleton_projection=true."""
     tau: float = 1.0
     nu: object = 0.3
     E: float = 1.0
     source: int = _lib.SOURCE_ELAS_SIN
     source_qp: object = None
     quad_degree: int = None
+    bulk_to_skeleton_projection: bool = False
     physics: int = ELASTICITY
 
 
@@ -553,7 +582,8 @@ def _problem(weakform, spaces):
     return _lib.Problem(int(weakform.physics), spaces.model.D, int(spaces.space),
                         int(spaces.k_flux), int(spaces.k_pot), int(spaces.k_trace), int(qd),
                         float(weakform.tau), float(young),
-                        _lib.SOURCE_QP if weakform.source_qp is not None else int(weakform.source))
+                        _lib.SOURCE_QP if weakform.source_qp is not None else int(weakform.source),
+                        int(bool(getattr(weakform, "bulk_to_skeleton_projection", False))))
 
 
 def _create_mesh(ctx, weakform, spaces):
@@ -563,9 +593,30 @@ def _create_mesh(ctx, weakform, spaces):
     coef = weakform.kappa if weakform.physics == DARCY else weakform.nu
     coef = np.ascontiguousarray(np.broadcast_to(np.asarray(coef, np.float64), (nc,)))
     prob = _problem(weakform, spaces)
+    if isinstance(model, AffineDiscreteModel):
+        x0 = np.ascontiguousarray(model.x0, np.float64)
+        # [cell][D][D] with J[c][a][e] = d x_a / d xi_e, stored column-major per cell
+        jac = np.ascontiguousarray(np.asarray(model.J, np.float64).transpose(0, 2, 1))
+        cf = np.ascontiguousarray(model.cell_facets, np.int32)
+        if model.facet_dirichlet is None:
+            cnt = np.bincount(cf.ravel(), minlength=int(model.nfacets))
+            fd = np.ascontiguousarray(cnt == 1, np.int8)
+        else:
+            fd = np.ascontiguousarray(model.facet_dirichlet, np.int8)
+        fp = None if model.facet_perm is None else np.ascontiguousarray(model.facet_perm, np.int8)
+        desc = _lib.MeshDesc(nc, int(model.nfacets), x0.ctypes.data, jac.ctypes.data,
+                             cf.ctypes.data, fd.ctypes.data, coef.ctypes.data,
+                             None if fp is None else fp.ctypes.data)
+        h = C.c_void_p()
+        check(lib().hdg_mesh_create(ctx.h, C.byref(prob), C.byref(desc), C.byref(h)), ctx.h,
+              "mesh_create")
+        return h
     cells = (C.c_int64 * 3)(*(list(model.partition) + [1] * (3 - D)))
     dom = (C.c_double * 6)(*(list(model.domain) + [0.0] * (6 - 2 * D)))
     h = C.c_void_p()
     check(lib().hdg_mesh_create_cartesian(ctx.h, C.byref(prob), cells, dom, coef.ctypes.data,
                                           C.byref(h)), ctx.h, "mesh_create")
     return h
+
+
+from .nonlinear import FEFunction, FESolver, HybridFEOperator, NLSolver, solve_newton  # noqa: E402

@@ -46,6 +46,7 @@ EXPORTS = [
     "hdg_batch_launch", "hdg_batch_condense_assemble", "hdg_batch_recover",
     "hdg_mesh_refinement_stats", "hdg_profile_start", "hdg_profile_stop",
     "hdg_footprint_cartesian", "hdg_mesh_factor_layout", "hdg_mesh_set_family",
+    "hdg_mesh_cell_quadrature", "hdg_static_condensation_noalloc",
 ]
 
 BATCH_AUTOTUNE = 1
@@ -83,6 +84,7 @@ class Problem(C.Structure):
         ("physics", C.c_int), ("dim", C.c_int), ("space", C.c_int), ("k_flux", C.c_int),
         ("k_pot", C.c_int), ("k_trace", C.c_int), ("quad_degree", C.c_int),
         ("tau", C.c_double), ("young", C.c_double), ("source", C.c_int),
+        ("projection", C.c_int),
     ]
 
 
@@ -90,6 +92,7 @@ class MeshDesc(C.Structure):
     _fields_ = [
         ("ncells", C.c_int64), ("nfacets", C.c_int64), ("x0", C.c_void_p), ("jac", C.c_void_p),
         ("cell_facets", C.c_void_p), ("facet_dirichlet", C.c_void_p), ("coef", C.c_void_p),
+        ("facet_perm", C.c_void_p),
     ]
 
 
@@ -151,6 +154,8 @@ def lib():
         L.hdg_memcpy_d2h.argtypes = [vp, vp, vp, i64]
         L.hdg_memcpy_h2d.argtypes = [vp, vp, vp, i64]
         L.hdg_static_condensation.argtypes = [vp, C.POINTER(Fields), i64, vp, vp, vp, vp, vp]
+        L.hdg_static_condensation_noalloc.argtypes = [vp, C.POINTER(Fields), i64, vp, vp, vp,
+                                                      vp, vp]
         L.hdg_backward_static_condensation.argtypes = [vp, C.POINTER(Fields), i64, vp, vp, vp,
                                                        vp, vp]
         L.hdg_mesh_create.argtypes = [vp, C.POINTER(Problem), C.POINTER(MeshDesc), C.POINTER(vp)]
@@ -160,6 +165,7 @@ def lib():
         L.hdg_mesh_set_dirichlet.argtypes = [vp, vp]
         L.hdg_mesh_set_dirichlet_mms.argtypes = [vp]
         L.hdg_mesh_set_source.argtypes = [vp, vp]
+        L.hdg_mesh_cell_quadrature.argtypes = [vp, vp, vp]
         L.hdg_condense_assemble.argtypes = [vp, i32, C.POINTER(i64)]
         L.hdg_skeleton_system.argtypes = [vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp),
                                           C.POINTER(vp)]

@@ -121,6 +121,19 @@ int hdg_static_condensation(hdg_ctx* ctx, const hdg_fields* fields, int64_t ncel
                             const double* A, const double* b, double* S, double* g,
                             int32_t* info);
 
+/* hdg_static_condensation that allocates nothing: no reference counterpart.
+ * For callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, checks and results (to the FP64 bar of 1e-12; the kernel family
+ * may differ) as hdg_static_condensation, for cell systems of at most 175 rows
+ * whose kernel needs no workspace (HDG_ERR_UNSUPPORTED otherwise).  Its
+ * scratch (field permutation, singular-cell counter) is a fixed 1,280-byte
+ * device array per GPU, held until the process ends: calls that overlap on
+ * one GPU race on it. */
+int hdg_static_condensation_noalloc(hdg_ctx* ctx, const hdg_fields* fields, int64_t ncells,
+                                    const double* A, const double* b, double* S, double* g,
+                                    int32_t* info);
+
 /* BackwardStaticCondensationMap.evaluate! over ncells cells
  * (replaces src/BackwardStaticCondensationMap.jl:61-102):
  *   x    device [ncells][nb] boundary (skeleton) values of each cell;
@@ -145,17 +158,39 @@ typedef struct {
   double tau;       /* HDG stabilisation */
   double young;     /* elasticity E */
   int source;       /* hdg_source */
+  /* elasticity: 1 = the bulk-to-skeleton L2 projection P_m of the displacement
+   * in the tau terms (src/GridapAPIExtensions.jl:453-819, test/P_m.jl:4-23,
+   * bulk_to_skeleton_projection=true of test/LinearElasticityHDGTests.jl:
+   * 124-150), 0 = P = identity.  Ignored for Darcy. */
+  int projection;
 } hdg_problem;
 
 /* Mesh of affine cells (parallelograms / parallelepipeds, which includes every
  * CartesianDiscreteModel).  All arrays are host memory, copied at creation.
  *   x0[ncells*dim]           cell origin (first vertex);
  *   jac[ncells*dim*dim]      column-major Jacobian of the affine cell map;
- *   cell_facets[ncells*2dim] facet ids in Gridap's local facet order;
+ *   cell_facets[ncells*2dim] facet ids in Gridap's local facet order (local
+ *                            facet f fixes reference direction dim-1-f/2 at
+ *                            f%2; its coordinates are the other directions,
+ *                            ascending);
  *   facet_dirichlet[nfacets] 1 if the facet carries Dirichlet trace dofs;
- *   coef[ncells]             kappa (Darcy) or Poisson ratio (elasticity).
+ *   coef[ncells]             kappa (Darcy) or Poisson ratio (elasticity);
+ *   facet_perm[ncells*2dim]  orientation of each (cell, local facet): the code
+ *                            p of the map from the cell's local facet
+ *                            coordinates (s[,t]) to the facet's own ones, in
+ *                            which the facet's trace dofs are numbered (the
+ *                            reference's pindex, src/SkeletonArrays.jl:28-37,
+ *                            src/Skeleton.jl:587-612).  2D: 0 s' = s,
+ *                            1 s' = 1-s.  3D: (a,b) = (t,s) if p&1 else (s,t),
+ *                            then s' = 1-a if p&2 else a, t' = 1-b if p&4 else
+ *                            b.  NULL: every code 0 (a CartesianDiscreteModel).
+ *                            A code under which the trace nodes are not mapped
+ *                            onto themselves (P_k traces with k >= 2 on hexahedra
+ *                            flipped) is HDG_ERR_UNSUPPORTED.
  * Owner signs n.n_o follow from cell_facets: the first (lowest id) cell
- * around a facet owns it (src/Skeleton.jl:135-140). */
+ * around a facet owns it (src/Skeleton.jl:135-140).  The condensed blocks of
+ * hdg_cell_blocks keep each cell's local trace node order; cell_dofs maps
+ * them to the facets' own numbering. */
 typedef struct {
   int64_t ncells;
   int64_t nfacets;
@@ -164,6 +199,7 @@ typedef struct {
   const int32_t* cell_facets;
   const int8_t* facet_dirichlet;
   const double* coef;
+  const int8_t* facet_perm;
 } hdg_mesh_desc;
 
 typedef struct {
@@ -203,6 +239,12 @@ int hdg_mesh_get_info(const hdg_mesh* mesh, hdg_mesh_info* info);
  * manufactured trace at the facet nodes instead. */
 int hdg_mesh_set_dirichlet(hdg_mesh* mesh, const double* lam_dir);
 int hdg_mesh_set_dirichlet_mms(hdg_mesh* mesh);
+/* The cell quadrature rule (Gauss of degree quad_degree, tensor points with
+ * the first reference direction fastest): points host [nq_cell][dim] on the
+ * reference cell [0,1]^dim, weights host [nq_cell].  The points at which
+ * hdg_mesh_set_source expects the source (x = x0 + J xi).  Replaces Gridap's
+ * CellQuadrature(trian, 2(k+1)) points (src/GridapTmpModifications.jl:13-40). */
+int hdg_mesh_cell_quadrature(const hdg_mesh* mesh, double* points, double* weights);
 /* Source values at cell quadrature points, device [ncells][nq_cell][ncomp]
  * (Darcy: f_1..f_D, g; elasticity: f_1..f_D); used when source == HDG_SOURCE_QP.
  * The pointer is borrowed until the next set or mesh destruction. */

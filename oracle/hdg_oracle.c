@@ -893,10 +893,11 @@ static void elas_cell(const orc_problem* p, const ctx_t* c, const double* x0, co
         }
       }
       for (int comp = 0; comp < D; comp++) {
-        /* tau omega.u */
-        for (int j = 0; j < nq; j++)
-          for (int i = 0; i < nq; i++)
-            skel[AIDX(off_u + comp * nq + i, off_u + comp * nq + j)] += (tau * psi[i] * psi[j]) * dS;
+        /* tau omega.u (with P_m: added after the facet's quadrature loop) */
+        if (!p->projection)
+          for (int j = 0; j < nq; j++)
+            for (int i = 0; i < nq; i++)
+              skel[AIDX(off_u + comp * nq + i, off_u + comp * nq + j)] += (tau * psi[i] * psi[j]) * dS;
         /* - tau omega.uhat */
         for (int m = 0; m < nm; m++)
           for (int i = 0; i < nq; i++)
@@ -910,6 +911,36 @@ static void elas_cell(const orc_problem* p, const ctx_t* c, const double* x0, co
           for (int m = 0; m < nm; m++)
             skel[AIDX(lo + comp * nm + m, lo + comp * nm + m2)] += (tau * mu[m] * mu[m2]) * dS;
       }
+    }
+    if (p->projection) {
+      /* P_m (test/P_m.jl:4-23, src/GridapAPIExtensions.jl:453-500): the facet
+       * trace of u is replaced by its L2 projection onto the trace space,
+       * dofs = A \ B with A = (mu, mu')_F and B = (mu, u)_F, so
+       * tau (omega, P_m u)_F = tau B^T A^-1 B per displacement component. */
+      double* Bf = (double*)calloc((size_t)nm * nq, sizeof(double));
+      double* Af = (double*)calloc((size_t)nm * nm, sizeof(double));
+      double* X = (double*)malloc(sizeof(double) * (size_t)nm * nq);
+      for (int q = 0; q < c->nqf; q++) {
+        const double* psi = c->psi_f + (fc * c->nqf + q) * nq;
+        const double* mu = c->mu_f + (fc * c->nqf + q) * nm;
+        double dS = c->wf[q] * fac;
+        for (int j = 0; j < nq; j++)
+          for (int m = 0; m < nm; m++) Bf[m + (size_t)j * nm] += mu[m] * psi[j] * dS;
+        for (int m2 = 0; m2 < nm; m2++)
+          for (int m = 0; m < nm; m++) Af[m + (size_t)m2 * nm] += mu[m] * mu[m2] * dS;
+      }
+      memcpy(X, Bf, sizeof(double) * (size_t)nm * nq);
+      orc_dense_solve(nm, nq, Af, X); /* X = A^-1 B */
+      for (int comp = 0; comp < D; comp++)
+        for (int j = 0; j < nq; j++)
+          for (int i = 0; i < nq; i++) {
+            double v = 0.0;
+            for (int m = 0; m < nm; m++) v += Bf[m + (size_t)i * nm] * X[m + (size_t)j * nm];
+            skel[AIDX(off_u + comp * nq + i, off_u + comp * nq + j)] += tau * v;
+          }
+      free(Bf);
+      free(Af);
+      free(X);
     }
   }
   for (size_t t = 0; t < (size_t)n * n; t++) A[t] += skel[t];
@@ -1414,4 +1445,141 @@ int orc_condense_recover_stored(int ni, int nb, int64_t ncells, const double* A,
     free(ipiv);
   }
   return bad;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Facet orientation (src/SkeletonArrays.jl:28-37 pindex, src/Skeleton.jl:    */
+/* 587-612): each (cell, local facet) carries a code p of the map from the   */
+/* cell's local facet coordinates to the facet's own coordinates.  2D: p=0   */
+/* s'=s, p=1 s'=1-s.  3D: (a,b) = (t,s) if p&1 else (s,t); s' = 1-a if p&2,   */
+/* t' = 1-b if p&4.  The trace dofs of a facet are numbered in its own frame, */
+/* so node i of the cell's local facet is own node perm_p(i).                */
+/* ------------------------------------------------------------------------ */
+static void orient_coords(int D, int p, const double* y, double* yo) {
+  if (D == 2) {
+    yo[0] = (p & 1) ? 1.0 - y[0] : y[0];
+    return;
+  }
+  double a = (p & 1) ? y[1] : y[0], b = (p & 1) ? y[0] : y[1];
+  yo[0] = (p & 2) ? 1.0 - a : a;
+  yo[1] = (p & 4) ? 1.0 - b : b;
+}
+
+int orc_facet_node_perm(const orc_problem* p, int code, int* perm) {
+  int D = p->D;
+  if (code < 0 || code >= (D == 2 ? 2 : 8)) return -1;
+  space_t s;
+  if (space_init(&s, D - 1, p->k_trace, p->space == ORC_SPACE_P)) return -1;
+  int rc = 0;
+  for (int i = 0; i < s.n && !rc; i++) {
+    double yo[2] = {0, 0};
+    orient_coords(D, code, s.nodes[i], yo);
+    int found = -1;
+    for (int j = 0; j < s.n; j++) {
+      double d = 0.0;
+      for (int a = 0; a < D - 1; a++) d += fabs(s.nodes[j][a] - yo[a]);
+      if (d < 1e-12) found = j;
+    }
+    if (found < 0) rc = -2; /* the node set is not invariant under this map */
+    perm[i] = found;
+  }
+  space_free(&s);
+  return rc;
+}
+
+int orc_trace_dofs_oriented(const orc_problem* p, int64_t ncells, int64_t nfacets,
+                            const int32_t* cell_facets, const int8_t* cell_facet_perm,
+                            const int8_t* facet_dirichlet, int32_t* cell_dofs, int64_t* nfree,
+                            int64_t* ndir) {
+  int D = p->D, nF = 2 * D;
+  int n_flux, n_pot, n_tf, ni, nb;
+  orc_sizes(p, &n_flux, &n_pot, &n_tf, &ni, &nb);
+  int ncomp = p->phys == ORC_PHYS_DARCY ? 1 : D;
+  int nm = n_tf / ncomp;
+  int* perm = (int*)malloc(sizeof(int) * 8 * nm);
+  for (int code = 0; code < (D == 2 ? 2 : 8); code++)
+    if (orc_facet_node_perm(p, code, perm + code * nm))
+      for (int i = 0; i < nm; i++) perm[code * nm + i] = -1;
+  int rc = orc_trace_dofs(D, n_tf, ncells, nfacets, cell_facets, facet_dirichlet, cell_dofs, nfree,
+                          ndir);
+  for (int64_t c = 0; c < ncells && !rc; c++)
+    for (int lf = 0; lf < nF && !rc; lf++) {
+      int code = cell_facet_perm ? cell_facet_perm[c * nF + lf] : 0;
+      if (!code) continue;
+      if (perm[code * nm] < 0) {
+        rc = -2;
+        break;
+      }
+      int32_t* d = cell_dofs + (c * nF + lf) * n_tf;
+      int32_t own[64];
+      for (int m = 0; m < n_tf; m++) own[m] = d[m]; /* own order: dof = comp*nm + node */
+      for (int comp = 0; comp < ncomp; comp++)
+        for (int i = 0; i < nm; i++) d[comp * nm + i] = own[comp * nm + perm[code * nm + i]];
+    }
+  free(perm);
+  return rc;
+}
+
+/* Dirichlet values in each facet's own node order: own node j is node
+ * perm^-1(j) of the owner cell's local facet. */
+int orc_dirichlet_values_oriented(const orc_problem* p, int64_t nfacets,
+                                  const int32_t* facet_cells, const int8_t* facet_lpos,
+                                  const int8_t* cell_facet_perm, const int8_t* facet_dirichlet,
+                                  const double* x0, const double* J, double* lam_dir) {
+  int D = p->D, nF = 2 * D;
+  space_t s;
+  if (space_init(&s, D - 1, p->k_trace, p->space == ORC_SPACE_P)) return -1;
+  int nm = s.n;
+  int* perm = (int*)malloc(sizeof(int) * 8 * nm);
+  int* inv = (int*)malloc(sizeof(int) * nm);
+  for (int code = 0; code < (D == 2 ? 2 : 8); code++)
+    if (orc_facet_node_perm(p, code, perm + code * nm))
+      for (int i = 0; i < nm; i++) perm[code * nm + i] = -1;
+  int64_t k = 0;
+  int rc = 0;
+  for (int64_t f = 0; f < nfacets && !rc; f++) {
+    if (!facet_dirichlet[f]) continue;
+    int64_t c = facet_cells[2 * f];
+    int lf = facet_lpos[2 * f];
+    int code = cell_facet_perm ? cell_facet_perm[c * nF + lf] : 0;
+    if (perm[code * nm] < 0) {
+      rc = -2;
+      break;
+    }
+    for (int i = 0; i < nm; i++) inv[perm[code * nm + i]] = i;
+    for (int j = 0; j < nm; j++) {
+      double xi[3] = {0, 0, 0}, x[3] = {0, 0, 0};
+      facet_to_cell(D, lf, s.nodes[inv[j]], xi);
+      for (int a = 0; a < D; a++) {
+        x[a] = x0[c * D + a];
+        for (int e = 0; e < D; e++) x[a] += J[c * D * D + a + D * e] * xi[e];
+      }
+      if (p->phys == ORC_PHYS_DARCY) {
+        double u, q[3], ff[3], g;
+        darcy_mms(p->mms, D, x, 1.0, &u, q, ff, &g);
+        lam_dir[k++] = u;
+      } else {
+        double u[3], sg[9], ff[3];
+        elas_mms(p->mms, D, x, p->E, p->nu, u, sg, ff);
+        for (int comp = 0; comp < D; comp++) lam_dir[k + comp * nm + j] = u[comp];
+      }
+    }
+    if (p->phys != ORC_PHYS_DARCY) k += (int64_t)D * nm;
+  }
+  free(perm);
+  free(inv);
+  space_free(&s);
+  return rc;
+}
+
+/* Nodes of the scalar Lagrange space of order k on the reference d-cube, in
+ * the oracle's node order: nodes[i*d + a]. */
+int orc_space_nodes(int d, int k, int pspace, double* nodes) {
+  space_t s;
+  if (space_init(&s, d, k, pspace)) return -1;
+  for (int i = 0; i < s.n; i++)
+    for (int a = 0; a < d; a++) nodes[i * d + a] = s.nodes[i][a];
+  int n = s.n;
+  space_free(&s);
+  return n;
 }

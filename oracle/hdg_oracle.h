@@ -43,6 +43,8 @@ typedef struct {
   double tau;        /* stabilisation */
   double E, nu;      /* elasticity only */
   int mms;           /* ORC_MMS_* or elasticity MMS id */
+  int projection;    /* elasticity: 1 = bulk-to-skeleton L2 projection P_m in the
+                        tau terms (test/P_m.jl:4-23), 0 = identity */
 } orc_problem;
 
 /* Sizes of the local blocks. */
@@ -142,6 +144,28 @@ int orc_set_threads(int n);
 int orc_condense_recover_stored(int ni, int nb, int64_t ncells, const double* A, const double* b,
                                 const double* xb, double* S, double* g, double* xi,
                                 int32_t* info);
+
+/* Facet orientation (pindex, src/SkeletonArrays.jl:28-37): code p of the map
+ * from a cell's local facet coordinates (s[,t]) to the facet's own ones.
+ * 2D: 0 identity, 1 s' = 1-s.  3D: (a,b) = (t,s) if p&1 else (s,t);
+ * s' = 1-a if p&2 else a; t' = 1-b if p&4 else b.
+ * orc_facet_node_perm: perm[i] = own index of the cell-local trace node i
+ * (-2 if the trace node set is not invariant under the map). */
+int orc_facet_node_perm(const orc_problem* p, int code, int* perm);
+/* orc_trace_dofs with per-(cell, local facet) orientation codes
+ * cell_facet_perm [ncells*2D] (NULL = all identity). */
+int orc_trace_dofs_oriented(const orc_problem* p, int64_t ncells, int64_t nfacets,
+                            const int32_t* cell_facets, const int8_t* cell_facet_perm,
+                            const int8_t* facet_dirichlet, int32_t* cell_dofs, int64_t* nfree,
+                            int64_t* ndir);
+/* Nodes of the scalar Lagrange space of order k on the reference d-cube
+ * (nodes[i*d + a]); returns their number. */
+int orc_space_nodes(int d, int k, int pspace, double* nodes);
+/* orc_dirichlet_values with each facet's values in its own node order. */
+int orc_dirichlet_values_oriented(const orc_problem* p, int64_t nfacets,
+                                  const int32_t* facet_cells, const int8_t* facet_lpos,
+                                  const int8_t* cell_facet_perm, const int8_t* facet_dirichlet,
+                                  const double* x0, const double* J, double* lam_dir);
 
 #ifdef __cplusplus
 }

@@ -32,6 +32,7 @@ class _Problem(C.Structure):
         ("phys", C.c_int), ("D", C.c_int), ("space", C.c_int), ("k_flux", C.c_int),
         ("k_pot", C.c_int), ("k_trace", C.c_int), ("quad_degree", C.c_int),
         ("tau", C.c_double), ("E", C.c_double), ("nu", C.c_double), ("mms", C.c_int),
+        ("projection", C.c_int),
     ]
 
 
@@ -68,6 +69,7 @@ class Problem:
     E: float = 1.0
     nu: float = 0.3
     mms: int = MMS_LINEAR
+    projection: int = 0   # elasticity: bulk-to-skeleton L2 projection P_m
 
     @staticmethod
     def darcy(D, k, space=SPACE_Q, mms=MMS_LINEAR, tau=1.0, k_pot=None, k_trace=None,
@@ -84,14 +86,16 @@ class Problem:
         return Problem(PHYS_DARCY, D, SPACE_P, 1, 0, 1, 4, 1.0, 1.0, 0.3, MMS_KAT)
 
     @staticmethod
-    def elasticity(D, k, tau, nu=0.3, E=1.0, mms=ELAS_MMS_SIN):
+    def elasticity(D, k, tau, nu=0.3, E=1.0, mms=ELAS_MMS_SIN, projection=0):
         """test/LinearElasticityHDGTests.jl:88-94: P_k sym stress, P_{k+1}
-        displacement, P_k trace, degree 2(k+1)."""
-        return Problem(PHYS_ELASTICITY, D, SPACE_P, k, k + 1, k, 2 * (k + 1), tau, E, nu, mms)
+        displacement, P_k trace, degree 2(k+1); projection = the
+        bulk_to_skeleton_projection flag (P_m, :124-150)."""
+        return Problem(PHYS_ELASTICITY, D, SPACE_P, k, k + 1, k, 2 * (k + 1), tau, E, nu, mms,
+                       int(projection))
 
     def c(self):
         return _Problem(self.phys, self.D, self.space, self.k_flux, self.k_pot, self.k_trace,
-                        self.quad_degree, self.tau, self.E, self.nu, self.mms)
+                        self.quad_degree, self.tau, self.E, self.nu, self.mms, self.projection)
 
     def sizes(self):
         v = [C.c_int() for _ in range(5)]
@@ -119,6 +123,77 @@ class Mesh:
     facet_boundary: np.ndarray
     x0: np.ndarray
     J: np.ndarray
+    facet_perm: np.ndarray = None   # [ncells*2D] orientation codes (None = identity)
+
+
+def general_mesh(D, x0, J, cell_facets, nfacets, facet_perm=None):
+    """Topology of a conforming mesh of affine cells given by its cell->facet
+    ids in local facet order: facet -> (cell, local position) in ascending cell
+    order, owner = first (lowest id) cell around a facet
+    (src/Skeleton.jl:135-140), owner signs, boundary facets."""
+    nc = len(x0) // D
+    nF = 2 * D
+    cf = np.ascontiguousarray(cell_facets, np.int32)
+    fc = -np.ones(nfacets * 2, np.int32)
+    fl = -np.ones(nfacets * 2, np.int8)
+    cs = np.ones(nc * nF, np.int8)
+    for c in range(nc):
+        for lf in range(nF):
+            f = cf[c * nF + lf]
+            if fc[2 * f] < 0:
+                fc[2 * f], fl[2 * f] = c, lf
+            else:
+                assert fc[2 * f + 1] < 0, "facet shared by more than two cells"
+                fc[2 * f + 1], fl[2 * f + 1] = c, lf
+                cs[c * nF + lf] = -1
+    fb = (fc[1::2] < 0).astype(np.int8)
+    fp = None if facet_perm is None else np.ascontiguousarray(facet_perm, np.int8)
+    return Mesh(D, (nc,), (), nc, nfacets, cf, cs, fc, fl, fb,
+                np.ascontiguousarray(x0, np.float64), np.ascontiguousarray(J, np.float64), fp)
+
+
+def facet_node_perm(prob: Problem, code):
+    """perm[i] = the facet's own index of the cell-local trace node i under
+    orientation code `code` (None if the node set is not invariant)."""
+    nm = prob.sizes()["n_tf"] // (1 if prob.phys == PHYS_DARCY else prob.D)
+    perm = np.zeros(nm, np.int32)
+    rc = lib().orc_facet_node_perm(C.byref(prob.c()), int(code), _p(perm))
+    return None if rc else perm
+
+
+def space_nodes(d, k, pspace):
+    """Reference nodes (n, d) of the scalar Lagrange space of order k."""
+    buf = np.zeros(4096 * max(d, 1))
+    n = lib().orc_space_nodes(int(d), int(k), int(pspace), _p(buf))
+    assert n > 0
+    return buf[:n * d].reshape(n, d)
+
+
+def trace_dofs_oriented(prob: Problem, mesh: Mesh, facet_dirichlet=None):
+    """trace_dofs honouring mesh.facet_perm (cell-local trace nodes mapped to
+    each facet's own node order)."""
+    if facet_dirichlet is None:
+        facet_dirichlet = mesh.facet_boundary.copy()
+    fd = np.ascontiguousarray(facet_dirichlet, np.int8)
+    nb = prob.sizes()["nb"]
+    cd = np.zeros(mesh.ncells * nb, np.int32)
+    nf, nd = C.c_int64(), C.c_int64()
+    rc = lib().orc_trace_dofs_oriented(C.byref(prob.c()), C.c_int64(mesh.ncells),
+                                       C.c_int64(mesh.nfacets), _p(mesh.cell_facets),
+                                       _p(mesh.facet_perm), _p(fd), _p(cd), C.byref(nf),
+                                       C.byref(nd))
+    assert rc == 0, "orientation not supported by the trace space"
+    return cd.reshape(mesh.ncells, nb), nf.value, nd.value, fd
+
+
+def dirichlet_values_oriented(prob: Problem, mesh: Mesh, facet_dirichlet, ndir):
+    out = np.zeros(max(ndir, 1))
+    rc = lib().orc_dirichlet_values_oriented(
+        C.byref(prob.c()), C.c_int64(mesh.nfacets), _p(mesh.facet_cells), _p(mesh.facet_lpos),
+        _p(mesh.facet_perm), _p(np.ascontiguousarray(facet_dirichlet, np.int8)), _p(mesh.x0),
+        _p(mesh.J), _p(out))
+    assert rc == 0
+    return out[:ndir]
 
 
 def cartesian(D, n, domain=None):
@@ -294,9 +369,14 @@ class Solution:
 def hdg_solve(prob: Problem, n, coef=None, domain=None, lam=None):
     """Full reference flow: condense -> assemble -> host sparse solve -> recover.
     If lam is given (free trace values), the skeleton solve is skipped."""
+    return hdg_solve_mesh(prob, cartesian(prob.D, n, domain), coef, lam)
+
+
+def hdg_solve_mesh(prob: Problem, mesh: Mesh, coef=None, lam=None):
+    """hdg_solve on any mesh of affine cells (general_mesh or cartesian),
+    honouring its facet orientation codes."""
     import scipy.sparse as sp
     import scipy.sparse.linalg as spla
-    mesh = cartesian(prob.D, n, domain)
     s = prob.sizes()
     ni, nb = s["ni"], s["nb"]
     if coef is None:
@@ -304,8 +384,12 @@ def hdg_solve(prob: Problem, n, coef=None, domain=None, lam=None):
     A, b = cell_systems(prob, mesh, coef)
     S, g, info = condense(A, b, ni, nb)
     assert not info.any(), "singular A_ii"
-    cd, nfree, ndir, fd = trace_dofs(mesh, s["n_tf"])
-    lam_dir = dirichlet_values(prob, mesh, fd, ndir)
+    if mesh.facet_perm is None:
+        cd, nfree, ndir, fd = trace_dofs(mesh, s["n_tf"])
+        lam_dir = dirichlet_values(prob, mesh, fd, ndir)
+    else:
+        cd, nfree, ndir, fd = trace_dofs_oriented(prob, mesh)
+        lam_dir = dirichlet_values_oriented(prob, mesh, fd, ndir)
     colptr, rowval, nzval, rhs = assemble(S, g, cd, nfree, lam_dir)
     if lam is None:
         K = sp.csc_matrix((nzval, rowval, colptr), shape=(nfree, nfree))

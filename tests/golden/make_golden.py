@@ -271,8 +271,27 @@ def sym_ij(D, c):
             t += 1
 
 
-def elasticity_cell_exact(D, k, h, E, nu, tau):
-    """Exact A of the elasticity HDG form (no projection) on a box cell, zero load."""
+def frac_solve(A, B):
+    """Exact A^-1 B (Fraction Gauss-Jordan); A n x n, B n x m (lists of rows)."""
+    n, m = len(A), len(B[0])
+    M = [list(A[i]) + list(B[i]) for i in range(n)]
+    for j in range(n):
+        p = next(i for i in range(j, n) if M[i][j] != 0)
+        M[j], M[p] = M[p], M[j]
+        piv = M[j][j]
+        M[j] = [v / piv for v in M[j]]
+        for i in range(n):
+            if i != j and M[i][j] != 0:
+                f = M[i][j]
+                M[i] = [vi - f * vj for vi, vj in zip(M[i], M[j])]
+    return [row[n:n + m] for row in M]
+
+
+def elasticity_cell_exact(D, k, h, E, nu, tau, projection=False):
+    """Exact A of the elasticity HDG form on a box cell, zero load; with
+    projection, the tau (omega, u) facet block uses the bulk-to-skeleton L2
+    projection P_m of test/P_m.jl:4-23: tau (omega, P_m u)_F = tau B^T A^-1 B
+    with A = (mu, mu')_F, B = (mu, u)_F (exact rational inverse)."""
     phi, _ = nodal_basis(D, k, True)       # stress scalar P_k
     psi, _ = nodal_basis(D, k + 1, True)   # displacement scalar P_{k+1}
     mu, _ = nodal_basis(D - 1, k, True)    # trace scalar P_k
@@ -339,10 +358,17 @@ def elasticity_cell_exact(D, k, h, E, nu, tau):
                         v = I(pmul(phF[i], mu[m])) * vn[comp]
                         A[ca * npf + i][lo + comp * nm + m] += -v
                         A[lo + comp * nm + m][ca * npf + i] += v
+        if projection:
+            Am = [[I(pmul(mu[m], mu[m2])) for m2 in range(nm)] for m in range(nm)]
+            Bm = [[I(pmul(mu[m], psF[j])) for j in range(nps)] for m in range(nm)]
+            X = frac_solve(Am, Bm)
+            PP = [[sum(Bm[m][i] * X[m][j] for m in range(nm)) for j in range(nps)]
+                  for i in range(nps)]
         for comp in range(D):
             for i in range(nps):
                 for j in range(nps):
-                    A[na + comp * nps + i][na + comp * nps + j] += tau * I(pmul(psF[i], psF[j]))
+                    A[na + comp * nps + i][na + comp * nps + j] += tau * (
+                        PP[i][j] if projection else I(pmul(psF[i], psF[j])))
                 for m in range(nm):
                     v = tau * I(pmul(psF[i], mu[m]))
                     A[na + comp * nps + i][lo + comp * nm + m] += -v
@@ -412,6 +438,14 @@ def main():
         A = elasticity_cell_exact(D, k, h, Fr(1), nu, tau)
         np.savez(os.path.join(HERE, f"{name}.npz"), A=A, D=D, k=k,
                  h=np.array([float(v) for v in h]), nu=float(nu), tau=float(tau))
+        print("wrote", name, A.shape)
+    # the same with the bulk-to-skeleton projection P_m
+    for D, k, h, nu, tau, name in ((2, 1, (Fr(1, 2), Fr(1, 4)), Fr(3, 10), Fr(4), "elas_p1_2d_pm"),
+                                   (3, 2, (Fr(1, 4), Fr(1, 2), Fr(1, 8)), Fr(1, 3), Fr(3),
+                                    "elas_p2_3d_pm")):
+        A = elasticity_cell_exact(D, k, h, Fr(1), nu, tau, projection=True)
+        np.savez(os.path.join(HERE, f"{name}.npz"), A=A, D=D, k=k,
+                 h=np.array([float(v) for v in h]), nu=float(nu), tau=float(tau), projection=1)
         print("wrote", name, A.shape)
     # --- LAPACK condensation --------------------------------------------
     import scipy.linalg as sl

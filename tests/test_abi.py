@@ -80,3 +80,13 @@ def test_product_path_fails_loudly_without_gpu():
     import gridaphybrid as gh
     with pytest.raises(gh.HDGError):
         gh.Context(0)
+
+
+def test_newton_solver_options():
+    """NLSolver accepts :newton only (the reference's NLSolver(method=:newton));
+    the Newton path's host types import without a GPU."""
+    import gridaphybrid as gh
+    assert gh.NLSolver().method == "newton"
+    with pytest.raises(gh.HDGError):
+        gh.NLSolver(method="trust_region")
+    assert gh.FESolver().nls.iterations > 0

@@ -323,3 +323,29 @@ def test_darcy_3d_k3_large(ctx):
     fv = op.recover(ref.lam).cpu().numpy()
     assert rel(fv, ref.free_values) < 1e-11
     op.close()
+
+
+@pytest.mark.parametrize("D,k,n", [(2, 1, 4), (3, 1, 2), (3, 2, 2)])
+def test_elasticity_projection_parity(ctx, D, k, n):
+    """Elasticity HDG with the bulk-to-skeleton projection P_m
+    (test/P_m.jl:4-23, bulk_to_skeleton_projection=true of
+    test/LinearElasticityHDGTests.jl:124-150) against the oracle."""
+    import gridaphybrid as gh
+    print(f"elasticity with P_m D={D} k={k} n={n}", flush=True)
+    nc = n ** D
+    nu = np.random.default_rng(6).uniform(0.2, 0.4, nc)
+    prob = O.Problem.elasticity(D, k, tau=float(n), nu=0.3, mms=O.ELAS_MMS_SIN, projection=1)
+    ref = O.hdg_solve(prob, (n,) * D, coef=nu)
+    model = gh.CartesianDiscreteModel(tuple([0.0, 1.0] * D), tuple([n] * D))
+    X = gh.HDGSpaces.elasticity(model, k)
+    wf = gh.ElasticityHDG(tau=float(n), nu=nu, E=1.0, source=12, bulk_to_skeleton_projection=True)
+    op = gh.HybridAffineFEOperator(wf, X, X, [1, 2], [3], ctx=ctx)
+    A, b = _device_cell_systems(op, 0, nc)
+    assert rel(A, ref.A) < RTOL and rel(b, ref.b) < RTOL
+    S, g, cd = op.cell_blocks()
+    assert rel(S, ref.S) < 1e-11 and rel(g, ref.g) < 1e-11
+    K, rhs = op.skeleton_system()
+    assert rel(K.data, ref.nzval) < 1e-11 and rel(rhs, ref.rhs) < 1e-11
+    sol = gh.solve(op)
+    assert rel(sol.free_values, ref.free_values) < 1e-10
+    op.close()

@@ -49,11 +49,16 @@ def test_darcy_cell_system_exact(name):
     assert rel(b[0], g["b"]) < 1e-13
 
 
-@pytest.mark.parametrize("name", ["elas_p1_2d", "elas_p1_3d", "elas_p2_3d"])
+@pytest.mark.parametrize("name", ["elas_p1_2d", "elas_p1_3d", "elas_p2_3d", "elas_p1_2d_pm",
+                                  "elas_p2_3d_pm"])
 def test_elasticity_cell_system_exact(name):
+    """Elasticity HDG cell systems against exact rational integration, without
+    and with the bulk-to-skeleton projection P_m (test/P_m.jl:4-23)."""
     g = np.load(os.path.join(GOLD, f"{name}.npz"))
     D, k = int(g["D"]), int(g["k"])
-    prob = O.Problem.elasticity(D, k, tau=float(g["tau"]), nu=float(g["nu"]), mms=0)
+    pm = int(g["projection"]) if "projection" in g.files else 0
+    prob = O.Problem.elasticity(D, k, tau=float(g["tau"]), nu=float(g["nu"]), mms=0,
+                                projection=pm)
     mesh = one_box_cell(D, np.zeros(D), g["h"], float(g["nu"]), np.ones(2 * D))
     A, b = O.cell_systems(prob, mesh, np.array([float(g["nu"])]))
     assert rel(A[0], g["A"]) < 1e-13
@@ -289,3 +294,96 @@ def test_self_variation_reorder_is_exact_permutation():
     S, gg, xi, info = O.solve_reordered(g["A"], g["b"], g["x"], 6, 3, perm)
     assert not info.any()
     assert rel(S, g["S"]) < 1e-13 and rel(gg, g["g"]) < 1e-13 and rel(xi, g["xi"]) < 1e-13
+
+
+@pytest.mark.parametrize("D,k,n", [(2, 1, 4), (2, 2, 3), (2, 3, 3), (3, 1, 2), (3, 2, 2)])
+def test_oriented_mesh_reproduces_unrotated_solution(D, k, n):
+    """Facet orientation (pindex, src/SkeletonArrays.jl:28-37): re-parametrising
+    cells by rotations of the reference cell permutes their local facets and
+    flips their facet frames, but not the discrete problem.  The skeleton
+    solution (numbered in each facet's own frame) must be the same, and every
+    cell's flux and potential must be the same nodal values in permuted
+    order (the flux components are physical)."""
+    import meshgen
+    kappa = O.lognormal_kappa(n ** D, seed=31)
+    prob = O.Problem.darcy(D, k, mms=O.MMS_SINPROD)
+    x0, J, cf, nf, perm, rot = meshgen.sheared_rotated_mesh(D, n, seed=D * 10 + k, frac=0.6)
+    x0r, Jr, cfr, _, _, _ = meshgen.sheared_rotated_mesh(D, n, seed=D * 10 + k, frac=0.0)
+    assert any(r is not None for r in rot) and perm.any()
+    ref = O.hdg_solve_mesh(prob, meshgen.oracle_mesh(x0r, Jr, cfr, nf, None), coef=kappa)
+    got = O.hdg_solve_mesh(prob, meshgen.oracle_mesh(x0, J, cf, nf, perm), coef=kappa)
+    assert got.nfree == ref.nfree and len(got.rowval) == len(ref.rowval)
+    assert rel(got.lam, ref.lam) < 1e-11
+    s = prob.sizes()
+    na, ni = s["n_flux"], s["ni"]
+    npn = na // D
+    nodes = O.space_nodes(D, k, 0)
+    for c in range(n ** D):
+        idx = np.arange(npn) if rot[c] is None else meshgen.node_map(nodes, *rot[c])
+        qi = np.concatenate([a * npn + idx for a in range(D)])
+        assert rel(got.xi[c, :na], ref.xi[c, qi]) < 1e-10
+        assert rel(got.xi[c, na:ni], ref.xi[c, na:ni][idx]) < 1e-10
+
+
+def test_orientation_codes_on_trace_nodes():
+    """Every orientation code maps Q_k trace nodes onto themselves (a
+    permutation); P_2 trace nodes on a hexahedron's facet are not invariant
+    under a flip, only under the swap of the facet coordinates."""
+    for D, k in ((2, 1), (2, 3), (3, 1), (3, 2), (3, 3)):
+        prob = O.Problem.darcy(D, k)
+        for p in range(2 if D == 2 else 8):
+            perm = O.facet_node_perm(prob, p)
+            assert perm is not None and sorted(perm) == list(range(len(perm)))
+            if p == 0:
+                assert list(perm) == list(range(len(perm)))
+    pe = O.Problem.elasticity(3, 2, tau=1.0)
+    assert O.facet_node_perm(pe, 1) is not None
+    assert O.facet_node_perm(pe, 2) is None
+
+
+@pytest.mark.parametrize("D,k", [(2, 1), (2, 2), (3, 1), (3, 2)])
+def test_projection_reproduces_trace_space_functions(D, k):
+    """P_m is the L2 projection onto the trace space (test/P_m.jl:4-23; the
+    reference checks (mu, u - P_m u)_F = 0 at test/LinearElasticityHDGTests.jl:
+    124-130): a displacement whose facet traces lie in P_k is left unchanged,
+    so on the interpolant of a linear field the projected and unprojected cell
+    systems agree; on a field of degree k+1 they differ."""
+    mesh = O.cartesian(D, (1,) * D)
+    mesh.J[:] = (np.diag([0.5, 0.25, 0.125][:D])).T.ravel()
+    A0, _ = O.cell_systems(O.Problem.elasticity(D, k, tau=2.0, mms=0), mesh, np.array([0.3]))
+    A1, _ = O.cell_systems(O.Problem.elasticity(D, k, tau=2.0, mms=0, projection=1), mesh,
+                           np.array([0.3]))
+    s = O.Problem.elasticity(D, k, tau=2.0).sizes()
+    na, ni, nq = s["n_flux"], s["ni"], s["n_pot"] // D
+    nodes = O.space_nodes(D, k + 1, 1) * np.array([0.5, 0.25, 0.125][:D])
+    dA = A1[0] - A0[0]
+    assert np.max(np.abs(dA[:na])) == 0.0 and np.max(np.abs(dA[ni:])) == 0.0
+    lin = np.concatenate([nodes @ np.arange(1, D + 1) + c for c in range(D)])
+    assert np.max(np.abs(dA[:, na:ni] @ lin)) < 1e-14 * np.max(np.abs(A0[0]))
+    top = np.concatenate([nodes[:, 0] ** (k + 1)] * D)
+    assert np.max(np.abs(dA[:, na:ni] @ top)) > 1e-6
+
+
+def test_projection_linear_displacement_exact():
+    """With P_m the linear displacement (exact in the spaces) is still
+    reproduced to round-off (test/LinearElasticityHDGTests.jl:70-71)."""
+    for D, n in ((2, 3), (3, 2)):
+        p = O.Problem.elasticity(D, 1, tau=float(n), nu=0.3, mms=O.ELAS_MMS_LINEAR, projection=1)
+        sol = O.hdg_solve(p, (n,) * D, coef=np.full(n ** D, 0.3))
+        err = O.l2_errors(p, sol.mesh, sol.coef, sol.xi)
+        assert err[0] < 1e-11 and err[1] < 1e-11
+        assert rel(sol.free_values, O.monolithic_solve(sol)) < 1e-10
+
+
+def test_projection_convergence():
+    """With P_m, the stress and displacement errors of the smooth solution
+    fall at least at rate k+1 and k+1 (test/LinearElasticityHDGTests.jl:212-221
+    reports the slopes of this convergence study)."""
+    errs = []
+    for n in (4, 8, 16):
+        p = O.Problem.elasticity(2, 1, tau=float(n), nu=0.3, mms=O.ELAS_MMS_SIN, projection=1)
+        sol = O.hdg_solve(p, (n, n), coef=np.full(n * n, 0.3))
+        errs.append(O.l2_errors(p, sol.mesh, sol.coef, sol.xi))
+    r = np.log2(np.array(errs[1]) / np.array(errs[2]))
+    print("P_m convergence rates (stress, displacement):", r)
+    assert r[0] > 1.7 and r[1] > 1.7
