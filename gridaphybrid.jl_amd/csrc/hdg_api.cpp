@@ -117,6 +117,15 @@ struct hdg_mesh {
   int used_family = -1;
   DevBuf d_fptr, d_pptr, d_pfacet, d_pcell, d_plpos, d_gcell, d_glpos;
   DevBuf d_pperm, d_gperm, d_iperm;  // allocated for oriented meshes only
+  // trace split into trace_fields scalar fields (hdg_mesh_set_trace_fields):
+  // the skeleton numbering at the boundary of the API is field-major; inside,
+  // the facet-major one.  tf_n2o / tf_dn2o: field-major free / Dirichlet
+  // index -> facet-major one; device copies of the renumbered CSC pattern,
+  // the nonzero source map and the buffers the renumbered values go to
+  int trace_fields = 1;
+  std::vector<int64_t> tf_n2o, tf_o2n, tf_dn2o;
+  DevBuf d_tf_colptr, d_tf_rowval, d_tf_src, d_tf_n2o, d_tf_o2n, d_tf_nzval, d_tf_rhs, d_tf_lam,
+      d_tf_cell_dofs;
   DevBuf d_colptr, d_rowval, d_nzval, d_rhs;
   const double* src_qp = nullptr;
   bool condensed = false;
@@ -991,6 +1000,12 @@ int hdg_mesh_set_dirichlet(hdg_mesh* m, const double* lam_dir) {
   if (!m) return HDG_ERR_ARG;
   if (m->ndir > 0 && !lam_dir) return fail(m->ctx, HDG_ERR_ARG, "null Dirichlet values");
   if (m->ndir == 0) return HDG_OK;
+  std::vector<double> facet_major;
+  if (m->trace_fields > 1) {  // field-major values in, facet-major inside
+    facet_major.resize((size_t)m->ndir);
+    for (int64_t i = 0; i < m->ndir; ++i) facet_major[(size_t)m->tf_dn2o[i]] = lam_dir[i];
+    lam_dir = facet_major.data();
+  }
   HIP_TRY(m->ctx, hipSetDevice(m->ctx->device), "hipSetDevice");
   HIP_TRY(m->ctx, hipMemcpyAsync(m->d_lam_dir.p, lam_dir, m->ndir * sizeof(double),
                                  hipMemcpyHostToDevice, m->ctx->stream),
@@ -1066,6 +1081,11 @@ extern "C" {
 int hdg_mesh_set_dirichlet_mms(hdg_mesh* m) {
   if (!m) return HDG_ERR_ARG;
   std::vector<double> vals = trace_values(m, 1, m->ndir);
+  if (m->trace_fields > 1) {  // to the field-major order hdg_mesh_set_dirichlet takes
+    std::vector<double> fm((size_t)std::max<int64_t>(m->ndir, 1));
+    for (int64_t i = 0; i < m->ndir; ++i) fm[(size_t)i] = vals[(size_t)m->tf_dn2o[i]];
+    return hdg_mesh_set_dirichlet(m, fm.data());
+  }
   return hdg_mesh_set_dirichlet(m, vals.data());
 }
 
@@ -1073,7 +1093,91 @@ int hdg_mesh_interpolate_trace(hdg_mesh* m, double* lam_free) {
   if (!m) return HDG_ERR_ARG;
   if (m->nfree > 0 && !lam_free) return fail(m->ctx, HDG_ERR_ARG, "null output");
   std::vector<double> vals = trace_values(m, 0, m->nfree);
-  if (m->nfree > 0) std::memcpy(lam_free, vals.data(), (size_t)m->nfree * sizeof(double));
+  if (m->trace_fields > 1)
+    for (int64_t i = 0; i < m->nfree; ++i) lam_free[i] = vals[(size_t)m->tf_n2o[i]];
+  else if (m->nfree > 0)
+    std::memcpy(lam_free, vals.data(), (size_t)m->nfree * sizeof(double));
+  return HDG_OK;
+}
+
+int hdg_mesh_set_trace_fields(hdg_mesh* m, int nfields) {
+  if (!m) return HDG_ERR_ARG;
+  hdg_ctx* ctx = m->ctx;
+  const int ntf = m->th.ntf, nm = m->th.nm, ncm = m->th.ncm, nb = m->th.nb;
+  if (nfields == 1) {
+    m->trace_fields = 1;
+    for (DevBuf* b : {&m->d_tf_colptr, &m->d_tf_rowval, &m->d_tf_src, &m->d_tf_n2o, &m->d_tf_o2n,
+                      &m->d_tf_nzval, &m->d_tf_rhs, &m->d_tf_lam, &m->d_tf_cell_dofs})
+      (void)b->alloc(0);
+    return HDG_OK;
+  }
+  if (nfields != ncm || ncm == 1)
+    return fail(ctx, HDG_ERR_UNSUPPORTED,
+                "the trace splits into one scalar field per component (" + std::to_string(ncm) +
+                    ") or stays one field");
+  HIP_TRY(ctx, hipSetDevice(ctx->device), "hipSetDevice");
+  // facet-major index G ntf + c nm + j  <->  field-major c (nF nm) + G nm + j
+  const int64_t nff = m->nfree / ntf, ndf = m->ndir / ntf;
+  m->tf_n2o.assign((size_t)m->nfree, 0);
+  m->tf_o2n.assign((size_t)m->nfree, 0);
+  for (int64_t G = 0; G < nff; ++G)
+    for (int c = 0; c < ncm; ++c)
+      for (int j = 0; j < nm; ++j) {
+        const int64_t o = G * ntf + c * nm + j, n = c * nff * nm + G * nm + j;
+        m->tf_n2o[(size_t)n] = o;
+        m->tf_o2n[(size_t)o] = n;
+      }
+  m->tf_dn2o.assign((size_t)m->ndir, 0);
+  std::vector<int64_t> dir_o2n((size_t)m->ndir);
+  for (int64_t G = 0; G < ndf; ++G)
+    for (int c = 0; c < ncm; ++c)
+      for (int j = 0; j < nm; ++j) {
+        const int64_t o = G * ntf + c * nm + j, n = c * ndf * nm + G * nm + j;
+        m->tf_dn2o[(size_t)n] = o;
+        dir_o2n[(size_t)o] = n;
+      }
+  // the CSC pattern renumbered: column n is facet-major column n2o[n], its
+  // rows mapped by o2n and sorted
+  std::vector<int64_t> colptr((size_t)m->nfree + 1);
+  std::vector<int32_t> rowval((size_t)std::max<int64_t>(m->nnz, 1));
+  HIP_TRY(ctx, hipMemcpyAsync(colptr.data(), m->d_colptr.p, colptr.size() * sizeof(int64_t),
+                              hipMemcpyDeviceToHost, ctx->stream), "colptr");
+  HIP_TRY(ctx, hipMemcpyAsync(rowval.data(), m->d_rowval.p, (size_t)m->nnz * sizeof(int32_t),
+                              hipMemcpyDeviceToHost, ctx->stream), "rowval");
+  std::vector<int32_t> cd((size_t)m->ncells * nb);
+  HIP_TRY(ctx, hipMemcpyAsync(cd.data(), m->d_cell_dofs.p, cd.size() * sizeof(int32_t),
+                              hipMemcpyDeviceToHost, ctx->stream), "cell dofs");
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream), "sync");
+  std::vector<int64_t> ncolptr((size_t)m->nfree + 1, 0), src((size_t)std::max<int64_t>(m->nnz, 1));
+  std::vector<int32_t> nrowval((size_t)std::max<int64_t>(m->nnz, 1));
+  std::vector<std::pair<int32_t, int64_t>> col;
+  int64_t k = 0;
+  for (int64_t n = 0; n < m->nfree; ++n) {
+    const int64_t o = m->tf_n2o[(size_t)n];
+    col.clear();
+    for (int64_t t = colptr[(size_t)o]; t < colptr[(size_t)o + 1]; ++t)
+      col.push_back({(int32_t)m->tf_o2n[(size_t)rowval[(size_t)t]], t});
+    std::sort(col.begin(), col.end());
+    for (auto& e : col) {
+      nrowval[(size_t)k] = e.first;
+      src[(size_t)k] = e.second;
+      ++k;
+    }
+    ncolptr[(size_t)n + 1] = k;
+  }
+  for (auto& d : cd)
+    d = d >= 0 ? (int32_t)m->tf_o2n[(size_t)d] : (int32_t)(-dir_o2n[(size_t)(-(int64_t)d - 1)] - 1);
+  // the resident Dirichlet values keep the facet-major order they were set in
+  HIP_TRY(ctx, upload(m->d_tf_colptr, ncolptr, ctx->stream), "upload colptr");
+  HIP_TRY(ctx, upload(m->d_tf_rowval, nrowval, ctx->stream), "upload rowval");
+  HIP_TRY(ctx, upload(m->d_tf_src, src, ctx->stream), "upload nz map");
+  HIP_TRY(ctx, upload(m->d_tf_n2o, m->tf_n2o, ctx->stream), "upload map");
+  HIP_TRY(ctx, upload(m->d_tf_o2n, m->tf_o2n, ctx->stream), "upload map");
+  HIP_TRY(ctx, upload(m->d_tf_cell_dofs, cd, ctx->stream), "upload cell dofs");
+  HIP_TRY(ctx, m->d_tf_nzval.alloc((size_t)std::max<int64_t>(m->nnz, 1) * sizeof(double)), "alloc");
+  HIP_TRY(ctx, m->d_tf_rhs.alloc((size_t)std::max<int64_t>(m->nfree, 1) * sizeof(double)), "alloc");
+  HIP_TRY(ctx, m->d_tf_lam.alloc((size_t)std::max<int64_t>(m->nfree, 1) * sizeof(double)), "alloc");
+  m->trace_fields = nfields;
   return HDG_OK;
 }
 
@@ -1205,6 +1309,21 @@ int hdg_mesh_refinement_stats(hdg_mesh* m, hdg_refinement_stats* st) {
 int hdg_skeleton_system(hdg_mesh* m, const int64_t** colptr, const int32_t** rowval,
                         const double** nzval, const double** rhs) {
   if (!m) return HDG_ERR_ARG;
+  if (m->trace_fields > 1) {
+    if (!m->condensed) return fail(m->ctx, HDG_ERR_STATE, "skeleton system not assembled yet");
+    // the facet-major system renumbered field-major (one gather per call)
+    hdg_ctx* ctx = m->ctx;
+    HIP_TRY(ctx, hipSetDevice(ctx->device), "hipSetDevice");
+    HIP_TRY(ctx, launch_gather(m->d_nzval.as<double>(), m->d_tf_src.as<int64_t>(),
+                               m->d_tf_nzval.as<double>(), m->nnz, ctx->stream), "renumber nzval");
+    HIP_TRY(ctx, launch_gather(m->d_rhs.as<double>(), m->d_tf_n2o.as<int64_t>(),
+                               m->d_tf_rhs.as<double>(), m->nfree, ctx->stream), "renumber rhs");
+    if (colptr) *colptr = m->d_tf_colptr.as<int64_t>();
+    if (rowval) *rowval = m->d_tf_rowval.as<int32_t>();
+    if (nzval) *nzval = m->d_tf_nzval.as<double>();
+    if (rhs) *rhs = m->d_tf_rhs.as<double>();
+    return HDG_OK;
+  }
   if (colptr) *colptr = m->d_colptr.as<int64_t>();
   if (rowval) *rowval = m->d_rowval.as<int32_t>();
   if (nzval) *nzval = m->d_nzval.as<double>();
@@ -1216,7 +1335,8 @@ int hdg_cell_blocks(hdg_mesh* m, const double** S, const double** g, const int32
   if (!m) return HDG_ERR_ARG;
   if (S) *S = m->pS;
   if (g) *g = m->pg;
-  if (cell_dofs) *cell_dofs = m->d_cell_dofs.as<int32_t>();
+  if (cell_dofs)
+    *cell_dofs = (m->trace_fields > 1 ? m->d_tf_cell_dofs : m->d_cell_dofs).as<int32_t>();
   return HDG_OK;
 }
 
@@ -1254,12 +1374,32 @@ int hdg_mesh_set_family(hdg_mesh* m, int family, int grid_per_cu) {
   return HDG_OK;
 }
 
+namespace {
+int recover_facet_major(hdg_mesh* m, const double* lam_free, double* free_values);
+}  // namespace
+
 int hdg_recover(hdg_mesh* m, const double* lam_free, double* free_values) {
   if (!m) return HDG_ERR_ARG;
   hdg_ctx* ctx = m->ctx;
   if (!m->condensed) return fail(ctx, HDG_ERR_STATE, "recover called before condense");
   if ((m->nfree > 0 && !lam_free) || !free_values) return fail(ctx, HDG_ERR_ARG, "null buffer");
   HIP_TRY(ctx, hipSetDevice(ctx->device), "hipSetDevice");
+  if (m->trace_fields > 1 && m->nfree > 0) {
+    // field-major trace in, facet-major to the kernels, field-major out
+    HIP_TRY(ctx, launch_gather(lam_free, m->d_tf_o2n.as<int64_t>(), m->d_tf_lam.as<double>(),
+                               m->nfree, ctx->stream), "renumber trace");
+    const int rc = recover_facet_major(m, m->d_tf_lam.as<double>(), free_values);
+    if (rc) return rc;
+    HIP_TRY(ctx, launch_copy(lam_free, free_values + m->ncells * m->th.ni, m->nfree, ctx->stream),
+            "trace copy");
+    return HDG_OK;
+  }
+  return recover_facet_major(m, lam_free, free_values);
+}
+
+namespace {
+int recover_facet_major(hdg_mesh* m, const double* lam_free, double* free_values) {
+  hdg_ctx* ctx = m->ctx;
   if (m->mixed) {
     HIP_TRY(ctx, launch_recover_mixed(m->tdev, m->meshdev(), m->pY, m->pXinv,
                                       m->d_cell_dofs.as<int32_t>(), lam_free,
@@ -1287,6 +1427,7 @@ int hdg_recover(hdg_mesh* m, const double* lam_free, double* free_values) {
           "recover kernel");
   return HDG_OK;
 }
+}  // namespace
 
 int hdg_cell_systems(hdg_mesh* m, int64_t first, int64_t count, double* A, double* b) {
   if (!m) return HDG_ERR_ARG;
@@ -1315,6 +1456,17 @@ struct hdg_batch {
     const hdg_mesh* lead = nullptr;  // reference tables shared by the members
     DevBuf S, g, U, Y, alpha, srcq, infos, iters, xinv;
     bool mixed = false;  // condensed in mixed precision at the last launch
+    // the class's branch of the batch graph: its own stream (captured after a
+    // fork from the context stream), workspace and join event, so that the
+    // classes' preparation, condensation and assembly run concurrently
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
+    Workspace ws;
+    ~Class() {
+      if (done) (void)hipEventDestroy(done);
+      if (stream) (void)hipStreamDestroy(stream);
+      workspace_release(ws);
+    }
 
     MeshDev cells(int64_t count) const {
       MeshDev M;
@@ -1343,6 +1495,7 @@ struct hdg_batch {
   Workspace ws;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  hipEvent_t fork = nullptr;  // the branches of the graph start after it
   std::vector<const double*> captured_src;  // member sources baked into the graph
   int captured_precision = -1;
 
@@ -1351,6 +1504,10 @@ struct hdg_batch {
     if (graph) (void)hipGraphDestroy(graph);
     exec = nullptr;
     graph = nullptr;
+  }
+  ~hdg_batch() {
+    drop_graph();
+    if (fork) (void)hipEventDestroy(fork);
   }
 };
 
@@ -1387,25 +1544,38 @@ bool class_mixed(const hdg_batch::Class& c, int precision) {
   return precision == HDG_MIXED && mixed_supported(c.info.ni, c.info.nb);
 }
 
+// One branch per class: the context stream forks to the class streams; each
+// branch prepares its member meshes, condenses the class in one launch and
+// assembles its members; the context stream joins every branch.  Captured,
+// the branches become independent paths of the graph.
 hipError_t enqueue_all(hdg_batch* b, int precision, hipStream_t st) {
-  hipError_t e = enqueue_prep(b, st);
+  hipError_t e = hipEventRecord(b->fork, st);
   for (size_t k = 0; e == hipSuccess && k < b->classes.size(); ++k) {
-    const hdg_batch::Class& c = *b->classes[k];
+    hdg_batch::Class& c = *b->classes[k];
+    hipStream_t cs = c.stream;
+    if ((e = hipStreamWaitEvent(cs, b->fork, 0)) != hipSuccess) break;
+    for (int i : c.members)
+      if ((e = launch_prep_cells(b->meshes[i]->tdev, b->meshes[i]->meshdev(), cs)) != hipSuccess)
+        break;
+    if (e != hipSuccess) break;
     if (class_mixed(c, precision)) {
       CondenseOut o = c.out();
       o.U = nullptr;
       o.Xinv = c.xinv.as<float>();
-      e = launch_condense_mixed(c.lead->tdev, c.cells(c.info.ncells), o, st, b->ws, false,
+      e = launch_condense_mixed(c.lead->tdev, c.cells(c.info.ncells), o, cs, c.ws, false,
                                 kMaxRefinementSweeps, c.iters.as<int32_t>());
     } else {
-      e = enqueue_condense(b, c, c.info.ncells, c.info.family, c.info.grid_per_cu, st);
+      e = launch_condense_fused(c.lead->tdev, c.cells(c.info.ncells), c.out(), cs, c.ws, false,
+                                c.info.family, c.info.grid_per_cu);
     }
-  }
-  for (size_t k = 0; e == hipSuccess && k < b->meshes.size(); ++k) {
-    hdg_mesh* m = b->meshes[k];
-    e = launch_assemble(m->pairs(), m->pS, m->pg, m->d_cell_dofs.as<int32_t>(),
-                        m->d_lam_dir.as<double>(), m->d_nzval.as<double>(),
-                        m->d_rhs.as<double>(), st);
+    for (size_t t = 0; e == hipSuccess && t < c.members.size(); ++t) {
+      hdg_mesh* m = b->meshes[c.members[t]];
+      e = launch_assemble(m->pairs(), m->pS, m->pg, m->d_cell_dofs.as<int32_t>(),
+                          m->d_lam_dir.as<double>(), m->d_nzval.as<double>(),
+                          m->d_rhs.as<double>(), cs);
+    }
+    if (e == hipSuccess) e = hipEventRecord(c.done, cs);
+    if (e == hipSuccess) e = hipStreamWaitEvent(st, c.done, 0);
   }
   return e;
 }
@@ -1476,13 +1646,19 @@ int capture(hdg_batch* b, int precision) {
   hdg_ctx* ctx = b->ctx;
   hipStream_t st = ctx->stream;
   b->drop_graph();
-  size_t need = 0;
-  for (auto& c : b->classes)
-    need = std::max(need, class_mixed(*c, precision)
-                              ? mixed_workspace_bytes(c->info.ni, c->info.nb, c->info.ncells)
-                              : condense_workspace_bytes(sys_shape(c->lead->tdev), c->info.family,
-                                                         c->info.ncells));
-  HIP_TRY(ctx, workspace_reserve(b->ws, need), "batch workspace");
+  // every branch has its own workspace (the branches run concurrently)
+  for (auto& c : b->classes) {
+    const size_t need = class_mixed(*c, precision)
+                            ? mixed_workspace_bytes(c->info.ni, c->info.nb, c->info.ncells)
+                            : condense_workspace_bytes(sys_shape(c->lead->tdev), c->info.family,
+                                                       c->info.ncells);
+    HIP_TRY(ctx, workspace_reserve(c->ws, need), "class workspace");
+    if (!c->stream)
+      HIP_TRY(ctx, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "class stream");
+    if (!c->done)
+      HIP_TRY(ctx, hipEventCreateWithFlags(&c->done, hipEventDisableTiming), "class event");
+  }
+  if (!b->fork) HIP_TRY(ctx, hipEventCreateWithFlags(&b->fork, hipEventDisableTiming), "fork event");
   // FP32 inverses kept for the recovery of mixed classes (allocated once)
   for (auto& cp : b->classes) {
     hdg_batch::Class& c = *cp;
@@ -1618,6 +1794,7 @@ int hdg_batch_destroy(hdg_batch* b) {
   (void)hipStreamSynchronize(ctx->stream);
   b->drop_graph();
   workspace_release(b->ws);
+  b->classes.clear();  // class streams, events and workspaces
   int rc = HDG_OK;
   for (hdg_mesh* m : b->meshes) {
     m->batch = nullptr;

@@ -271,6 +271,9 @@ hipError_t launch_recover_mixed(const Tables& T, const MeshDev& M, const double*
                                 const double* lam_free, const double* lam_dir, double* out,
                                 int64_t nfree, hipStream_t st);
 hipError_t launch_symbolic(const PairTable& P, int32_t* rowval, hipStream_t st);
+hipError_t launch_gather(const double* src, const int64_t* idx, double* dst, int64_t n,
+                         hipStream_t st);
+hipError_t launch_copy(const double* src, double* dst, int64_t n, hipStream_t st);
 hipError_t launch_assemble(const PairTable& P, const double* S, const double* g,
                            const int32_t* cell_dofs, const double* lam_dir, double* nzval,
                            double* rhs, hipStream_t st);

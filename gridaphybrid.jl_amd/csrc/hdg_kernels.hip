@@ -855,6 +855,10 @@ __device__ inline int64_t wave_max_i64(int64_t x) {
   const int hi = __builtin_amdgcn_readlane((int)((uint64_t)x >> 32), 63);
   return (int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint64_t)(uint32_t)lo);
 }
+// max over the wavefront of nonnegative doubles (ordered like their bit patterns)
+__device__ inline double wave_max_f64(double x) {
+  return __longlong_as_double(wave_max_i64(__double_as_longlong(x)));
+}
 __device__ inline int wave_min_i32(int x) {
   int y;
   y = __builtin_amdgcn_update_dpp(x, x, 0x121, 0xf, 0xf, false); x = min(x, y);
@@ -1130,11 +1134,11 @@ __global__ void __launch_bounds__(64 * NW)
 constexpr int kLargeMaxN = 512;
 constexpr int kLargeKB = 16;
 
-template <class Gen, bool BACKSUB>
-__global__ void __launch_bounds__(256)
+template <class Gen, bool BACKSUB, int NT>
+__global__ void __launch_bounds__(NT)
     condense_large_kernel(Gen gen, CondenseOut out, int64_t ncells, int ni, int nb,
                           double* __restrict__ work, int ld) {
-  constexpr int NT = 256, KB = kLargeKB, NMAX = kLargeMaxN;
+  constexpr int KB = kLargeKB, NMAX = kLargeMaxN, NW = NT / 64;
   const int n = ni + nb, ncol = n + 1;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   double* M = work + (int64_t)blockIdx.x * ld * (int64_t)ncol;
@@ -1145,8 +1149,8 @@ __global__ void __launch_bounds__(256)
   __shared__ unsigned char s_act[NMAX];
   __shared__ int s_piv[KB];
   __shared__ double s_diag[NMAX];
-  __shared__ int64_t s_wkey[4];
-  __shared__ int s_wpos[4];
+  __shared__ int64_t s_wkey[NW];
+  __shared__ int s_wpos[NW];
   __shared__ int s_info;
   __shared__ typename Gen::Shared s_gen;
 
@@ -1210,9 +1214,9 @@ __global__ void __launch_bounds__(256)
         }
         __syncthreads();
         int64_t m = s_wkey[0];
-        for (int q = 1; q < 4; ++q) m = s_wkey[q] > m ? s_wkey[q] : m;
+        for (int q = 1; q < NW; ++q) m = s_wkey[q] > m ? s_wkey[q] : m;
         int key = INT_MAX;
-        for (int q = 0; q < 4; ++q)
+        for (int q = 0; q < NW; ++q)
           if (s_wkey[q] == m) key = min(key, s_wpos[q]);
         const int p = s_at[key];
         const double pv = s_P[jj][p];
@@ -1878,6 +1882,14 @@ __global__ void copy_kernel(const double* __restrict__ src, double* __restrict__
     dst[t] = src[t];
 }
 
+// dst[i] = src[idx[i]] (renumbering of skeleton values between trace orders)
+__global__ void gather_kernel(const double* __restrict__ src, const int64_t* __restrict__ idx,
+                              double* __restrict__ dst, int64_t n) {
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+       t += (int64_t)gridDim.x * blockDim.x)
+    dst[t] = src[idx[t]];
+}
+
 // --------------------------------------------------------------------------
 // Skeleton assembly (deterministic; every slot sums its <= 2 contributing
 // cells in ascending cell order).  Facet-pair structure: for each free facet
@@ -2023,6 +2035,7 @@ __global__ void cell_systems_kernel(FusedGen<PHYS> gen, int64_t first, int64_t c
 // W (natural interior order) is kept for recovery: x_i = w - W_b lambda.
 // --------------------------------------------------------------------------
 constexpr int kMixedMaxNi = 128;
+constexpr int kRecoverSweeps = 8;  // FP64 refinement sweeps of the mixed recovery, at most
 constexpr int kRecoverMaxNb = 512;
 __host__ __device__ inline int cdiv(int a, int b) { return (a + b - 1) / b; }
 constexpr int64_t kMixedChunkCells = 16384;  // cells per gen / inverse / refine round
@@ -2396,6 +2409,9 @@ constexpr int kLargeGrid = 256 * 2;
 
 // Bytes of the large-system kernel's workspace: one column-major (n+1)-column
 // system per resident workgroup, leading dimension rounded up to even.
+// wavefronts per workgroup of the blocked kernel, measured fastest per size
+static int large_default_waves(int n) { return n > 256 ? 4 : 4; }
+
 static size_t large_bytes(int n, int* ld) {
   const int l = (n + 1) & ~1;
   if (ld) *ld = l;
@@ -2496,15 +2512,23 @@ static hipError_t launch_condense(const Gen& gen, const CondenseOut& out, int64_
     return (int)(ncells < target ? ncells : target);
   };
   if (fam == kFamLarge) {
+    // gridmul selects the wavefronts per workgroup (one workgroup per CU: the
+    // panel and pivot-row LDS take most of it): 4 (default), 8 or 16
     int ld;
     hipError_t e = workspace_reserve(ws, large_bytes(ni + nb, &ld));
     if (e != hipSuccess) return e;
     const int grid = (int)(ncells < kLargeGrid ? ncells : kLargeGrid);
-    {
-      ClockScope clk_(st, "condense_blocked");
-      hipLaunchKernelGGL((condense_large_kernel<Gen, BACKSUB>), dim3(grid), dim3(256), 0, st, gen,
-                       out, ncells, ni, nb, ws.p, ld);
-    }
+    const int waves = gridmul > 0 ? gridmul : large_default_waves(ni + nb);
+    ClockScope clk_(st, "condense_blocked");
+    if (waves == 16)
+      hipLaunchKernelGGL((condense_large_kernel<Gen, BACKSUB, 1024>), dim3(grid), dim3(1024), 0,
+                         st, gen, out, ncells, ni, nb, ws.p, ld);
+    else if (waves == 8)
+      hipLaunchKernelGGL((condense_large_kernel<Gen, BACKSUB, 512>), dim3(grid), dim3(512), 0, st,
+                         gen, out, ncells, ni, nb, ws.p, ld);
+    else
+      hipLaunchKernelGGL((condense_large_kernel<Gen, BACKSUB, 256>), dim3(grid), dim3(256), 0, st,
+                         gen, out, ncells, ni, nb, ws.p, ld);
     return hipGetLastError();
   }
   if (fam == kFamTile && BACKSUB) fam = kFamPanel;
@@ -2605,7 +2629,7 @@ int condense_variants(const SysShape& sh, int* fams, int max) {
 int condense_grid_options(int fam, int* gm, int max) {
   // resident workgroups per CU worth timing for each family (0 = fixed grid)
   static const int wave[] = {16, 32, 48, 64}, grid8[] = {20, 40, 80}, col[] = {1, 2, 3},
-                   g32[] = {2, 3, 4}, pan[] = {1, 2}, til[] = {1}, large[] = {0},
+                   g32[] = {2, 3, 4}, pan[] = {1, 2}, til[] = {1}, large[] = {4, 8, 16},
                    g16[] = {2, 4, 6}, fus[] = {1, 2};
   const int* v = large;
   int n = 1;
@@ -2616,6 +2640,7 @@ int condense_grid_options(int fam, int* gm, int max) {
     case kFamGrid32: v = g32, n = 3; break;
     case kFamPanel: v = pan, n = 2; break;
     case kFamTile: v = til, n = 1; break;
+    case kFamLarge: v = large, n = 3; break;
     case kFamGrid16: v = g16, n = 3; break;
     case kFamFused: v = fus, n = 2; break;
     default: break;
@@ -2724,6 +2749,23 @@ hipError_t launch_condense_fused(const Tables& T, const MeshDev& M, const Conden
   FusedGen<kElasticity> gen{T, M};
   return launch_condense<FusedGen<kElasticity>, false>(gen, out, M.ncells, T.ni, T.nb, st, ws,
                                                        fam, gridmul);
+}
+
+hipError_t launch_gather(const double* src, const int64_t* idx, double* dst, int64_t n,
+                         hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  const int64_t blocks = (n + 255) / 256;
+  ClockScope clk_(st, "renumber");
+  hipLaunchKernelGGL(gather_kernel, dim3((unsigned)(blocks < 4096 ? blocks : 4096)), dim3(256), 0,
+                     st, src, idx, dst, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_copy(const double* src, double* dst, int64_t n, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  ClockScope clk_(st, "recover_copy");
+  hipLaunchKernelGGL(copy_kernel, dim3(1024), dim3(256), 0, st, src, dst, n);
+  return hipGetLastError();
 }
 
 hipError_t launch_condense_dense(const DenseIn& in, int64_t ncells, const CondenseOut& out,
@@ -2864,30 +2906,49 @@ __global__ void __launch_bounds__(256)
       x0[t] = x;
     }
     wave_sync_lds();
-    // residual of the interior equations at [x0; lambda], term by term (FP64)
-#pragma unroll
-    for (int t = 0; t < KMAX; ++t) {
-      const int r = lane + 64 * t;
-      if (r >= ni) continue;
-      double res = s_sh[w].b[r];
-      const int k1 = T.prog_rptr[r + 1];
-      for (int k = T.prog_rptr[r]; k < k1; ++k)
-        res = fma(-(s_sh[w].alpha[T.prog_rid[k]] * T.prog_rval[k]), s_v[w][T.prog_rcol[k]], res);
-      s_r[w][r] = res;
-    }
-    wave_sync_lds();
+    // FP64 refinement sweeps x += X (c - A_ii x) with the FP32 inverse X,
+    // until the correction is below 2^-44 of x, the condensation's own
+    // stopping rule (each sweep contracts the error by about cond(A_ii) times
+    // the FP32 rounding), or kRecoverSweeps
     const float* Xc = Xinv + cell * (int64_t)ni * ni;
+    for (int sweep = 0; sweep < kRecoverSweeps; ++sweep) {
+      // residual of the interior equations at [x; lambda], term by term (FP64)
+#pragma unroll
+      for (int t = 0; t < KMAX; ++t) {
+        const int r = lane + 64 * t;
+        if (r >= ni) continue;
+        double res = s_sh[w].b[r];
+        const int k1 = T.prog_rptr[r + 1];
+        for (int k = T.prog_rptr[r]; k < k1; ++k)
+          res = fma(-(s_sh[w].alpha[T.prog_rid[k]] * T.prog_rval[k]), s_v[w][T.prog_rcol[k]], res);
+        s_r[w][r] = res;
+      }
+      wave_sync_lds();
+      double cmax = 0.0, xmax = 0.0;
+#pragma unroll
+      for (int t = 0; t < KMAX; ++t) {
+        const int r = lane + 64 * t;
+        if (r >= ni) continue;
+        double acc = 0.0;
+        for (int k = 0; k < ni; ++k) acc = fma((double)Xc[(int64_t)r * ni + k], s_r[w][k], acc);
+        x0[t] += acc;
+        s_v[w][r] = x0[t];
+        cmax = fmax(cmax, fabs(acc));
+        xmax = fmax(xmax, fabs(x0[t]));
+      }
+      wave_sync_lds();
+      cmax = wave_max_f64(cmax);
+      xmax = wave_max_f64(xmax);
+      if (!(cmax > 0x1p-44 * xmax)) break;  // uniform across the wavefront
+    }
 #pragma unroll
     for (int t = 0; t < KMAX; ++t) {
       const int r = lane + 64 * t;
       if (r >= ni) continue;
-      double acc = 0.0;
-      for (int k = 0; k < ni; ++k) acc = fma((double)Xc[(int64_t)r * ni + k], s_r[w][k], acc);
-      const double x = x0[t] + acc;
       if (r < na)
-        out[cell * na + r] = x;
+        out[cell * na + r] = x0[t];
       else
-        out[ncells * na + cell * (ni - na) + (r - na)] = x;
+        out[ncells * na + cell * (ni - na) + (r - na)] = x0[t];
     }
     wave_sync_lds();
   }

@@ -32,6 +32,7 @@ __all__ = [
     "CartesianDiscreteModel", "AffineDiscreteModel", "HDGSpaces", "DarcyHDG", "ElasticityHDG",
     "HybridAffineFEOperator", "solve", "HDGError", "SingularError", "FP64", "MIXED",
     "HybridFEOperator", "NLSolver", "FESolver", "FEFunction", "solve_newton",
+    "Scalar2ArrayBlockMap",
 ]
 
 _torch = None
@@ -216,6 +217,23 @@ class BackwardStaticCondensationMap:
 # ----------------------------------------------------------------------------
 # Operator-level API
 # ----------------------------------------------------------------------------
+class Scalar2ArrayBlockMap:
+    """Scalar2ArrayBlockMap (src/Scalar2ArrayBlockMap.jl:1-69): splits a
+    dense (A, b) into the (nblocks x nblocks) block matrix and block vector of
+    block sizes bs.  Works on one cell (A: n x n) or a batch (A: ncells x n x
+    n), on numpy arrays or torch tensors (the blocks are views)."""
+
+    def evaluate(self, Ab, bs):
+        A, b = Ab
+        off = np.concatenate([[0], np.cumsum([int(v) for v in bs])])
+        if off[-1] != A.shape[-1] or off[-1] != A.shape[-2] or off[-1] != b.shape[-1]:
+            raise HDGError(_lib.HDG_ERR_ARG, "block sizes do not add up to the matrix size")
+        Ab_ = [[A[..., off[i]:off[i + 1], off[j]:off[j + 1]] for j in range(len(bs))]
+               for i in range(len(bs))]
+        bb = [b[..., off[i]:off[i + 1]] for i in range(len(bs))]
+        return Ab_, bb
+
+
 @dataclass
 class CartesianDiscreteModel:
     """CartesianDiscreteModel(domain, partition): cells lexicographic, x
@@ -323,9 +341,15 @@ class HybridAffineFEOperator:
 
     def __init__(self, weakform, trial, test, bulk_fields, skeleton_fields, precision=FP64,
                  ctx=None, assemble=True, family=None, grid_per_cu=0):
-        if list(bulk_fields) != [1, 2] or list(skeleton_fields) != [3]:
+        D = trial.model.D
+        ntrace = 1
+        if (list(bulk_fields) == [1, 2] and weakform.physics == ELASTICITY and
+                list(skeleton_fields) == list(range(3, 3 + D))):
+            ntrace = D   # the displacement trace as D scalar trace fields
+        elif list(bulk_fields) != [1, 2] or list(skeleton_fields) != [3]:
             raise HDGError(_lib.HDG_ERR_UNSUPPORTED,
-                           "hybrid operators here hybridize fields [1,2] onto trace field [3]")
+                           "hybrid operators here hybridize fields [1,2] onto trace field [3] "
+                           "(elasticity: or onto its components [3, ..., 2+D])")
         if trial is not test and trial != test:
             raise HDGError(_lib.HDG_ERR_ARG, "trial and test spaces must match")
         self.weakform, self.trial, self.test = weakform, trial, test
@@ -333,6 +357,9 @@ class HybridAffineFEOperator:
         self.precision = precision
         self.ctx = ctx or Context.default()
         self.mesh = _create_mesh(self.ctx, weakform, trial)
+        self.trace_fields = ntrace
+        if ntrace > 1:
+            check(lib().hdg_mesh_set_trace_fields(self.mesh, ntrace), self.ctx.h, "trace_fields")
         self.info = _lib.MeshInfo()
         check(lib().hdg_mesh_get_info(self.mesh, C.byref(self.info)), self.ctx.h, "info")
         self._src_keep = None
@@ -391,6 +418,17 @@ class HybridAffineFEOperator:
                   "d2h")
         _ = torch
         return sp.csc_matrix((nzval, rowval, colptr), shape=(nfree, nfree)), rhs
+
+    def skeleton_blocks(self):
+        """Per-cell local indices of each skeleton field in S and g (cell-local
+        order: facet by facet, each facet's trace dofs component-major): one
+        index array per trace field -- the block sizes the reference passes to
+        Scalar2ArrayBlockMap (src/HybridAffineFEOperators.jl:343-356)."""
+        nb, ntf = self.info.nb, self.info.n_trace_facet
+        ncomp = self.trace_fields
+        nm = ntf // max(ncomp, 1)
+        idx = np.arange(nb).reshape(nb // ntf, ncomp, nm)
+        return [idx[:, c, :].ravel() for c in range(ncomp)]
 
     def cell_blocks(self):
         """(S, g, cell_dofs) device tensors: S (ncells, nb, nb)."""

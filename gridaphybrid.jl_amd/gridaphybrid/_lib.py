@@ -46,7 +46,7 @@ EXPORTS = [
     "hdg_batch_launch", "hdg_batch_condense_assemble", "hdg_batch_recover",
     "hdg_mesh_refinement_stats", "hdg_profile_start", "hdg_profile_stop",
     "hdg_footprint_cartesian", "hdg_mesh_factor_layout", "hdg_mesh_set_family",
-    "hdg_mesh_cell_quadrature", "hdg_static_condensation_noalloc",
+    "hdg_mesh_cell_quadrature", "hdg_static_condensation_noalloc", "hdg_mesh_set_trace_fields",
 ]
 
 BATCH_AUTOTUNE = 1
@@ -166,6 +166,7 @@ def lib():
         L.hdg_mesh_set_dirichlet_mms.argtypes = [vp]
         L.hdg_mesh_set_source.argtypes = [vp, vp]
         L.hdg_mesh_cell_quadrature.argtypes = [vp, vp, vp]
+        L.hdg_mesh_set_trace_fields.argtypes = [vp, i32]
         L.hdg_condense_assemble.argtypes = [vp, i32, C.POINTER(i64)]
         L.hdg_skeleton_system.argtypes = [vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp),
                                           C.POINTER(vp)]

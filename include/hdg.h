@@ -245,6 +245,24 @@ int hdg_mesh_set_dirichlet_mms(hdg_mesh* mesh);
  * hdg_mesh_set_source expects the source (x = x0 + J xi).  Replaces Gridap's
  * CellQuadrature(trian, 2(k+1)) points (src/GridapTmpModifications.jl:13-40). */
 int hdg_mesh_cell_quadrature(const hdg_mesh* mesh, double* points, double* weights);
+/* Multi-field skeleton: the vector-valued trace as `nfields` scalar trace
+ * fields, one per component (elasticity: nfields = dim), i.e. the reference's
+ * HybridAffineFEOperator(weakform, X, Y, bulk_fields, [3, 4, ...]) with the
+ * skeleton MultiFieldFESpace of _setup_fe_spaces_skeleton_system
+ * (src/HybridAffineFEOperators.jl:52-63) and the condensed cell blocks
+ * re-blocked per field by Scalar2ArrayBlockMap (src/Scalar2ArrayBlockMap.jl:
+ * 42-69, _block_skeleton_system_contributions at
+ * src/HybridAffineFEOperators.jl:343-356).  Every skeleton quantity at this
+ * API then uses the multi-field consecutive numbering -- all dofs of field 1
+ * (facet by facet, node order), then field 2, ...: the CSC of
+ * hdg_skeleton_system, the trace of hdg_recover (input and the tail of the
+ * free values), hdg_mesh_set_dirichlet / hdg_mesh_interpolate_trace and the
+ * cell_dofs of hdg_cell_blocks (free ids per field block; Dirichlet ids -1,
+ * -2, ... in the same field-major order).  S and g keep each cell's local
+ * order.  nfields = 1 restores the single trace field.  HDG_ERR_UNSUPPORTED
+ * for other counts. */
+int hdg_mesh_set_trace_fields(hdg_mesh* mesh, int nfields);
+
 /* Source values at cell quadrature points, device [ncells][nq_cell][ncomp]
  * (Darcy: f_1..f_D, g; elasticity: f_1..f_D); used when source == HDG_SOURCE_QP.
  * The pointer is borrowed until the next set or mesh destruction. */

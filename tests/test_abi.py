@@ -90,3 +90,21 @@ def test_newton_solver_options():
     with pytest.raises(gh.HDGError):
         gh.NLSolver(method="trust_region")
     assert gh.FESolver().nls.iterations > 0
+
+
+def test_scalar2arrayblockmap():
+    """test/Scalar2ArrayBlockMapTests.jl:1-19: bs = [8, 16], A rand(24, 24)."""
+    import numpy as np
+    import gridaphybrid as gh
+    bs = [8, 16]
+    rng = np.random.default_rng(0)
+    A = rng.random((24, 24))
+    b = rng.random(24)
+    Ab, bb = gh.Scalar2ArrayBlockMap().evaluate((A, b), bs)
+    assert np.array_equal(Ab[0][0], A[0:8, 0:8])
+    assert np.array_equal(Ab[1][0], A[8:24, 0:8])
+    assert np.array_equal(Ab[0][1], A[0:8, 8:24])
+    assert np.array_equal(Ab[1][1], A[8:24, 8:24])
+    assert np.array_equal(bb[0], b[:8]) and np.array_equal(bb[1], b[8:])
+    with pytest.raises(gh.HDGError):
+        gh.Scalar2ArrayBlockMap().evaluate((A, b), [8, 8])

@@ -18,7 +18,9 @@ more than that.  On identical cell systems (the GPU's element matrices fed to
 the oracle) S and g must agree to 1e-12 (1e-10 for the mixed path), and the
 local solves must be backward stable by LAPACK's own test ratio (dget02):
 ||c - A_ii x|| / (||A_ii|| ||x|| eps) below 30, with the residual formed in
-extended precision.  Their forward distance to the correctly rounded solution
+extended precision.  The mixed path stops its FP64 refinement when the
+correction falls below 2^-44 of the solution (2^9 eps), so its bar is a ratio
+of 1000: a relative residual of 1e-13, far inside its 1e-10 target.  Their forward distance to the correctly rounded solution
 is printed beside the oracle's (both are of the order cond(A_ii) eps).
 """
 import ctypes as C
@@ -169,7 +171,7 @@ def test_cfg3_end_to_end(ctx, oracle_side, precision):
         # the plain FP64 bar is 1e-12, the mixed path's 1e-10 after refinement
         bar = FLOOR if prec == gh.FP64 else 1e-10
         assert same["S"] <= bar and same["g"] <= bar, same
-        assert ratio["gpu"] < 30.0, ratio
+        assert ratio["gpu"] < (30.0 if prec == gh.FP64 else 1000.0), ratio
         assert np.all(np.isfinite(gpu_x))
     finally:
         op.close()
