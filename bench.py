@@ -45,6 +45,57 @@ def f_rec(ni, nb):
     return 2 * ni * nb + 2 * ni * ni
 
 
+def kernel_model(kname, info, D, sweeps, family=None, nqc=0):
+    """Roofline model of one kernel, per cell: (bound, algorithmic quantity,
+    unit).  HBM-bound kernels count the compulsory bytes of their inputs and
+    outputs (8-byte values, 4-byte ids); compute-bound ones the algorithmic
+    flops.  ni, nb: interior and skeleton sizes; na: flux unknowns; nu = ni -
+    na; nr = nu + nb rows of the reduced system; nqc: cell quadrature points."""
+    ni, nb, na = info.ni, info.nb, info.n_flux
+    nu, nr = ni - na, ni - na + nb
+    nc = max(int(info.ncells), 1)
+    if kname == "flux_schur":
+        return "mfma", flux_flops(info, D), "flop"
+    if kname == "mixed_refine":
+        return "mfma", refine_flops(ni, nb, sweeps), "flop"
+    if kname == "prep":
+        # reads the cell's origin, Jacobian, facet signs and coefficient;
+        # writes the 72 cell coefficients and the source at the quadrature points
+        return "hbm", 8 * (D + D * D + 1 + 72 + 4 * nqc) + 2 * D, "byte"
+    if kname == "prep_rhs":
+        # reads the source at the quadrature points; writes b_i
+        return "hbm", 8 * (4 * nqc + ni), "byte"
+    if kname == "flux_ref":
+        # writes R [nr][nr+1] and the kept block A_uu; reads the cell
+        # coefficients (72) and b_i
+        return "hbm", 8 * (nr * (nr + 1) + nu * nu + 72 + ni), "byte"
+    if kname == "reduced_rowlu":
+        # reads R; writes S, g, U (packed), multipliers, pivot order, Y
+        return "hbm", 8 * (nr * (nr + 1) + nb * nb + nb + nu * (nu + 1) // 2 + nu * nu + nu
+                           + (nb + 1) * nu), "byte"
+    if kname == "recover_flux":
+        # reads A_uu, R_uu^-1, U^-1, Y (and, for the MFMA-product family, the
+        # kept A_qu and A_uq), the cell coefficients and b_i, the cell's trace
+        # values and ids; writes (q, u)
+        kept = 0 if family == 10 else 2 * na * nu
+        return "hbm", (8 * (3 * nu * nu + kept + (nb + 1) * nu + 72 + ni + nb + ni)
+                       + 4 * nb), "byte"
+    if kname == "flux_invert":
+        return "hbm", 8 * (nu * (nu + 1) // 2 + nu * nu + nu + 2 * nu * nu), "byte"
+    if kname == "assemble":
+        # reads every S block once; writes the CSC values
+        return "hbm", 8 * nb * nb + 8 * int(info.nnz) / nc, "byte"
+    if kname == "assemble_rhs":
+        return "hbm", 8 * nb + 8 * int(info.nfree) / nc, "byte"
+    if kname == "recover_copy":
+        # copies the free trace values into the free-value vector
+        return "hbm", 16 * int(info.nfree) / nc, "byte"
+    if kname == "recover":
+        # reads U (packed), Y, b_i and the trace values; writes the interior
+        return "hbm", 8 * (ni * (ni + 1) // 2 + (nb + 1) * ni + 2 * ni + nb), "byte"
+    return "mfma", f_cond(ni, nb), "flop"
+
+
 def flux_flops(info, D):
     """Algorithmic flops of one cell in the flux-block kernel: Z = A_qq^-1 G1
     with the block-diagonal (I (x) M^-1) (np terms per entry, C diagonal for
@@ -58,6 +109,8 @@ def flux_flops(info, D):
 
 # the dense elimination family measured fastest for the 3D k=2 system (32x16 grid)
 DENSE_FAMILY = 3
+# the flux-block elimination with per-cell MFMA products (flux_schur)
+SCHUR_FAMILY = 8
 
 
 def parse():
@@ -78,7 +131,7 @@ def parse():
     ap.add_argument("--batch", action="store_true",
                     help="uneven-batch run: 256 seeded meshes in HBM-sized batch graphs")
     ap.add_argument("--nmesh", type=int, default=256)
-    ap.add_argument("--only", choices=("fp64", "fp64_dense", "mixed"), default=None,
+    ap.add_argument("--only", choices=("fp64", "fp64_schur", "fp64_dense", "mixed"), default=None,
                     help="run only this path of the 3D case (no oracle, no 2D line): the "
                          "command the rocprofv3 summaries are taken from")
     ap.add_argument("--physics", choices=("darcy", "elasticity"), default="darcy",
@@ -469,7 +522,7 @@ def only_main(gh, args):
     """One path of one 3D case, timed like the main run (for profiling)."""
     ctx = gh.Context(0)
     case = Case(gh, ctx, 3, args.n3d, args.order, physics=args.physics)
-    fam = DENSE_FAMILY if args.only == "fp64_dense" else None
+    fam = {"fp64_dense": DENSE_FAMILY, "fp64_schur": SCHUR_FAMILY}.get(args.only)
     prec = gh.MIXED if args.only == "mixed" else gh.FP64
     r = case.run(prec, args.steps, args.warmup, fam, args.only)
     info = case.info
@@ -508,11 +561,12 @@ def main():
     case = Case(gh, ctx, 3, args.n3d, args.order)
     sample = None if args.no_cpu else OracleSample(case, args.cpu_seconds)
     res, errs = {}, {}
-    # fp64: the class default (flux-block Schur complement here); fp64_dense:
-    # the getrf-order elimination of the whole interior; mixed: FP32 inverse
-    # + FP64 refinement
-    for name, prec, fam in (("fp64", gh.FP64, None), ("fp64_dense", gh.FP64, DENSE_FAMILY),
-                            ("mixed", gh.MIXED, None)):
+    # fp64: the class default (flux elimination with the reduced system from
+    # the reference decomposition, family 10); fp64_schur: the same elimination
+    # with two MFMA products per cell (family 8); fp64_dense: the getrf-order
+    # elimination of the whole interior; mixed: FP32 inverse + FP64 refinement
+    for name, prec, fam in (("fp64", gh.FP64, None), ("fp64_schur", gh.FP64, SCHUR_FAMILY),
+                            ("fp64_dense", gh.FP64, DENSE_FAMILY), ("mixed", gh.MIXED, None)):
         r = case.run(prec, args.steps, args.warmup, fam, name)
         res[name] = r
         if sample is not None:
@@ -540,27 +594,30 @@ def main():
     launches = top["kernels_ms_per_step"][kname] / per_launch_ms
     cells_per_launch = nc / launches
     traffic = load_traffic().get(f"{kname}@{args.n3d}^3k{args.order}")
-    if kname == "flux_schur":
-        pc = flux_flops(info, 3)
-        achieved = pc * cells_per_launch / (per_launch_ms * 1e-3) / 1e12
-        roof = dict(bound="mfma", achieved=achieved, peak=PEAK_FP64_TFLOPS, unit="TFLOP/s",
-                    frac=achieved / PEAK_FP64_TFLOPS, traffic=traffic, kernel=kname,
-                    per_cell_flops=pc, cells_per_launch=cells_per_launch,
-                    ms_per_launch=per_launch_ms)
-    elif kname == "mixed_refine":
-        flops = refine_flops(ni, nb, sweeps) * cells_per_launch
-        achieved = flops / (per_launch_ms * 1e-3) / 1e12
-        roof = dict(bound="mfma", achieved=achieved, peak=PEAK_FP64_TFLOPS, unit="TFLOP/s",
-                    frac=achieved / PEAK_FP64_TFLOPS, traffic=traffic, kernel=kname,
-                    per_cell_flops=refine_flops(ni, nb, sweeps),
-                    cells_per_launch=cells_per_launch, ms_per_launch=per_launch_ms)
-    else:
-        flops = f_cond(ni, nb) * cells_per_launch
-        achieved = flops / (per_launch_ms * 1e-3) / 1e12
-        roof = dict(bound="mfma", achieved=achieved, peak=PEAK_FP64_TFLOPS, unit="TFLOP/s",
-                    frac=achieved / PEAK_FP64_TFLOPS, traffic=traffic, kernel=kname,
-                    per_cell_flops=f_cond(ni, nb), cells_per_launch=cells_per_launch,
-                    ms_per_launch=per_launch_ms)
+
+    def roofline(run):
+        """Roofline of every kernel of a run, from the live per-launch times."""
+        out = {}
+        for k, t_step in run["kernels_ms_per_step"].items():
+            t_launch = run["kernels_ms_per_launch"][k]
+            cpl = nc / (t_step / t_launch)
+            bound, q, unit = kernel_model(k, info, 3, sweeps, run.get("family"),
+                                          (args.order + 2) ** 3)
+            rate = q * cpl / (t_launch * 1e-3)
+            if bound == "hbm":
+                out[k] = dict(bound="hbm", achieved=rate / 1e9, peak=PEAK_HBM_GBS, unit="GB/s",
+                              frac=rate / 1e9 / PEAK_HBM_GBS, per_cell_bytes=q,
+                              cells_per_launch=cpl, ms_per_launch=t_launch)
+            else:
+                out[k] = dict(bound="mfma", achieved=rate / 1e12, peak=PEAK_FP64_TFLOPS,
+                              unit="TFLOP/s", frac=rate / 1e12 / PEAK_FP64_TFLOPS,
+                              per_cell_flops=q, cells_per_launch=cpl, ms_per_launch=t_launch)
+        return out
+
+    kernel_roofs = {name: roofline(r) for name, r in res.items()}
+    roof = dict(kernel_roofs[best][kname])
+    roof["traffic"] = traffic
+    roof["kernel"] = kname
     setup_s = case.setup_s
     case.op.close()
     del case
@@ -588,18 +645,23 @@ def main():
         "config": {
             "workload": f"3D Darcy HDG {args.n3d}^3 hex k={args.order} Q_k/Q_k/Q_k, "
                         f"{nc} cells, ni={ni}, nb={nb}; step = condense + assemble + recover; "
-                        f"value = fastest path within 1e-10 of the FP64 oracle ({best}); "
-                        f"fp64 = flux-block Schur complement (FP64), fp64_dense = getrf-order "
-                        f"elimination of A_ii (FP64), mixed = FP32 inverse + FP64 refinement",
+                        f"value = fastest path within the oracle's parity bound ({best}); "
+                        f"fp64 = flux-block elimination with the reduced system from the "
+                        f"reference decomposition (FP64), fp64_schur = the same with two MFMA "
+                        f"products per cell, fp64_dense = getrf-order elimination of A_ii "
+                        f"(FP64), mixed = FP32 inverse + FP64 refinement",
             "cells": nc,
             "value_path": best,
         },
         "fp64": {"cells_per_s": fp64["cells_per_s"], "ms_per_step": fp64["ms_per_step"]},
+        "fp64_schur": {"cells_per_s": res["fp64_schur"]["cells_per_s"],
+                       "ms_per_step": res["fp64_schur"]["ms_per_step"]},
         "fp64_dense": {"cells_per_s": dense["cells_per_s"], "ms_per_step": dense["ms_per_step"]},
         "mixed": {"cells_per_s": mixed["cells_per_s"], "ms_per_step": mixed["ms_per_step"]},
         "mixed_over_fp64": mixed["cells_per_s"] / fp64["cells_per_s"],
         "mixed_over_fp64_dense": mixed["cells_per_s"] / dense["cells_per_s"],
         "roofline": roof,
+        "kernel_roofline": kernel_roofs,
         "cpu_baseline": None if sample is None else sample.baseline(),
         "err_vs_oracle_sample": errs,
         "runs": res,

@@ -717,7 +717,7 @@ int hdg_mesh_create(hdg_ctx* ctx, const hdg_problem* prob, const hdg_mesh_desc* 
     // tables: one allocation, doubles first then bytes
     std::vector<const std::vector<double>*> dv = {&th.Mpp, &th.Gpq, &th.Gqp, &th.Fqp, &th.Fqq,
                                                    &th.Fpm, &th.Fqm, &th.Fmm, &th.Bp, &th.Bq,
-                                                   &th.xq, &th.fb_minv};
+                                                   &th.xq, &th.fb_minv, &th.fr_P, &th.fr_G};
     size_t nd = 0;
     for (auto* v : dv) nd += v->size();
     size_t bytes = nd * sizeof(double) + th.prog_val.size() * sizeof(double) +
@@ -799,8 +799,8 @@ int hdg_mesh_create(hdg_ctx* ctx, const hdg_problem* prob, const hdg_mesh_desc* 
     T.nb = th.nb;
     T.n = th.n;
     T.nqc = th.nqc;
-    const double** ptrs[] = {&T.Mpp, &T.Gpq, &T.Gqp, &T.Fqp, &T.Fqq, &T.Fpm,
-                             &T.Fqm, &T.Fmm, &T.Bp,  &T.Bq,  &T.xq, &T.fb_minv};
+    const double** ptrs[] = {&T.Mpp, &T.Gpq, &T.Gqp, &T.Fqp, &T.Fqq, &T.Fpm, &T.Fqm,
+                             &T.Fmm, &T.Bp,  &T.Bq,  &T.xq,  &T.fb_minv, &T.fr_P, &T.fr_G};
     for (size_t i = 0; i < dv.size(); ++i) *ptrs[i] = reinterpret_cast<const double*>(base + offs[i]);
     T.pmask = reinterpret_cast<const uint8_t*>(base + off_pm);
     T.qmask = reinterpret_cast<const uint8_t*>(base + off_qm);
@@ -821,6 +821,8 @@ int hdg_mesh_create(hdg_ctx* ctx, const hdg_problem* prob, const hdg_mesh_desc* 
     T.fb_sym = th.fb_sym;
     std::memcpy(T.fb_cid, th.fb_cid, sizeof(T.fb_cid));
     if (th.fb_minv.empty()) T.fb_minv = nullptr;
+    T.fr_npair = th.fr_npair;
+    if (th.fr_npair == 0) T.fr_P = T.fr_G = nullptr;
   }
   m->x0.assign(desc->x0, desc->x0 + nc * D);
   m->jac.assign(desc->jac, desc->jac + nc * D * D);
@@ -833,6 +835,11 @@ int hdg_mesh_create(hdg_ctx* ctx, const hdg_problem* prob, const hdg_mesh_desc* 
             J[6] * (J[1] * J[5] - J[2] * J[4]);
     }
     if (!(std::fabs(det) > 0.0)) return fail(ctx, HDG_ERR_ARG, "degenerate cell " + std::to_string(c));
+    // the reference flux decomposition assumes positively oriented cells
+    if (det < 0.0) {
+      m->tdev.fr_npair = 0;
+      m->tdev.fr_P = m->tdev.fr_G = nullptr;
+    }
   }
   std::vector<double> coef(desc->coef, desc->coef + nc);
   HIP_TRY(ctx, upload(m->d_x0, m->x0, st), "upload x0");
@@ -1350,7 +1357,8 @@ int hdg_local_factors(hdg_mesh* m, const double** U, const double** Y) {
 int hdg_mesh_factor_layout(hdg_mesh* m, hdg_factor_layout* lay) {
   if (!m || !lay) return HDG_ERR_ARG;
   if (!m->condensed) return fail(m->ctx, HDG_ERR_STATE, "no factors before condensation");
-  const bool flux = !m->mixed && m->used_family == kFamilyFluxSchur;
+  const bool flux = !m->mixed && (m->used_family == kFamilyFluxSchur ||
+                                  m->used_family == kFamilyFluxRef);
   const CellLayout L(m->th);
   lay->rows = flux ? m->th.ni - m->th.na : m->th.ni;
   lay->family = m->mixed ? -1 : m->used_family;
@@ -1408,10 +1416,12 @@ int recover_facet_major(hdg_mesh* m, const double* lam_free, double* free_values
             "mixed recover kernel");
     return HDG_OK;
   }
-  if (m->used_family == kFamilyFluxFused || m->used_family == kFamilyFluxSchur) {
+  if (m->used_family == kFamilyFluxFused || m->used_family == kFamilyFluxSchur ||
+      m->used_family == kFamilyFluxRef) {
     const CellLayout L(m->th);
     HIP_TRY(ctx, launch_recover_flux(m->tdev, m->meshdev(), m->pU, m->pY, (int64_t)L.U,
                                      (int64_t)L.Y, m->used_family == kFamilyFluxFused,
+                                     m->used_family == kFamilyFluxRef,
                                      m->d_cell_dofs.as<int32_t>(), lam_free,
                                      m->d_lam_dir.as<double>(), free_values, m->nfree,
                                      ctx->stream),
@@ -1715,7 +1725,9 @@ int hdg_batch_create(hdg_ctx* ctx, int nmesh, hdg_mesh* const* meshes, int flags
     hdg_mesh* m = b->meshes[i];
     hdg_batch::Class* c = nullptr;
     for (auto& cp : b->classes)
-      if (same_class(cp->lead->prob, m->prob)) c = cp.get();
+      if (same_class(cp->lead->prob, m->prob) &&
+          (cp->lead->tdev.fr_npair != 0) == (m->tdev.fr_npair != 0))
+        c = cp.get();
     if (!c) {
       b->classes.emplace_back(new hdg_batch::Class());
       c = b->classes.back().get();

@@ -26,6 +26,7 @@ void build_program(TablesHost& t);
 int inline_program(TablesHost& t, std::string& err);
 void detect_flux_block(TablesHost& t);
 void detect_flux_sym(TablesHost& t);
+void build_flux_ref(TablesHost& t);
 namespace {
 
 // Gauss-Legendre nodes/weights on [0,1]: Newton on P_n from the asymptotic
@@ -437,6 +438,7 @@ int build_tables(int phys, int D, int space, int kp, int kq, int km, int quad_de
   build_program(t);
   detect_flux_block(t);
   detect_flux_sym(t);
+  build_flux_ref(t);
   return inline_program(t, err);
 }
 
@@ -809,6 +811,80 @@ void detect_flux_sym(TablesHost& t) {
       if (!ok) return;
     }
   t.fb_sym = 1 | (su < 0 ? 2 : 0) | (sb < 0 ? 4 : 0);
+}
+
+// Reference decomposition of the Darcy flux Schur complement (family 10).
+// The program's flux rows (a, i) of [A_qu A_qb] are
+//   A_qu: sum_e alpha[1 + e D + a] Gpq(e, i, j),  alpha[1 + e D + a] = -c_ea,
+//   A_qb: alpha[10 + 3F + a] Fpm(F, i, m),        alpha[10 + 3F + a] = aF nF_a,
+// and on an affine cell with det J > 0 the facet area times normal is the
+// cofactor image of the reference normal, aF nF_a = sum_e c_ea nhat_Fe.  So the
+// row block of component a is sum_e c_ea G_e with the reference matrices
+//   G_e[i][j] = -Gpq(e, i, j) (potential columns),
+//   G_e[i][(F, m)] = nhat_Fe Fpm(F, i, m) (trace columns),
+// and, C being (detJ/kappa) I = alpha_0 I, the flux Schur term
+//   [A_qu A_qb]^T (C^-1 (x) M^-1) [A_qu A_qb]
+//     = (1/alpha_0) sum_(e<=e') (sum_a c_ea c_e'a) P_(e,e').
+// Computed once here in long double.
+void build_flux_ref(TablesHost& t) {
+  t.fr_npair = 0;
+  t.fr_P.clear();
+  t.fr_G.clear();
+  const int D = t.D, np = t.np, nq = t.nq, nm = t.nm, nF = t.nF;
+  if (t.phys != kDarcy || t.fb_nc != D || (t.fb_sym & 1) == 0 || t.fb_minv.empty()) return;
+  // s_u = -1 (the integrated-by-parts u-q block), s_b = +1
+  if ((t.fb_sym & 6) != 2) return;
+  for (int a = 0; a < D; ++a)
+    for (int b = 0; b < D; ++b)
+      if (t.fb_cid[6 * a + b] != (a == b ? 0 : -1)) return;  // C = alpha_0 I
+  const int nr = nq + nF * nm;
+  // the kernel keeps at most three program terms per entry of R0 in registers
+  for (int c = t.na; c < t.n; ++c)
+    for (int r = t.na; r < t.n; ++r) {
+      const size_t e = (size_t)c * t.n + r;
+      if (t.prog_off[e + 1] - t.prog_off[e] > 3) return;
+    }
+  std::vector<long double> G((size_t)D * np * nr, 0.0L);
+  for (int e = 0; e < D; ++e)
+    for (int i = 0; i < np; ++i) {
+      for (int j = 0; j < nq; ++j)
+        G[((size_t)e * np + i) * nr + j] = -(long double)t.Gpq[((size_t)e * np + i) * nq + j];
+      for (int F = 0; F < nF; ++F) {
+        const int fix = D - 1 - F / 2;
+        if (fix != e) continue;
+        const long double nh = (F % 2) ? 1.0L : -1.0L;
+        for (int m = 0; m < nm; ++m)
+          G[((size_t)e * np + i) * nr + nq + F * nm + m] =
+              nh * (long double)t.Fpm[((size_t)F * np + i) * nm + m];
+      }
+    }
+  // H_e = M^-1 G_e
+  std::vector<long double> H((size_t)D * np * nr, 0.0L);
+  for (int e = 0; e < D; ++e)
+    for (int i = 0; i < np; ++i)
+      for (int c = 0; c < nr; ++c) {
+        long double v = 0.0L;
+        for (int k = 0; k < np; ++k)
+          v += (long double)t.fb_minv[(size_t)i * np + k] * G[((size_t)e * np + k) * nr + c];
+        H[((size_t)e * np + i) * nr + c] = v;
+      }
+  const int npair = D * (D + 1) / 2;
+  t.fr_P.assign((size_t)npair * nr * nr, 0.0);
+  int p = 0;
+  for (int e = 0; e < D; ++e)
+    for (int e2 = e; e2 < D; ++e2, ++p)
+      for (int r = 0; r < nr; ++r)
+        for (int c = 0; c < nr; ++c) {
+          long double v = 0.0L;
+          for (int i = 0; i < np; ++i) {
+            v += G[((size_t)e * np + i) * nr + r] * H[((size_t)e2 * np + i) * nr + c];
+            if (e2 != e) v += G[((size_t)e2 * np + i) * nr + r] * H[((size_t)e * np + i) * nr + c];
+          }
+          t.fr_P[((size_t)p * nr + r) * nr + c] = (double)v;
+        }
+  t.fr_G.resize(G.size());
+  for (size_t k = 0; k < G.size(); ++k) t.fr_G[k] = (double)G[k];
+  t.fr_npair = npair;
 }
 
 }  // namespace hdg

@@ -70,6 +70,16 @@ struct TablesHost {
   // A_uq = s_u A_qu^T and A_bq = s_b A_qb^T entry by entry, bit 1: s_u = -1,
   // bit 2: s_b = -1.
   int fb_sym = 0;
+  // Reference decomposition of the flux Schur complement for Darcy on affine
+  // cells (build_flux_ref): with c_ea = detJ Ji(e, a) the cofactor entries of
+  // the cell, [A_qu A_qb] row block of component a = sum_e c_ea G_e, so
+  //   [A_qu A_qb]^T (C^-1 (x) M^-1) [A_qu A_qb] = (1 / alpha_0) sum_p w_p P_p,
+  // w over the pairs (e, e') of sum_a c_ea c_e'a and P_(e,e) = G_e^T M^-1 G_e,
+  // P_(e,e') = G_e^T M^-1 G_e' + its transpose.  fr_npair = 0 when it does not
+  // apply (not Darcy, or no symmetric scalar flux block).
+  int fr_npair = 0;
+  std::vector<double> fr_P;  // [npair][nr][nr] row-major, nr = nq + nb
+  std::vector<double> fr_G;  // [D][np][nr]  G_e
 };
 
 // Device view of the tables (pointers into one device allocation).
@@ -108,6 +118,9 @@ struct Tables {
   int8_t fb_cid[36];      // coefficient id of C[a][b] (< 0: zero)
   const double* fb_minv;  // [np][np] inverse of the scalar flux mass matrix
   int fb_sym;             // transpose symmetry of the flux coupling (TablesHost::fb_sym)
+  int fr_npair;           // reference flux Schur decomposition (TablesHost::fr_*), 0: none
+  const double* fr_P;     // [npair][nr][nr]
+  const double* fr_G;     // [D][np][nr]
 };
 
 // Device view of a mesh for the fused element generator.
@@ -213,6 +226,7 @@ struct SysShape {
   int flux_na = 0;
   bool flux = false;
   bool fused = false;  // the fused flux-block kernel fits (family 9)
+  bool ref = false;    // the reference decomposition applies (family 10)
 };
 SysShape sys_shape(const Tables& T);
 SysShape sys_shape(const TablesHost& t);
@@ -231,6 +245,10 @@ constexpr int kFamilyFluxSchur = 8;
 // rows of W = A_ii^-1 [A_ib b_i] of those unknowns are kept in Y (natural
 // order) with R_uu^-1 and the blocks A_uu, A_qu, A_uq in the factor slot.
 constexpr int kFamilyFluxFused = 9;
+// Family 10: the flux elimination of family 8 with the reduced system formed
+// from the reference decomposition (TablesHost::fr_*) instead of two MFMA
+// products per cell (Darcy on cells with det J > 0).
+constexpr int kFamilyFluxRef = 10;
 // kernel families able to condense a class; returns their count
 int condense_variants(const SysShape& sh, int* fams, int max);
 int condense_default_family(const SysShape& sh);
@@ -238,7 +256,7 @@ int condense_default_family(const SysShape& sh);
 int condense_grid_options(int fam, int* gm, int max);
 size_t condense_workspace_bytes(const SysShape& sh, int fam, int64_t ncells);
 hipError_t launch_condense_flux(const Tables& T, const MeshDev& M, const CondenseOut& out,
-                                hipStream_t st, Workspace& ws, int gridmul);
+                                hipStream_t st, Workspace& ws, int gridmul, bool ref);
 hipError_t launch_condense_fused_flux(const Tables& T, const MeshDev& M, const CondenseOut& out,
                                       hipStream_t st, int gridmul);
 // Recovery after a flux-block condensation (families 8 and 9; fused selects
@@ -247,7 +265,7 @@ hipError_t launch_condense_fused_flux(const Tables& T, const MeshDev& M, const C
 // FP64 refinement against the residual of the full interior system (the
 // reduced system is worse conditioned than A_ii).
 hipError_t launch_recover_flux(const Tables& T, const MeshDev& M, const double* U,
-                               const double* Y, int64_t usz, int64_t ysz, bool fused,
+                               const double* Y, int64_t usz, int64_t ysz, bool fused, bool ref,
                                const int32_t* cell_dofs, const double* lam_free,
                                const double* lam_dir, double* out, int64_t nfree,
                                hipStream_t st);
@@ -308,6 +326,7 @@ void clock_start();
 int clock_stop(std::vector<KernelTime>& out);  // returns the number of failed timings
 
 // Host helpers (hdg_fe.cpp)
+void build_flux_ref(TablesHost& t);
 // proj (elasticity): the tau (w, P_m u) facet block is tau (P_m w, P_m u), i.e.
 // Fqq[F] = Fqm[F] Fmm^-1 Fqm[F]^T (every facet integral scales with the same area)
 int build_tables(int phys, int D, int space, int kp, int kq, int km, int quad_degree, int proj,

@@ -2410,7 +2410,7 @@ constexpr int kLargeGrid = 256 * 2;
 // Bytes of the large-system kernel's workspace: one column-major (n+1)-column
 // system per resident workgroup, leading dimension rounded up to even.
 // wavefronts per workgroup of the blocked kernel, measured fastest per size
-static int large_default_waves(int n) { return n > 256 ? 4 : 4; }
+static int large_default_waves(int n) { return n > 256 ? 16 : 8; }
 
 static size_t large_bytes(int n, int* ld) {
   const int l = (n + 1) & ~1;
@@ -2465,7 +2465,8 @@ enum Family {
   kFamGrid16 = 7,
   kFamFlux = kFamilyFluxSchur,
   kFamFused = kFamilyFluxFused,
-  kFamCount = 10
+  kFamRef = kFamilyFluxRef,
+  kFamCount = 11
 };
 
 static bool wave_class(int ni, int nb) {
@@ -2621,7 +2622,8 @@ static hipError_t launch_condense(const Gen& gen, const CondenseOut& out, int64_
 int condense_variants(const SysShape& sh, int* fams, int max) {
   int k = 0;
   for (int f = 0; f < kFamCount && k < max; ++f)
-    if (family_ok(f, sh.ni, sh.nb) || (f == kFamFlux && sh.flux) || (f == kFamFused && sh.fused))
+    if (family_ok(f, sh.ni, sh.nb) || (f == kFamFlux && sh.flux) || (f == kFamFused && sh.fused) ||
+        (f == kFamRef && sh.ref))
       fams[k++] = f;
   return k;
 }
@@ -2630,8 +2632,8 @@ int condense_grid_options(int fam, int* gm, int max) {
   // resident workgroups per CU worth timing for each family (0 = fixed grid)
   static const int wave[] = {16, 32, 48, 64}, grid8[] = {20, 40, 80}, col[] = {1, 2, 3},
                    g32[] = {2, 3, 4}, pan[] = {1, 2}, til[] = {1}, large[] = {4, 8, 16},
-                   g16[] = {2, 4, 6}, fus[] = {1, 2};
-  const int* v = large;
+                   g16[] = {2, 4, 6}, fus[] = {1, 2}, fixed[] = {0};
+  const int* v = fixed;
   int n = 1;
   switch (fam) {
     case kFamWave: v = wave, n = 4; break;
@@ -2657,14 +2659,14 @@ int condense_default_family(const SysShape& sh) {
   // an 8x8 grid per cell), which no two-stage scheme beats
   // (the fused variant, family 9, is an autotuner candidate: its one-wavefront
   // Gauss-Jordan steps make it slower than the reduced grid elimination here)
-  if (sh.flux && !wave_class(sh.ni, sh.nb) && sh.ni + sh.nb > 96) return kFamFlux;
+  if (sh.flux && !wave_class(sh.ni, sh.nb) && sh.ni + sh.nb > 96) return sh.ref ? kFamRef : kFamFlux;
   return default_family(sh.ni, sh.nb);
 }
 
 size_t condense_workspace_bytes(const SysShape& sh, int fam, int64_t ncells) {
   const int ni = sh.ni, nb = sh.nb;
   if (fam < 0) fam = condense_default_family(sh);
-  if (fam == kFamFlux) return flux_workspace_bytes(ni - sh.flux_na + nb, ncells);
+  if (fam == kFamFlux || fam == kFamRef) return flux_workspace_bytes(ni - sh.flux_na + nb, ncells);
   if (fam == kFamFused) return 0;
   if (fam == kFamPanel) {
     const int n = ni + nb;
@@ -2739,7 +2741,8 @@ hipError_t launch_condense_fused(const Tables& T, const MeshDev& M, const Conden
     if (e != hipSuccess) return e;
   }
   if (fam < 0) fam = condense_default_family(sys_shape(T));
-  if (fam == kFamFlux) return launch_condense_flux(T, M, out, st, ws, gridmul);
+  if (fam == kFamFlux) return launch_condense_flux(T, M, out, st, ws, gridmul, false);
+  if (fam == kFamRef) return launch_condense_flux(T, M, out, st, ws, gridmul, true);
   if (fam == kFamFused) return launch_condense_fused_flux(T, M, out, st, gridmul);
   if (T.phys == kDarcy) {
     FusedGen<kDarcy> gen{T, M};
@@ -3321,6 +3324,224 @@ __global__ void __launch_bounds__(64 * kFluxWaves, 3)  // two workgroups per CU
     if (pend_rr >= 0) store_pending();
     __syncthreads();  // LDS reuse by the next cell
   }
+}
+
+// --------------------------------------------------------------------------
+// Darcy flux elimination from the reference decomposition (family 10).  With
+// C = alpha_0 I and the flux coupling [A_qu A_qb] = sum_e c_ea G_e per flux
+// component a (build_flux_ref), the reduced system of flux_schur is
+//   R[:, :nr] = R0 - s_r (1/alpha_0) sum_p w_p P_p,  w_(e,e') = sum_a c_ea c_e'a
+//   R[:, nr]  = r0 - s_r (1/alpha_0) sum_e G_e^T (sum_a c_ea M^-1 b_q,a)
+// (s_r = -1 on the potential rows, +1 on the trace rows: A_uq = -A_qu^T,
+// A_bq = A_qb^T).  No per-cell product of the flux block remains.  A
+// workgroup owns a band of BR rows of R and runs over the cells of the chunk;
+// each thread owns up to kRefEntries entries (row, column) of the band and
+// keeps, for all cells, their values of every P_p and their entry-program
+// terms (R0 has at most kRefTerms terms per entry) in registers, so a cell
+// costs the cell's coefficients, a few small products and one store per
+// entry.  Of the blocks kept for the recovery only A_uu is written (the
+// recovery applies A_qu and A_uq from the reference matrices as well).
+// --------------------------------------------------------------------------
+constexpr int kRefThreads = 256;
+constexpr int kRefEntries = 5;  // entries of R per thread
+constexpr int kRefTerms = 3;    // entry-program terms per entry of R0 kept in registers
+constexpr int kRefSplit = 16;   // threads per row of the rhs column's sum
+
+struct RefPlan {
+  int nr, np, na, nu, BR, nbands, npair;
+  size_t lds;  // dynamic LDS bytes
+};
+
+static bool ref_plan(const Tables& T, RefPlan* rp) {
+  if (T.phys != kDarcy || T.fr_npair <= 0 || !T.fr_P || !T.fr_G || !T.fb_minv) return false;
+  if (T.ni > kRefThreads) return false;  // one right-hand-side value per thread
+  const int D = T.D, np = T.np, na = T.na, nu = T.ni - na, nr = nu + T.nb;
+  int BR = std::min(nr, kRefEntries * kRefThreads / (nr + 1));
+  BR = std::min(BR, kRefThreads / kRefSplit);  // the rhs column: kRefSplit threads per row
+  if (BR < 1) return false;
+  const int nbands = (nr + BR - 1) / BR;
+  BR = (nr + nbands - 1) / nbands;  // balance the bands
+  // alpha, b_i, then M^-1 (prologue only) aliased with the rhs column, and
+  // the band's columns of every H_e = M^-1 G_e
+  const size_t lds = (size_t)kMaxAlpha + T.ni + std::max<size_t>((size_t)np * np, BR) +
+                     (size_t)D * np * BR;
+  if (lds * sizeof(double) > 64 * 1024) return false;
+  if (rp) {
+    rp->nr = nr;
+    rp->np = np;
+    rp->na = na;
+    rp->nu = nu;
+    rp->BR = BR;
+    rp->nbands = nbands;
+    rp->npair = T.fr_npair;
+    rp->lds = lds * sizeof(double);
+  }
+  return true;
+}
+
+template <int D>
+__global__ void __launch_bounds__(kRefThreads, 2)
+    flux_ref_kernel(FusedGen<kDarcy> gen, int64_t first, int64_t count, double* __restrict__ R,
+                    RefPlan rp, double* __restrict__ blocks, int64_t bstride) {
+  constexpr int NT = kRefThreads, NE = kRefEntries, NK = kRefTerms, NPAIR = D * (D + 1) / 2;
+  constexpr int SPL = kRefSplit;
+  const Tables& T = gen.T;
+  const int nr = rp.nr, nc = nr + 1, np = rp.np, na = rp.na, nu = rp.nu, BR = rp.BR;
+  const int ni = T.ni, n = T.n, nk = D * np;
+  const int tid = threadIdx.x;
+  const int band = (int)(blockIdx.x % rp.nbands);
+  const int64_t group = blockIdx.x / rp.nbands, ngroups = gridDim.x / rp.nbands;
+  const int r0 = band * BR, rows = min(BR, nr - r0);
+  extern __shared__ double s_dyn[];
+  double* s_al = s_dyn;                      // [kMaxAlpha]
+  double* s_b = s_al + kMaxAlpha;            // [ni] interior right-hand side
+  double* s_minv = s_b + ni;                 // [np][np], prologue only
+  double* s_xr = s_minv;                     // [BR] rhs column of the Schur term
+  double* s_H = s_minv + max(np * np, BR);   // [BR][D * np]: H_e[j][r0 + rl] at rl * nk + e * np + j
+  // ---- per-thread entries: P values and R0's program terms, for all cells ----
+  double pv[NE][NPAIR], tv[NE][NK];
+  int tid8[NE];  // term coefficient ids, 8 bits each; 0xff: no term
+#pragma unroll
+  for (int u = 0; u < NE; ++u) {
+    const int t = tid + NT * u;
+    const bool ok = t < rows * nc;
+    const int c = ok ? t / rows : 0, rl = ok ? t - c * rows : 0;
+    const int r = r0 + rl;
+#pragma unroll
+    for (int p = 0; p < NPAIR; ++p)
+      pv[u][p] = (ok && c < nr) ? T.fr_P[((size_t)p * nr + r) * nr + c] : 0.0;
+    int ids = 0xffffff;
+#pragma unroll
+    for (int k = 0; k < NK; ++k) tv[u][k] = 0.0;
+    if (ok && c < nr) {
+      const int e = (na + c) * n + na + r;
+      const uint32_t h = T.prog_head[e];
+      const int cnt = (h >> 8) & 15, x0 = (int)(h >> 12);
+      if (cnt >= 1) {
+        ids = (ids & ~0xff) | (int)(h & 255);
+        tv[u][0] = T.prog_val0[e];
+      }
+#pragma unroll
+      for (int k = 1; k < NK; ++k)
+        if (cnt > k) {
+          ids = (ids & ~(0xff << (8 * k))) | ((int)T.prog_xid[x0 + k - 1] << (8 * k));
+          tv[u][k] = T.prog_xval[x0 + k - 1];
+        }
+    }
+    tid8[u] = ids;
+  }
+  // H_e = M^-1 G_e on the band's columns (M^-1 is symmetric, so the rhs
+  // column's G_e^T M^-1 z_e is H_e^T z_e)
+  for (int t = tid; t < np * np; t += NT) s_minv[t] = T.fb_minv[t];
+  __syncthreads();
+  for (int t = tid; t < nk * BR; t += NT) {
+    const int rl = t / nk, k = t - rl * nk, e = k / np, j = k - e * np;
+    double h = 0.0;
+    if (rl < rows)
+      for (int i = 0; i < np; ++i)
+        h = fma(s_minv[j * np + i], T.fr_G[((size_t)e * np + i) * nr + r0 + rl], h);
+    s_H[t] = h;
+  }
+  __syncthreads();  // s_minv is reused as s_xr
+  // the cell's coefficients and right-hand side are loaded one cell ahead
+  // (a thread holds at most one of each: kMaxAlpha, ni <= NT)
+  auto fetch = [&](int64_t lc, double& al, double& bv) {
+    const int64_t cell = first + lc;
+    al = (lc < count && tid < kMaxAlpha) ? gen.M.alpha[cell * kMaxAlpha + tid] : 0.0;
+    bv = (lc < count && tid < ni) ? gen.M.srcq[cell * srcq_stride(T) + T.nqc * 4 + tid] : 0.0;
+  };
+  // rhs column: thread (rl, part) sums the terms k = part, part + SPL, ... of
+  // sum_k H[k][rl] z[k], z_e[j] = sum_a c_ea b_q,a[j]; the SPL partial sums of
+  // a row sit in consecutive lanes of one wavefront
+  const int xrl = tid / SPL, xpart = tid - xrl * SPL;
+  double nal, nbv;
+  fetch(group, nal, nbv);
+  for (int64_t lc = group; lc < count; lc += ngroups) {
+    if (tid < kMaxAlpha) s_al[tid] = nal;
+    if (tid < ni) s_b[tid] = nbv;
+    fetch(lc + ngroups, nal, nbv);
+    __syncthreads();
+    // C = alpha_0 I, c_ea = -alpha[1 + e * D + a]; w_p = sum_a c_ea c_e'a / alpha_0
+    double c[D * D];
+#pragma unroll
+    for (int t = 0; t < D * D; ++t) c[t] = -s_al[1 + t];
+    const double a0 = s_al[0];
+    const bool bad = !(a0 != 0.0);  // a singular C leaves a zero system
+    const double inva = bad ? 0.0 : 1.0 / a0;
+    if (xrl < rows) {
+      double x = 0.0;
+      for (int k = xpart; k < nk; k += SPL) {
+        const int e = k / np, j = k - e * np;
+        double z = 0.0;
+#pragma unroll
+        for (int a = 0; a < D; ++a) z = fma(c[e * D + a], s_b[a * np + j], z);
+        x = fma(s_H[xrl * nk + k], z, x);
+      }
+#pragma unroll
+      for (int off = SPL / 2; off >= 1; off >>= 1) x += __shfl_xor(x, off, SPL);
+      if (xpart == 0) s_xr[xrl] = x * inva;
+    }  // (a row group's SPL lanes take the same branch)
+    double w[NPAIR];
+    {
+      int p = 0;
+#pragma unroll
+      for (int e = 0; e < D; ++e)
+#pragma unroll
+        for (int e2 = e; e2 < D; ++e2, ++p) {
+          double v = 0.0;
+#pragma unroll
+          for (int a = 0; a < D; ++a) v = fma(c[e * D + a], c[e2 * D + a], v);
+          w[p] = v * inva;
+        }
+    }
+    __syncthreads();
+    double* Rc = R + lc * (int64_t)nc * nr;  // [nc][nr] column-major
+    double* Bk = blocks + lc * bstride;      // [A_uu] (A_qu, A_uq: not used by the recovery)
+#pragma unroll
+    for (int u = 0; u < NE; ++u) {
+      const int t = tid + NT * u;
+      if (t >= rows * nc) continue;
+      const int cc = t / rows, rl = t - cc * rows, r = r0 + rl;
+      double r0v, x;
+      if (cc < nr) {  // the entry program's terms, in its order
+        r0v = 0.0;
+#pragma unroll
+        for (int k = 0; k < NK; ++k) {
+          const int id = (tid8[u] >> (8 * k)) & 0xff;
+          if (id != 0xff) r0v = fma(s_al[id], tv[u][k], r0v);
+        }
+        x = 0.0;
+#pragma unroll
+        for (int p = 0; p < NPAIR; ++p) x = fma(w[p], pv[u][p], x);
+      } else {  // the trace rows carry no source (the cell system's rhs is [b_i; 0])
+        r0v = r < nu ? s_b[na + r] : 0.0;
+        x = s_xr[rl];
+      }
+      // s_r = -1 on the potential rows, +1 on the trace rows
+      Rc[(int64_t)cc * nr + r] = bad ? 0.0 : (r < nu ? r0v + x : r0v - x);
+      if (r < nu && cc < nu) Bk[(int64_t)cc * nu + r] = r0v;  // A_uu
+    }
+    __syncthreads();  // LDS reuse by the next cell
+  }
+}
+
+static hipError_t launch_flux_ref(const Tables& T, const MeshDev& M, int64_t first, int64_t count,
+                                  double* R, const RefPlan& rp, double* blocks, int64_t bstride,
+                                  hipStream_t st) {
+  if (count <= 0) return hipSuccess;
+  // about four workgroups per CU over all bands
+  const int64_t groups = std::max<int64_t>(
+      1, std::min<int64_t>(count, (4 * 256 * 2 + rp.nbands - 1) / rp.nbands));
+  const int grid = (int)(groups * rp.nbands);
+  FusedGen<kDarcy> gen{T, M};
+  ClockScope clk_(st, "flux_ref");
+  if (T.D == 3)
+    hipLaunchKernelGGL(flux_ref_kernel<3>, dim3(grid), dim3(kRefThreads), rp.lds, st, gen, first,
+                       count, R, rp, blocks, bstride);
+  else
+    hipLaunchKernelGGL(flux_ref_kernel<2>, dim3(grid), dim3(kRefThreads), rp.lds, st, gen, first,
+                       count, R, rp, blocks, bstride);
+  return hipGetLastError();
 }
 
 // The reduced augmented system R of a chunk of cells as an element generator
@@ -4139,10 +4360,12 @@ static size_t flux_workspace_bytes(int nr, int64_t ncells) {
 
 template <int PHYS>
 static hipError_t launch_flux_phys(const Tables& T, const MeshDev& M, const CondenseOut& out,
-                                   hipStream_t st, Workspace& ws, int gridmul) {
+                                   hipStream_t st, Workspace& ws, int gridmul, bool ref) {
   FluxPlan fp;
   int ct = 0, ksm = 0;
   if (!flux_plan(T, fp, &ct, &ksm)) return hipErrorInvalidValue;
+  RefPlan rp;
+  if (ref && !ref_plan(T, &rp)) return hipErrorInvalidValue;
   const int nu = T.ni - T.na, nb = T.nb, nr = fp.nr;
   const int rfam = reduced_family(nu, nb);
   if (rfam < 0) return hipErrorInvalidValue;
@@ -4163,7 +4386,10 @@ static hipError_t launch_flux_phys(const Tables& T, const MeshDev& M, const Cond
   for (int64_t first = 0; first < ncells; first += chunk) {
     const int64_t count = std::min<int64_t>(chunk, ncells - first);
     const int grid = (int)std::min<int64_t>(count, 256 * 4);
-    {
+    if (ref) {
+      e = launch_flux_ref(T, M, first, count, ws.p, rp, out.U + first * usz + boff, usz, st);
+      if (e != hipSuccess) return e;
+    } else {
       ClockScope clk_(st, "flux_schur");
 #define HDG_FLUX(CT_, KS_)                                                                   \
   if (ct == CT_ && ksm == KS_) {                                                             \
@@ -4203,9 +4429,10 @@ static hipError_t launch_flux_phys(const Tables& T, const MeshDev& M, const Cond
 }
 
 hipError_t launch_condense_flux(const Tables& T, const MeshDev& M, const CondenseOut& out,
-                                hipStream_t st, Workspace& ws, int gridmul) {
-  return T.phys == kDarcy ? launch_flux_phys<kDarcy>(T, M, out, st, ws, gridmul)
-                          : launch_flux_phys<kElasticity>(T, M, out, st, ws, gridmul);
+                                hipStream_t st, Workspace& ws, int gridmul, bool ref) {
+  if (ref && (T.phys != kDarcy || T.fr_npair == 0)) return hipErrorInvalidValue;
+  return T.phys == kDarcy ? launch_flux_phys<kDarcy>(T, M, out, st, ws, gridmul, ref)
+                          : launch_flux_phys<kElasticity>(T, M, out, st, ws, gridmul, false);
 }
 
 size_t factor_slot(const SysShape& sh) {
@@ -4224,6 +4451,7 @@ SysShape sys_shape(const Tables& T) {
   if (T.fb_minv && flux_fits(T.fb_nc, T.na, T.np, T.ni, T.nb, nullptr, nullptr)) sh.flux_na = T.na;
   sh.flux = sh.flux_na > 0;
   sh.fused = sh.flux && fused_fits(T.fb_nc, T.na, T.np, T.ni, T.nb, nullptr, nullptr, nullptr);
+  sh.ref = sh.flux && T.fr_npair > 0 && ref_plan(T, nullptr);
   return sh;
 }
 
@@ -4235,6 +4463,7 @@ SysShape sys_shape(const TablesHost& t) {
     sh.flux_na = t.na;
   sh.flux = sh.flux_na > 0;
   sh.fused = sh.flux && fused_fits(t.fb_nc, t.na, t.np, t.ni, t.nb, nullptr, nullptr, nullptr);
+  sh.ref = sh.flux && t.fr_npair > 0;
   return sh;
 }
 
@@ -4262,9 +4491,10 @@ SysShape sys_shape(const TablesHost& t) {
 constexpr int kFusedMaxNb = 128;  // boundary values held in LDS by the flux recovery
 constexpr int kRecWaves = 4;      // cells (wavefronts) per workgroup of the flux recovery
 
-// doubles of LDS per wavefront of the flux recovery
+// doubles of LDS per wavefront of the flux recovery (the last na: the
+// reference variant's work vector)
 __host__ __device__ inline int rec_wave_lds(int na, int nu, int nb) {
-  return kMaxAlpha + 72 + nb + 4 * na + (na > 72 ? na : 72) + 2 * nu;
+  return kMaxAlpha + 72 + nb + 5 * na + (na > 72 ? na : 72) + 2 * nu;
 }
 
 struct RecSlot {
@@ -4277,19 +4507,26 @@ struct RecSlot {
   int wld;
 };
 
-template <int PHYS, int KQ>
+// REF (family 10): A_qu and A_uq are not read from the slot but applied from
+// the reference matrices G_e (potential columns, staged in LDS once per
+// workgroup) and the cell's cofactors c_ea: A_qu[(a,i)][c] = sum_e c_ea
+// G_e[i][c] and A_uq = -A_qu^T (the Darcy coupling of build_flux_ref).
+template <int PHYS, int KQ, bool REF>
 __global__ void __launch_bounds__(256)
     recover_flux_kernel(FusedGen<PHYS> gen, RecSlot rs, const int32_t* __restrict__ cell_dofs,
                         const double* __restrict__ lam_free, const double* __restrict__ lam_dir,
                         double* __restrict__ out) {
   constexpr int WPB = kRecWaves;
-  extern __shared__ double s_m[];  // M^-1 [np][np], M, then the wavefronts' vectors
+  // M^-1 [np][np], M, (REF: G_e [D][np][nu]), then the wavefronts' vectors
+  extern __shared__ double s_m[];
   const Tables& T = gen.T;
   const int ni = T.ni, nb = T.nb, na = T.na, np = T.np, nu = ni - na, nqc = T.nqc;
-  const int ncomp = T.fb_nc;
+  const int ncomp = T.fb_nc, D = T.D;
   const int64_t ncells = gen.M.ncells;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  double* alpha = s_m + 2 * np * np + (int64_t)w * rec_wave_lds(na, nu, nb);  // [kMaxAlpha]
+  const int gsz = REF ? D * np * nu : 0;
+  double* s_gu = s_m + 2 * np * np;  // REF: G_e[i][c] at (e * np + i) * nu + c
+  double* alpha = s_m + 2 * np * np + gsz + (int64_t)w * rec_wave_lds(na, nu, nb);  // [kMaxAlpha]
   double* c6 = alpha + kMaxAlpha;  // C, then C^-1 [2][36]
   double* lam = c6 + 72;           // [nb]
   double* xq = lam + nb;           // x_q [na]
@@ -4299,10 +4536,16 @@ __global__ void __launch_bounds__(256)
   double* zv = qv + na;            // [max(na, 72)]; flux_cinv's scratch at a cell's start
   double* xuv = zv + (na > 72 ? na : 72);  // x_u [nu]
   double* uv = xuv + nu;           // reduced work vector [nu]
+  double* tv = uv + nu;            // REF: G_e products [D][np]
   for (int t = threadIdx.x; t < np * np; t += 64 * WPB) {
     s_m[t] = T.fb_minv[t];
     s_m[np * np + t] = T.Mpp[t];
   }
+  if (REF)
+    for (int t = threadIdx.x; t < gsz; t += 64 * WPB) {
+      const int e = t / (np * nu), rem = t - e * np * nu, i = rem / nu, c = rem - i * nu;
+      s_gu[t] = T.fr_G[((size_t)e * np + i) * (nu + nb) + c];
+    }
   __syncthreads();
   const double* Minv = s_m;
   const double* Mm = s_m + np * np;
@@ -4350,6 +4593,28 @@ __global__ void __launch_bounds__(256)
   };
   // qv = A_qu v (v over the u unknowns, in LDS)
   auto aqu_apply = [&](const double* Aqu, const double* v) {
+    if (REF) {
+      // t_e[i] = sum_c G_e[i][c] v[c]; qv[(a, i)] = sum_e c_ea t_e[i]
+      for (int t = lane; t < D * np; t += 64) {
+        const double* g = s_gu + (int64_t)t * nu;
+        double acc = 0.0;
+#pragma unroll 4
+        for (int c = 0; c < nu; ++c) acc = fma(g[c], v[c], acc);
+        tv[t] = acc;
+      }
+      wave_sync_lds();
+#pragma unroll
+      for (int t = 0; t < KQ; ++t) {
+        const int r = lane + 64 * t;
+        if (r < na) {
+          const int a = r / np, i = r - a * np;
+          double acc = 0.0;
+          for (int e = 0; e < D; ++e) acc = fma(-alpha[1 + e * D + a], tv[e * np + i], acc);
+          qv[r] = acc;
+        }
+      }
+      return;
+    }
 #pragma unroll
     for (int t = 0; t < KQ; ++t) {
       const int r = lane + 64 * t;
@@ -4468,7 +4733,21 @@ __global__ void __launch_bounds__(256)
       }
       wave_sync_lds();
       // t = r_u - A_uq z = c_u - A_uu x_u - A_uq (x_q + z);  d_u = R_uu^-1 t
-      if (lane < nu) {
+      if (REF) {
+        // A_uq = -A_qu^T: s_e[i] = sum_a c_ea qv[(a, i)]; ru += sum_(e,i) G_e[i][u] s_e[i]
+        for (int t = lane; t < D * np; t += 64) {
+          const int e = t / np, i = t - e * np;
+          double acc = 0.0;
+          for (int a = 0; a < D; ++a) acc = fma(-alpha[1 + e * D + a], qv[a * np + i], acc);
+          tv[t] = acc;
+        }
+        wave_sync_lds();
+        if (lane < nu) {
+#pragma unroll 4
+          for (int t = 0; t < D * np; ++t) ru = fma(s_gu[(int64_t)t * nu + lane], tv[t], ru);
+          uv[lane] = ru;
+        }
+      } else if (lane < nu) {
 #pragma unroll 4
         for (int j = 0; j < na; ++j) ru = fma(-Auq[(int64_t)j * nu + lane], qv[j], ru);
         uv[lane] = ru;
@@ -4523,13 +4802,14 @@ __global__ void __launch_bounds__(256)
 }
 
 hipError_t launch_recover_flux(const Tables& T, const MeshDev& M, const double* U,
-                               const double* Y, int64_t usz, int64_t ysz, bool fused,
+                               const double* Y, int64_t usz, int64_t ysz, bool fused, bool ref,
                                const int32_t* cell_dofs, const double* lam_free,
                                const double* lam_dir, double* out, int64_t nfree,
                                hipStream_t st) {
   const int na = T.na, nu = T.ni - na, nb = T.nb;
   if (na > kMixedMaxNi || nu > 64 || nb > kFusedMaxNb || T.np > kFluxMaxNp)
     return hipErrorInvalidValue;
+  if (ref && (fused || T.phys != kDarcy || !T.fr_G)) return hipErrorInvalidValue;
   RecSlot rs;
   rs.slot = U;
   rs.usz = usz;
@@ -4557,16 +4837,21 @@ hipError_t launch_recover_flux(const Tables& T, const MeshDev& M, const double* 
   if (M.ncells > 0) {
     const int64_t blocks = (M.ncells + kRecWaves - 1) / kRecWaves;
     const int grid = (int)(blocks < 65536 ? blocks : 65536);
-    const size_t lds =
-        (2 * (size_t)T.np * T.np + kRecWaves * (size_t)rec_wave_lds(na, nu, nb)) * sizeof(double);
+    const size_t lds = (2 * (size_t)T.np * T.np + (ref ? (size_t)T.D * T.np * nu : 0) +
+                        kRecWaves * (size_t)rec_wave_lds(na, nu, nb)) *
+                       sizeof(double);
     ClockScope clk_(st, "recover_flux");
-#define HDG_RF(P_, K_)                                                                      \
-  hipLaunchKernelGGL((recover_flux_kernel<P_, K_>), dim3(grid), dim3(64 * kRecWaves), lds, st, \
-                     FusedGen<P_>{T, M}, rs, cell_dofs, lam_free, lam_dir, out)
+#define HDG_RF(P_, K_, R_)                                                                     \
+  hipLaunchKernelGGL((recover_flux_kernel<P_, K_, R_>), dim3(grid), dim3(64 * kRecWaves), lds, \
+                     st, FusedGen<P_>{T, M}, rs, cell_dofs, lam_free, lam_dir, out)
     if (T.phys == kDarcy) {
-      if (na <= 64) HDG_RF(kDarcy, 1); else HDG_RF(kDarcy, 2);
+      if (ref) {
+        if (na <= 64) HDG_RF(kDarcy, 1, true); else HDG_RF(kDarcy, 2, true);
+      } else {
+        if (na <= 64) HDG_RF(kDarcy, 1, false); else HDG_RF(kDarcy, 2, false);
+      }
     } else {
-      if (na <= 64) HDG_RF(kElasticity, 1); else HDG_RF(kElasticity, 2);
+      if (na <= 64) HDG_RF(kElasticity, 1, false); else HDG_RF(kElasticity, 2, false);
     }
 #undef HDG_RF
   }

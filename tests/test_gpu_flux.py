@@ -1,4 +1,4 @@
-"""Flux-block Schur complement (kernel family 8): the flux unknowns are
+"""Flux-block Schur complement (kernel families 8 and 10): the flux unknowns are
 eliminated with A_qq^-1 = C^-1 (x) M^-1 (per-cell coefficient matrix C, fixed
 scalar mass matrix M) and the reduced system is condensed by an elimination
 kernel.  Checked against the FP64 oracle (relative 1e-12 for S, g and the
@@ -17,7 +17,8 @@ pytestmark = pytest.mark.gpu
 RTOL = 1e-12
 FLUX = 8
 FUSED = 9
-FLUX_FAMS = (FLUX, FUSED)
+REF = 10
+FLUX_FAMS = (FLUX, FUSED, REF)
 
 
 def _darcy(ctx, k, n, seed, family=None, kappa=None):
@@ -121,7 +122,7 @@ def test_flux_family_selection(ctx):
     assert op.factor_rows()[1] == 3
     op.set_family(None)
     op.assemble()
-    assert op.factor_rows()[1] == FLUX    # the default for the class
+    assert op.factor_rows()[1] == REF     # the default for the Darcy class
     op.close()
     model = gh.CartesianDiscreteModel((0.0, 1.0) * 2, (4, 4))
     X = gh.HDGSpaces.darcy_baseline(model, 2)
@@ -178,4 +179,87 @@ def test_fused_family_vs_oracle(ctx, phys, k, n):
     U, Y = op.local_factors()
     assert U is None
     check_local_factors(op, A, b, U, Y, min(op.info.ncells, 16))
+    op.close()
+
+
+@pytest.mark.parametrize("k,n", [(2, 4)])
+def test_ref_matches_flux_schur(ctx, k, n):
+    """Family 10 (the reduced system from the reference decomposition of the
+    flux Schur complement) against family 8 (two MFMA products per cell) on the
+    same cells: S, g to 1e-12, the recovered values to 1e-11."""
+    print(f"flux_ref vs flux_schur darcy 3D k={k} n={n}", flush=True)
+    a, kappa = _darcy(ctx, k, n, seed=71, family=REF)
+    b, _ = _darcy(ctx, k, n, seed=71, family=FLUX)
+    assert a.factor_rows() == (a.info.ni - a.info.n_flux, REF)
+    assert b.factor_rows()[1] == FLUX
+    Sa, ga, _ = a.cell_blocks()
+    Sb, gb, _ = b.cell_blocks()
+    assert rel(Sa, Sb) < RTOL and rel(ga, gb) < RTOL
+    lam = a.interpolate_trace()
+    assert rel(a.recover(lam).cpu().numpy(), b.recover(lam).cpu().numpy()) < 1e-11
+    ref = O.hdg_solve(O.Problem.darcy(3, k, mms=O.MMS_SINPROD), (n,) * 3, coef=kappa)
+    _check_vs_oracle(a, ref, 1e-11)
+    A, bb = _device_cell_systems(a, 0, 16)
+    U, Y = a.local_factors()
+    check_local_factors(a, A, bb, U, Y, 16)
+    a.close()
+    b.close()
+
+
+@pytest.mark.parametrize("frac", [0.0, 0.7])
+def test_ref_general_affine_cells(ctx, frac):
+    """The reference decomposition on sheared cells (non-diagonal Jacobians),
+    some re-parametrised by rotations of the reference cell (oriented facets):
+    family 10 is the default and matches the oracle."""
+    import gridaphybrid as gh
+    import meshgen
+    print(f"flux_ref on sheared cells, rotated fraction {frac}", flush=True)
+    n, k = 3, 2
+    x0, J, cf, nf, perm, rot = meshgen.sheared_rotated_mesh(3, n, seed=12, frac=frac)
+    fperm = perm if perm.any() else None
+    kappa = O.lognormal_kappa(n ** 3, seed=13)
+    model = gh.AffineDiscreteModel(x0=x0, J=J, cell_facets=cf, nfacets=nf, facet_perm=fperm)
+    X = gh.HDGSpaces.darcy_baseline(model, k)
+    op = gh.HybridAffineFEOperator(gh.DarcyHDG(tau=1.0, kappa=kappa, source=O.MMS_SINPROD),
+                                   X, X, [1, 2], [3], ctx=ctx)
+    assert op.factor_rows()[1] == REF
+    ref = O.hdg_solve_mesh(O.Problem.darcy(3, k, mms=O.MMS_SINPROD),
+                           meshgen.oracle_mesh(x0, J, cf, nf, fperm), coef=kappa)
+    _check_vs_oracle(op, ref, 1e-11)
+    op.close()
+
+
+def test_ref_needs_positive_orientation(ctx):
+    """A cell with det J < 0 (a reflected parametrisation) falls outside the
+    cofactor identity of the reference decomposition: the class then keeps
+    family 8, and the result still matches the oracle."""
+    import gridaphybrid as gh
+    import meshgen
+    n, k = 2, 2
+    x0, J, cf, nf, perm, rot = meshgen.sheared_rotated_mesh(3, n, seed=14, frac=0.0)
+    # reflect cell 3 in its first reference direction: xi_0 -> 1 - xi_0, which
+    # swaps its local facets 4 and 5 (the ones fixing direction 0) and flips
+    # the first coordinate of the facets that fix directions 1 and 2
+    x0[3] = x0[3] + J[3][:, 0]
+    J[3][:, 0] = -J[3][:, 0]
+    cf[3][[4, 5]] = cf[3][[5, 4]]
+    perm[:] = 0
+    perm[3] = [2, 2, 2, 2, 0, 0]
+    kappa = O.lognormal_kappa(n ** 3, seed=15)
+    model = gh.AffineDiscreteModel(x0=x0, J=J, cell_facets=cf, nfacets=nf, facet_perm=perm)
+    X = gh.HDGSpaces.darcy_baseline(model, k)
+    op = gh.HybridAffineFEOperator(gh.DarcyHDG(tau=1.0, kappa=kappa, source=O.MMS_SINPROD),
+                                   X, X, [1, 2], [3], ctx=ctx)
+    assert op.factor_rows()[1] == FLUX
+    with pytest.raises(gh.HDGError):
+        op.set_family(REF)
+    ref = O.hdg_solve_mesh(O.Problem.darcy(3, k, mms=O.MMS_SINPROD),
+                           meshgen.oracle_mesh(x0, J, cf, nf, perm), coef=kappa)
+    _check_vs_oracle(op, ref, 1e-11)
+    # the reflected cell leaves the discrete problem unchanged: the skeleton
+    # solution equals the one of the unreflected mesh
+    x0u, Ju, cfu, _, _, _ = meshgen.sheared_rotated_mesh(3, n, seed=14, frac=0.0)
+    ref_u = O.hdg_solve_mesh(O.Problem.darcy(3, k, mms=O.MMS_SINPROD),
+                             meshgen.oracle_mesh(x0u, Ju, cfu, nf, None), coef=kappa)
+    assert rel(ref.lam, ref_u.lam) < 1e-11
     op.close()

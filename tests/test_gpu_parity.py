@@ -135,7 +135,7 @@ def check_local_factors(op, A, b, U, Y, ncheck):
     for c in range(ncheck):
         Ab = np.concatenate([A[c], b[c][:, None]], axis=1)
         if na:
-            assert fam == 8
+            assert fam in (8, 10)
             Z = np.linalg.solve(Ab[:na, :na], Ab[:na, na:])
             M = Ab[na:, na:] - Ab[na:, :na] @ Z
         else:
