@@ -136,6 +136,11 @@ def parse():
                          "command the rocprofv3 summaries are taken from")
     ap.add_argument("--physics", choices=("darcy", "elasticity"), default="darcy",
                     help="physics of the --only case")
+    ap.add_argument("--family", type=int, default=None,
+                    help="FP64 kernel family of the --only case (default: the class default)")
+    ap.add_argument("--grid-per-cu", type=int, default=0,
+                    help="launch shape of that family (0: its default; the blocked "
+                         "family: wavefronts per workgroup, 100 + w for 32-column panels)")
     ap.add_argument("--copy-gib", type=float, default=0.0,
                     help="only run one plain device copy of this many GiB (the reference "
                          "kernel that scales the FETCH_SIZE / WRITE_SIZE counters)")
@@ -189,13 +194,14 @@ class Case:
         if ev:
             ev[3].record()
 
-    def run(self, precision, steps, warmup, family=None, name=None):
+    def run(self, precision, steps, warmup, family=None, name=None, grid_per_cu=0):
         """Times `steps` steps after `warmup`; the library's kernel clock (HIP
         events around every launch, on the context stream) runs over the
-        timed steps.  family: FP64 kernel family (None: the class default)."""
+        timed steps.  family: FP64 kernel family (None: the class default);
+        grid_per_cu: the family's launch shape (0: its default)."""
         import torch
         ctx = self.ctx
-        self.op.set_family(family)
+        self.op.set_family(family, grid_per_cu)
         with ctx.stream():
             for _ in range(warmup):
                 self.step(precision)
@@ -468,9 +474,14 @@ def batch_main(gh, args):
     res = {"fp64": [], "mixed": []}
     cells = 0
     classes = {}
-    for w in waves:
+    for wi, w in enumerate(waves):
+        t0 = time.perf_counter()
         ops = [sv.build_operator(gh, ctx, r) for r in w]
+        t1 = time.perf_counter()
         batch = gh.HybridOperatorBatch(ops, autotune=True)
+        print(f"[batch] graph {wi + 1}/{len(waves)}: {len(w)} meshes, "
+              f"{sum(op.info.ncells for op in ops)} cells; operators {t1 - t0:.1f} s, "
+              f"batch + autotune {time.perf_counter() - t1:.1f} s", file=sys.stderr, flush=True)
         for c in batch.classes():
             key = f"{'elas' if c['physics'] else 'darcy'}{c['dim']}d ni={c['ni']} nb={c['nb']}"
             classes.setdefault(key, []).append({k: c[k] for k in
@@ -494,6 +505,8 @@ def batch_main(gh, args):
             ctx.synchronize()
             res[name].append(dict(meshes=len(w), cells=sum(op.info.ncells for op in ops),
                                   graph_ms=e0.elapsed_time(e1), recover_ms=e1.elapsed_time(e2)))
+            print(f"[batch] graph {wi + 1} {name}: {res[name][-1]['graph_ms']:.1f} ms + "
+                  f"recovery {res[name][-1]['recover_ms']:.1f} ms", file=sys.stderr, flush=True)
         cells += sum(op.info.ncells for op in ops)
         batch.close()
         for op in ops:
@@ -523,10 +536,13 @@ def only_main(gh, args):
     ctx = gh.Context(0)
     case = Case(gh, ctx, 3, args.n3d, args.order, physics=args.physics)
     fam = {"fp64_dense": DENSE_FAMILY, "fp64_schur": SCHUR_FAMILY}.get(args.only)
+    if args.family is not None:
+        fam = args.family
     prec = gh.MIXED if args.only == "mixed" else gh.FP64
-    r = case.run(prec, args.steps, args.warmup, fam, args.only)
+    r = case.run(prec, args.steps, args.warmup, fam, args.only, args.grid_per_cu)
     info = case.info
     print(json.dumps({"only": args.only, "physics": args.physics, "n3d": args.n3d,
+                      "grid_per_cu": args.grid_per_cu,
                       "order": args.order, "cells": info.ncells, "ni": info.ni, "nb": info.nb,
                       "cells_per_s": r["cells_per_s"], "ms_per_step": r["ms_per_step"],
                       "family": r["family"], "kernels_ms_per_launch": r["kernels_ms_per_launch"],

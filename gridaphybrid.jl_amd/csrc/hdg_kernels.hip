@@ -1124,27 +1124,33 @@ __global__ void __launch_bounds__(64 * NW)
 }
 
 // --------------------------------------------------------------------------
-// Blocked elimination for large systems (up to 512 rows): the cell system
+// Blocked elimination for large systems (up to NMAX rows): the cell system
 // lives in a per-workgroup HBM workspace (column-major, leading dimension
 // ld); panels of KB columns are factored in LDS (partial pivoting over the
 // interior rows, LAPACK position bookkeeping), the panel's pivot rows are
 // brought to U form by forward substitution (U12), and the remaining active
-// rows get one rank-KB update per panel from register tiles.
+// rows get one rank-KB update per panel from register tiles.  Each panel
+// reads and writes the active trailing matrix once, so the workspace traffic
+// per cell falls with KB: the 32-column variant (systems up to 416 rows)
+// stages the pivot rows of U12 in LDS 128 trailing columns at a time (the
+// rest wait in the workspace's pivot rows) so that the 32-column panel fits.
 // --------------------------------------------------------------------------
 constexpr int kLargeMaxN = 512;
 constexpr int kLargeKB = 16;
+constexpr int kLarge32MaxN = 416;  // 32-column panels: systems up to this many rows
 
-template <class Gen, bool BACKSUB, int NT>
+template <class Gen, bool BACKSUB, int NT, int KB = kLargeKB, int NMAX = kLargeMaxN>
 __global__ void __launch_bounds__(NT)
     condense_large_kernel(Gen gen, CondenseOut out, int64_t ncells, int ni, int nb,
                           double* __restrict__ work, int ld) {
-  constexpr int KB = kLargeKB, NMAX = kLargeMaxN, NW = NT / 64;
+  constexpr int NW = NT / 64;
+  constexpr int CWU = KB == kLargeKB ? NMAX : 128;  // trailing columns of U12 staged in LDS
   const int n = ni + nb, ncol = n + 1;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   double* M = work + (int64_t)blockIdx.x * ld * (int64_t)ncol;
 
   __shared__ double s_P[KB][NMAX];        // panel columns (all rows)
-  __shared__ double s_U[KB][NMAX + 1];    // pivot rows x trailing columns
+  __shared__ double s_U[KB][CWU + 1];     // pivot rows x (a chunk of the) trailing columns
   __shared__ int s_pos[NMAX], s_at[NMAX];
   __shared__ unsigned char s_act[NMAX];
   __shared__ int s_piv[KB];
@@ -1251,75 +1257,128 @@ __global__ void __launch_bounds__(NT)
         if (out.U) out.U[cell * ustride(out, ni) + col * (col + 1) / 2 + k] = v;
         if (BACKSUB) M[(int64_t)col * ld + s_piv[jj]] = v;  // U row kept for back substitution
       }
-      for (int c = jt + tid; c < ncol; c += NT) {
-        double u[KB], mv[KB];
-#pragma unroll
-        for (int jj = 0; jj < KB; ++jj)  // the pivot rows' entries, all loads in flight at once
-          mv[jj] = jj < kb ? M[(int64_t)c * ld + s_piv[jj]] : 0.0;
-#pragma unroll
-        for (int jj = 0; jj < KB; ++jj) {
-          if (jj < kb) {
-            double v = mv[jj];
-            for (int ii = 0; ii < jj; ++ii) v = fma(-s_P[ii][s_piv[jj]], u[ii], v);
-            u[jj] = v;
-            s_U[jj][c - jt] = v;
-          }
-        }
-        for (int jj = 0; jj < kb; ++jj) {
-          const int k = j0 + jj;
-          if (c < ni) {
-            if (out.U) out.U[cell * ustride(out, ni) + c * (c + 1) / 2 + k] = u[jj];
-          } else {
-            if (out.Y) out.Y[cell * ystride(out, ni, nb) + (int64_t)(c - ni) * ni + k] = u[jj];
-          }
-          if (BACKSUB) M[(int64_t)c * ld + s_piv[jj]] = u[jj];
-        }
-      }
-      __syncthreads();
-      // ---- rank-kb update of the active rows on the trailing columns ----
       const int ntr = ncol - jt;
-      const int rt = (n + 3) / 4, ct = (ntr + 3) / 4;
-      for (int tile = tid; tile < rt * ct; tile += NT) {
-        const int tr4 = (tile % rt) * 4, tc4 = (tile / rt) * 4;
-        double acc[4][4];
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-#pragma unroll
-          for (int q = 0; q < 4; ++q) acc[i][q] = 0.0;
-        bool rowok[4];
-#pragma unroll
-        for (int i = 0; i < 4; ++i) rowok[i] = tr4 + i < n && s_act[tr4 + i];
-        if (!(rowok[0] | rowok[1] | rowok[2] | rowok[3])) continue;
-        // the tile's entries load first, in flight during the rank-kb products
-        double mt[4][4];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const int c = jt + tc4 + q;
+      const int rt = (n + 3) / 4;
+      // rank-kb update of the active rows on trailing columns [c0, c0 + cw),
+      // with their U12 in s_U
+      auto update_chunk = [&](int c0, int cw) {
+        const int ct = (cw + 3) / 4;
+        for (int tile = tid; tile < rt * ct; tile += NT) {
+          const int tr4 = (tile % rt) * 4, tc4 = (tile / rt) * 4;
+          double acc[4][4];
 #pragma unroll
           for (int i = 0; i < 4; ++i)
-            mt[i][q] = (c < ncol && rowok[i]) ? M[(int64_t)c * ld + tr4 + i] : 0.0;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) acc[i][q] = 0.0;
+          bool rowok[4];
+#pragma unroll
+          for (int i = 0; i < 4; ++i) rowok[i] = tr4 + i < n && s_act[tr4 + i];
+          if (!(rowok[0] | rowok[1] | rowok[2] | rowok[3])) continue;
+          // the tile's entries load first, in flight during the rank-kb products
+          double mt[4][4];
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const int c = jt + c0 + tc4 + q;
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+              mt[i][q] = (tc4 + q < cw && rowok[i]) ? M[(int64_t)c * ld + tr4 + i] : 0.0;
+          }
+          for (int jj = 0; jj < kb; ++jj) {
+            double l[4], u[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) l[i] = rowok[i] ? s_P[jj][tr4 + i] : 0.0;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) u[q] = tc4 + q < cw ? s_U[jj][tc4 + q] : 0.0;
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+              for (int q = 0; q < 4; ++q) acc[i][q] = fma(l[i], u[q], acc[i][q]);
+          }
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const int c = jt + c0 + tc4 + q;
+            if (tc4 + q >= cw) continue;
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+              if (rowok[i]) M[(int64_t)c * ld + tr4 + i] = mt[i][q] - acc[i][q];
+          }
         }
-        for (int jj = 0; jj < kb; ++jj) {
-          double l[4], u[4];
-#pragma unroll
-          for (int i = 0; i < 4; ++i) l[i] = rowok[i] ? s_P[jj][tr4 + i] : 0.0;
-#pragma unroll
-          for (int q = 0; q < 4; ++q) u[q] = tc4 + q < ntr ? s_U[jj][tc4 + q] : 0.0;
-#pragma unroll
-          for (int i = 0; i < 4; ++i)
-#pragma unroll
-            for (int q = 0; q < 4; ++q) acc[i][q] = fma(l[i], u[q], acc[i][q]);
+      };
+      // the U12 value of pivot row jj, trailing column c: the outputs (and the
+      // workspace for the back substitution)
+      auto put_u12 = [&](int c, int jj, double u) {
+        const int k = j0 + jj;
+        if (c < ni) {
+          if (out.U) out.U[cell * ustride(out, ni) + c * (c + 1) / 2 + k] = u;
+        } else {
+          if (out.Y) out.Y[cell * ystride(out, ni, nb) + (int64_t)(c - ni) * ni + k] = u;
         }
+        if (BACKSUB) M[(int64_t)c * ld + s_piv[jj]] = u;
+      };
+      if constexpr (KB == kLargeKB) {
+        // U12 = L11^-1 A12, a thread per trailing column, all in LDS at once
+        for (int c = jt + tid; c < ncol; c += NT) {
+          double u[KB], mv[KB];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const int c = jt + tc4 + q;
-          if (c >= ncol) continue;
+          for (int jj = 0; jj < KB; ++jj)  // the pivot rows' entries, all loads in flight at once
+            mv[jj] = jj < kb ? M[(int64_t)c * ld + s_piv[jj]] : 0.0;
 #pragma unroll
-          for (int i = 0; i < 4; ++i)
-            if (rowok[i]) M[(int64_t)c * ld + tr4 + i] = mt[i][q] - acc[i][q];
+          for (int jj = 0; jj < KB; ++jj) {
+            if (jj < kb) {
+              double v = mv[jj];
+              for (int ii = 0; ii < jj; ++ii) v = fma(-s_P[ii][s_piv[jj]], u[ii], v);
+              u[jj] = v;
+              s_U[jj][c - jt] = v;
+            }
+          }
+          for (int jj = 0; jj < kb; ++jj) put_u12(c, jj, u[jj]);
+        }
+        __syncthreads();
+        update_chunk(0, ntr);
+        __syncthreads();
+      } else {
+        // chunks of CWU trailing columns: U12 of the chunk into s_U, then its
+        // update.  A wavefront owns 8 columns of the chunk; lane (column
+        // lane / 8, rl = lane % 8) holds the pivot rows rl + 8 m, and row jj's
+        // final value reaches the column's lanes by a shuffle (each entry
+        // subtracts the rows above it in increasing order, as in the
+        // thread-per-column substitution)
+        constexpr int RPL = KB / 8;
+        const int rl = lane & 7;
+        int prow[RPL];
+#pragma unroll
+        for (int m = 0; m < RPL; ++m) prow[m] = rl + 8 * m < kb ? s_piv[rl + 8 * m] : 0;
+        for (int c0 = 0; c0 < ntr; c0 += CWU) {
+          const int cw = min(CWU, ntr - c0);
+          for (int cb = wv * 8; cb < cw; cb += NW * 8) {  // wave-uniform
+            const int cc = cb + (lane >> 3), c = jt + c0 + cc;
+            const bool colok = cc < cw;
+            double v[RPL];
+#pragma unroll
+            for (int m = 0; m < RPL; ++m)
+              v[m] = (colok && rl + 8 * m < kb) ? M[(int64_t)c * ld + prow[m]] : 0.0;
+#pragma unroll
+            for (int jj = 0; jj < KB; ++jj) {
+              const double u = __shfl(v[jj >> 3], (lane & ~7) | (jj & 7), 64);
+#pragma unroll
+              for (int m = 0; m < RPL; ++m) {
+                const int j2 = rl + 8 * m;
+                if (j2 > jj && j2 < kb) v[m] = fma(-s_P[jj][prow[m]], u, v[m]);
+              }
+            }
+#pragma unroll
+            for (int m = 0; m < RPL; ++m) {
+              const int j2 = rl + 8 * m;
+              if (!colok || j2 >= kb) continue;
+              s_U[j2][cc] = v[m];
+              put_u12(c, j2, v[m]);
+            }
+          }
+          __syncthreads();
+          update_chunk(c0, cw);
+          __syncthreads();
         }
       }
-      __syncthreads();
     }
 
     // ---- back substitution (backward map only: single rhs column) -------
@@ -2410,7 +2469,10 @@ constexpr int kLargeGrid = 256 * 2;
 // Bytes of the large-system kernel's workspace: one column-major (n+1)-column
 // system per resident workgroup, leading dimension rounded up to even.
 // wavefronts per workgroup of the blocked kernel, measured fastest per size
-static int large_default_waves(int n) { return n > 256 ? 16 : 8; }
+// wavefronts per workgroup of the blocked elimination (100 + w: 32-column panels)
+static int large_default_waves(int n) {
+  return n > 256 ? (n <= kLarge32MaxN ? 116 : 16) : 8;
+}
 
 static size_t large_bytes(int n, int* ld) {
   const int l = (n + 1) & ~1;
@@ -2519,9 +2581,19 @@ static hipError_t launch_condense(const Gen& gen, const CondenseOut& out, int64_
     hipError_t e = workspace_reserve(ws, large_bytes(ni + nb, &ld));
     if (e != hipSuccess) return e;
     const int grid = (int)(ncells < kLargeGrid ? ncells : kLargeGrid);
-    const int waves = gridmul > 0 ? gridmul : large_default_waves(ni + nb);
+    int waves = gridmul > 0 ? gridmul : large_default_waves(ni + nb);
+    // 100 + waves: 32-column panels (systems up to kLarge32MaxN rows)
+    const bool kb32 = waves > 100 && ni + nb <= kLarge32MaxN;
+    if (waves > 100) waves -= 100;
     ClockScope clk_(st, "condense_blocked");
-    if (waves == 16)
+    if (kb32) {
+      if (waves == 16)
+        hipLaunchKernelGGL((condense_large_kernel<Gen, BACKSUB, 1024, 32, kLarge32MaxN>),
+                           dim3(grid), dim3(1024), 0, st, gen, out, ncells, ni, nb, ws.p, ld);
+      else
+        hipLaunchKernelGGL((condense_large_kernel<Gen, BACKSUB, 512, 32, kLarge32MaxN>),
+                           dim3(grid), dim3(512), 0, st, gen, out, ncells, ni, nb, ws.p, ld);
+    } else if (waves == 16)
       hipLaunchKernelGGL((condense_large_kernel<Gen, BACKSUB, 1024>), dim3(grid), dim3(1024), 0,
                          st, gen, out, ncells, ni, nb, ws.p, ld);
     else if (waves == 8)
@@ -2631,7 +2703,8 @@ int condense_variants(const SysShape& sh, int* fams, int max) {
 int condense_grid_options(int fam, int* gm, int max) {
   // resident workgroups per CU worth timing for each family (0 = fixed grid)
   static const int wave[] = {16, 32, 48, 64}, grid8[] = {20, 40, 80}, col[] = {1, 2, 3},
-                   g32[] = {2, 3, 4}, pan[] = {1, 2}, til[] = {1}, large[] = {4, 8, 16},
+                   g32[] = {2, 3, 4}, pan[] = {1, 2}, til[] = {1},
+                   large[] = {4, 8, 16, 108, 116},
                    g16[] = {2, 4, 6}, fus[] = {1, 2}, fixed[] = {0};
   const int* v = fixed;
   int n = 1;
@@ -2642,7 +2715,7 @@ int condense_grid_options(int fam, int* gm, int max) {
     case kFamGrid32: v = g32, n = 3; break;
     case kFamPanel: v = pan, n = 2; break;
     case kFamTile: v = til, n = 1; break;
-    case kFamLarge: v = large, n = 3; break;
+    case kFamLarge: v = large, n = 5; break;
     case kFamGrid16: v = g16, n = 3; break;
     case kFamFused: v = fus, n = 2; break;
     default: break;
@@ -2910,10 +2983,12 @@ __global__ void __launch_bounds__(256)
     }
     wave_sync_lds();
     // FP64 refinement sweeps x += X (c - A_ii x) with the FP32 inverse X,
-    // until the correction is below 2^-44 of x, the condensation's own
-    // stopping rule (each sweep contracts the error by about cond(A_ii) times
-    // the FP32 rounding), or kRecoverSweeps
+    // with the condensation's own stopping rule: the correction is below
+    // 2^-44 of x, or it has stalled at the FP64 floor of the residual (below
+    // 1e-10 of x and no longer halving; each sweep contracts the error by
+    // about cond(A_ii) times the FP32 rounding), or kRecoverSweeps
     const float* Xc = Xinv + cell * (int64_t)ni * ni;
+    double cprev = 0.0;
     for (int sweep = 0; sweep < kRecoverSweeps; ++sweep) {
       // residual of the interior equations at [x; lambda], term by term (FP64)
 #pragma unroll
@@ -2942,7 +3017,10 @@ __global__ void __launch_bounds__(256)
       wave_sync_lds();
       cmax = wave_max_f64(cmax);
       xmax = wave_max_f64(xmax);
-      if (!(cmax > 0x1p-44 * xmax)) break;  // uniform across the wavefront
+      // (uniform across the wavefront)
+      if (!(cmax > 0x1p-44 * xmax)) break;
+      if (sweep >= 1 && cmax <= 1e-10 * xmax && cmax > 0.5 * cprev) break;
+      cprev = cmax;
     }
 #pragma unroll
     for (int t = 0; t < KMAX; ++t) {
@@ -3356,6 +3434,7 @@ static bool ref_plan(const Tables& T, RefPlan* rp) {
   if (T.phys != kDarcy || T.fr_npair <= 0 || !T.fr_P || !T.fr_G || !T.fb_minv) return false;
   if (T.ni > kRefThreads) return false;  // one right-hand-side value per thread
   const int D = T.D, np = T.np, na = T.na, nu = T.ni - na, nr = nu + T.nb;
+  if (nr + 1 >= (1 << 23)) return false;  // entry columns packed above 8 bits of band row
   int BR = std::min(nr, kRefEntries * kRefThreads / (nr + 1));
   BR = std::min(BR, kRefThreads / kRefSplit);  // the rhs column: kRefSplit threads per row
   if (BR < 1) return false;
@@ -3401,12 +3480,14 @@ __global__ void __launch_bounds__(kRefThreads, 2)
   // ---- per-thread entries: P values and R0's program terms, for all cells ----
   double pv[NE][NPAIR], tv[NE][NK];
   int tid8[NE];  // term coefficient ids, 8 bits each; 0xff: no term
+  int ent[NE];   // (column << 8) | band row of the entry, -1: none
 #pragma unroll
   for (int u = 0; u < NE; ++u) {
     const int t = tid + NT * u;
     const bool ok = t < rows * nc;
     const int c = ok ? t / rows : 0, rl = ok ? t - c * rows : 0;
     const int r = r0 + rl;
+    ent[u] = ok ? (c << 8) | rl : -1;
 #pragma unroll
     for (int p = 0; p < NPAIR; ++p)
       pv[u][p] = (ok && c < nr) ? T.fr_P[((size_t)p * nr + r) * nr + c] : 0.0;
@@ -3470,12 +3551,15 @@ __global__ void __launch_bounds__(kRefThreads, 2)
     const double inva = bad ? 0.0 : 1.0 / a0;
     if (xrl < rows) {
       double x = 0.0;
+      int e = 0, j = xpart;  // k = e * np + j
+      while (j >= np) j -= np, ++e;
       for (int k = xpart; k < nk; k += SPL) {
-        const int e = k / np, j = k - e * np;
         double z = 0.0;
 #pragma unroll
         for (int a = 0; a < D; ++a) z = fma(c[e * D + a], s_b[a * np + j], z);
         x = fma(s_H[xrl * nk + k], z, x);
+        j += SPL;
+        while (j >= np) j -= np, ++e;
       }
 #pragma unroll
       for (int off = SPL / 2; off >= 1; off >>= 1) x += __shfl_xor(x, off, SPL);
@@ -3499,9 +3583,8 @@ __global__ void __launch_bounds__(kRefThreads, 2)
     double* Bk = blocks + lc * bstride;      // [A_uu] (A_qu, A_uq: not used by the recovery)
 #pragma unroll
     for (int u = 0; u < NE; ++u) {
-      const int t = tid + NT * u;
-      if (t >= rows * nc) continue;
-      const int cc = t / rows, rl = t - cc * rows, r = r0 + rl;
+      if (ent[u] < 0) continue;
+      const int cc = ent[u] >> 8, rl = ent[u] & 255, r = r0 + rl;
       double r0v, x;
       if (cc < nr) {  // the entry program's terms, in its order
         r0v = 0.0;

@@ -325,6 +325,29 @@ def test_darcy_3d_k3_large(ctx):
     op.close()
 
 
+@pytest.mark.parametrize("shape", [4, 8, 16, 108, 116])
+def test_blocked_launch_shapes(ctx, shape):
+    """Every launch shape of the blocked elimination (16-column panels with 4,
+    8 or 16 wavefronts per workgroup; 32-column panels, shape 100 + w) on the
+    3D Darcy k=3 class (ni=256, nb=96) against the oracle."""
+    print(f"blocked elimination, launch shape {shape}", flush=True)
+    nc = 8
+    kappa = O.lognormal_kappa(nc, seed=9)
+    prob = O.Problem.darcy(3, 3, mms=O.MMS_SINPROD)
+    ref = O.hdg_solve(prob, (2, 2, 2), coef=kappa)
+    op = _operator(ctx, 3, 3, 2, kappa=kappa, mms=O.MMS_SINPROD)
+    op.set_family(4, shape)
+    op.assemble()
+    assert op.factor_rows()[1] == 4
+    S, g, cd = op.cell_blocks()
+    assert rel(S, ref.S) < RTOL and rel(g, ref.g) < RTOL
+    K, rhs = op.skeleton_system()
+    assert rel(K.data, ref.nzval) < RTOL and rel(rhs, ref.rhs) < RTOL
+    fv = op.recover(ref.lam).cpu().numpy()
+    assert rel(fv, ref.free_values) < 1e-11
+    op.close()
+
+
 @pytest.mark.parametrize("D,k,n", [(2, 1, 4), (3, 1, 2), (3, 2, 2)])
 def test_elasticity_projection_parity(ctx, D, k, n):
     """Elasticity HDG with the bulk-to-skeleton projection P_m
