@@ -128,6 +128,11 @@ def parse():
                     help="serving run: this many Poisson requests (rate --rate) instead of the "
                          "64^3 step")
     ap.add_argument("--rate", type=float, default=200.0)
+    ap.add_argument("--serve-loops", default="fp64,mixed,per_request",
+                    help="serving loops to run: fp64 and mixed continuous batching, "
+                         "per_request (one request at a time, FP64)")
+    ap.add_argument("--baseline-sample", type=int, default=200,
+                    help="requests served by the one-request-at-a-time baseline")
     ap.add_argument("--batch", action="store_true",
                     help="uneven-batch run: 256 seeded meshes in HBM-sized batch graphs")
     ap.add_argument("--nmesh", type=int, default=256)
@@ -406,27 +411,38 @@ def serve_main(gh, args):
     the host."""
     from gridaphybrid import serving as sv
     reqs = sv.draw_requests(args.serve, args.rate, SEED)
+    loops = args.serve_loops.split(",")
+    # the one-request-at-a-time baseline serves the first `baseline_sample`
+    # requests of the same stream (all of them for short streams)
+    nbase = min(args.serve, args.baseline_sample)
 
-    def fresh():
+    def fresh(n):
         return [sv.SolveRequest(r.rid, r.arrival, r.physics, r.dim, r.order, r.n, r.seed)
-                for r in reqs]
+                for r in reqs[:n]]
 
     out = {}
-    for name, kw in (("continuous_batching_fp64", dict(precision=gh.FP64, streams=2)),
-                     ("continuous_batching_mixed", dict(precision=gh.MIXED, streams=2)),
-                     ("per_request_fp64", dict(precision=gh.FP64, streams=1, batching=False))):
-        srv = sv.Server(gh, hbm_budget=200 << 30, **kw)
-        rs = fresh()
+    for key, name, kw, n in (
+            ("fp64", "continuous_batching_fp64", dict(precision=gh.FP64, streams=2), args.serve),
+            ("mixed", "continuous_batching_mixed", dict(precision=gh.MIXED, streams=2),
+             args.serve),
+            ("per_request", "per_request_fp64",
+             dict(precision=gh.FP64, streams=1, batching=False), nbase)):
+        if key not in loops:
+            continue
+        srv = sv.Server(gh, hbm_budget=200 << 30, keep_results=False, **kw)
+        rs = fresh(n)
         t0 = time.perf_counter()
         srv.run(rs, progress=lambda m: print(f"[{name}] {m}", file=sys.stderr, flush=True))
         wall = time.perf_counter() - t0
         summ = sv.summarize(rs, wall)
         summ["rounds"] = len(srv.rounds)
         summ["mean_round"] = float(np.mean(srv.rounds)) if srv.rounds else 0.0
+        summ["host_s"] = {k: round(v, 3) for k, v in srv.host_s.items()}
         out[name] = summ
         srv.close()
-    best = max(("continuous_batching_fp64", "continuous_batching_mixed"),
-               key=lambda n: out[n]["cells_per_s"] or 0.0)
+        print(f"[{name}] {json.dumps(summ)}", file=sys.stderr, flush=True)
+    cont = [n for n in ("continuous_batching_fp64", "continuous_batching_mixed") if n in out]
+    best = max(cont or out, key=lambda n: out[n]["cells_per_s"] or 0.0)
     m = out[best]
     line = {
         "metric": METRIC,
@@ -446,6 +462,7 @@ def serve_main(gh, args):
                                f"k in 1..3; skeleton = manufactured-trace interpolant; "
                                f"value = {best}"},
         "p50_ms": m["p50_ms"], "p99_ms": m["p99_ms"],
+        "baseline_sample": nbase,
         "serving": out,
     }
     print(json.dumps(line))

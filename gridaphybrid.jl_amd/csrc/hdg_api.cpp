@@ -34,19 +34,20 @@ struct hdg_ctx {
 
 namespace {
 
-// RAII device buffer
+// RAII device buffer (blocks come from and return to the library's device
+// cache: see dev_alloc)
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;
-  ~DevBuf() {
-    if (p) (void)hipFree(p);
-  }
+  ~DevBuf() { dev_free(p, bytes); }
   hipError_t alloc(size_t b) {
-    if (p) (void)hipFree(p);
+    dev_free(p, bytes);
     p = nullptr;
     bytes = b;
     if (b == 0) return hipSuccess;
-    return hipMalloc(&p, b);
+    hipError_t e = dev_alloc(&p, b);
+    if (e != hipSuccess) bytes = 0;
+    return e;
   }
   template <class T>
   T* as() const {
@@ -227,7 +228,12 @@ int hdg_ctx_create(int device, hdg_ctx** out) {
   c->device = device;
   if (hipSetDevice(device) != hipSuccess) return HDG_ERR_HIP;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return HDG_ERR_HIP;
-  if (hipMalloc(&c->d_count, sizeof(int64_t)) != hipSuccess) return HDG_ERR_ALLOC;
+  if (hipMalloc(&c->d_count, sizeof(int64_t)) != hipSuccess) {
+    (void)hipStreamDestroy(c->stream);
+    return HDG_ERR_ALLOC;
+  }
+  // the library's kernels run on this stream: freed blocks wait for it before reuse
+  dev_stream_register(c->stream);
   // the allocation-free entry point's scratch on this device (its failure only
   // disables that entry point, which then reports it)
   c->scratch_status = scratch_init(device);
@@ -242,7 +248,10 @@ int hdg_ctx_destroy(hdg_ctx* ctx) {
   if (ctx->d_count) (void)hipFree(ctx->d_count);
   if (ctx->d_perm) (void)hipFree(ctx->d_perm);
   workspace_release(ctx->ws);
-  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  if (ctx->stream) {
+    dev_stream_unregister(ctx->stream);
+    (void)hipStreamDestroy(ctx->stream);
+  }
   delete ctx;
   return HDG_OK;
 }

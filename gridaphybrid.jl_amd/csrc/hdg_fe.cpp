@@ -15,8 +15,11 @@
 // total degree <= k with the lattice nodes of that order (nodal basis from a
 // Vandermonde solve in centred monomials).
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <utility>
 
 #include "hdg_internal.h"
@@ -285,8 +288,36 @@ void facet_point(int D, int f, const double* s, double* xi) {
 
 }  // namespace
 
+static int build_tables_uncached(int phys, int D, int space, int kp, int kq, int km,
+                                 int quad_degree, int proj, TablesHost& t, std::string& err);
+
+// The tables of a class depend only on its description, and meshes (and the
+// footprint queries of a serving loop) ask for the same few classes again and
+// again: they are built once per process (up to 40 ms for 3D Darcy k=3) and
+// copied from then on.
 int build_tables(int phys, int D, int space, int kp, int kq, int km, int quad_degree, int proj,
                  TablesHost& t, std::string& err) {
+  static std::mutex mu;
+  static std::map<std::array<int, 8>, TablesHost> cache;
+  const std::array<int, 8> key{phys, D, space, kp, kq, km, quad_degree, proj};
+  {
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = cache.find(key);
+    if (it != cache.end()) {
+      t = it->second;
+      return 0;
+    }
+  }
+  const int rc = build_tables_uncached(phys, D, space, kp, kq, km, quad_degree, proj, t, err);
+  if (rc == 0) {
+    std::lock_guard<std::mutex> lock(mu);
+    cache.emplace(key, t);
+  }
+  return rc;
+}
+
+static int build_tables_uncached(int phys, int D, int space, int kp, int kq, int km,
+                                 int quad_degree, int proj, TablesHost& t, std::string& err) {
   if (D != 2 && D != 3) {
     err = "dimension must be 2 or 3";
     return -1;

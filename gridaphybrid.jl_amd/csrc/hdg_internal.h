@@ -200,6 +200,19 @@ struct Workspace {
 hipError_t workspace_reserve(Workspace& w, size_t bytes);
 void workspace_release(Workspace& w);
 
+// Device memory of the library's buffers and workspaces.  hipFree waits for
+// the whole device, so a freed block goes to a per-device cache and is handed
+// out again for a request of at most its size and at least half of it (a
+// serving loop frees and allocates every round while another round runs); a
+// failed hipMalloc empties the device's cache and retries.
+// A cached block is handed out again only once the work queued, when it was
+// freed, on every stream registered here has completed.
+hipError_t dev_alloc(void** p, size_t bytes);
+void dev_free(void* p, size_t bytes);
+void dev_cache_flush(int device);
+void dev_stream_register(hipStream_t s);
+void dev_stream_unregister(hipStream_t s);
+
 // Kernel launchers (hdg_kernels.hip)
 bool condense_supported(int n);
 // fam: kernel family (-1 = default for the size; see condense_variants);

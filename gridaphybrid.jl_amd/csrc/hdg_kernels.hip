@@ -18,6 +18,9 @@
 #include <algorithm>
 #include <atomic>
 #include <climits>
+#include <map>
+#include <mutex>
+#include <vector>
 
 #include "hdg_internal.h"
 
@@ -2497,20 +2500,123 @@ static int panel_grid(int n, int64_t ncells, int gridmul) {
   return (int)std::min<int64_t>(ncells, 256 * per);
 }
 
+namespace {
+// A cached block and the events recorded, when it was freed, on every live
+// stream of the library: it is handed out again only once all of them have
+// completed (work queued before the free may still be using it).
+struct CachedBlock {
+  void* p;
+  std::vector<hipEvent_t> ev;
+};
+struct DevCache {
+  std::mutex mu;
+  std::multimap<size_t, CachedBlock> free[64];  // per device, by size
+  std::vector<hipStream_t> streams[64];         // live streams of the library, per device
+};
+DevCache& dev_cache() {
+  static DevCache c;
+  return c;
+}
+int current_device() {
+  int d = 0;
+  if (hipGetDevice(&d) != hipSuccess || d < 0 || d >= 64) d = 0;
+  return d;
+}
+void drop_events(CachedBlock& b) {
+  for (hipEvent_t e : b.ev) (void)hipEventDestroy(e);
+  b.ev.clear();
+}
+}  // namespace
+
+void dev_stream_register(hipStream_t s) {
+  DevCache& c = dev_cache();
+  std::lock_guard<std::mutex> lock(c.mu);
+  c.streams[current_device()].push_back(s);
+}
+
+void dev_stream_unregister(hipStream_t s) {
+  DevCache& c = dev_cache();
+  std::lock_guard<std::mutex> lock(c.mu);
+  auto& v = c.streams[current_device()];
+  v.erase(std::remove(v.begin(), v.end(), s), v.end());
+}
+
+void dev_cache_flush(int device) {
+  DevCache& c = dev_cache();
+  std::lock_guard<std::mutex> lock(c.mu);
+  for (auto& kv : c.free[device]) {
+    drop_events(kv.second);
+    (void)hipFree(kv.second.p);  // (waits for the device)
+  }
+  c.free[device].clear();
+}
+
+hipError_t dev_alloc(void** p, size_t bytes) {
+  *p = nullptr;
+  if (bytes == 0) return hipSuccess;
+  const int dev = current_device();
+  DevCache& c = dev_cache();
+  {
+    std::lock_guard<std::mutex> lock(c.mu);
+    auto& fl = c.free[dev];
+    for (auto it = fl.lower_bound(bytes); it != fl.end() && it->first / 2 <= bytes; ++it) {
+      bool ready = true;
+      for (hipEvent_t e : it->second.ev) ready = ready && hipEventQuery(e) == hipSuccess;
+      if (!ready) continue;
+      *p = it->second.p;
+      drop_events(it->second);
+      fl.erase(it);
+      return hipSuccess;
+    }
+  }
+  hipError_t e = hipMalloc(p, bytes);
+  if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) {
+    (void)hipGetLastError();
+    dev_cache_flush(dev);
+    e = hipMalloc(p, bytes);
+  }
+  if (e != hipSuccess) *p = nullptr;
+  return e;
+}
+
+// The block keeps the size it was requested with: a later request of at most
+// that size and at least half of it reuses it.
+void dev_free(void* p, size_t bytes) {
+  if (!p) return;
+  const int dev = current_device();
+  DevCache& c = dev_cache();
+  std::lock_guard<std::mutex> lock(c.mu);
+  CachedBlock b{p, {}};
+  for (hipStream_t s : c.streams[dev]) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) continue;
+    hipEvent_t e = nullptr;
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess ||
+        hipEventRecord(e, s) != hipSuccess) {
+      if (e) (void)hipEventDestroy(e);
+      drop_events(b);
+      (void)hipGetLastError();
+      (void)hipFree(p);  // no way to order the reuse: give the block back (waits)
+      return;
+    }
+    b.ev.push_back(e);
+  }
+  c.free[dev].emplace(bytes, std::move(b));
+}
+
 hipError_t workspace_reserve(Workspace& w, size_t bytes) {
   if (bytes <= w.bytes) return hipSuccess;
   workspace_release(w);
-  hipError_t e = hipMalloc(&w.p, bytes);
-  if (e != hipSuccess) {
-    w.p = nullptr;
-    return e;
-  }
+  void* p = nullptr;
+  hipError_t e = dev_alloc(&p, bytes);
+  if (e != hipSuccess) return e;
+  w.p = static_cast<double*>(p);
   w.bytes = bytes;
   return hipSuccess;
 }
 
 void workspace_release(Workspace& w) {
-  if (w.p) (void)hipFree(w.p);
+  dev_free(w.p, w.bytes);
   w.p = nullptr;
   w.bytes = 0;
 }
@@ -3535,6 +3641,8 @@ __global__ void __launch_bounds__(kRefThreads, 2)
   // sum_k H[k][rl] z[k], z_e[j] = sum_a c_ea b_q,a[j]; the SPL partial sums of
   // a row sit in consecutive lanes of one wavefront
   const int xrl = tid / SPL, xpart = tid - xrl * SPL;
+  int xe0 = 0, xj0 = xpart;  // the thread's first term k = xe0 * np + xj0
+  while (xj0 >= np) xj0 -= np, ++xe0;
   double nal, nbv;
   fetch(group, nal, nbv);
   for (int64_t lc = group; lc < count; lc += ngroups) {
@@ -3550,14 +3658,15 @@ __global__ void __launch_bounds__(kRefThreads, 2)
     const bool bad = !(a0 != 0.0);  // a singular C leaves a zero system
     const double inva = bad ? 0.0 : 1.0 / a0;
     if (xrl < rows) {
+      // (c_ea read from LDS: a register array indexed by the runtime e would
+      // become a chain of selects; -z is exact, so z = -sum_a alpha b)
       double x = 0.0;
-      int e = 0, j = xpart;  // k = e * np + j
-      while (j >= np) j -= np, ++e;
+      int e = xe0, j = xj0;  // k = e * np + j
       for (int k = xpart; k < nk; k += SPL) {
-        double z = 0.0;
+        double zn = 0.0;
 #pragma unroll
-        for (int a = 0; a < D; ++a) z = fma(c[e * D + a], s_b[a * np + j], z);
-        x = fma(s_H[xrl * nk + k], z, x);
+        for (int a = 0; a < D; ++a) zn = fma(s_al[1 + e * D + a], s_b[a * np + j], zn);
+        x = fma(s_H[xrl * nk + k], -zn, x);
         j += SPL;
         while (j >= np) j -= np, ++e;
       }

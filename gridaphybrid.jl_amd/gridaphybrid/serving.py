@@ -117,12 +117,19 @@ class _Round:
         import torch
         gh = server.gh
         self.server, self.ctx, self.reqs = server, ctx, reqs
+        t0 = time.perf_counter()
         self.ops = [build_operator(gh, ctx, r) for r in reqs]
         for r, op in zip(reqs, self.ops):
             r.admitted = now
             r.cells = op.info.ncells
+        t1 = time.perf_counter()
         self.batch = gh.HybridOperatorBatch(self.ops, autotune=False)
+        t2 = time.perf_counter()
         self.batch.launch(server.precision)
+        t3 = time.perf_counter()
+        server.host_s["operators"] += t1 - t0
+        server.host_s["batch_graph"] += t2 - t1
+        server.host_s["launch"] += t3 - t2
         self.ev = torch.cuda.Event()
         self.ev.record(ctx.torch_stream)
         self.stage = "condense"
@@ -134,6 +141,7 @@ class _Round:
         if not self.ev.query():
             return False
         ctx = self.ctx
+        t0 = time.perf_counter()
         if self.stage == "condense":
             lams, outs = [], []
             with ctx.stream():
@@ -153,6 +161,7 @@ class _Round:
                 self.ev.record(ctx.torch_stream)
             self.lams = lams
             self.stage = "recover"
+            self.server.host_s["skeleton_and_recover"] += time.perf_counter() - t0
             return False
         for r, op, out in zip(self.reqs, self.ops, self.outs):
             r.result = out.cpu().numpy()
@@ -164,26 +173,36 @@ class _Round:
                 r.result = op.recover(self.lams[self.ops.index(op)]).cpu().numpy()
                 r.fp64_fallback = True
             r.done = self.server.clock()
+            if not self.server.keep_results:  # the solution reached the host; keep its norm
+                r.extra["norm"] = float(np.linalg.norm(r.result))
+                r.result = None
+        self.server.host_s["results_to_host"] += time.perf_counter() - t0
         return True
 
     def release(self):
+        t0 = time.perf_counter()
         self.batch.close()
         for op in self.ops:
             op.close()
+        self.server.host_s["release"] += time.perf_counter() - t0
 
 
 class Server:
     """Continuous-batching HDG solve server on one GPU."""
 
     def __init__(self, gh, hbm_budget=200 << 30, precision=MIXED, streams=2, max_batch=64,
-                 skeleton="interpolant", batching=True):
+                 skeleton="interpolant", batching=True, keep_results=True):
         self.gh = gh
+        self.keep_results = keep_results
         self.ctxs = [gh.Context(0) for _ in range(streams)]
         self.budget = int(hbm_budget)
         self.precision = precision
         self.max_batch = max_batch if batching else 1
         self.skeleton = skeleton
         self.t0 = None
+        # host seconds per phase of the loop (the GPU works behind most of them)
+        self.host_s = dict.fromkeys(("footprint", "operators", "batch_graph", "launch",
+                                     "skeleton_and_recover", "results_to_host", "release"), 0.0)
 
     def clock(self):
         return time.perf_counter() - self.t0
@@ -205,7 +224,9 @@ class Server:
             now = self.clock()
             while pending and pending[0].arrival <= now:
                 r = pending.popleft()
+                tf = time.perf_counter()
                 r.footprint = footprint(r, self.precision)
+                self.host_s["footprint"] += time.perf_counter() - tf
                 if r.footprint > self.budget:
                     self.rejected.append(r)
                 else:
@@ -261,11 +282,14 @@ class Server:
 
 
 def summarize(requests, wall=None):
+    """Latency percentiles, throughput (cells/s and requests/s over the span
+    from the first arrival to the last completion, or `wall`)."""
     lat = np.array([r.latency for r in requests if r.latency is not None])
     cells = sum(r.cells for r in requests if r.done is not None)
     done = [r for r in requests if r.done is not None]
     span = wall if wall is not None else (max(r.done for r in done) - min(r.arrival for r in done))
     return dict(requests=len(done), rejected=len(requests) - len(done),
+                requests_per_s=len(done) / span if span > 0 else None,
                 p50_ms=float(np.percentile(lat, 50) * 1e3) if len(lat) else None,
                 p99_ms=float(np.percentile(lat, 99) * 1e3) if len(lat) else None,
                 mean_ms=float(lat.mean() * 1e3) if len(lat) else None,
