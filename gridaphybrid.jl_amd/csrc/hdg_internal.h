@@ -203,8 +203,9 @@ void workspace_release(Workspace& w);
 // Device memory of the library's buffers and workspaces.  hipFree waits for
 // the whole device, so a freed block goes to a per-device cache and is handed
 // out again for a request of at most its size and at least half of it (a
-// serving loop frees and allocates every round while another round runs); a
-// failed hipMalloc empties the device's cache and retries.
+// serving loop frees and allocates every round while another round runs).
+// The cache is emptied (hipFree) when an allocation would leave less than a
+// few GB of device memory free, and before retrying a failed hipMalloc.
 // A cached block is handed out again only once the work queued, when it was
 // freed, on every stream registered here has completed.
 hipError_t dev_alloc(void** p, size_t bytes);

@@ -2500,6 +2500,9 @@ static int panel_grid(int n, int64_t ncells, int gridmul) {
   return (int)std::min<int64_t>(ncells, 256 * per);
 }
 
+// free device memory kept beyond an allocation before cached blocks are returned
+constexpr size_t kDevFreeMargin = size_t(4) << 30;
+
 namespace {
 // A cached block and the events recorded, when it was freed, on every live
 // stream of the library: it is handed out again only once all of them have
@@ -2534,11 +2537,18 @@ void dev_stream_register(hipStream_t s) {
   c.streams[current_device()].push_back(s);
 }
 
+// The last context of a device going away returns the device's cached blocks.
 void dev_stream_unregister(hipStream_t s) {
+  const int dev = current_device();
   DevCache& c = dev_cache();
-  std::lock_guard<std::mutex> lock(c.mu);
-  auto& v = c.streams[current_device()];
-  v.erase(std::remove(v.begin(), v.end(), s), v.end());
+  bool last = false;
+  {
+    std::lock_guard<std::mutex> lock(c.mu);
+    auto& v = c.streams[dev];
+    v.erase(std::remove(v.begin(), v.end(), s), v.end());
+    last = v.empty();
+  }
+  if (last) dev_cache_flush(dev);
 }
 
 void dev_cache_flush(int device) {
@@ -2569,8 +2579,14 @@ hipError_t dev_alloc(void** p, size_t bytes) {
       return hipSuccess;
     }
   }
+  // keep a margin of free device memory for the runtime and other users: the
+  // cached blocks go back first when an allocation would eat into it
+  size_t fr = 0, tot = 0;
+  if (hipMemGetInfo(&fr, &tot) == hipSuccess && fr < bytes + kDevFreeMargin) {
+    dev_cache_flush(dev);
+  }
   hipError_t e = hipMalloc(p, bytes);
-  if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) {
+  if (e != hipSuccess) {
     (void)hipGetLastError();
     dev_cache_flush(dev);
     e = hipMalloc(p, bytes);

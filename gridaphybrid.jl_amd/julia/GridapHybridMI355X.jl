@@ -343,11 +343,15 @@ function GridapHybrid.HybridAffineFEOperator(wf::HDGWeakForm, trial::MultiFieldF
                                              bulk_fields::Vector{<:Integer},
                                              skeleton_fields::Vector{<:Integer};
                                              precision=HDG_FP64, ctx::Context=default_context())
-  (bulk_fields == [1, 2] && skeleton_fields == [3]) ||
-    error("the GPU path hybridizes fields [1,2] onto [3]; pass the weak form as a plain " *
-          "Function to use GridapHybrid's own operator")
   model = get_background_model(get_triangulation(trial.spaces[1]))
   D = num_cell_dims(model)
+  el = wf isa ElasticityHDGWeakForm
+  # elasticity: the displacement trace as one vector field [3] or as one
+  # scalar field per component [3, ..., 2+D] (src/HybridAffineFEOperators.jl:343-356)
+  ntrace = el && skeleton_fields == collect(3:2 + D) ? D : 1
+  (bulk_fields == [1, 2] && (skeleton_fields == [3] || ntrace > 1)) ||
+    error("the GPU path hybridizes fields [1,2] onto [3] (elasticity: or onto [3, ..., 2+D]); " *
+          "pass the weak form as a plain Function to use GridapHybrid's own operator")
   is_n_cube(first(get_polytopes(model))) ||
     error("the GPU path integrates parallelogram/parallelepiped cells only")
   grid = get_grid(model)
@@ -370,7 +374,6 @@ function GridapHybrid.HybridAffineFEOperator(wf::HDGWeakForm, trial::MultiFieldF
   L = trial.spaces[3]
   fids = get_cell_dof_ids(L)
   fdir = Int8[any(<(0), fids[f]) for f in 1:nf]
-  el = wf isa ElasticityHDGWeakForm
   coef = el ? (wf.ν isa Float64 ? fill(wf.ν, nc) : wf.ν) : (wf.κ isa Float64 ? fill(wf.κ, nc) : wf.κ)
   length(coef) == nc || error("one coefficient per cell expected")
   k = wf.order
@@ -386,14 +389,18 @@ function GridapHybrid.HybridAffineFEOperator(wf::HDGWeakForm, trial::MultiFieldF
                      (Ptr{Cvoid}, Ref{HdgProblem}, Ref{HdgMeshDesc}, Ptr{Ptr{Cvoid}}),
                      ctx.h, prob, desc, h), "hdg_mesh_create")
   end
+  ntrace > 1 && check(ctx, ccall((:hdg_mesh_set_trace_fields, libhdg), Cint, (Ptr{Cvoid}, Cint),
+                                 h[], ntrace), "hdg_mesh_set_trace_fields")
   info = Ref{HdgMeshInfo}()
   check(ctx, ccall((:hdg_mesh_get_info, libhdg), Cint, (Ptr{Cvoid}, Ref{HdgMeshInfo}), h[], info),
         "hdg_mesh_get_info")
   num_free_dofs(trial) == info[].nfree_total ||
     error("the spaces of X do not match the weak form's orders ($(num_free_dofs(trial)) free " *
           "dofs, the GPU path has $(info[].nfree_total))")
-  # Dirichlet values of the trial trace space, in Gridap's Dirichlet numbering
-  lam_dir = Vector{Float64}(get_dirichlet_dof_values(L))
+  # Dirichlet values of the trial trace space(s), in Gridap's Dirichlet numbering
+  lam_dir = ntrace > 1 ?
+    reduce(vcat, [Vector{Float64}(get_dirichlet_dof_values(trial.spaces[2 + i])) for i in 1:ntrace]) :
+    Vector{Float64}(get_dirichlet_dof_values(L))
   check(ctx, ccall((:hdg_mesh_set_dirichlet, libhdg), Cint, (Ptr{Cvoid}, Ptr{Float64}), h[],
                    lam_dir), "hdg_mesh_set_dirichlet")
   # sources at the cell quadrature points, x = x0 + J xi
