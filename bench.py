@@ -131,6 +131,8 @@ def parse():
     ap.add_argument("--serve-loops", default="fp64,mixed,per_request",
                     help="serving loops to run: fp64 and mixed continuous batching, "
                          "per_request (one request at a time, FP64)")
+    ap.add_argument("--serve-budget-gib", type=float, default=160.0,
+                    help="device memory the serving loop admits requests into (footprints)")
     ap.add_argument("--baseline-sample", type=int, default=200,
                     help="requests served by the one-request-at-a-time baseline")
     ap.add_argument("--batch", action="store_true",
@@ -429,7 +431,8 @@ def serve_main(gh, args):
              dict(precision=gh.FP64, streams=1, batching=False), nbase)):
         if key not in loops:
             continue
-        srv = sv.Server(gh, hbm_budget=200 << 30, keep_results=False, **kw)
+        srv = sv.Server(gh, hbm_budget=int(args.serve_budget_gib * 2**30), keep_results=False,
+                        **kw)
         rs = fresh(n)
         t0 = time.perf_counter()
         srv.run(rs, progress=lambda m: print(f"[{name}] {m}", file=sys.stderr, flush=True))
@@ -438,6 +441,7 @@ def serve_main(gh, args):
         summ["rounds"] = len(srv.rounds)
         summ["mean_round"] = float(np.mean(srv.rounds)) if srv.rounds else 0.0
         summ["host_s"] = {k: round(v, 3) for k, v in srv.host_s.items()}
+        summ["memory_retries"] = srv.memory_retries
         out[name] = summ
         srv.close()
         print(f"[{name}] {json.dumps(summ)}", file=sys.stderr, flush=True)

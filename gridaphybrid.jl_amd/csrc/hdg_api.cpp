@@ -29,23 +29,26 @@ struct hdg_ctx {
   int32_t* d_perm = nullptr;   // field permutation of the map-level API
   int perm_cap = 0;
   Workspace ws;                // large-system kernel workspace of this context
+  int pool = -1;               // block pool of this context's buffers
   hipError_t scratch_status = hipErrorNotInitialized;  // scratch_init on this device
 };
 
 namespace {
 
-// RAII device buffer (blocks come from and return to the library's device
-// cache: see dev_alloc)
+// RAII device buffer (blocks come from and return to the context's block
+// pool in effect: see PoolScope)
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;
-  ~DevBuf() { dev_free(p, bytes); }
+  int pool = -1;  // the block pool it came from
+  ~DevBuf() { dev_free(p, bytes, pool); }
   hipError_t alloc(size_t b) {
-    dev_free(p, bytes);
+    dev_free(p, bytes, pool);
     p = nullptr;
+    pool = -1;
     bytes = b;
     if (b == 0) return hipSuccess;
-    hipError_t e = dev_alloc(&p, b);
+    hipError_t e = dev_alloc(&p, b, &pool);
     if (e != hipSuccess) bytes = 0;
     return e;
   }
@@ -232,8 +235,7 @@ int hdg_ctx_create(int device, hdg_ctx** out) {
     (void)hipStreamDestroy(c->stream);
     return HDG_ERR_ALLOC;
   }
-  // the library's kernels run on this stream: freed blocks wait for it before reuse
-  dev_stream_register(c->stream);
+  c->pool = dev_pool_create(device);
   // the allocation-free entry point's scratch on this device (its failure only
   // disables that entry point, which then reports it)
   c->scratch_status = scratch_init(device);
@@ -248,10 +250,8 @@ int hdg_ctx_destroy(hdg_ctx* ctx) {
   if (ctx->d_count) (void)hipFree(ctx->d_count);
   if (ctx->d_perm) (void)hipFree(ctx->d_perm);
   workspace_release(ctx->ws);
-  if (ctx->stream) {
-    dev_stream_unregister(ctx->stream);
-    (void)hipStreamDestroy(ctx->stream);
-  }
+  dev_pool_destroy(ctx->pool);  // (the stream is idle)
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return HDG_OK;
 }
@@ -259,6 +259,13 @@ int hdg_ctx_destroy(hdg_ctx* ctx) {
 const char* hdg_last_error(const hdg_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
 
 void* hdg_ctx_stream(hdg_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+int hdg_ctx_memory(hdg_ctx* ctx, int64_t* live_bytes, int64_t* pooled_bytes,
+                   int64_t* unpooled_bytes) {
+  if (!ctx || !live_bytes || !pooled_bytes || !unpooled_bytes) return HDG_ERR_ARG;
+  dev_pool_stats(ctx->pool, live_bytes, pooled_bytes, unpooled_bytes);
+  return HDG_OK;
+}
 
 int hdg_ctx_synchronize(hdg_ctx* ctx) {
   if (!ctx) return HDG_ERR_ARG;
@@ -378,6 +385,7 @@ int hdg_static_condensation(hdg_ctx* ctx, const hdg_fields* fields, int64_t ncel
                             const double* A, const double* b, double* S, double* g,
                             int32_t* info) {
   if (!ctx) return HDG_ERR_ARG;
+  PoolScope pool_(ctx->pool);
   if (ncells < 0) return fail(ctx, HDG_ERR_ARG, "ncells < 0");
   std::vector<int32_t> perm;
   int ni, nb;
@@ -479,6 +487,7 @@ int hdg_backward_static_condensation(hdg_ctx* ctx, const hdg_fields* fields, int
                                      const double* A, const double* b, const double* x,
                                      double* xi, int32_t* info) {
   if (!ctx) return HDG_ERR_ARG;
+  PoolScope pool_(ctx->pool);
   if (ncells < 0) return fail(ctx, HDG_ERR_ARG, "ncells < 0");
   std::vector<int32_t> perm;
   int ni, nb;
@@ -506,6 +515,7 @@ int hdg_backward_static_condensation(hdg_ctx* ctx, const hdg_fields* fields, int
 int hdg_mesh_create(hdg_ctx* ctx, const hdg_problem* prob, const hdg_mesh_desc* desc,
                     hdg_mesh** out) {
   if (!ctx) return HDG_ERR_ARG;
+  PoolScope pool_(ctx->pool);
   if (!prob || !desc || !out) return fail(ctx, HDG_ERR_ARG, "null argument");
   *out = nullptr;
   const int D = prob->dim;
@@ -1119,6 +1129,7 @@ int hdg_mesh_interpolate_trace(hdg_mesh* m, double* lam_free) {
 int hdg_mesh_set_trace_fields(hdg_mesh* m, int nfields) {
   if (!m) return HDG_ERR_ARG;
   hdg_ctx* ctx = m->ctx;
+  PoolScope pool_(ctx->pool);
   const int ntf = m->th.ntf, nm = m->th.nm, ncm = m->th.ncm, nb = m->th.nb;
   if (nfields == 1) {
     m->trace_fields = 1;
@@ -1217,6 +1228,7 @@ int hdg_mesh_set_source(hdg_mesh* m, const double* src_qp) {
 int hdg_condense(hdg_mesh* m, int precision) {
   if (!m) return HDG_ERR_ARG;
   hdg_ctx* ctx = m->ctx;
+  PoolScope pool_(ctx->pool);
   if (precision != HDG_FP64 && precision != HDG_MIXED)
     return fail(ctx, HDG_ERR_ARG, "unknown precision");
   if (m->prob.source == HDG_SOURCE_QP && !m->src_qp)
@@ -1263,6 +1275,7 @@ int hdg_condense(hdg_mesh* m, int precision) {
 int hdg_assemble(hdg_mesh* m) {
   if (!m) return HDG_ERR_ARG;
   hdg_ctx* ctx = m->ctx;
+  PoolScope pool_(ctx->pool);
   if (!m->condensed) return fail(ctx, HDG_ERR_STATE, "assemble called before condense");
   HIP_TRY(ctx, hipSetDevice(ctx->device), "hipSetDevice");
   HIP_TRY(ctx, launch_assemble(m->pairs(), m->pS, m->pg,
@@ -1302,6 +1315,7 @@ int hdg_condense_assemble(hdg_mesh* m, int precision, int64_t* n_singular) {
 int hdg_mesh_refinement_stats(hdg_mesh* m, hdg_refinement_stats* st) {
   if (!m || !st) return HDG_ERR_ARG;
   hdg_ctx* ctx = m->ctx;
+  PoolScope pool_(ctx->pool);
   std::memset(st, 0, sizeof(*st));
   st->mixed = m->mixed ? 1 : 0;
   st->fp64_fallback = m->fp64_fallback ? 1 : 0;
@@ -1329,6 +1343,7 @@ int hdg_skeleton_system(hdg_mesh* m, const int64_t** colptr, const int32_t** row
     if (!m->condensed) return fail(m->ctx, HDG_ERR_STATE, "skeleton system not assembled yet");
     // the facet-major system renumbered field-major (one gather per call)
     hdg_ctx* ctx = m->ctx;
+    PoolScope pool_(ctx->pool);
     HIP_TRY(ctx, hipSetDevice(ctx->device), "hipSetDevice");
     HIP_TRY(ctx, launch_gather(m->d_nzval.as<double>(), m->d_tf_src.as<int64_t>(),
                                m->d_tf_nzval.as<double>(), m->nnz, ctx->stream), "renumber nzval");
@@ -1398,6 +1413,7 @@ int recover_facet_major(hdg_mesh* m, const double* lam_free, double* free_values
 int hdg_recover(hdg_mesh* m, const double* lam_free, double* free_values) {
   if (!m) return HDG_ERR_ARG;
   hdg_ctx* ctx = m->ctx;
+  PoolScope pool_(ctx->pool);
   if (!m->condensed) return fail(ctx, HDG_ERR_STATE, "recover called before condense");
   if ((m->nfree > 0 && !lam_free) || !free_values) return fail(ctx, HDG_ERR_ARG, "null buffer");
   HIP_TRY(ctx, hipSetDevice(ctx->device), "hipSetDevice");
@@ -1417,6 +1433,7 @@ int hdg_recover(hdg_mesh* m, const double* lam_free, double* free_values) {
 namespace {
 int recover_facet_major(hdg_mesh* m, const double* lam_free, double* free_values) {
   hdg_ctx* ctx = m->ctx;
+  PoolScope pool_(ctx->pool);
   if (m->mixed) {
     HIP_TRY(ctx, launch_recover_mixed(m->tdev, m->meshdev(), m->pY, m->pXinv,
                                       m->d_cell_dofs.as<int32_t>(), lam_free,
@@ -1451,6 +1468,7 @@ int recover_facet_major(hdg_mesh* m, const double* lam_free, double* free_values
 int hdg_cell_systems(hdg_mesh* m, int64_t first, int64_t count, double* A, double* b) {
   if (!m) return HDG_ERR_ARG;
   hdg_ctx* ctx = m->ctx;
+  PoolScope pool_(ctx->pool);
   if (first < 0 || count < 0 || first + count > m->ncells || (count > 0 && (!A || !b)))
     return fail(ctx, HDG_ERR_ARG, "cell range");
   if (m->prob.source == HDG_SOURCE_QP && !m->src_qp)
@@ -1604,6 +1622,7 @@ hipError_t enqueue_all(hdg_batch* b, int precision, hipStream_t st) {
 // run.
 int autotune(hdg_batch* b) {
   hdg_ctx* ctx = b->ctx;
+  PoolScope pool_(ctx->pool);
   hipStream_t st = ctx->stream;
   hipEvent_t e0, e1;
   HIP_TRY(ctx, hipEventCreate(&e0), "event");
@@ -1663,6 +1682,7 @@ int autotune(hdg_batch* b) {
 
 int capture(hdg_batch* b, int precision) {
   hdg_ctx* ctx = b->ctx;
+  PoolScope pool_(ctx->pool);
   hipStream_t st = ctx->stream;
   b->drop_graph();
   // every branch has its own workspace (the branches run concurrently)
@@ -1715,6 +1735,7 @@ extern "C" {
 int hdg_batch_create(hdg_ctx* ctx, int nmesh, hdg_mesh* const* meshes, int flags,
                      hdg_batch** out) {
   if (!ctx) return HDG_ERR_ARG;
+  PoolScope pool_(ctx->pool);
   if (!out || !meshes || nmesh <= 0) return fail(ctx, HDG_ERR_ARG, "empty batch");
   *out = nullptr;
   if (flags & ~HDG_BATCH_AUTOTUNE) return fail(ctx, HDG_ERR_ARG, "unknown batch flags");
@@ -1811,6 +1832,7 @@ int hdg_batch_create(hdg_ctx* ctx, int nmesh, hdg_mesh* const* meshes, int flags
 int hdg_batch_destroy(hdg_batch* b) {
   if (!b) return HDG_ERR_ARG;
   hdg_ctx* ctx = b->ctx;
+  PoolScope pool_(ctx->pool);
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
   b->drop_graph();
@@ -1840,6 +1862,7 @@ int hdg_batch_get_classes(const hdg_batch* b, int max, hdg_batch_class* classes,
 int hdg_batch_launch(hdg_batch* b, int precision) {
   if (!b) return HDG_ERR_ARG;
   hdg_ctx* ctx = b->ctx;
+  PoolScope pool_(ctx->pool);
   if (precision != HDG_FP64 && precision != HDG_MIXED)
     return fail(ctx, HDG_ERR_ARG, "unknown precision");
   if (!sources_ready(b)) return fail(ctx, HDG_ERR_STATE, "source values at quadrature points not set");

@@ -196,23 +196,34 @@ namespace hdg {
 struct Workspace {
   double* p = nullptr;
   size_t bytes = 0;
+  int pool = -1;  // the block pool it came from
 };
 hipError_t workspace_reserve(Workspace& w, size_t bytes);
 void workspace_release(Workspace& w);
 
 // Device memory of the library's buffers and workspaces.  hipFree waits for
-// the whole device, so a freed block goes to a per-device cache and is handed
-// out again for a request of at most its size and at least half of it (a
-// serving loop frees and allocates every round while another round runs).
-// The cache is emptied (hipFree) when an allocation would leave less than a
-// few GB of device memory free, and before retrying a failed hipMalloc.
-// A cached block is handed out again only once the work queued, when it was
-// freed, on every stream registered here has completed.
-hipError_t dev_alloc(void** p, size_t bytes);
-void dev_free(void* p, size_t bytes);
+// the whole device (a serving loop frees and allocates every round while
+// another round runs), so each context has a pool of freed blocks: a block
+// freed by the context's objects serves the context's later allocations of at
+// most its size and at least half of it.  The context's buffers are used only
+// by work on its one stream, so the reuse is ordered behind their last use.
+// The pool in effect is the innermost PoolScope of the calling thread (none:
+// plain hipMalloc / hipFree); a buffer returns to the pool it came from.  The
+// device's pools are emptied (hipFree) when an allocation would leave less
+// than 8 GB of device memory free, and before retrying a failed hipMalloc.
+struct PoolScope {
+  explicit PoolScope(int pool);
+  ~PoolScope();
+  PoolScope(const PoolScope&) = delete;
+  PoolScope& operator=(const PoolScope&) = delete;
+  int prev;
+};
+int dev_pool_create(int device);
+void dev_pool_destroy(int pool);
+hipError_t dev_alloc(void** p, size_t bytes, int* pool);
+void dev_free(void* p, size_t bytes, int pool);
 void dev_cache_flush(int device);
-void dev_stream_register(hipStream_t s);
-void dev_stream_unregister(hipStream_t s);
+void dev_pool_stats(int pool, int64_t* live, int64_t* pooled, int64_t* unpooled);
 
 // Kernel launchers (hdg_kernels.hip)
 bool condense_supported(int n);

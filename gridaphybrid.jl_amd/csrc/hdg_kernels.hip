@@ -2501,130 +2501,173 @@ static int panel_grid(int n, int64_t ncells, int gridmul) {
 }
 
 // free device memory kept beyond an allocation before cached blocks are returned
-constexpr size_t kDevFreeMargin = size_t(4) << 30;
+constexpr size_t kDevFreeMargin = size_t(8) << 30;
 
 namespace {
-// A cached block and the events recorded, when it was freed, on every live
-// stream of the library: it is handed out again only once all of them have
-// completed (work queued before the free may still be using it).
-struct CachedBlock {
-  void* p;
-  std::vector<hipEvent_t> ev;
+// Per-context block pools.  A context's buffers are only ever touched by work
+// on the context's one stream (its kernels, copies and graph replays), so a
+// block freed into the context's pool can be handed to the context's next
+// allocation at once: that allocation's work is queued behind whatever still
+// used the block.  No pool is shared between contexts.
+struct Pool {
+  int device = 0;
+  bool live = false;
+  std::multimap<size_t, void*> free;  // cached blocks by the size they were requested with
+  int64_t live_bytes = 0, pooled_bytes = 0;
 };
 struct DevCache {
   std::mutex mu;
-  std::multimap<size_t, CachedBlock> free[64];  // per device, by size
-  std::vector<hipStream_t> streams[64];         // live streams of the library, per device
+  std::vector<Pool> pools;
+  int64_t unpooled_bytes = 0;
 };
 DevCache& dev_cache() {
   static DevCache c;
   return c;
 }
-int current_device() {
-  int d = 0;
-  if (hipGetDevice(&d) != hipSuccess || d < 0 || d >= 64) d = 0;
-  return d;
-}
-void drop_events(CachedBlock& b) {
-  for (hipEvent_t e : b.ev) (void)hipEventDestroy(e);
-  b.ev.clear();
+thread_local int tl_pool = -1;
+// hipFree of every cached block of a device; caller holds the lock.  The
+// runtime hands freed memory back once the device has drained, so a flush
+// that freed anything waits for the device: the free memory it reports
+// afterwards is what an allocation can get.
+void flush_device(DevCache& c, int device) {
+  bool any = false;
+  for (Pool& pl : c.pools) {
+    if (pl.device != device) continue;
+    for (auto& kv : pl.free) (void)hipFree(kv.second);
+    any = any || !pl.free.empty();
+    pl.free.clear();
+    pl.pooled_bytes = 0;
+  }
+  if (any) (void)hipDeviceSynchronize();
+  (void)hipGetLastError();
 }
 }  // namespace
 
-void dev_stream_register(hipStream_t s) {
+PoolScope::PoolScope(int pool) : prev(tl_pool) { tl_pool = pool; }
+PoolScope::~PoolScope() { tl_pool = prev; }
+
+int dev_pool_create(int device) {
   DevCache& c = dev_cache();
   std::lock_guard<std::mutex> lock(c.mu);
-  c.streams[current_device()].push_back(s);
+  c.pools.emplace_back();
+  c.pools.back().device = device;
+  c.pools.back().live = true;
+  return (int)c.pools.size() - 1;
 }
 
-// The last context of a device going away returns the device's cached blocks.
-void dev_stream_unregister(hipStream_t s) {
-  const int dev = current_device();
+// The context's stream is idle (synchronized) when its pool goes away.
+void dev_pool_destroy(int pool) {
   DevCache& c = dev_cache();
-  bool last = false;
-  {
-    std::lock_guard<std::mutex> lock(c.mu);
-    auto& v = c.streams[dev];
-    v.erase(std::remove(v.begin(), v.end(), s), v.end());
-    last = v.empty();
-  }
-  if (last) dev_cache_flush(dev);
+  std::lock_guard<std::mutex> lock(c.mu);
+  if (pool < 0 || pool >= (int)c.pools.size()) return;
+  Pool& pl = c.pools[pool];
+  for (auto& kv : pl.free) (void)hipFree(kv.second);
+  pl.free.clear();
+  pl.pooled_bytes = 0;
+  pl.live = false;
+  (void)hipGetLastError();
+}
+
+void dev_pool_stats(int pool, int64_t* live, int64_t* pooled, int64_t* unpooled) {
+  DevCache& c = dev_cache();
+  std::lock_guard<std::mutex> lock(c.mu);
+  const bool ok = pool >= 0 && pool < (int)c.pools.size();
+  *live = ok ? c.pools[pool].live_bytes : 0;
+  *pooled = ok ? c.pools[pool].pooled_bytes : 0;
+  *unpooled = c.unpooled_bytes;
 }
 
 void dev_cache_flush(int device) {
   DevCache& c = dev_cache();
   std::lock_guard<std::mutex> lock(c.mu);
-  for (auto& kv : c.free[device]) {
-    drop_events(kv.second);
-    (void)hipFree(kv.second.p);  // (waits for the device)
-  }
-  c.free[device].clear();
+  flush_device(c, device);
 }
 
-hipError_t dev_alloc(void** p, size_t bytes) {
+hipError_t dev_alloc(void** p, size_t bytes, int* pool_out) {
   *p = nullptr;
+  const int pool = tl_pool;
+  *pool_out = -1;
   if (bytes == 0) return hipSuccess;
-  const int dev = current_device();
   DevCache& c = dev_cache();
+  int device = 0;
   {
     std::lock_guard<std::mutex> lock(c.mu);
-    auto& fl = c.free[dev];
-    for (auto it = fl.lower_bound(bytes); it != fl.end() && it->first / 2 <= bytes; ++it) {
-      bool ready = true;
-      for (hipEvent_t e : it->second.ev) ready = ready && hipEventQuery(e) == hipSuccess;
-      if (!ready) continue;
-      *p = it->second.p;
-      drop_events(it->second);
-      fl.erase(it);
-      return hipSuccess;
+    if (pool >= 0 && pool < (int)c.pools.size() && c.pools[pool].live) {
+      Pool& pl = c.pools[pool];
+      device = pl.device;
+      auto it = pl.free.lower_bound(bytes);
+      if (it != pl.free.end() && it->first / 2 <= bytes) {
+        *p = it->second;
+        pl.pooled_bytes -= (int64_t)it->first;
+        pl.live_bytes += (int64_t)bytes;
+        pl.free.erase(it);
+        *pool_out = pool;
+        return hipSuccess;
+      }
+      // keep a margin of free device memory for the runtime and other users
+      // (the runtime's own allocations, such as scratch, fail hard): the
+      // device's cached blocks go back first when the allocation would eat
+      // into it, and an allocation that would still eat into it is refused
+      size_t fr = 0, tot = 0;
+      if (hipMemGetInfo(&fr, &tot) == hipSuccess && fr < bytes + kDevFreeMargin) {
+        flush_device(c, device);
+        if (hipMemGetInfo(&fr, &tot) == hipSuccess && fr < bytes + kDevFreeMargin) {
+          (void)hipGetLastError();
+          return hipErrorOutOfMemory;
+        }
+      }
+      (void)hipGetLastError();
     }
-  }
-  // keep a margin of free device memory for the runtime and other users: the
-  // cached blocks go back first when an allocation would eat into it
-  size_t fr = 0, tot = 0;
-  if (hipMemGetInfo(&fr, &tot) == hipSuccess && fr < bytes + kDevFreeMargin) {
-    dev_cache_flush(dev);
   }
   hipError_t e = hipMalloc(p, bytes);
   if (e != hipSuccess) {
     (void)hipGetLastError();
-    dev_cache_flush(dev);
+    {
+      std::lock_guard<std::mutex> lock(c.mu);
+      flush_device(c, device);
+    }
     e = hipMalloc(p, bytes);
   }
-  if (e != hipSuccess) *p = nullptr;
-  return e;
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    *p = nullptr;
+    return e;
+  }
+  std::lock_guard<std::mutex> lock(c.mu);
+  if (pool >= 0 && pool < (int)c.pools.size() && c.pools[pool].live) {
+    *pool_out = pool;
+    c.pools[pool].live_bytes += (int64_t)bytes;
+  } else {
+    c.unpooled_bytes += (int64_t)bytes;
+  }
+  return hipSuccess;
 }
 
 // The block keeps the size it was requested with: a later request of at most
-// that size and at least half of it reuses it.
-void dev_free(void* p, size_t bytes) {
+// that size and at least half of it, from the same pool, reuses it.
+void dev_free(void* p, size_t bytes, int pool) {
   if (!p) return;
-  const int dev = current_device();
   DevCache& c = dev_cache();
-  std::lock_guard<std::mutex> lock(c.mu);
-  CachedBlock b{p, {}};
-  for (hipStream_t s : c.streams[dev]) {
-    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-    if (hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) continue;
-    hipEvent_t e = nullptr;
-    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess ||
-        hipEventRecord(e, s) != hipSuccess) {
-      if (e) (void)hipEventDestroy(e);
-      drop_events(b);
-      (void)hipGetLastError();
-      (void)hipFree(p);  // no way to order the reuse: give the block back (waits)
+  {
+    std::lock_guard<std::mutex> lock(c.mu);
+    if (pool >= 0 && pool < (int)c.pools.size() && c.pools[pool].live) {
+      Pool& pl = c.pools[pool];
+      pl.free.emplace(bytes, p);
+      pl.live_bytes -= (int64_t)bytes;
+      pl.pooled_bytes += (int64_t)bytes;
       return;
     }
-    b.ev.push_back(e);
+    if (pool < 0) c.unpooled_bytes -= (int64_t)bytes;
   }
-  c.free[dev].emplace(bytes, std::move(b));
+  (void)hipFree(p);
+  (void)hipGetLastError();
 }
 
 hipError_t workspace_reserve(Workspace& w, size_t bytes) {
   if (bytes <= w.bytes) return hipSuccess;
   workspace_release(w);
   void* p = nullptr;
-  hipError_t e = dev_alloc(&p, bytes);
+  hipError_t e = dev_alloc(&p, bytes, &w.pool);
   if (e != hipSuccess) return e;
   w.p = static_cast<double*>(p);
   w.bytes = bytes;
@@ -2632,9 +2675,10 @@ hipError_t workspace_reserve(Workspace& w, size_t bytes) {
 }
 
 void workspace_release(Workspace& w) {
-  dev_free(w.p, w.bytes);
+  dev_free(w.p, w.bytes, w.pool);
   w.p = nullptr;
   w.bytes = 0;
+  w.pool = -1;
 }
 
 // Kernel families for a local system size (see condense_variants()).
@@ -2854,7 +2898,8 @@ int condense_default_family(const SysShape& sh) {
   // an 8x8 grid per cell), which no two-stage scheme beats
   // (the fused variant, family 9, is an autotuner candidate: its one-wavefront
   // Gauss-Jordan steps make it slower than the reduced grid elimination here)
-  if (sh.flux && !wave_class(sh.ni, sh.nb) && sh.ni + sh.nb > 96) return sh.ref ? kFamRef : kFamFlux;
+  if ((sh.flux || sh.ref) && !wave_class(sh.ni, sh.nb) && sh.ni + sh.nb > 96)
+    return sh.ref ? kFamRef : kFamFlux;
   return default_family(sh.ni, sh.nb);
 }
 
@@ -3216,6 +3261,8 @@ hipError_t launch_recover_mixed(const Tables& T, const MeshDev& M, const double*
 // --------------------------------------------------------------------------
 constexpr int kFluxWaves = 6;
 constexpr int kFluxMaxNp = 64;
+constexpr int kFusedMaxNb = 128;  // boundary values held in LDS by the flux recovery
+constexpr int kRecMaxNa = 192;    // flux rows of the flux recovery (three per lane)
 constexpr int kFluxTileLd = 17;  // leading dimension of a generated row tile
 
 struct FluxPlan {
@@ -3552,10 +3599,9 @@ struct RefPlan {
   size_t lds;  // dynamic LDS bytes
 };
 
-static bool ref_plan(const Tables& T, RefPlan* rp) {
-  if (T.phys != kDarcy || T.fr_npair <= 0 || !T.fr_P || !T.fr_G || !T.fb_minv) return false;
-  if (T.ni > kRefThreads) return false;  // one right-hand-side value per thread
-  const int D = T.D, np = T.np, na = T.na, nu = T.ni - na, nr = nu + T.nb;
+static bool ref_plan_sizes(int D, int np, int na, int ni, int nb, int npair, RefPlan* rp) {
+  if (ni > kRefThreads) return false;  // one right-hand-side value per thread
+  const int nu = ni - na, nr = nu + nb;
   if (nr + 1 >= (1 << 23)) return false;  // entry columns packed above 8 bits of band row
   int BR = std::min(nr, kRefEntries * kRefThreads / (nr + 1));
   BR = std::min(BR, kRefThreads / kRefSplit);  // the rhs column: kRefSplit threads per row
@@ -3564,7 +3610,7 @@ static bool ref_plan(const Tables& T, RefPlan* rp) {
   BR = (nr + nbands - 1) / nbands;  // balance the bands
   // alpha, b_i, then M^-1 (prologue only) aliased with the rhs column, and
   // the band's columns of every H_e = M^-1 G_e
-  const size_t lds = (size_t)kMaxAlpha + T.ni + std::max<size_t>((size_t)np * np, BR) +
+  const size_t lds = (size_t)kMaxAlpha + ni + std::max<size_t>((size_t)np * np, BR) +
                      (size_t)D * np * BR;
   if (lds * sizeof(double) > 64 * 1024) return false;
   if (rp) {
@@ -3574,10 +3620,15 @@ static bool ref_plan(const Tables& T, RefPlan* rp) {
     rp->nu = nu;
     rp->BR = BR;
     rp->nbands = nbands;
-    rp->npair = T.fr_npair;
+    rp->npair = npair;
     rp->lds = lds * sizeof(double);
   }
   return true;
+}
+
+static bool ref_plan(const Tables& T, RefPlan* rp) {
+  if (T.phys != kDarcy || T.fr_npair <= 0 || !T.fr_P || !T.fr_G || !T.fb_minv) return false;
+  return ref_plan_sizes(T.D, T.np, T.na, T.ni, T.nb, T.fr_npair, rp);
 }
 
 template <int D>
@@ -4571,10 +4622,13 @@ static hipError_t launch_flux_phys(const Tables& T, const MeshDev& M, const Cond
                                    hipStream_t st, Workspace& ws, int gridmul, bool ref) {
   FluxPlan fp;
   int ct = 0, ksm = 0;
-  if (!flux_plan(T, fp, &ct, &ksm)) return hipErrorInvalidValue;
   RefPlan rp;
-  if (ref && !ref_plan(T, &rp)) return hipErrorInvalidValue;
-  const int nu = T.ni - T.na, nb = T.nb, nr = fp.nr;
+  if (ref) {
+    if (!sys_shape(T).ref || !ref_plan(T, &rp)) return hipErrorInvalidValue;
+  } else if (!flux_plan(T, fp, &ct, &ksm)) {
+    return hipErrorInvalidValue;
+  }
+  const int nu = T.ni - T.na, nb = T.nb, nr = nu + nb;
   const int rfam = reduced_family(nu, nb);
   if (rfam < 0) return hipErrorInvalidValue;
   const int64_t ncells = M.ncells;
@@ -4583,7 +4637,7 @@ static hipError_t launch_flux_phys(const Tables& T, const MeshDev& M, const Cond
   const int64_t chunk = std::min<int64_t>(ncells, flux_chunk(nr));
   hipError_t e = workspace_reserve(ws, flux_workspace_bytes(nr, ncells));
   if (e != hipSuccess) return e;
-  const size_t lds = flux_lds(fp, T.np);
+  const size_t lds = ref ? 0 : flux_lds(fp, T.np);
   if (lds > 160 * 1024 - sizeof(FusedShared) - 1024) return hipErrorInvalidValue;
   FusedGen<PHYS> gen{T, M};
   // reduced factors and kept blocks in the per-cell slots of the class
@@ -4645,21 +4699,34 @@ hipError_t launch_condense_flux(const Tables& T, const MeshDev& M, const Condens
 
 size_t factor_slot(const SysShape& sh) {
   const size_t dense = (size_t)sh.ni * (sh.ni + 1) / 2;
-  if (!sh.flux) return dense;
+  if (sh.flux_na <= 0) return dense;
   const size_t m = (size_t)(sh.ni - sh.flux_na), na = (size_t)sh.flux_na;
   size_t s = std::max(dense, (size_t)flux_slot((int)na, (int)m).size);
   if (sh.fused) s = std::max(s, fused_slot(sh.flux_na, (int)m));
   return s;
 }
 
+// Family 10 has kernels for the sizes: the band kernel's plan, the reduced
+// elimination (row-per-lane or grid), the inverses (nu <= 64) and the
+// recovery (one wavefront per cell, at most kRecMaxNa flux rows).
+static bool ref_fits(int phys, int D, int fb_nc, int na, int np, int ni, int nb, int npair) {
+  if (phys != kDarcy || npair <= 0 || fb_nc != D || np > kFluxMaxNp || na <= 0 || na >= ni)
+    return false;
+  const int nu = ni - na;
+  if (nu > 64 || na > kRecMaxNa || nb > kFusedMaxNb) return false;
+  if (reduced_family(nu, nb) < 0) return false;
+  return ref_plan_sizes(D, np, na, ni, nb, npair, nullptr);
+}
+
 SysShape sys_shape(const Tables& T) {
   SysShape sh;
   sh.ni = T.ni;
   sh.nb = T.nb;
-  if (T.fb_minv && flux_fits(T.fb_nc, T.na, T.np, T.ni, T.nb, nullptr, nullptr)) sh.flux_na = T.na;
-  sh.flux = sh.flux_na > 0;
+  sh.flux = T.fb_minv && flux_fits(T.fb_nc, T.na, T.np, T.ni, T.nb, nullptr, nullptr);
+  sh.ref = T.fb_minv && T.fr_P && T.fr_G &&
+           ref_fits(T.phys, T.D, T.fb_nc, T.na, T.np, T.ni, T.nb, T.fr_npair);
+  sh.flux_na = sh.flux || sh.ref ? T.na : 0;
   sh.fused = sh.flux && fused_fits(T.fb_nc, T.na, T.np, T.ni, T.nb, nullptr, nullptr, nullptr);
-  sh.ref = sh.flux && T.fr_npair > 0 && ref_plan(T, nullptr);
   return sh;
 }
 
@@ -4667,11 +4734,11 @@ SysShape sys_shape(const TablesHost& t) {
   SysShape sh;
   sh.ni = t.ni;
   sh.nb = t.nb;
-  if (!t.fb_minv.empty() && flux_fits(t.fb_nc, t.na, t.np, t.ni, t.nb, nullptr, nullptr))
-    sh.flux_na = t.na;
-  sh.flux = sh.flux_na > 0;
+  sh.flux = !t.fb_minv.empty() && flux_fits(t.fb_nc, t.na, t.np, t.ni, t.nb, nullptr, nullptr);
+  sh.ref = !t.fb_minv.empty() && !t.fr_P.empty() &&
+           ref_fits(t.phys, t.D, t.fb_nc, t.na, t.np, t.ni, t.nb, t.fr_npair);
+  sh.flux_na = sh.flux || sh.ref ? t.na : 0;
   sh.fused = sh.flux && fused_fits(t.fb_nc, t.na, t.np, t.ni, t.nb, nullptr, nullptr, nullptr);
-  sh.ref = sh.flux && t.fr_npair > 0;
   return sh;
 }
 
@@ -4696,7 +4763,6 @@ SysShape sys_shape(const TablesHost& t) {
 // r_u - A_uq z is formed as c_u - A_uu x_u - A_uq (x_q + z), whose rounding is
 // that of the residual itself.  It stops once a correction is below 2^-48 of
 // the solution.
-constexpr int kFusedMaxNb = 128;  // boundary values held in LDS by the flux recovery
 constexpr int kRecWaves = 4;      // cells (wavefronts) per workgroup of the flux recovery
 
 // doubles of LDS per wavefront of the flux recovery (the last na: the
@@ -4716,24 +4782,29 @@ struct RecSlot {
 };
 
 // REF (family 10): A_qu and A_uq are not read from the slot but applied from
-// the reference matrices G_e (potential columns, staged in LDS once per
-// workgroup) and the cell's cofactors c_ea: A_qu[(a,i)][c] = sum_e c_ea
-// G_e[i][c] and A_uq = -A_qu^T (the Darcy coupling of build_flux_ref).
+// the reference matrices G_e (potential columns) and the cell's cofactors
+// c_ea: A_qu[(a,i)][c] = sum_e c_ea G_e[i][c] and A_uq = -A_qu^T (the Darcy
+// coupling of build_flux_ref).  G_e is staged in LDS once per workgroup
+// (g_lds), or, where it does not fit (3D k=3), read from the table, which
+// every cell shares (L2-resident).
 template <int PHYS, int KQ, bool REF>
 __global__ void __launch_bounds__(256)
     recover_flux_kernel(FusedGen<PHYS> gen, RecSlot rs, const int32_t* __restrict__ cell_dofs,
                         const double* __restrict__ lam_free, const double* __restrict__ lam_dir,
-                        double* __restrict__ out) {
+                        double* __restrict__ out, bool g_lds) {
   constexpr int WPB = kRecWaves;
-  // M^-1 [np][np], M, (REF: G_e [D][np][nu]), then the wavefronts' vectors
+  // M^-1 [np][np], M, (REF with g_lds: G_e [D][np][nu]), then the wavefronts' vectors
   extern __shared__ double s_m[];
   const Tables& T = gen.T;
   const int ni = T.ni, nb = T.nb, na = T.na, np = T.np, nu = ni - na, nqc = T.nqc;
   const int ncomp = T.fb_nc, D = T.D;
   const int64_t ncells = gen.M.ncells;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int gsz = REF ? D * np * nu : 0;
-  double* s_gu = s_m + 2 * np * np;  // REF: G_e[i][c] at (e * np + i) * nu + c
+  const int gsz = REF && g_lds ? D * np * nu : 0;
+  double* s_gu = s_m + 2 * np * np;  // REF with g_lds: G_e[i][c] at (e * np + i) * nu + c
+  // G_e[i][c] at gsrc[(e * np + i) * gld + c]
+  const double* gsrc = g_lds ? s_gu : T.fr_G;
+  const int gld = g_lds ? nu : nu + nb;
   double* alpha = s_m + 2 * np * np + gsz + (int64_t)w * rec_wave_lds(na, nu, nb);  // [kMaxAlpha]
   double* c6 = alpha + kMaxAlpha;  // C, then C^-1 [2][36]
   double* lam = c6 + 72;           // [nb]
@@ -4749,7 +4820,7 @@ __global__ void __launch_bounds__(256)
     s_m[t] = T.fb_minv[t];
     s_m[np * np + t] = T.Mpp[t];
   }
-  if (REF)
+  if (REF && g_lds)
     for (int t = threadIdx.x; t < gsz; t += 64 * WPB) {
       const int e = t / (np * nu), rem = t - e * np * nu, i = rem / nu, c = rem - i * nu;
       s_gu[t] = T.fr_G[((size_t)e * np + i) * (nu + nb) + c];
@@ -4804,7 +4875,7 @@ __global__ void __launch_bounds__(256)
     if (REF) {
       // t_e[i] = sum_c G_e[i][c] v[c]; qv[(a, i)] = sum_e c_ea t_e[i]
       for (int t = lane; t < D * np; t += 64) {
-        const double* g = s_gu + (int64_t)t * nu;
+        const double* g = gsrc + (int64_t)t * gld;
         double acc = 0.0;
 #pragma unroll 4
         for (int c = 0; c < nu; ++c) acc = fma(g[c], v[c], acc);
@@ -4952,7 +5023,7 @@ __global__ void __launch_bounds__(256)
         wave_sync_lds();
         if (lane < nu) {
 #pragma unroll 4
-          for (int t = 0; t < D * np; ++t) ru = fma(s_gu[(int64_t)t * nu + lane], tv[t], ru);
+          for (int t = 0; t < D * np; ++t) ru = fma(gsrc[(int64_t)t * gld + lane], tv[t], ru);
           uv[lane] = ru;
         }
       } else if (lane < nu) {
@@ -5015,7 +5086,7 @@ hipError_t launch_recover_flux(const Tables& T, const MeshDev& M, const double* 
                                const double* lam_dir, double* out, int64_t nfree,
                                hipStream_t st) {
   const int na = T.na, nu = T.ni - na, nb = T.nb;
-  if (na > kMixedMaxNi || nu > 64 || nb > kFusedMaxNb || T.np > kFluxMaxNp)
+  if (na > (ref ? kRecMaxNa : kMixedMaxNi) || nu > 64 || nb > kFusedMaxNb || T.np > kFluxMaxNp)
     return hipErrorInvalidValue;
   if (ref && (fused || T.phys != kDarcy || !T.fr_G)) return hipErrorInvalidValue;
   RecSlot rs;
@@ -5045,16 +5116,22 @@ hipError_t launch_recover_flux(const Tables& T, const MeshDev& M, const double* 
   if (M.ncells > 0) {
     const int64_t blocks = (M.ncells + kRecWaves - 1) / kRecWaves;
     const int grid = (int)(blocks < 65536 ? blocks : 65536);
-    const size_t lds = (2 * (size_t)T.np * T.np + (ref ? (size_t)T.D * T.np * nu : 0) +
-                        kRecWaves * (size_t)rec_wave_lds(na, nu, nb)) *
-                       sizeof(double);
+    const size_t base = (2 * (size_t)T.np * T.np + kRecWaves * (size_t)rec_wave_lds(na, nu, nb)) *
+                        sizeof(double);
+    const size_t gbytes = (size_t)T.D * T.np * nu * sizeof(double);
+    // G_e in LDS while two workgroups still fit a CU
+    const bool g_lds = ref && base + gbytes <= 64 * 1024;
+    const size_t lds = base + (g_lds ? gbytes : 0);
+    if (lds > 160 * 1024) return hipErrorInvalidValue;
     ClockScope clk_(st, "recover_flux");
 #define HDG_RF(P_, K_, R_)                                                                     \
   hipLaunchKernelGGL((recover_flux_kernel<P_, K_, R_>), dim3(grid), dim3(64 * kRecWaves), lds, \
-                     st, FusedGen<P_>{T, M}, rs, cell_dofs, lam_free, lam_dir, out)
+                     st, FusedGen<P_>{T, M}, rs, cell_dofs, lam_free, lam_dir, out, g_lds)
     if (T.phys == kDarcy) {
       if (ref) {
-        if (na <= 64) HDG_RF(kDarcy, 1, true); else HDG_RF(kDarcy, 2, true);
+        if (na <= 64) HDG_RF(kDarcy, 1, true);
+        else if (na <= 128) HDG_RF(kDarcy, 2, true);
+        else HDG_RF(kDarcy, 3, true);
       } else {
         if (na <= 64) HDG_RF(kDarcy, 1, false); else HDG_RF(kDarcy, 2, false);
       }

@@ -36,7 +36,7 @@ SOURCE_QP = 100
 # every symbol include/hdg.h declares
 EXPORTS = [
     "hdg_ctx_create", "hdg_ctx_destroy", "hdg_last_error", "hdg_ctx_stream",
-    "hdg_ctx_synchronize", "hdg_memcpy_d2h", "hdg_memcpy_h2d", "hdg_static_condensation",
+    "hdg_ctx_synchronize", "hdg_ctx_memory", "hdg_memcpy_d2h", "hdg_memcpy_h2d", "hdg_static_condensation",
     "hdg_backward_static_condensation", "hdg_mesh_create", "hdg_mesh_create_cartesian",
     "hdg_mesh_destroy", "hdg_mesh_get_info", "hdg_mesh_set_dirichlet",
     "hdg_mesh_set_dirichlet_mms", "hdg_mesh_set_source", "hdg_condense_assemble",
@@ -153,6 +153,8 @@ def lib():
         L.hdg_ctx_create.argtypes = [i32, C.POINTER(vp)]
         L.hdg_ctx_destroy.argtypes = [vp]
         L.hdg_ctx_synchronize.argtypes = [vp]
+        L.hdg_ctx_memory.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                                     C.POINTER(C.c_int64)]
         L.hdg_memcpy_d2h.argtypes = [vp, vp, vp, i64]
         L.hdg_memcpy_h2d.argtypes = [vp, vp, vp, i64]
         L.hdg_static_condensation.argtypes = [vp, C.POINTER(Fields), i64, vp, vp, vp, vp, vp]

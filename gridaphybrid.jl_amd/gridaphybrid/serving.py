@@ -32,7 +32,11 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import _lib
-from ._lib import FP64, MIXED, check, lib
+from ._lib import FP64, MIXED, HDGError, check, lib
+
+
+def _out_of_memory(err):
+    return err.code == _lib.HDG_ERR_ALLOC or "out of memory" in str(err)
 
 
 @dataclass
@@ -117,16 +121,22 @@ class _Round:
         import torch
         gh = server.gh
         self.server, self.ctx, self.reqs = server, ctx, reqs
+        self.ops, self.batch = [], None
         t0 = time.perf_counter()
-        self.ops = [build_operator(gh, ctx, r) for r in reqs]
+        try:
+            for r in reqs:
+                self.ops.append(build_operator(gh, ctx, r))
+            t1 = time.perf_counter()
+            self.batch = gh.HybridOperatorBatch(self.ops, autotune=False)
+            t2 = time.perf_counter()
+            self.batch.launch(server.precision)
+            t3 = time.perf_counter()
+        except Exception:
+            self.release()  # (the caller decides whether to retry the requests)
+            raise
         for r, op in zip(reqs, self.ops):
             r.admitted = now
             r.cells = op.info.ncells
-        t1 = time.perf_counter()
-        self.batch = gh.HybridOperatorBatch(self.ops, autotune=False)
-        t2 = time.perf_counter()
-        self.batch.launch(server.precision)
-        t3 = time.perf_counter()
         server.host_s["operators"] += t1 - t0
         server.host_s["batch_graph"] += t2 - t1
         server.host_s["launch"] += t3 - t2
@@ -181,7 +191,8 @@ class _Round:
 
     def release(self):
         t0 = time.perf_counter()
-        self.batch.close()
+        if self.batch is not None:
+            self.batch.close()
         for op in self.ops:
             op.close()
         self.server.host_s["release"] += time.perf_counter() - t0
@@ -216,6 +227,9 @@ class Server:
         inflight, retired = [], []
         used = 0
         busy = set()
+        blocked = False   # an admission ran out of device memory: wait for a round
+        self.memory_retries = 0
+        self.max_round = self.max_batch  # halved by an admission that ran out of memory alone
         self.rejected = []
         self.rounds = []
         self.t0 = time.perf_counter()
@@ -238,15 +252,18 @@ class Server:
                     busy.discard(id(rd.ctx))
                     retired.append(rd)
                     progressed = True
+                    blocked = False
             if not inflight and retired:
                 for rd in retired:
                     rd.release()
                     used -= rd.bytes
                 retired = []
             free = [c for c in self.ctxs if id(c) not in busy]
-            if waiting and free:
+            if blocked and not inflight:
+                blocked = False   # every retired round has been released just above
+            if waiting and free and not blocked:
                 take, need = [], 0
-                while (waiting and len(take) < self.max_batch and
+                while (waiting and len(take) < min(self.max_batch, self.max_round) and
                        used + need + waiting[0].footprint <= self.budget):
                     r = waiting.popleft()
                     take.append(r)
@@ -255,16 +272,43 @@ class Server:
                     continue  # space is released at the top of the next pass
                 if take:
                     ctx = free[0]
-                    rd = _Round(self, ctx, take, self.clock())
+                    try:
+                        rd = _Round(self, ctx, take, self.clock())
+                    except HDGError as err:
+                        # the device is fuller than the footprints say (the
+                        # runtime's own memory is not in them): give the
+                        # requests back and admit again once a round is done,
+                        # or, with nothing on the device, as a smaller round
+                        if not _out_of_memory(err) or (len(take) == 1 and not inflight):
+                            raise
+                        for r in reversed(take):
+                            waiting.appendleft(r)
+                        if not inflight:
+                            self.max_round = max(1, len(take) // 2)
+                        self.memory_retries += 1
+                        blocked = bool(inflight)
+                        continue
                     self.rounds.append(len(take))
                     inflight.append(rd)
                     busy.add(id(ctx))
                     used += rd.bytes
                     progressed = True
             if progress and self.clock() - last_report > 5.0:
+                import torch
                 last_report = self.clock()
+                free, total = torch.cuda.mem_get_info()
+                live = pooled = 0
+                for cx in self.ctxs:
+                    a, b, u = C.c_int64(), C.c_int64(), C.c_int64()
+                    check(lib().hdg_ctx_memory(cx.h, C.byref(a), C.byref(b), C.byref(u)),
+                          cx.h, "memory")
+                    live, pooled = live + a.value, pooled + b.value
                 progress(f"t={last_report:.1f}s done={sum(r.done is not None for r in requests)} "
-                         f"waiting={len(waiting)} inflight={len(inflight)}")
+                         f"waiting={len(waiting)} inflight={len(inflight)} "
+                         f"admitted={used / 2**30:.1f}GiB live={live / 2**30:.1f}GiB "
+                         f"pooled={pooled / 2**30:.1f}GiB unpooled={u.value / 2**30:.1f}GiB "
+                         f"free={free / 2**30:.1f}GiB "
+                         f"torch={torch.cuda.memory_reserved() / 2**30:.1f}GiB")
             if not progressed:
                 wait = 2e-4
                 if pending and not waiting and not inflight:

@@ -73,6 +73,13 @@ const char* hdg_last_error(const hdg_ctx* ctx);
 /* The stream all work of the context is enqueued on (a hipStream_t). */
 void* hdg_ctx_stream(hdg_ctx* ctx);
 int hdg_ctx_synchronize(hdg_ctx* ctx);
+/* Device memory held for the context's meshes, batches and workspaces (bytes
+ * live in its buffers, bytes of freed blocks its pool keeps for reuse) and,
+ * process-wide, bytes allocated outside any context's pool.  No counterpart
+ * in the reference (Julia's GC owns its arrays); for callers that admit work
+ * by device memory, such as a serving loop. */
+int hdg_ctx_memory(hdg_ctx* ctx, int64_t* live_bytes, int64_t* pooled_bytes,
+                   int64_t* unpooled_bytes);
 /* Kernel timing.  Between hdg_profile_start and hdg_profile_stop every kernel
  * launch of the library (outside stream capture) is bracketed by two HIP
  * events on the stream it runs on; hdg_profile_stop synchronises and returns

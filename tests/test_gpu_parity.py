@@ -311,13 +311,17 @@ def test_elasticity_operator_parity(ctx, D, k, n):
 
 
 def test_darcy_3d_k3_large(ctx):
-    """3D Darcy HDG k=3: ni=256, nb=96 (blocked kernel)."""
+    """3D Darcy HDG k=3: ni=256, nb=96, n_flux=192 (the flux elimination from
+    reference products with a 160-row reduced system, the recovery reading
+    G_e from the table; the blocked kernel is tested in
+    test_blocked_launch_shapes)."""
     print("darcy 3D k=3", flush=True)
     nc = 8
     kappa = O.lognormal_kappa(nc, seed=8)
     prob = O.Problem.darcy(3, 3, mms=O.MMS_SINPROD)
     ref = O.hdg_solve(prob, (2, 2, 2), coef=kappa)
     op = _operator(ctx, 3, 3, 2, kappa=kappa, mms=O.MMS_SINPROD)
+    assert op.factor_rows() == (64, 10)
     S, g, cd = op.cell_blocks()
     assert rel(S, ref.S) < RTOL and rel(g, ref.g) < RTOL
     fv = op.recover(ref.lam).cpu().numpy()
