@@ -60,17 +60,20 @@ def kernel_model(kname, info, D, sweeps, family=None, nqc=0):
         return "mfma", refine_flops(ni, nb, sweeps), "flop"
     if kname == "prep":
         # reads the cell's origin, Jacobian, facet signs and coefficient;
-        # writes the 72 cell coefficients and the source at the quadrature points
-        return "hbm", 8 * (D + D * D + 1 + 72 + 4 * nqc) + 2 * D, "byte"
+        # writes the 72 cell coefficients
+        return "hbm", 8 * (D + D * D + 1 + 72) + 2 * D, "byte"
     if kname == "prep_rhs":
-        # reads the source at the quadrature points; writes b_i
-        return "hbm", 8 * (4 * nqc + ni), "byte"
+        # reads the cell's origin, Jacobian and coefficient (the built-in
+        # source is evaluated at the quadrature points); writes b_i
+        return "hbm", 8 * (D + D * D + 1 + ni), "byte"
     if kname == "flux_ref":
         # writes R [nr][nr+1] and the kept block A_uu; reads the cell
         # coefficients (72) and b_i
         return "hbm", 8 * (nr * (nr + 1) + nu * nu + 72 + ni), "byte"
-    if kname == "reduced_rowlu":
-        # reads R; writes S, g, U (packed), multipliers, pivot order, Y
+    if kname == "reduced_rowlu" or (kname == "condense_grid" and family in (8, 10)):
+        # the reduced elimination of the flux families (row-per-lane, or the
+        # grid kernel for other sizes): reads R; writes S, g, U (packed),
+        # multipliers, pivot order, Y
         return "hbm", 8 * (nr * (nr + 1) + nb * nb + nb + nu * (nu + 1) // 2 + nu * nu + nu
                            + (nb + 1) * nu), "byte"
     if kname == "recover_flux":
@@ -480,10 +483,12 @@ def batch_main(gh, args):
     import torch
     from gridaphybrid import serving as sv
     reqs = sv.draw_requests(args.nmesh, 1.0, SEED + 1)
-    budget = 200 << 30
+    budget = int(args.serve_budget_gib * (1 << 30))
+    # each graph runs both paths, so a mesh is sized by the larger footprint
+    foot = {id(r): max(sv.footprint(r, gh.FP64), sv.footprint(r, gh.MIXED)) for r in reqs}
     waves, cur, used = [], [], 0
-    for r in sorted(reqs, key=lambda r: -sv.footprint(r, gh.FP64)):
-        f = sv.footprint(r, gh.FP64)
+    for r in sorted(reqs, key=lambda r: -foot[id(r)]):
+        f = foot[id(r)]
         if cur and used + f > budget:
             waves.append(cur)
             cur, used = [], 0

@@ -179,7 +179,7 @@ struct CellLayout {
         U(factor_slot(sys_shape(t))),
         Y((size_t)(t.nb + 1) * t.ni),
         alpha(72),
-        srcq((size_t)t.nqc * 4 + t.ni) {}  // source at quadrature points, then b_i
+        srcq((size_t)t.ni) {}  // interior right-hand side b_i
 };
 
 // Allocates the mesh's own per-cell state (S, g, U, Y, alpha, srcq, info) and
@@ -1001,7 +1001,9 @@ int hdg_mesh_destroy(hdg_mesh* mesh) {
     return fail(mesh->ctx, HDG_ERR_STATE, "mesh belongs to a batch: destroy the batch first");
   (void)hipSetDevice(mesh->ctx->device);
   (void)hipStreamSynchronize(mesh->ctx->stream);
+  const int pool = mesh->ctx->pool;
   delete mesh;
+  dev_pool_idle(pool);  // the stream is idle: the freed blocks may serve any context
   return HDG_OK;
 }
 
@@ -1849,6 +1851,7 @@ int hdg_batch_destroy(hdg_batch* b) {
     m->mixed = false;
   }
   delete b;
+  dev_pool_idle(ctx->pool);  // the stream is idle: the freed blocks may serve any context
   return rc;
 }
 

@@ -132,7 +132,7 @@ struct MeshDev {
   const double* coef;      // [ncells] kappa or nu
   const double* src;       // [ncells][nqc][ncomp] or null (user source values)
   double* alpha;           // [ncells][72] per-cell entry-program coefficients (scratch)
-  double* srcq;            // [ncells][nqc][4] |det J| * source at quadrature points (scratch)
+  double* srcq;            // [ncells][ni] interior right-hand side b_i (scratch)
   int source;              // hdg_source
   double tau;
   double young;
@@ -208,8 +208,11 @@ void workspace_release(Workspace& w);
 // most its size and at least half of it.  The context's buffers are used only
 // by work on its one stream, so the reuse is ordered behind their last use.
 // The pool in effect is the innermost PoolScope of the calling thread (none:
-// plain hipMalloc / hipFree); a buffer returns to the pool it came from.  The
-// device's pools are emptied (hipFree) when an allocation would leave less
+// plain hipMalloc / hipFree); a buffer returns to the pool it came from.
+// dev_pool_idle, called once the context's stream has been synchronized,
+// hands the pool's blocks to the device's idle list, which serves any
+// context (no queued work uses those blocks any more).  The device's pools
+// and idle list are emptied (hipFree) when an allocation would leave less
 // than 8 GB of device memory free, and before retrying a failed hipMalloc.
 struct PoolScope {
   explicit PoolScope(int pool);
@@ -220,6 +223,7 @@ struct PoolScope {
 };
 int dev_pool_create(int device);
 void dev_pool_destroy(int pool);
+void dev_pool_idle(int pool);
 hipError_t dev_alloc(void** p, size_t bytes, int* pool);
 void dev_free(void* p, size_t bytes, int pool);
 void dev_cache_flush(int device);

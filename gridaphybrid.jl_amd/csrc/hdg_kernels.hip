@@ -41,6 +41,16 @@ struct CellGeo {
   double K[6][6];       // elasticity: v_a : A(sigma_b) per unit mass
 };
 
+// det J of an affine cell (J column-major D x D in the first D*D entries)
+__device__ inline double jac_det(const double* J, int D) {
+  if (D == 2) return J[0] * J[3] - J[2] * J[1];
+#define JM(r, c) J[(r) + 3 * (c)]
+  return JM(0, 0) * (JM(1, 1) * JM(2, 2) - JM(2, 1) * JM(1, 2)) -
+         JM(0, 1) * (JM(1, 0) * JM(2, 2) - JM(2, 0) * JM(1, 2)) +
+         JM(0, 2) * (JM(1, 0) * JM(2, 1) - JM(2, 0) * JM(1, 1));
+#undef JM
+}
+
 __device__ inline void cell_geometry(const MeshDev& m, int D, int64_t cell, CellGeo& g) {
   // fixed-bound loops with guards keep every local array in registers
   double J[9], Ji[9];
@@ -50,9 +60,8 @@ __device__ inline void cell_geometry(const MeshDev& m, int D, int64_t cell, Cell
   for (int a = 0; a < 3; ++a) g.x0[a] = a < D ? m.x0[cell * D + a] : 0.0;
 #pragma unroll
   for (int t = 0; t < 9; ++t) g.J[t] = J[t];
-  double det;
+  const double det = jac_det(J, D);
   if (D == 2) {
-    det = J[0] * J[3] - J[2] * J[1];
     Ji[0] = J[3] / det;
     Ji[3] = J[0] / det;
     Ji[2] = -J[2] / det;
@@ -61,9 +70,6 @@ __device__ inline void cell_geometry(const MeshDev& m, int D, int64_t cell, Cell
     for (int t = 4; t < 9; ++t) Ji[t] = 0.0;
   } else {
 #define JM(r, c) J[(r) + 3 * (c)]
-    det = JM(0, 0) * (JM(1, 1) * JM(2, 2) - JM(2, 1) * JM(1, 2)) -
-          JM(0, 1) * (JM(1, 0) * JM(2, 2) - JM(2, 0) * JM(1, 2)) +
-          JM(0, 2) * (JM(1, 0) * JM(2, 1) - JM(2, 0) * JM(1, 1));
     Ji[0 + 3 * 0] = (JM(1, 1) * JM(2, 2) - JM(1, 2) * JM(2, 1)) / det;
     Ji[0 + 3 * 1] = (JM(0, 2) * JM(2, 1) - JM(0, 1) * JM(2, 2)) / det;
     Ji[0 + 3 * 2] = (JM(0, 1) * JM(1, 2) - JM(0, 2) * JM(1, 1)) / det;
@@ -118,12 +124,18 @@ __device__ inline void darcy_source(int src, int D, const double* x, double kapp
   }
 }
 
+// x[i] for a run-time i < 3, by selects (a dynamic index would put x in
+// scratch memory)
+__device__ inline double sel3(const double* x, int i) {
+  return i == 0 ? x[0] : (i == 1 ? x[1] : x[2]);
+}
+
 __device__ inline double elas_h(int D, const double* x, int i, int j, int k) {
   // d2 u_i / dx_j dx_k of u_i = sin(pi (x_i + x_{i+1}/2))
   const int i1 = (i + 1) % D;
   const double dj = (j == i ? 1.0 : 0.0) + (j == i1 ? 0.5 : 0.0);
   const double dk = (k == i ? 1.0 : 0.0) + (k == i1 ? 0.5 : 0.0);
-  return -M_PI * M_PI * sin(M_PI * (x[i] + 0.5 * x[i1])) * dj * dk;
+  return -M_PI * M_PI * sin(M_PI * (sel3(x, i) + 0.5 * sel3(x, i1))) * dj * dk;
 }
 
 __device__ inline void elas_source(int src, int D, const double* x, double E, double nu,
@@ -168,95 +180,124 @@ constexpr int kMaxAlpha = 72;
 
 // Per-cell preparation (runs at the start of every condensation): the
 // affine pull-back coefficients alpha of the entry program (layout fixed by
-// build_program in hdg_fe.cpp) and the source values at the cell quadrature
-// points, pre-multiplied by |det J|.  One thread per cell.
-// Per-cell stride of the source scratch: |det J| * source at the cell
-// quadrature points [nqc][4], then the interior right-hand side b_i [ni].
+// build_program in hdg_fe.cpp).  One thread per cell computes its
+// coefficients into LDS; the wavefront then stores its 64 cells' rows as one
+// contiguous, coalesced range.
+// Per-cell stride of the right-hand-side scratch: the interior b_i [ni].
 __device__ __host__ inline int64_t srcq_stride(const Tables& T) {
-  return (int64_t)T.nqc * 4 + T.ni;
+  return (int64_t)T.ni;
 }
 
+constexpr int kPrepCells = 64;  // cells (threads) per prep workgroup
+
 template <int PHYS>
-__global__ void __launch_bounds__(128) prep_kernel(Tables T, MeshDev M) {
+__global__ void __launch_bounds__(kPrepCells) prep_kernel(Tables T, MeshDev M) {
+  __shared__ double s_al[kPrepCells][kMaxAlpha + 1];  // odd row stride: no bank conflicts
   const int D = T.D;
-  for (int64_t cell = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; cell < M.ncells;
-       cell += (int64_t)gridDim.x * blockDim.x) {
-    CellGeo g;
-    cell_geometry(M, D, cell, g);
-    double* al = M.alpha + cell * kMaxAlpha;
-    if (PHYS == kDarcy) {
-      al[0] = g.detJ / g.coef;
-      for (int t = 1; t < 10; ++t) {
-        const int e = (t - 1) / D, a = (t - 1) % D;
-        al[t] = e < D ? -g.detJ * g.Ji[e + D * a] : 0.0;
-      }
-      for (int t = 10; t < 28; ++t) {
-        const int F = (t - 10) / 3, a = (t - 10) % 3;
-        al[t] = (F < 2 * D && a < D) ? g.aF[F] * g.nF[F][a] : 0.0;
-      }
-      for (int t = 28; t < kMaxAlpha; ++t) {
-        const int F = t - 28;
-        al[t] = F < 2 * D ? M.tau * g.sF[F] * g.aF[F] : 0.0;
-      }
-    } else {
-      const int nsym = D * (D + 1) / 2;
-      const double nu = g.coef, c1 = (1.0 + nu) / M.young, c2 = nu / M.young;
-      for (int t = 0; t < 36; ++t) {
-        const int a = t / 6, b = t % 6;
-        double v = 0.0;
-        if (a < nsym && b < nsym) {
-          int ia, ja, ib, jb;
-          sym_pair(D, a, ia, ja);
-          sym_pair(D, b, ib, jb);
-          const double dd = (a == b) ? (ia == ja ? 1.0 : 2.0) : 0.0;
-          const double tra = ia == ja ? 1.0 : 0.0, trb = ib == jb ? 1.0 : 0.0;
-          v = (c1 * dd - c2 * tra * trb) * g.detJ;
+  for (int64_t cell0 = (int64_t)blockIdx.x * kPrepCells; cell0 < M.ncells;
+       cell0 += (int64_t)gridDim.x * kPrepCells) {
+    const int64_t cell = cell0 + threadIdx.x;
+    const int cnt = (int)(M.ncells - cell0 < kPrepCells ? M.ncells - cell0 : kPrepCells);
+    if (cell < M.ncells) {
+      CellGeo g;
+      cell_geometry(M, D, cell, g);
+      double* al = s_al[threadIdx.x];
+      if (PHYS == kDarcy) {
+        al[0] = g.detJ / g.coef;
+        for (int t = 1; t < 10; ++t) {
+          const int e = (t - 1) / D, a = (t - 1) % D;
+          al[t] = e < D ? -g.detJ * g.Ji[e + D * a] : 0.0;
         }
-        al[t] = v;
-      }
-      for (int t = 36; t < 45; ++t) {
-        const int e = (t - 36) / D, d = (t - 36) % D;
-        al[t] = e < D ? g.detJ * g.Ji[e + D * d] : 0.0;
-      }
-      for (int t = 45; t < 63; ++t) {
-        const int F = (t - 45) / 3, d = (t - 45) % 3;
-        al[t] = (F < 2 * D && d < D) ? g.aF[F] * g.nF[F][d] : 0.0;
-      }
-      for (int t = 63; t < kMaxAlpha; ++t) {
-        const int F = t - 63;
-        al[t] = F < 2 * D ? M.tau * g.aF[F] : 0.0;
+        for (int t = 10; t < 28; ++t) {
+          const int F = (t - 10) / 3, a = (t - 10) % 3;
+          al[t] = (F < 2 * D && a < D) ? g.aF[F] * g.nF[F][a] : 0.0;
+        }
+        for (int t = 28; t < kMaxAlpha; ++t) {
+          const int F = t - 28;
+          al[t] = F < 2 * D ? M.tau * g.sF[F] * g.aF[F] : 0.0;
+        }
+      } else {
+        const int nsym = D * (D + 1) / 2;
+        const double nu = g.coef, c1 = (1.0 + nu) / M.young, c2 = nu / M.young;
+        for (int t = 0; t < 36; ++t) {
+          const int a = t / 6, b = t % 6;
+          double v = 0.0;
+          if (a < nsym && b < nsym) {
+            int ia, ja, ib, jb;
+            sym_pair(D, a, ia, ja);
+            sym_pair(D, b, ib, jb);
+            const double dd = (a == b) ? (ia == ja ? 1.0 : 2.0) : 0.0;
+            const double tra = ia == ja ? 1.0 : 0.0, trb = ib == jb ? 1.0 : 0.0;
+            v = (c1 * dd - c2 * tra * trb) * g.detJ;
+          }
+          al[t] = v;
+        }
+        for (int t = 36; t < 45; ++t) {
+          const int e = (t - 36) / D, d = (t - 36) % D;
+          al[t] = e < D ? g.detJ * g.Ji[e + D * d] : 0.0;
+        }
+        for (int t = 45; t < 63; ++t) {
+          const int F = (t - 45) / 3, d = (t - 45) % 3;
+          al[t] = (F < 2 * D && d < D) ? g.aF[F] * g.nF[F][d] : 0.0;
+        }
+        for (int t = 63; t < kMaxAlpha; ++t) {
+          const int F = t - 63;
+          al[t] = F < 2 * D ? M.tau * g.aF[F] : 0.0;
+        }
       }
     }
-    const int ncomp = PHYS == kDarcy ? D + 1 : D;
-    double* sq = M.srcq + cell * srcq_stride(T);
-    for (int q = 0; q < T.nqc; ++q) {
-      double v[4] = {0.0, 0.0, 0.0, 0.0};
-      if (M.source == 100) {
-        const double* p = M.src + (cell * T.nqc + q) * ncomp;
-        for (int c = 0; c < 4; ++c)
-          if (c < ncomp) v[c] = p[c];
-      } else if (M.source != 0) {
-        double x[3];
-        for (int a = 0; a < 3; ++a) {
-          double t = g.x0[a];
-          for (int e = 0; e < 3; ++e)
-            if (a < D && e < D) t += g.J[a + D * e] * T.xq[3 * q + e];
-          x[a] = t;
-        }
-        double f[3], gs = 0.0;
-        if (PHYS == kDarcy) {
-          darcy_source(M.source, D, x, g.coef, f, &gs);
-          for (int c = 0; c < 3; ++c)
-            if (c < D) v[c] = f[c];
-          v[D] = gs;
-        } else {
-          elas_source(M.source, D, x, M.young, g.coef, f);
-          for (int c = 0; c < 3; ++c) v[c] = f[c];
-        }
-      }
-      for (int c = 0; c < 4; ++c) sq[q * 4 + c] = v[c] * g.detJ;
+    __syncthreads();
+    double* dst = M.alpha + cell0 * kMaxAlpha;
+    for (int i = threadIdx.x; i < cnt * kMaxAlpha; i += kPrepCells)
+      dst[i] = s_al[i / kMaxAlpha][i % kMaxAlpha];
+    __syncthreads();
+  }
+}
+
+// |det J| times the source at cell quadrature point q (components [4]): the
+// values given per quadrature point (source 100) or a built-in manufactured
+// source at x = x0 + J xi_q.
+template <int PHYS>
+__device__ inline void cell_source(const Tables& T, const MeshDev& M, int64_t cell, int q,
+                                   double* v) {
+  const int D = T.D;
+  const int ncomp = PHYS == kDarcy ? D + 1 : D;
+  double J[9], x0[3];
+#pragma unroll
+  for (int t = 0; t < 9; ++t) J[t] = t < D * D ? M.jac[cell * D * D + t] : 0.0;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) x0[a] = a < D ? M.x0[cell * D + a] : 0.0;
+  const double detJ = fabs(jac_det(J, D));
+#pragma unroll
+  for (int c = 0; c < 4; ++c) v[c] = 0.0;
+  if (M.source == 100) {
+    const double* p = M.src + (cell * T.nqc + q) * ncomp;
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+      if (c < ncomp) v[c] = p[c];
+  } else if (M.source != 0) {
+    double x[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      double t = x0[a];
+#pragma unroll
+      for (int e = 0; e < 3; ++e)  // J[a + D e], by constant indices
+        if (a < D && e < D) t += (D == 2 ? J[a + 2 * e] : J[a + 3 * e]) * T.xq[3 * q + e];
+      x[a] = t;
+    }
+    double f[3], gs = 0.0;
+    if (PHYS == kDarcy) {
+      darcy_source(M.source, D, x, M.coef[cell], f, &gs);
+#pragma unroll
+      for (int c = 0; c < 4; ++c) v[c] = c < D ? f[c < 3 ? c : 2] : (c == D ? gs : 0.0);
+    } else {
+      elas_source(M.source, D, x, M.young, M.coef[cell], f);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) v[c] = f[c];
     }
   }
+#pragma unroll
+  for (int c = 0; c < 4; ++c) v[c] *= detJ;
 }
 
 // Per-cell strides of the factor outputs (see CondenseOut).
@@ -286,7 +327,7 @@ struct FusedGen {
     for (int t = threadIdx.x; t < kMaxAlpha; t += blockDim.x)
       sh.alpha[t] = M.alpha[cell * kMaxAlpha + t];
     const double* sq = M.srcq + cell * srcq_stride(T);
-    for (int t = threadIdx.x; t < T.ni; t += blockDim.x) sh.b[t] = sq[T.nqc * 4 + t];
+    for (int t = threadIdx.x; t < T.ni; t += blockDim.x) sh.b[t] = sq[t];
   }
 
   __device__ double entry(const Shared& sh, int row, int col) const {
@@ -2515,9 +2556,16 @@ struct Pool {
   std::multimap<size_t, void*> free;  // cached blocks by the size they were requested with
   int64_t live_bytes = 0, pooled_bytes = 0;
 };
+// Blocks of a device that no queued work uses any more (their context's
+// stream was synchronized after they were freed): any context may reuse them.
+struct IdleBlocks {
+  std::multimap<size_t, void*> free;
+  int64_t bytes = 0;
+};
 struct DevCache {
   std::mutex mu;
   std::vector<Pool> pools;
+  std::map<int, IdleBlocks> idle;  // by device
   int64_t unpooled_bytes = 0;
 };
 DevCache& dev_cache() {
@@ -2538,6 +2586,11 @@ void flush_device(DevCache& c, int device) {
     pl.free.clear();
     pl.pooled_bytes = 0;
   }
+  IdleBlocks& ib = c.idle[device];
+  for (auto& kv : ib.free) (void)hipFree(kv.second);
+  any = any || !ib.free.empty();
+  ib.free.clear();
+  ib.bytes = 0;
   if (any) (void)hipDeviceSynchronize();
   (void)hipGetLastError();
 }
@@ -2568,12 +2621,24 @@ void dev_pool_destroy(int pool) {
   (void)hipGetLastError();
 }
 
+void dev_pool_idle(int pool) {
+  DevCache& c = dev_cache();
+  std::lock_guard<std::mutex> lock(c.mu);
+  if (pool < 0 || pool >= (int)c.pools.size() || !c.pools[pool].live) return;
+  Pool& pl = c.pools[pool];
+  IdleBlocks& ib = c.idle[pl.device];
+  for (auto& kv : pl.free) ib.free.emplace(kv.first, kv.second);
+  ib.bytes += pl.pooled_bytes;
+  pl.free.clear();
+  pl.pooled_bytes = 0;
+}
+
 void dev_pool_stats(int pool, int64_t* live, int64_t* pooled, int64_t* unpooled) {
   DevCache& c = dev_cache();
   std::lock_guard<std::mutex> lock(c.mu);
   const bool ok = pool >= 0 && pool < (int)c.pools.size();
   *live = ok ? c.pools[pool].live_bytes : 0;
-  *pooled = ok ? c.pools[pool].pooled_bytes : 0;
+  *pooled = ok ? c.pools[pool].pooled_bytes + c.idle[c.pools[pool].device].bytes : 0;
   *unpooled = c.unpooled_bytes;
 }
 
@@ -2601,6 +2666,16 @@ hipError_t dev_alloc(void** p, size_t bytes, int* pool_out) {
         pl.pooled_bytes -= (int64_t)it->first;
         pl.live_bytes += (int64_t)bytes;
         pl.free.erase(it);
+        *pool_out = pool;
+        return hipSuccess;
+      }
+      IdleBlocks& ib = c.idle[device];
+      auto jt = ib.free.lower_bound(bytes);
+      if (jt != ib.free.end() && jt->first / 2 <= bytes) {
+        *p = jt->second;
+        ib.bytes -= (int64_t)jt->first;
+        pl.live_bytes += (int64_t)bytes;
+        ib.free.erase(jt);
         *pool_out = pool;
         return hipSuccess;
       }
@@ -2920,10 +2995,11 @@ bool condense_supported(int n) {
   return pick_bucket(n, b) || (n > 64 && n <= kLargeMaxN);
 }
 
-// Interior right-hand side b_i of every cell, one wavefront per cell and a
-// lane per row: FusedGen::rhs (the quadrature of the source against the test
-// basis) evaluated once and kept in the source scratch, so the generators read
-// one value per right-hand-side entry.
+// Interior right-hand side b_i of every cell, one wavefront per cell: a lane
+// per cell quadrature point evaluates |det J| times the source there, then a
+// lane per row runs FusedGen::rhs (the quadrature of the source against the
+// test basis); b_i is kept in the per-cell scratch, so the generators read one
+// value per right-hand-side entry.
 template <int PHYS>
 __global__ void __launch_bounds__(256) cell_rhs_kernel(FusedGen<PHYS> gen) {
   constexpr int WPB = 4;
@@ -2933,9 +3009,14 @@ __global__ void __launch_bounds__(256) cell_rhs_kernel(FusedGen<PHYS> gen) {
   for (int64_t cell = (int64_t)blockIdx.x * WPB + w; cell < gen.M.ncells;
        cell += (int64_t)gridDim.x * WPB) {
     double* sq = gen.M.srcq + cell * srcq_stride(T);
-    for (int t = lane; t < T.nqc * 4; t += 64) s_src[w][t >> 2][t & 3] = sq[t];
+    for (int q = lane; q < T.nqc; q += 64) {
+      double v[4];
+      cell_source<PHYS>(T, gen.M, cell, q, v);
+#pragma unroll
+      for (int c = 0; c < 4; ++c) s_src[w][q][c] = v[c];
+    }
     wave_sync_lds();
-    for (int r = lane; r < T.ni; r += 64) sq[T.nqc * 4 + r] = gen.rhs(s_src[w], r);
+    for (int r = lane; r < T.ni; r += 64) sq[r] = gen.rhs(s_src[w], r);
     wave_sync_lds();
   }
 }
@@ -2943,18 +3024,15 @@ __global__ void __launch_bounds__(256) cell_rhs_kernel(FusedGen<PHYS> gen) {
 static hipError_t launch_prep(const Tables& T, const MeshDev& M, hipStream_t st) {
   if (M.ncells <= 0) return hipSuccess;
   if (T.ni > kMaxRhsRows) return hipErrorInvalidValue;
-  const int64_t blocks = (M.ncells + 127) / 128;
-  const int grid = (int)(blocks < 8192 ? blocks : 8192);
-  if (T.phys == kDarcy)
-    {
-      ClockScope clk_(st, "prep");
-      hipLaunchKernelGGL(prep_kernel<kDarcy>, dim3(grid), dim3(128), 0, st, T, M);
-    }
-  else
-    {
-      ClockScope clk_(st, "prep");
-      hipLaunchKernelGGL(prep_kernel<kElasticity>, dim3(grid), dim3(128), 0, st, T, M);
-    }
+  const int64_t blocks = (M.ncells + kPrepCells - 1) / kPrepCells;
+  const int grid = (int)(blocks < 16384 ? blocks : 16384);
+  {
+    ClockScope clk_(st, "prep");
+    if (T.phys == kDarcy)
+      hipLaunchKernelGGL(prep_kernel<kDarcy>, dim3(grid), dim3(kPrepCells), 0, st, T, M);
+    else
+      hipLaunchKernelGGL(prep_kernel<kElasticity>, dim3(grid), dim3(kPrepCells), 0, st, T, M);
+  }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   const int64_t wblocks = (M.ncells + 3) / 4;
@@ -3129,7 +3207,7 @@ __global__ void __launch_bounds__(256)
        cell += (int64_t)gridDim.x * WPB) {
     for (int t = lane; t < kMaxAlpha; t += 64) s_sh[w].alpha[t] = gen.M.alpha[cell * kMaxAlpha + t];
     for (int t = lane; t < ni; t += 64)
-      s_sh[w].b[t] = gen.M.srcq[cell * srcq_stride(T) + nqc * 4 + t];
+      s_sh[w].b[t] = gen.M.srcq[cell * srcq_stride(T) + t];
     for (int j = lane; j < nb; j += 64) {
       const int32_t d = cell_dofs[cell * nb + j];
       s_v[w][ni + j] = d >= 0 ? lam_free[d] : lam_dir[-(int64_t)d - 1];
@@ -3609,9 +3687,9 @@ static bool ref_plan_sizes(int D, int np, int na, int ni, int nb, int npair, Ref
   const int nbands = (nr + BR - 1) / BR;
   BR = (nr + nbands - 1) / nbands;  // balance the bands
   // alpha, b_i, then M^-1 (prologue only) aliased with the rhs column, and
-  // the band's columns of every H_e = M^-1 G_e
+  // the band's columns of every H_e = M^-1 G_e (rows of odd length D np | 1)
   const size_t lds = (size_t)kMaxAlpha + ni + std::max<size_t>((size_t)np * np, BR) +
-                     (size_t)D * np * BR;
+                     (size_t)((D * np) | 1) * BR;
   if (lds * sizeof(double) > 64 * 1024) return false;
   if (rp) {
     rp->nr = nr;
@@ -3640,6 +3718,7 @@ __global__ void __launch_bounds__(kRefThreads, 2)
   const Tables& T = gen.T;
   const int nr = rp.nr, nc = nr + 1, np = rp.np, na = rp.na, nu = rp.nu, BR = rp.BR;
   const int ni = T.ni, n = T.n, nk = D * np;
+  const int hld = nk | 1;  // odd row stride of s_H: the rows of a wavefront hit distinct banks
   const int tid = threadIdx.x;
   const int band = (int)(blockIdx.x % rp.nbands);
   const int64_t group = blockIdx.x / rp.nbands, ngroups = gridDim.x / rp.nbands;
@@ -3649,7 +3728,7 @@ __global__ void __launch_bounds__(kRefThreads, 2)
   double* s_b = s_al + kMaxAlpha;            // [ni] interior right-hand side
   double* s_minv = s_b + ni;                 // [np][np], prologue only
   double* s_xr = s_minv;                     // [BR] rhs column of the Schur term
-  double* s_H = s_minv + max(np * np, BR);   // [BR][D * np]: H_e[j][r0 + rl] at rl * nk + e * np + j
+  double* s_H = s_minv + max(np * np, BR);   // [BR][hld]: H_e[j][r0 + rl] at rl * hld + e * np + j
   // ---- per-thread entries: P values and R0's program terms, for all cells ----
   double pv[NE][NPAIR], tv[NE][NK];
   int tid8[NE];  // term coefficient ids, 8 bits each; 0xff: no term
@@ -3694,7 +3773,7 @@ __global__ void __launch_bounds__(kRefThreads, 2)
     if (rl < rows)
       for (int i = 0; i < np; ++i)
         h = fma(s_minv[j * np + i], T.fr_G[((size_t)e * np + i) * nr + r0 + rl], h);
-    s_H[t] = h;
+    s_H[rl * hld + k] = h;
   }
   __syncthreads();  // s_minv is reused as s_xr
   // the cell's coefficients and right-hand side are loaded one cell ahead
@@ -3702,7 +3781,7 @@ __global__ void __launch_bounds__(kRefThreads, 2)
   auto fetch = [&](int64_t lc, double& al, double& bv) {
     const int64_t cell = first + lc;
     al = (lc < count && tid < kMaxAlpha) ? gen.M.alpha[cell * kMaxAlpha + tid] : 0.0;
-    bv = (lc < count && tid < ni) ? gen.M.srcq[cell * srcq_stride(T) + T.nqc * 4 + tid] : 0.0;
+    bv = (lc < count && tid < ni) ? gen.M.srcq[cell * srcq_stride(T) + tid] : 0.0;
   };
   // rhs column: thread (rl, part) sums the terms k = part, part + SPL, ... of
   // sum_k H[k][rl] z[k], z_e[j] = sum_a c_ea b_q,a[j]; the SPL partial sums of
@@ -3733,7 +3812,7 @@ __global__ void __launch_bounds__(kRefThreads, 2)
         double zn = 0.0;
 #pragma unroll
         for (int a = 0; a < D; ++a) zn = fma(s_al[1 + e * D + a], s_b[a * np + j], zn);
-        x = fma(s_H[xrl * nk + k], -zn, x);
+        x = fma(s_H[xrl * hld + k], -zn, x);
         j += SPL;
         while (j >= np) j -= np, ++e;
       }
@@ -4793,7 +4872,9 @@ __global__ void __launch_bounds__(256)
                         const double* __restrict__ lam_free, const double* __restrict__ lam_dir,
                         double* __restrict__ out, bool g_lds) {
   constexpr int WPB = kRecWaves;
-  // M^-1 [np][np], M, (REF with g_lds: G_e [D][np][nu]), then the wavefronts' vectors
+  // M^-1 and M transposed ([col][row]: lane i reads row i at stride 1, free of
+  // bank conflicts for any np), (REF with g_lds: G_e [D][np][nu]), then the
+  // wavefronts' vectors
   extern __shared__ double s_m[];
   const Tables& T = gen.T;
   const int ni = T.ni, nb = T.nb, na = T.na, np = T.np, nu = ni - na, nqc = T.nqc;
@@ -4817,8 +4898,9 @@ __global__ void __launch_bounds__(256)
   double* uv = xuv + nu;           // reduced work vector [nu]
   double* tv = uv + nu;            // REF: G_e products [D][np]
   for (int t = threadIdx.x; t < np * np; t += 64 * WPB) {
-    s_m[t] = T.fb_minv[t];
-    s_m[np * np + t] = T.Mpp[t];
+    const int i = t / np, j = t - i * np;
+    s_m[j * np + i] = T.fb_minv[t];
+    s_m[np * np + j * np + i] = T.Mpp[t];
   }
   if (REF && g_lds)
     for (int t = threadIdx.x; t < gsz; t += 64 * WPB) {
@@ -4830,7 +4912,7 @@ __global__ void __launch_bounds__(256)
   const double* Mm = s_m + np * np;
 
   // v <- (Cm (x) Mt) v over the flux rows (v in LDS): Mt = M^-1 with Cm = C^-1
-  // for a solve, Mt = M with Cm = C for a product
+  // for a solve, Mt = M with Cm = C for a product (Mt stored transposed)
   auto flux_apply = [&](double* v, const double* Mt, const double* Cm) {
     double y[KQ];
 #pragma unroll
@@ -4839,9 +4921,9 @@ __global__ void __launch_bounds__(256)
       double acc = 0.0;
       if (r < na) {
         const int a = r / np, i = r - a * np;
-        const double* mr = Mt + i * np;
+        const double* mc = Mt + i;
         const double* vr = v + a * np;
-        for (int j = 0; j < np; ++j) acc = fma(mr[j], vr[j], acc);
+        for (int j = 0; j < np; ++j) acc = fma(mc[j * np], vr[j], acc);
       }
       y[t] = acc;
     }
@@ -4928,7 +5010,7 @@ __global__ void __launch_bounds__(256)
     const double* Aqu = base + rs.aqu;
     const double* Auq = base + rs.auq;
     const double* Wc = rs.w + cell * rs.wsz;
-    const double* bi = gen.M.srcq + cell * srcq_stride(T) + nqc * 4;
+    const double* bi = gen.M.srcq + cell * srcq_stride(T);
     // ---- c = b_i - A_ib lambda and x_u = w_u - W_u lambda ------------------------
     double cu = 0.0;
 #pragma unroll

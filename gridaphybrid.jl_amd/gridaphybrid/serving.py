@@ -19,8 +19,9 @@ The skeleton solve of a request is either the host sparse direct solve
 host solve would dominate, the interpolant of the manufactured trace
 (``skeleton="interpolant"``), as in the 64^3 benchmark.
 
-Freeing device memory synchronises the device, so retired requests are
-released only when no round is in flight (or when admission needs the space).
+A finished round is released at once: its device blocks go back to its
+context's pool (libhdg's per-context block pools), which needs no device
+synchronisation, so the other context's round keeps the GPU busy meanwhile.
 """
 from __future__ import annotations
 
@@ -224,7 +225,7 @@ class Server:
         whole budget are rejected (done stays None)."""
         pending = deque(sorted(requests, key=lambda r: r.arrival))
         waiting = deque()
-        inflight, retired = [], []
+        inflight = []
         used = 0
         busy = set()
         blocked = False   # an admission ran out of device memory: wait for a round
@@ -248,19 +249,18 @@ class Server:
             progressed = False
             for rd in list(inflight):
                 if rd.advance(self.clock()):
+                    # a finished round gives its device memory back at once: it
+                    # returns to its context's pool without a device
+                    # synchronisation, while the other context keeps running
                     inflight.remove(rd)
                     busy.discard(id(rd.ctx))
-                    retired.append(rd)
-                    progressed = True
-                    blocked = False
-            if not inflight and retired:
-                for rd in retired:
                     rd.release()
                     used -= rd.bytes
-                retired = []
+                    progressed = True
+                    blocked = False
             free = [c for c in self.ctxs if id(c) not in busy]
             if blocked and not inflight:
-                blocked = False   # every retired round has been released just above
+                blocked = False
             if waiting and free and not blocked:
                 take, need = [], 0
                 while (waiting and len(take) < min(self.max_batch, self.max_round) and
@@ -268,8 +268,6 @@ class Server:
                     r = waiting.popleft()
                     take.append(r)
                     need += r.footprint
-                if not take and retired and not inflight:
-                    continue  # space is released at the top of the next pass
                 if take:
                     ctx = free[0]
                     try:
@@ -314,8 +312,6 @@ class Server:
                 if pending and not waiting and not inflight:
                     wait = max(0.0, min(0.05, pending[0].arrival - self.clock()))
                 time.sleep(wait)
-        for rd in retired:
-            rd.release()
         for c in self.ctxs:
             c.synchronize()
         return requests

@@ -74,7 +74,8 @@ const char* hdg_last_error(const hdg_ctx* ctx);
 void* hdg_ctx_stream(hdg_ctx* ctx);
 int hdg_ctx_synchronize(hdg_ctx* ctx);
 /* Device memory held for the context's meshes, batches and workspaces (bytes
- * live in its buffers, bytes of freed blocks its pool keeps for reuse) and,
+ * live in its buffers; bytes of freed blocks it can reuse: its pool's and the
+ * device's idle blocks, which every context of the device shares) and,
  * process-wide, bytes allocated outside any context's pool.  No counterpart
  * in the reference (Julia's GC owns its arrays); for callers that admit work
  * by device memory, such as a serving loop. */
