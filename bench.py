@@ -134,6 +134,8 @@ def parse():
     ap.add_argument("--serve-loops", default="fp64,mixed,per_request",
                     help="serving loops to run: fp64 and mixed continuous batching, "
                          "per_request (one request at a time, FP64)")
+    ap.add_argument("--serve-max-batch", type=int, default=2,
+                    help="requests per round of the continuous-batching loops")
     ap.add_argument("--serve-budget-gib", type=float, default=160.0,
                     help="device memory the serving loop admits requests into (footprints)")
     ap.add_argument("--baseline-sample", type=int, default=200,
@@ -427,9 +429,10 @@ def serve_main(gh, args):
 
     out = {}
     for key, name, kw, n in (
-            ("fp64", "continuous_batching_fp64", dict(precision=gh.FP64, streams=2), args.serve),
-            ("mixed", "continuous_batching_mixed", dict(precision=gh.MIXED, streams=2),
-             args.serve),
+            ("fp64", "continuous_batching_fp64",
+             dict(precision=gh.FP64, streams=2, max_batch=args.serve_max_batch), args.serve),
+            ("mixed", "continuous_batching_mixed",
+             dict(precision=gh.MIXED, streams=2, max_batch=args.serve_max_batch), args.serve),
             ("per_request", "per_request_fp64",
              dict(precision=gh.FP64, streams=1, batching=False), nbase)):
         if key not in loops:

@@ -1304,11 +1304,13 @@ __global__ void __launch_bounds__(NT)
       const int ntr = ncol - jt;
       const int rt = (n + 3) / 4;
       // rank-kb update of the active rows on trailing columns [c0, c0 + cw),
-      // with their U12 in s_U
+      // with their U12 in s_U.  A thread's 4 x 4 tile holds rows tr + rt i
+      // (i < 4): consecutive lanes take consecutive rows, so the workspace
+      // accesses are contiguous and the panel reads from LDS hit distinct banks.
       auto update_chunk = [&](int c0, int cw) {
         const int ct = (cw + 3) / 4;
         for (int tile = tid; tile < rt * ct; tile += NT) {
-          const int tr4 = (tile % rt) * 4, tc4 = (tile / rt) * 4;
+          const int tr = tile % rt, tc4 = (tile / rt) * 4;
           double acc[4][4];
 #pragma unroll
           for (int i = 0; i < 4; ++i)
@@ -1316,7 +1318,7 @@ __global__ void __launch_bounds__(NT)
             for (int q = 0; q < 4; ++q) acc[i][q] = 0.0;
           bool rowok[4];
 #pragma unroll
-          for (int i = 0; i < 4; ++i) rowok[i] = tr4 + i < n && s_act[tr4 + i];
+          for (int i = 0; i < 4; ++i) rowok[i] = tr + rt * i < n && s_act[tr + rt * i];
           if (!(rowok[0] | rowok[1] | rowok[2] | rowok[3])) continue;
           // the tile's entries load first, in flight during the rank-kb products
           double mt[4][4];
@@ -1325,12 +1327,12 @@ __global__ void __launch_bounds__(NT)
             const int c = jt + c0 + tc4 + q;
 #pragma unroll
             for (int i = 0; i < 4; ++i)
-              mt[i][q] = (tc4 + q < cw && rowok[i]) ? M[(int64_t)c * ld + tr4 + i] : 0.0;
+              mt[i][q] = (tc4 + q < cw && rowok[i]) ? M[(int64_t)c * ld + tr + rt * i] : 0.0;
           }
           for (int jj = 0; jj < kb; ++jj) {
             double l[4], u[4];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) l[i] = rowok[i] ? s_P[jj][tr4 + i] : 0.0;
+            for (int i = 0; i < 4; ++i) l[i] = rowok[i] ? s_P[jj][tr + rt * i] : 0.0;
 #pragma unroll
             for (int q = 0; q < 4; ++q) u[q] = tc4 + q < cw ? s_U[jj][tc4 + q] : 0.0;
 #pragma unroll
@@ -1344,7 +1346,7 @@ __global__ void __launch_bounds__(NT)
             if (tc4 + q >= cw) continue;
 #pragma unroll
             for (int i = 0; i < 4; ++i)
-              if (rowok[i]) M[(int64_t)c * ld + tr4 + i] = mt[i][q] - acc[i][q];
+              if (rowok[i]) M[(int64_t)c * ld + tr + rt * i] = mt[i][q] - acc[i][q];
           }
         }
       };
