@@ -22,6 +22,9 @@ host solve would dominate, the interpolant of the manufactured trace
 A finished round is released at once: its device blocks go back to its
 context's pool (libhdg's per-context block pools), which needs no device
 synchronisation, so the other context's round keeps the GPU busy meanwhile.
+By default each context is driven by a host thread of its own, so that one
+round's host work (mesh set-up, batch creation, result copies) overlaps the
+other's.
 """
 from __future__ import annotations
 
@@ -204,12 +207,18 @@ class Server:
     requests of one round: on MI355X with the uneven-batch stream (meshes of
     4k to 250k cells, each enough to fill the GPU), rounds of 2 served 28.6
     requests/s against 22 for rounds of up to 64, whose larger allocations
-    miss the block pools more often."""
+    miss the block pools more often; a host thread per context (`threaded`)
+    raised rounds of 2 from 26.4 to 34.2 requests/s."""
 
     def __init__(self, gh, hbm_budget=200 << 30, precision=MIXED, streams=2, max_batch=2,
-                 skeleton="interpolant", batching=True, keep_results=True):
+                 skeleton="interpolant", batching=True, keep_results=True, threaded=True):
         self.gh = gh
         self.keep_results = keep_results
+        # threaded: one host thread per context builds, launches, recovers and
+        # releases its own rounds, so one context's host work (mesh set-up,
+        # batch creation, result copies: libhdg and torch release the GIL)
+        # overlaps the other's; otherwise one thread drives every context
+        self.threaded = threaded and streams > 1
         self.ctxs = [gh.Context(0) for _ in range(streams)]
         self.budget = int(hbm_budget)
         self.precision = precision
@@ -227,6 +236,8 @@ class Server:
         """Serves `requests` (arrival times relative to the start) and returns
         them with latencies filled in; requests whose footprint exceeds the
         whole budget are rejected (done stays None)."""
+        if self.threaded:
+            return self._run_threaded(requests, progress)
         pending = deque(sorted(requests, key=lambda r: r.arrival))
         waiting = deque()
         inflight = []
@@ -318,6 +329,120 @@ class Server:
                 time.sleep(wait)
         for c in self.ctxs:
             c.synchronize()
+        return requests
+
+    def _report(self, requests, progress, used, inflight, waiting):
+        import torch
+        free, total = torch.cuda.mem_get_info()
+        live = pooled = 0
+        u = C.c_int64()
+        for cx in self.ctxs:
+            a, b = C.c_int64(), C.c_int64()
+            check(lib().hdg_ctx_memory(cx.h, C.byref(a), C.byref(b), C.byref(u)), cx.h, "memory")
+            live, pooled = live + a.value, pooled + b.value
+        progress(f"t={self.clock():.1f}s done={sum(r.done is not None for r in requests)} "
+                 f"waiting={waiting} inflight={inflight} "
+                 f"admitted={used / 2**30:.1f}GiB live={live / 2**30:.1f}GiB "
+                 f"pooled={pooled / 2**30:.1f}GiB unpooled={u.value / 2**30:.1f}GiB "
+                 f"free={free / 2**30:.1f}GiB "
+                 f"torch={torch.cuda.memory_reserved() / 2**30:.1f}GiB")
+
+    def _run_threaded(self, requests, progress=None):
+        """run() with one worker thread per context.  Admission (arrival order,
+        footprint budget, round size) is shared under a lock; each worker
+        admits a round, builds and launches it on its context, waits for its
+        GPU work, recovers, copies the results and releases the round."""
+        import threading
+        pending = deque(sorted(requests, key=lambda r: r.arrival))
+        waiting = deque()
+        st = dict(used=0, inflight=0, blocked=False)
+        lock = threading.Lock()
+        errors = []
+        self.memory_retries = 0
+        self.max_round = self.max_batch
+        self.rejected = []
+        self.rounds = []
+        self.t0 = time.perf_counter()
+
+        def arrivals():  # (under the lock)
+            now = self.clock()
+            while pending and pending[0].arrival <= now:
+                r = pending.popleft()
+                tf = time.perf_counter()
+                r.footprint = footprint(r, self.precision)
+                self.host_s["footprint"] += time.perf_counter() - tf
+                (self.rejected if r.footprint > self.budget else waiting).append(r)
+
+        def worker(ctx):
+            try:
+                while True:
+                    take, need, wait = [], 0, 2e-4
+                    with lock:
+                        if errors:
+                            return
+                        arrivals()
+                        if not pending and not waiting:
+                            return
+                        if not st["blocked"]:
+                            while (waiting and len(take) < min(self.max_batch, self.max_round)
+                                   and st["used"] + need + waiting[0].footprint <= self.budget):
+                                r = waiting.popleft()
+                                take.append(r)
+                                need += r.footprint
+                        if take:
+                            st["used"] += need
+                            st["inflight"] += 1
+                        elif pending and not waiting:
+                            wait = max(0.0, min(0.05, pending[0].arrival - self.clock()))
+                    if not take:
+                        time.sleep(wait)
+                        continue
+                    try:
+                        rd = _Round(self, ctx, take, self.clock())
+                    except HDGError as err:
+                        with lock:
+                            st["used"] -= need
+                            st["inflight"] -= 1
+                            if not _out_of_memory(err) or (len(take) == 1 and not st["inflight"]):
+                                raise
+                            # give the requests back; admit again once a round
+                            # is done, or, with nothing in flight, as a smaller round
+                            for r in reversed(take):
+                                waiting.appendleft(r)
+                            if not st["inflight"]:
+                                self.max_round = max(1, len(take) // 2)
+                            self.memory_retries += 1
+                            st["blocked"] = st["inflight"] > 0
+                        continue
+                    with lock:
+                        self.rounds.append(len(take))
+                    while not rd.advance(self.clock()):
+                        time.sleep(1e-4)
+                    rd.release()
+                    with lock:
+                        st["used"] -= rd.bytes
+                        st["inflight"] -= 1
+                        st["blocked"] = False
+            except BaseException as e:  # handed to the caller's thread
+                with lock:
+                    errors.append(e)
+
+        threads = [threading.Thread(target=worker, args=(c,), daemon=True) for c in self.ctxs]
+        for t in threads:
+            t.start()
+        last_report = 0.0
+        while any(t.is_alive() for t in threads):
+            for t in threads:
+                t.join(timeout=0.05)
+            if progress and self.clock() - last_report > 5.0:
+                last_report = self.clock()
+                with lock:
+                    used, inflight, nwait = st["used"], st["inflight"], len(waiting)
+                self._report(requests, progress, used, inflight, nwait)
+        for c in self.ctxs:
+            c.synchronize()
+        if errors:
+            raise errors[0]
         return requests
 
     def close(self):
