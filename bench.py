@@ -134,6 +134,11 @@ def parse():
     ap.add_argument("--serve-loops", default="fp64,mixed,per_request",
                     help="serving loops to run: fp64 and mixed continuous batching, "
                          "per_request (one request at a time, FP64)")
+    ap.add_argument("--stages", action="store_true",
+                    help="four-stage timing (condense+assemble, device skeleton solve, "
+                         "recover, total) of one 3D case (--n3d, --order, --physics)")
+    ap.add_argument("--stages-host-max", type=int, default=300000,
+                    help="largest skeleton system also solved by the host direct solver")
     ap.add_argument("--serve-max-batch", type=int, default=2,
                     help="requests per round of the continuous-batching loops")
     ap.add_argument("--serve-budget-gib", type=float, default=160.0,
@@ -579,6 +584,47 @@ def only_main(gh, args):
     case.op.close()
 
 
+def stages_main(gh, args):
+    """The four-stage split of the reference's experiments/driver.jl:19-41 on
+    one 3D case: condense+assemble, skeleton solve, recover, total.  The
+    skeleton solve is the device one (hdg_skeleton_solve to rtol 1e-10), and
+    the host sparse direct solve of the reference is timed beside it while the
+    system has at most --stages-host-max unknowns."""
+    import torch
+    import scipy.sparse.linalg as spla
+    ctx = gh.Context(0)
+    case = Case(gh, ctx, 3, args.n3d, args.order, physics=args.physics)
+    op, info = case.op, case.info
+    L = gh._lib.lib()
+    out = {"n3d": args.n3d, "order": args.order, "physics": args.physics,
+           "cells": info.ncells, "nfree": info.nfree, "nnz": info.nnz}
+    with ctx.stream():
+        for rep in range(2):  # the first pass warms up
+            t0 = time.perf_counter()
+            gh._lib.check(L.hdg_condense_assemble(op.mesh, gh.FP64, None), ctx.h, "condense")
+            ctx.synchronize()
+            t1 = time.perf_counter()
+            lam, it, res = op.skeleton_solve(rtol=1e-10)
+            t2 = time.perf_counter()
+            gh._lib.check(L.hdg_recover(op.mesh, lam.data_ptr(), case.out.data_ptr()), ctx.h,
+                          "recover")
+            ctx.synchronize()
+            t3 = time.perf_counter()
+    out.update(condense_assemble_s=t1 - t0, skeleton_solve_s=t2 - t1, recover_s=t3 - t2,
+               total_s=t3 - t0, solver="device MINRES + facet block-Jacobi, rtol 1e-10",
+               iterations=it, relres=res)
+    if info.nfree <= args.stages_host_max:
+        K, rhs = op.skeleton_system()
+        t4 = time.perf_counter()
+        lam_h = spla.spsolve(K.tocsc(), rhs)
+        out["host_direct_solve_s"] = time.perf_counter() - t4
+        lam_d = lam.cpu().numpy()
+        out["device_vs_direct_rel"] = float(np.max(np.abs(lam_d - lam_h)) / np.max(np.abs(lam_h)))
+    print(json.dumps(out))
+    _ = torch
+    case.op.close()
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -600,6 +646,8 @@ def main():
         return serve_main(gh, args)
     if args.batch:
         return batch_main(gh, args)
+    if args.stages:
+        return stages_main(gh, args)
     if args.only:
         return only_main(gh, args)
     ctx = gh.Context(0)

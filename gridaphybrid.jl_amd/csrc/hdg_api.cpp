@@ -1408,6 +1408,133 @@ int hdg_mesh_set_family(hdg_mesh* m, int family, int grid_per_cu) {
   return HDG_OK;
 }
 
+int hdg_skeleton_solve(hdg_mesh* m, double rtol, int maxit, double* lam_free, int* iters,
+                       double* relres) {
+  if (!m) return HDG_ERR_ARG;
+  hdg_ctx* ctx = m->ctx;
+  PoolScope pool_(ctx->pool);
+  if (!lam_free && m->nfree > 0) return fail(ctx, HDG_ERR_ARG, "null buffer");
+  if (!(rtol > 0.0)) return fail(ctx, HDG_ERR_ARG, "rtol must be positive");
+  const int64_t* colptr;
+  const int32_t* rowval;
+  const double *nzval, *rhs;
+  int rc = hdg_skeleton_system(m, &colptr, &rowval, &nzval, &rhs);
+  if (rc) return rc;
+  if (iters) *iters = 0;
+  if (relres) *relres = 0.0;
+  const int64_t n = m->nfree;
+  if (n == 0) return HDG_OK;
+  if (maxit <= 0) maxit = (int)std::min<int64_t>(10 * n, 50000);
+  hipStream_t st = ctx->stream;
+  // preconditioner: per diagonal block of one facet's trace dofs (of one
+  // field with several trace fields: their numbering is field-major, each
+  // facet's dofs of a field contiguous) the inverse of sign * block where that
+  // is positive definite, else the inverse of its |diagonal|: SPD
+  const int bsf = m->th.ntf / std::max(1, m->trace_fields);
+  const int bs = (bsf >= 1 && bsf <= block_jacobi_max() && n % bsf == 0) ? bsf : 1;
+  const int64_t nblk = n / bs;
+  // r1, r2, y, v, w (three), t, the block inverses and the dot scratch
+  DevBuf ws;
+  const size_t nd = multidot_scratch_doubles();
+  HIP_TRY(ctx, ws.alloc((8 * (size_t)n + (size_t)n * bs + nd) * sizeof(double)),
+          "alloc solve workspace");
+  double* r1 = ws.as<double>();
+  double *r2 = r1 + n, *y = r2 + n, *v = y + n, *wa = v + n, *wb = wa + n, *wc = wb + n,
+         *t = wc + n, *binv = t + n, *scr = binv + (size_t)n * bs;
+  double* x = lam_free;
+  HIP_TRY(ctx, launch_block_inv(colptr, rowval, nzval, binv, nblk, bs, st), "block inverses");
+  double h[3];
+  auto dots = [&](int k, const double* a0, const double* b0, const double* a1,
+                  const double* b1) -> hipError_t {
+    hipError_t e = launch_multidot(k, n, a0, b0, a1, b1, nullptr, nullptr, scr, st);
+    if (e != hipSuccess) return e;
+    e = hipMemcpyAsync(h, scr + nd - 3, 3 * sizeof(double), hipMemcpyDeviceToHost, st);
+    if (e != hipSuccess) return e;
+    return hipStreamSynchronize(st);
+  };
+  double res = 1.0;
+  // |b - K x| / |b|, from t = K x
+  auto true_residual = [&](double bnorm) -> hipError_t {
+    hipError_t e = launch_spmv_sym(colptr, rowval, nzval, x, t, n, 1.0, st);
+    if (e == hipSuccess)
+      e = launch_krylov_update(3, n, 0, 0, 0, 0, t, nullptr, rhs, t, nullptr, st);
+    if (e == hipSuccess) e = dots(1, t, t, nullptr, nullptr);
+    res = std::sqrt(h[0]) / bnorm;
+    return e;
+  };
+  // preconditioned MINRES (Paige and Saunders): symmetric K of any inertia,
+  // with the SPD block preconditioner above
+  HIP_TRY(ctx, hipMemsetAsync(x, 0, n * sizeof(double), st), "zero x");
+  HIP_TRY(ctx, hipMemsetAsync(wa, 0, n * sizeof(double), st), "zero w");
+  HIP_TRY(ctx, hipMemsetAsync(wb, 0, n * sizeof(double), st), "zero w");
+  HIP_TRY(ctx, hipMemcpyAsync(r1, rhs, n * sizeof(double), hipMemcpyDeviceToDevice, st), "r1");
+  HIP_TRY(ctx, hipMemcpyAsync(r2, rhs, n * sizeof(double), hipMemcpyDeviceToDevice, st), "r2");
+  HIP_TRY(ctx, launch_block_apply(binv, r1, y, n, bs, st), "precond");
+  HIP_TRY(ctx, dots(2, r1, y, r1, r1), "dot");
+  const double bnorm = std::sqrt(h[1]);
+  if (bnorm == 0.0) return HDG_OK;  // x = 0
+  if (!(h[0] > 0.0)) return fail(ctx, HDG_ERR_STATE, "skeleton solve: indefinite preconditioner");
+  const double beta1 = std::sqrt(h[0]);
+  double oldb = 0.0, beta = beta1, dbar = 0.0, epsln = 0.0, phibar = beta1, cs = -1.0, sn = 0.0;
+  double *w = wa, *w2 = wb, *w1 = wc;
+  int it = 0;
+  bool conv = false;
+  while (it < maxit && !conv) {
+    ++it;
+    HIP_TRY(ctx, launch_krylov_update(2, n, 1.0 / beta, 0, 0, 0, v, nullptr, y, nullptr, nullptr,
+                                      st), "v");
+    HIP_TRY(ctx, launch_spmv_sym(colptr, rowval, nzval, v, y, n, 1.0, st), "spmv");
+    if (it >= 2)
+      HIP_TRY(ctx, launch_krylov_update(0, n, beta / oldb, 0, 0, 0, r1, y, nullptr, nullptr,
+                                        nullptr, st), "y");
+    HIP_TRY(ctx, dots(1, v, y, nullptr, nullptr), "dot");
+    const double alfa = h[0];
+    HIP_TRY(ctx, launch_krylov_update(0, n, alfa / beta, 0, 0, 0, r2, y, nullptr, nullptr,
+                                      nullptr, st), "y");
+    std::swap(r1, r2);  // r1 <- r2, r2 <- y, y <- the old r1
+    std::swap(r2, y);
+    HIP_TRY(ctx, launch_block_apply(binv, r2, y, n, bs, st), "precond");
+    oldb = beta;
+    HIP_TRY(ctx, dots(1, r2, y, nullptr, nullptr), "dot");
+    if (h[0] < 0.0) return fail(ctx, HDG_ERR_STATE, "skeleton solve: indefinite preconditioner");
+    beta = std::sqrt(h[0]);
+    const double oldeps = epsln;
+    const double delta = cs * dbar + sn * alfa;
+    const double gbar = sn * dbar - cs * alfa;
+    epsln = sn * beta;
+    dbar = -cs * beta;
+    const double gamma = std::max(std::hypot(gbar, beta), 1e-300);
+    cs = gbar / gamma;
+    sn = beta / gamma;
+    const double phi = cs * phibar;
+    phibar = sn * phibar;
+    // w1 <- w2, w2 <- w, w <- (v - oldeps w1 - delta w2) / gamma; x += phi w
+    std::swap(w1, w2);  // w1 = old w2, w2 = old w1 (dead)
+    std::swap(w2, w);   // w2 = old w, w = dead buffer
+    HIP_TRY(ctx, launch_krylov_update(1, n, oldeps, delta, 1.0 / gamma, phi, w, x, v, w1, w2, st),
+            "w, x");
+    // phibar estimates the preconditioned residual norm; confirm with the
+    // true residual when it meets rtol, and every 100 iterations
+    if (phibar <= rtol * beta1 || it % 100 == 0 || beta == 0.0) {
+      HIP_TRY(ctx, true_residual(bnorm), "residual");
+      conv = res <= rtol;
+      if (beta == 0.0) break;  // the Krylov space is exhausted
+    }
+  }
+  if (!conv) HIP_TRY(ctx, true_residual(bnorm), "residual");
+  conv = res <= rtol;
+  if (iters) *iters = it;
+  if (relres) *relres = res;
+  if (!conv) {
+    char msg[160];
+    std::snprintf(msg, sizeof msg,
+                  "skeleton solve: relative residual %.3e after %d iterations (rtol %.1e)", res,
+                  it, rtol);
+    return fail(ctx, HDG_ERR_STATE, msg);
+  }
+  return HDG_OK;
+}
+
 namespace {
 int recover_facet_major(hdg_mesh* m, const double* lam_free, double* free_values);
 }  // namespace

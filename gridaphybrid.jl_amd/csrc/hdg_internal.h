@@ -318,6 +318,25 @@ hipError_t launch_recover_mixed(const Tables& T, const MeshDev& M, const double*
                                 const double* lam_free, const double* lam_dir, double* out,
                                 int64_t nfree, hipStream_t st);
 hipError_t launch_symbolic(const PairTable& P, int32_t* rowval, hipStream_t st);
+// device skeleton solve (symmetric CSC read as CSR; y = scale K x)
+hipError_t launch_spmv_sym(const int64_t* colptr, const int32_t* rowval, const double* nzval,
+                           const double* x, double* y, int64_t n, double scale, hipStream_t st);
+// SPD preconditioner blocks [nblk][bs][bs]: (s B)^-1 for each diagonal block
+// B with s B positive definite (s the sign of its diagonal), else
+// diag(1 / |B_ii|)
+hipError_t launch_block_inv(const int64_t* colptr, const int32_t* rowval, const double* nzval,
+                            double* binv, int64_t nblk, int bs, hipStream_t st);
+hipError_t launch_block_apply(const double* binv, const double* r, double* z, int64_t n, int bs,
+                              hipStream_t st);
+hipError_t launch_krylov_update(int op, int64_t n, double a, double b, double c, double d,
+                                double* u0, double* u1, const double* w0, const double* w1,
+                                const double* w2, hipStream_t st);
+int block_jacobi_max();
+size_t multidot_scratch_doubles();
+// nd (1..3) dot products (a_k, b_k); the sums land in scratch[3 * 1024 + k]
+hipError_t launch_multidot(int nd, int64_t n, const double* a0, const double* b0,
+                           const double* a1, const double* b1, const double* a2,
+                           const double* b2, double* scratch, hipStream_t st);
 hipError_t launch_gather(const double* src, const int64_t* idx, double* dst, int64_t n,
                          hipStream_t st);
 hipError_t launch_copy(const double* src, double* dst, int64_t n, hipStream_t st);

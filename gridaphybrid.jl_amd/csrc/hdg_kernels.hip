@@ -914,259 +914,6 @@ __device__ inline int wave_min_i32(int x) {
   return __builtin_amdgcn_readlane(x, 63);
 }
 
-template <int R, int CC, int NW, class Gen, bool BACKSUB>
-__global__ void __launch_bounds__(64 * NW)
-    condense_colwave_kernel(Gen gen, CondenseOut out, int64_t ncells, int ni, int nb) {
-  constexpr int NT = 64 * NW;
-  constexpr int NMAX = 64 * R;
-  static_assert(R <= 32, "row mask is 32 bits");
-  const int n = ni + nb, ncol = n + 1;
-  const int lane = threadIdx.x & 63;
-  const int w = threadIdx.x >> 6;
-
-  __shared__ double s_lcol[2][NMAX];
-  __shared__ double s_diag[NMAX];
-  __shared__ int s_pos[NMAX];
-  __shared__ int s_at[NMAX];
-  __shared__ int s_piv[2];
-  __shared__ int s_info;
-  __shared__ typename Gen::Shared s_gen;
-  __shared__ double s_slab[NW][NMAX];
-  __shared__ double s_urow[NW][CC];
-  __shared__ double s_colv[NMAX];
-
-  for (int64_t cell = blockIdx.x; cell < ncells; cell += gridDim.x) {
-    gen.begin(cell, s_gen);
-    for (int i = threadIdx.x; i < NMAX; i += NT) {
-      s_pos[i] = i;
-      s_at[i] = i;
-    }
-    if (threadIdx.x == 0) s_info = 0;
-    __syncthreads();
-
-    // ---- generate: slab c holds local column c of every wave -------------
-    double a[R][CC];
-    uint32_t act = 0;
-#pragma unroll
-    for (int r = 0; r < R; ++r)
-      if (lane + 64 * r < n) act |= 1u << r;
-#pragma unroll 1
-    for (int c = 0; c < CC; ++c) {
-      for (int t = threadIdx.x; t < n * NW; t += NT) {
-        const int cc = t / n, row = t - cc * n;
-        const int col = cc + NW * c;
-        s_slab[cc][row] = col < ncol ? GenCall<Gen>::entry(gen, s_gen, row, col, cell) : 0.0;
-      }
-      __syncthreads();
-      double v[R];
-#pragma unroll
-      for (int r = 0; r < R; ++r) v[r] = lane + 64 * r < n ? s_slab[w][lane + 64 * r] : 0.0;
-#pragma unroll
-      for (int c2 = 0; c2 < CC; ++c2)
-        if (c2 == c) {
-#pragma unroll
-          for (int r = 0; r < R; ++r) a[r][c2] = v[r];
-        }
-      __syncthreads();
-    }
-
-    // pivot search + multipliers + bookkeeping of step j on local column cj
-    // (executed by the wave owning column j; wave-uniform control flow)
-    auto pivot_step = [&](int j, int buf) {
-      double colv[R];
-#pragma unroll
-      for (int r = 0; r < R; ++r) colv[r] = s_colv[lane + 64 * r];
-      int64_t bkey = -1;  // |v| bit pattern (monotone for |v| >= 0); -1: no candidate
-      int bpos = INT_MAX;
-#pragma unroll
-      for (int r = 0; r < R; ++r) {
-        const int row = lane + 64 * r;
-        const bool cand = row < ni && ((act >> r) & 1u);
-        const int64_t key = cand ? dbits(fabs(colv[r])) : (int64_t)-1;
-        const int pr = cand ? s_pos[row] : INT_MAX;
-        const bool better = key > bkey || (key == bkey && pr < bpos);
-        bkey = better ? key : bkey;
-        bpos = better ? pr : bpos;
-      }
-      const int64_t m = wave_max_i64(bkey);
-      const int key = wave_min_i32(bkey == m ? bpos : INT_MAX);
-      const int p = __builtin_amdgcn_readfirstlane(s_at[key]);
-      const double pv = s_colv[p];
-      const double rinv = pv != 0.0 ? 1.0 / pv : 0.0;
-#pragma unroll
-      for (int r = 0; r < R; ++r) {
-        const int row = lane + 64 * r;
-        const bool use = ((act >> r) & 1u) && row != p;
-        s_lcol[buf][row] = use ? colv[r] * rinv : 0.0;
-      }
-      if (lane == 0) {
-        s_piv[buf] = p;
-        s_diag[j] = pv;
-        if (pv == 0.0 && s_info == 0) s_info = j + 1;
-        const int q = s_at[j];
-        s_at[key] = q;
-        s_pos[q] = key;
-        s_at[j] = p;
-        s_pos[p] = j;
-      }
-    };
-
-    if (ni > 0 && w == 0) {
-#pragma unroll
-      for (int r = 0; r < R; ++r) s_colv[lane + 64 * r] = a[r][0];
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-      pivot_step(0, 0);
-    }
-    __syncthreads();
-
-#pragma unroll 1
-    for (int j = 0; j < ni; ++j) {
-      const int buf = j & 1;
-      const int jl = j / NW;
-      const int p = __builtin_amdgcn_readfirstlane(s_piv[buf]);
-      const int pl = p >> 6, plane = p & 63;
-      if (lane == plane) act &= ~(1u << pl);
-      double l[R];
-#pragma unroll
-      for (int r = 0; r < R; ++r) {
-        const double lv = s_lcol[buf][lane + 64 * r];
-        l[r] = ((act >> r) & 1u) ? lv : 0.0;
-      }
-      // pivot row of this wave's columns: its owner lane stores it in the
-      // wave's LDS row buffer (pl is wave-uniform: a scalar switch, no
-      // per-column selects), then every lane reads it back by broadcast
-#pragma unroll
-      for (int r = 0; r < R; ++r)
-        if (r == pl) {  // wave-uniform branch
-          if (lane == plane) {
-#pragma unroll
-            for (int c = 0; c < CC; ++c) s_urow[w][c] = a[r][c];
-          }
-        }
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-      const int j1 = j + 1;
-      const bool look = j1 < ni && w == (j1 % NW);  // wave-uniform
-      const int c1 = j1 / NW;
-#pragma unroll
-      for (int c = 0; c < CC; ++c) {
-        if (c < jl) continue;  // local column fully eliminated
-        const double u = s_urow[w][c];
-#pragma unroll
-        for (int r = 0; r < R; ++r) a[r][c] = fma(-l[r], u, a[r][c]);
-        if (look && c == c1) {  // wave-uniform: stage column j+1 for its pivot search
-#pragma unroll
-          for (int r = 0; r < R; ++r) s_colv[lane + 64 * r] = a[r][c];
-        }
-      }
-      if (look) {
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        pivot_step(j1, buf ^ 1);
-      }
-      __syncthreads();
-    }
-    const int info = s_info;
-
-    // ---- back substitution (backward map only; single rhs column) -------
-    if (BACKSUB) {
-      const int rc = ni / NW, rw = ni % NW;  // rhs column ni lives in wave rw, local rc
-#pragma unroll 1
-      for (int k = ni - 1; k >= 0; --k) {
-        const int buf = k & 1;
-        const int p = __builtin_amdgcn_readfirstlane(s_at[k]);
-        const int pl = p >> 6, plane = p & 63;
-        const double d = s_diag[k];
-        // x_k in the rhs-owning wave, multipliers U(., k) from the column-k wave
-        if (w == rw) {
-#pragma unroll
-          for (int c = 0; c < CC; ++c)
-            if (c == rc) {
-              double sel = 0.0;
-#pragma unroll
-              for (int r = 0; r < R; ++r)
-                if (r == pl) sel = a[r][c];
-              const double x = d != 0.0 ? wave_readlane_f64(sel, plane) / d : 0.0;
-              if (lane == 0) s_lcol[buf][NMAX - 1] = x;  // not a row slot used here
-#pragma unroll
-              for (int r = 0; r < R; ++r)
-                if (r == pl && lane == plane) a[r][c] = x;
-            }
-        }
-        if (w == k % NW) {
-#pragma unroll
-          for (int c = 0; c < CC; ++c)
-            if (c == k / NW) {
-#pragma unroll
-              for (int r = 0; r < R; ++r) {
-                const int row = lane + 64 * r;
-                const bool use = row < ni && s_pos[row] < k;
-                if (row < NMAX - 1) s_lcol[buf][row] = use ? a[r][c] : 0.0;
-              }
-            }
-        }
-        __syncthreads();
-        if (w == rw) {
-          const double x = s_lcol[buf][NMAX - 1];
-#pragma unroll
-          for (int c = 0; c < CC; ++c)
-            if (c == rc) {
-#pragma unroll
-              for (int r = 0; r < R; ++r) {
-                const int row = lane + 64 * r;
-                if (row < ni && row < NMAX - 1) a[r][c] = fma(-s_lcol[buf][row], x, a[r][c]);
-              }
-            }
-        }
-        __syncthreads();
-      }
-    }
-
-    // ---- outputs, slab by slab --------------------------------------------
-#pragma unroll 1
-    for (int c = 0; c < CC; ++c) {
-      if (NW * c + NW <= ni && out.U == nullptr) continue;
-#pragma unroll
-      for (int c2 = 0; c2 < CC; ++c2)
-        if (c2 == c) {
-#pragma unroll
-          for (int r = 0; r < R; ++r) s_slab[w][lane + 64 * r] = a[r][c2];
-        }
-      __syncthreads();
-      for (int t = threadIdx.x; t < n * NW; t += NT) {
-        const int cc = t / n, row = t - cc * n;
-        const int col = cc + NW * c;
-        if (col >= ncol) continue;
-        const double v = s_slab[cc][row];
-        if (row >= ni) {
-          if (col < ni) continue;
-          if (col < n) {
-            if (out.S) out.S[(cell * nb + (col - ni)) * nb + (row - ni)] = v;
-          } else if (out.g) {
-            out.g[cell * nb + (row - ni)] = v;
-          }
-        } else {
-          const int k = s_pos[row];
-          if (col < ni) {
-            if (out.U && col >= k)
-              out.U[cell * ustride(out, ni) + col * (col + 1) / 2 + k] = v;
-          } else {
-            if (out.Y) out.Y[cell * ystride(out, ni, nb) + (int64_t)(col - ni) * ni + k] = v;
-            if (out.X && col == n) out.X[cell * ni + k] = v;
-          }
-        }
-      }
-      __syncthreads();
-    }
-    if (threadIdx.x == 0 && out.info) out.info[cell] = info;
-    __syncthreads();
-  }
-}
-
 // --------------------------------------------------------------------------
 // Blocked elimination for large systems (up to NMAX rows): the cell system
 // lives in a per-workgroup HBM workspace (column-major, leading dimension
@@ -1675,238 +1422,6 @@ __global__ void __launch_bounds__(256)
         if (out.S) out.S[(cell * nb + cb) * nb + rb] = v;
       } else if (out.g) {
         out.g[cell * nb + rb] = v;
-      }
-    }
-    if (tid == 0 && out.info) out.info[cell] = s_info;
-    __syncthreads();
-  }
-}
-
-// --------------------------------------------------------------------------
-// Register-resident blocked elimination (n <= 16 * 11): the cell system is a
-// grid of 16x16 FP64 tiles held in MFMA accumulator layout by the 16
-// wavefronts of the workgroup (tile t belongs to wavefront t % 16, register
-// slot t / 16; lane l, register v of a tile hold row 16 rt + l/16 + 4v,
-// column 16 ct + l%16).  Per panel of 16 interior columns: the panel's tile
-// column goes to LDS and is factored by wavefront 0 (getrf's pivots, LAPACK
-// position bookkeeping); the pivot rows' trailing parts are gathered from
-// the tiles into LDS and brought to U form (U12); every tile right of the
-// panel then takes C -= L21 U12 through v_mfma_f64_16x16x4f64.  Nothing of
-// the cell system goes through HBM; S and g are written from the tiles.
-// --------------------------------------------------------------------------
-constexpr int kTileMaxN = 16 * 11 - 1;  // n + 1 columns in at most 11 tile columns
-
-__device__ inline double dbl4_at(const dbl4& a, int v) {
-  return v == 0 ? a[0] : (v == 1 ? a[1] : (v == 2 ? a[2] : a[3]));
-}
-
-constexpr int kTileThreads = 512;
-
-template <int TPW, class Gen>
-__global__ void __launch_bounds__(kTileThreads, 1)
-    condense_tile_kernel(Gen gen, CondenseOut out, int64_t ncells, int ni, int nb) {
-  constexpr int NT = kTileThreads, NW = NT / 64, KB = 16, NMAX = 16 * 11, LDP = NMAX + 16;
-  const int n = ni + nb, ncol = n + 1;
-  const int RT = (n + 15) >> 4, CT = (ncol + 15) >> 4, ntiles = RT * CT;
-  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  const int i16 = lane & 15, q4 = lane >> 4;
-
-  __shared__ double s_P[KB][LDP];  // panel columns (all rows), then multipliers
-  __shared__ double s_U[KB][LDP];  // pivot rows x trailing columns (U12)
-  __shared__ double s_stage[NW][4][64];  // one generated tile per wavefront
-  __shared__ int s_pos[NMAX], s_at[NMAX];
-  __shared__ unsigned char s_act[NMAX];
-  __shared__ int s_piv[KB];
-  __shared__ int s_info;
-  __shared__ short s_rc[NW * TPW];  // tile (rt, ct) of slot (wavefront, i): rt | ct << 8
-  __shared__ typename Gen::Shared s_gen;
-  for (int t = tid; t < NW * TPW; t += NT) {
-    const int tt = (t % NW) + NW * (t / NW);
-    s_rc[t] = (short)(tt < ntiles ? (tt % RT) | ((tt / RT) << 8) : 0x7f7f);
-  }
-
-  for (int64_t cell = blockIdx.x; cell < ncells; cell += gridDim.x) {
-    gen.begin(cell, s_gen);
-    for (int i = tid; i < NMAX; i += NT) {
-      s_pos[i] = i;
-      s_at[i] = i;
-      s_act[i] = i < n ? 1 : 0;
-    }
-    if (tid == 0) s_info = 0;
-    __syncthreads();
-    // generate the wavefront's tiles one at a time through LDS (one call
-    // site of the entry program keeps the register budget for the tiles)
-    dbl4 acc[TPW];
-#pragma unroll 1
-    for (int i = 0; i < TPW; ++i) {
-      const int t = wv + NW * i;
-      const int rc = s_rc[wv + NW * i], rt = rc & 0xff, ct = rc >> 8;
-#pragma unroll 1
-      for (int v = 0; v < 4; ++v) {
-        const int row = 16 * rt + q4 + 4 * v, col = 16 * ct + i16;
-        s_stage[wv][v][lane] = (t < ntiles && row < n && col < ncol)
-                                   ? GenCall<Gen>::entry(gen, s_gen, row, col, cell)
-                                   : 0.0;
-      }
-      wave_sync_lds();
-      dbl4 g;
-#pragma unroll
-      for (int v = 0; v < 4; ++v) g[v] = s_stage[wv][v][lane];
-#pragma unroll
-      for (int ii = 0; ii < TPW; ++ii)
-        if (ii == i) acc[ii] = g;
-      wave_sync_lds();
-    }
-
-    for (int j0 = 0; j0 < ni; j0 += KB) {
-      const int kb = min(KB, ni - j0);
-      const int jt = j0 + kb;  // first trailing column
-      const int pc = j0 >> 4;  // tile column of the panel
-      // ---- panel tile column -> LDS ----
-#pragma unroll
-      for (int i = 0; i < TPW; ++i) {
-        const int t = wv + NW * i;
-        const int rc = s_rc[wv + NW * i], rt = rc & 0xff, ct = rc >> 8;
-        if (t < ntiles && ct == pc && i16 < kb) {
-#pragma unroll
-          for (int v = 0; v < 4; ++v) {
-            const int row = 16 * rt + q4 + 4 * v;
-            if (row < n) s_P[i16][row] = acc[i][v];
-          }
-        }
-      }
-      __syncthreads();
-      // ---- factor the panel: wavefront 0 only ----
-      if (wv == 0) {
-        for (int jj = 0; jj < kb; ++jj) {
-          const int j = j0 + jj;
-          int64_t bkey = -1;
-          int bpos = INT_MAX;
-          for (int row = lane; row < ni; row += 64) {
-            if (!s_act[row]) continue;
-            const int64_t key = dbits(fabs(s_P[jj][row]));
-            const int pr = s_pos[row];
-            if (key > bkey || (key == bkey && pr < bpos)) {
-              bkey = key;
-              bpos = pr;
-            }
-          }
-          const int64_t wm = wave_max_i64(bkey);
-          const int key = wave_min_i32(bkey == wm ? bpos : INT_MAX);
-          const int p = s_at[key];
-          const double pv = s_P[jj][p];
-          wave_sync_lds();
-          if (lane == 0) {
-            s_piv[jj] = p;
-            if (pv == 0.0 && s_info == 0) s_info = j + 1;
-            s_act[p] = 0;
-            const int q = s_at[j];
-            s_at[key] = q;
-            s_pos[q] = key;
-            s_at[j] = p;
-            s_pos[p] = j;
-          }
-          wave_sync_lds();
-          const double rinv = pv != 0.0 ? 1.0 / pv : 0.0;
-          // each active row with all its loads in flight
-          for (int row = lane; row < n; row += 64) {
-            if (!s_act[row]) continue;
-            double r[KB];
-#pragma unroll
-            for (int cc = 0; cc < KB; ++cc) r[cc] = cc >= jj && cc < kb ? s_P[cc][row] : 0.0;
-            const double l = r[jj % KB] * rinv;
-#pragma unroll
-            for (int cc = 0; cc < KB; ++cc) {
-              if (cc == jj) s_P[cc][row] = l;
-              if (cc > jj && cc < kb) s_P[cc][row] = fma(-l, s_P[cc][p], r[cc]);
-            }
-          }
-          wave_sync_lds();
-        }
-      }
-      __syncthreads();
-      // ---- pivot rows: panel part to U; trailing parts gathered into U12 ----
-      if (out.U) {
-        for (int t = tid; t < kb * KB; t += NT) {
-          const int jj = t / KB, cc = t - jj * KB;
-          if (cc >= kb || cc < jj) continue;
-          const int col = j0 + cc, k = j0 + jj;
-          out.U[cell * ustride(out, ni) + col * (col + 1) / 2 + k] =
-              s_P[cc][s_piv[jj]];
-        }
-      }
-#pragma unroll
-      for (int i = 0; i < TPW; ++i) {
-        const int t = wv + NW * i;
-        const int rc = s_rc[wv + NW * i], rt = rc & 0xff, ct = rc >> 8;
-        if (t >= ntiles || 16 * ct + 15 < jt) continue;
-        const int col = 16 * ct + i16;
-        for (int jj = 0; jj < kb; ++jj) {
-          const int rr = s_piv[jj] - 16 * rt;
-          if (rr < 0 || rr >= 16) continue;
-          if (q4 == (rr & 3) && col >= jt && col < ncol) s_U[jj][col - jt] = dbl4_at(acc[i], rr >> 2);
-        }
-      }
-      __syncthreads();
-      for (int c = jt + tid; c < ncol; c += NT) {
-        double u[KB];
-#pragma unroll
-        for (int jj = 0; jj < KB; ++jj) u[jj] = jj < kb ? s_U[jj][c - jt] : 0.0;
-#pragma unroll
-        for (int jj = 0; jj < KB; ++jj) {
-          if (jj < kb) {
-            const int pr = s_piv[jj];
-            double v = u[jj];
-#pragma unroll
-            for (int ii = 0; ii < jj; ++ii) v = fma(-s_P[ii][pr], u[ii], v);
-            u[jj] = v;
-            s_U[jj][c - jt] = v;
-            const int k = j0 + jj;
-            if (c < ni) {
-              if (out.U) out.U[cell * ustride(out, ni) + c * (c + 1) / 2 + k] = v;
-            } else if (out.Y) {
-              out.Y[cell * ystride(out, ni, nb) + (int64_t)(c - ni) * ni + k] = v;
-            }
-          }
-        }
-      }
-      __syncthreads();
-      // ---- trailing update C -= L21 U12 on every tile right of the panel ----
-#pragma unroll
-      for (int i = 0; i < TPW; ++i) {
-        const int t = wv + NW * i;
-        const int rc = s_rc[wv + NW * i], rt = rc & 0xff, ct = rc >> 8;
-        if (t >= ntiles || 16 * ct + 15 < jt) continue;
-        const int ra = 16 * rt + i16, col = 16 * ct + i16;
-        const bool aok = ra < n && s_act[ra];
-        if (__ballot(aok) == 0) continue;
-        const bool cok = col >= jt && col < ncol;
-#pragma unroll
-        for (int k4 = 0; k4 < KB / 4; ++k4) {
-          const int k = 4 * k4 + q4;
-          const double a = (aok && k < kb) ? -s_P[k][ra] : 0.0;
-          const double b = (cok && k < kb) ? s_U[k][col - jt] : 0.0;
-          acc[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[i], 0, 0, 0);
-        }
-      }
-      __syncthreads();
-    }
-    // ---- boundary rows: S and g straight from the tiles ----
-#pragma unroll
-    for (int i = 0; i < TPW; ++i) {
-      const int t = wv + NW * i;
-      const int rc = s_rc[wv + NW * i], rt = rc & 0xff, ct = rc >> 8;
-      if (t >= ntiles) continue;
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int row = 16 * rt + q4 + 4 * v, col = 16 * ct + i16;
-        if (row < ni || row >= n || col < ni || col >= ncol) continue;
-        const int rb = row - ni, cb = col - ni;
-        if (cb < nb) {
-          if (out.S) out.S[(cell * nb + cb) * nb + rb] = acc[i][v];
-        } else if (out.g) {
-          out.g[cell * nb + rb] = acc[i][v];
-        }
       }
     }
     if (tid == 0 && out.info) out.info[cell] = s_info;
@@ -2762,11 +2277,12 @@ void workspace_release(Workspace& w) {
 enum Family {
   kFamWave = 0,
   kFamGrid8 = 1,
-  kFamColwave = 2,
+  // 2: a column-set-per-wavefront elimination, and 6: a register-resident
+  // MFMA tile elimination, were autotuner candidates that never won; their
+  // numbers stay reserved
   kFamGrid32 = 3,
   kFamLarge = 4,
   kFamPanel = 5,
-  kFamTile = 6,
   kFamGrid16 = 7,
   kFamFlux = kFamilyFluxSchur,
   kFamFused = kFamilyFluxFused,
@@ -2785,11 +2301,9 @@ static bool family_ok(int fam, int ni, int nb) {
   switch (fam) {
     case kFamWave: return wave_class(ni, nb);
     case kFamGrid8: return pick_bucket(n, b) && b.pr == 8;
-    case kFamColwave: return n > 64 && n <= 192 && n + 1 <= 168;
     case kFamGrid32: return n <= 192 && n + 1 <= 176 && n > 71;
     case kFamLarge: return n > 64 && n <= kLargeMaxN;
     case kFamPanel: return n > 64 && n <= kPanelMaxN;
-    case kFamTile: return n > 64 && n <= kTileMaxN;
     case kFamGrid16: return n >= 48 && n + 1 <= 96;
   }
   return false;  // kFamFlux depends on the tables: launch_condense_fused
@@ -2847,25 +2361,6 @@ static hipError_t launch_condense(const Gen& gen, const CondenseOut& out, int64_
                          gen, out, ncells, ni, nb, ws.p, ld);
     return hipGetLastError();
   }
-  if (fam == kFamTile && BACKSUB) fam = kFamPanel;
-  if (fam == kFamTile) {
-    const int n = ni + nb;
-    const int tiles = ((n + 15) >> 4) * ((n + 16) >> 4);
-    const int grid = (int)std::min<int64_t>(ncells, 256 * (gridmul > 0 ? gridmul : 1));
-#define HDG_TILE(TPW_)                                                                        \
-  if (tiles <= (kTileThreads / 64) * TPW_) {                                                  \
-    ClockScope clk_(st, "condense_tile");                                                     \
-    hipLaunchKernelGGL((condense_tile_kernel<TPW_, Gen>), dim3(grid), dim3(kTileThreads), 0,  \
-                       st, gen, out, ncells, ni, nb);                                         \
-    return hipGetLastError();                                                                 \
-  }
-    HDG_TILE(4)
-    HDG_TILE(8)
-    HDG_TILE(12)
-    HDG_TILE(16)
-#undef HDG_TILE
-    return hipErrorInvalidValue;
-  }
   if (fam == kFamPanel) {
     const int n = ni + nb, grid = panel_grid(n, ncells, gridmul);
     int ld;
@@ -2900,15 +2395,6 @@ static hipError_t launch_condense(const Gen& gen, const CondenseOut& out, int64_
     HDG_WAVE(7, 8)
 #undef HDG_WAVE
     return hipErrorInvalidValue;
-  }
-  if (fam == kFamColwave) {
-    const int grid = grid_of(2);
-    {
-      ClockScope clk_(st, "condense_colwave");
-      hipLaunchKernelGGL((condense_colwave_kernel<3, 21, 8, Gen, BACKSUB>), dim3(grid), dim3(512),
-                       0, st, gen, out, ncells, ni, nb);
-    }
-    return hipGetLastError();
   }
   Bucket b;
   if (fam == kFamGrid32) {
@@ -2945,8 +2431,8 @@ int condense_variants(const SysShape& sh, int* fams, int max) {
 
 int condense_grid_options(int fam, int* gm, int max) {
   // resident workgroups per CU worth timing for each family (0 = fixed grid)
-  static const int wave[] = {16, 32, 48, 64}, grid8[] = {20, 40, 80}, col[] = {1, 2, 3},
-                   g32[] = {2, 3, 4}, pan[] = {1, 2}, til[] = {1},
+  static const int wave[] = {16, 32, 48, 64}, grid8[] = {20, 40, 80},
+                   g32[] = {2, 3, 4}, pan[] = {1, 2},
                    large[] = {4, 8, 16, 108, 116},
                    g16[] = {2, 4, 6}, fus[] = {1, 2}, fixed[] = {0};
   const int* v = fixed;
@@ -2954,10 +2440,8 @@ int condense_grid_options(int fam, int* gm, int max) {
   switch (fam) {
     case kFamWave: v = wave, n = 4; break;
     case kFamGrid8: v = grid8, n = 3; break;
-    case kFamColwave: v = col, n = 3; break;
     case kFamGrid32: v = g32, n = 3; break;
     case kFamPanel: v = pan, n = 2; break;
-    case kFamTile: v = til, n = 1; break;
     case kFamLarge: v = large, n = 5; break;
     case kFamGrid16: v = g16, n = 3; break;
     case kFamFused: v = fus, n = 2; break;
@@ -3081,6 +2565,248 @@ hipError_t launch_gather(const double* src, const int64_t* idx, double* dst, int
   ClockScope clk_(st, "renumber");
   hipLaunchKernelGGL(gather_kernel, dim3((unsigned)(blocks < 4096 ? blocks : 4096)), dim3(256), 0,
                      st, src, idx, dst, n);
+  return hipGetLastError();
+}
+
+// --------------------------------------------------------------------------
+// Device skeleton solve (MINRES with a facet block-Jacobi preconditioner)
+// on the assembled CSC.
+// The skeleton matrices of both weak forms are symmetric (the condensed
+// blocks S are), so column i of the CSC is read as row i: y = s K x is a
+// gather, one wavefront per row, with a fixed reduction order.
+// --------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) spmv_sym_kernel(const int64_t* __restrict__ colptr,
+                                                       const int32_t* __restrict__ rowval,
+                                                       const double* __restrict__ nzval,
+                                                       const double* __restrict__ x,
+                                                       double* __restrict__ y, int64_t n,
+                                                       double scale) {
+  const int lane = threadIdx.x & 63;
+  for (int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; i < n;
+       i += ((int64_t)gridDim.x * blockDim.x) >> 6) {
+    const int64_t k0 = colptr[i], k1 = colptr[i + 1];
+    double acc = 0.0;
+    for (int64_t k = k0 + lane; k < k1; k += 64) acc = fma(nzval[k], x[rowval[k]], acc);
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) acc += __shfl_xor(acc, off, 64);
+    if (lane == 0) y[i] = scale * acc;
+  }
+}
+
+constexpr int kBlkMax = 32;  // largest diagonal block of the block-Jacobi preconditioner
+
+// SPD preconditioner blocks: for each bs x bs diagonal block B of K (rows
+// [blk bs, blk bs + bs)) whose diagonal has one sign s, the inverse of s B
+// when s B is positive definite (Gauss-Jordan without pivoting on [s B | I]:
+// all pivots positive); otherwise diag(1 / |B_ii|).  One wavefront per block
+// in LDS, lane c owning column c.
+__global__ void __launch_bounds__(128) block_inv_kernel(const int64_t* __restrict__ colptr,
+                                                        const int32_t* __restrict__ rowval,
+                                                        const double* __restrict__ nzval,
+                                                        double* __restrict__ binv, int64_t nblk,
+                                                        int bs) {
+  __shared__ double s_a[2][kBlkMax][2 * kBlkMax + 1];
+  __shared__ double s_d[2][kBlkMax];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  double(*A)[2 * kBlkMax + 1] = s_a[w];
+  double* dg = s_d[w];
+  for (int64_t blk = (int64_t)blockIdx.x * 2 + w; blk < nblk; blk += (int64_t)gridDim.x * 2) {
+    const int64_t r0 = blk * bs;
+    if (lane < bs) {  // column r0 + lane of K restricted to the block's rows
+      for (int i = 0; i < bs; ++i) A[i][lane] = 0.0;
+      const int64_t c = r0 + lane;
+      int64_t lo = colptr[c], hi = colptr[c + 1];
+      while (lo < hi) {  // first row >= r0
+        const int64_t mid = (lo + hi) >> 1;
+        if (rowval[mid] < r0) lo = mid + 1;
+        else hi = mid;
+      }
+      for (int64_t k = lo; k < colptr[c + 1] && rowval[k] < r0 + bs; ++k)
+        A[rowval[k] - r0][lane] = nzval[k];
+    } else if (lane < 2 * bs) {
+      for (int i = 0; i < bs; ++i) A[i][lane] = i == lane - bs ? 1.0 : 0.0;
+    }
+    wave_sync_lds();
+    int pos = 0, neg = 0;
+    for (int i = 0; i < bs; ++i) {
+      pos += A[i][i] > 0.0;
+      neg += A[i][i] < 0.0;
+    }
+    if (lane < bs) dg[lane] = A[lane][lane];
+    const double sg = pos == bs ? 1.0 : (neg == bs ? -1.0 : 0.0);
+    wave_sync_lds();
+    if (lane < bs)
+      for (int i = 0; i < bs; ++i) A[i][lane] *= sg;
+    wave_sync_lds();
+    bool ok = sg != 0.0;
+    for (int k = 0; k < bs && ok; ++k) {
+      const double piv = A[k][k];
+      if (!(piv > 0.0)) {  // s B is not positive definite
+        ok = false;
+        break;
+      }
+      wave_sync_lds();
+      const double rinv = 1.0 / piv;
+      if (lane < 2 * bs) A[k][lane] *= rinv;
+      wave_sync_lds();
+      for (int i = 0; i < bs; ++i) {
+        if (i == k) continue;
+        const double f = A[i][k];
+        wave_sync_lds();
+        if (lane < 2 * bs) A[i][lane] = fma(-f, A[k][lane], A[i][lane]);
+        wave_sync_lds();
+      }
+    }
+    if (lane >= bs && lane < 2 * bs) {
+      const int j = lane - bs;
+      const double dj = fabs(dg[j]);
+      for (int i = 0; i < bs; ++i)
+        binv[(blk * bs + i) * bs + j] =
+            ok ? A[i][lane] : (i == j ? (dj > 0.0 ? 1.0 / dj : 1.0) : 0.0);
+    }
+    wave_sync_lds();
+  }
+}
+
+// z = blockdiag(binv) r, a thread per row
+__global__ void block_apply_kernel(const double* __restrict__ binv,
+                                   const double* __restrict__ r, double* __restrict__ z,
+                                   int64_t n, int bs) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r0 = i - i % bs;
+    const double* b = binv + i * bs;
+    double acc = 0.0;
+    for (int j = 0; j < bs; ++j) acc = fma(b[j], r[r0 + j], acc);
+    z[i] = acc;
+  }
+}
+
+// MINRES vector updates:
+// op 0: y = y - a u0                        (u0 = r1 or r2)
+// op 1: w = (v - a w1 - b w2) / c, x += d w (u0 = w, u1 = x, w0 = v, w1 = w1, w2 = w2;
+//       c, d in the extra arguments)
+// op 2: u0 = a w0
+// op 3: u0 = w0 - w1                        (the residual b - K x)
+__global__ void krylov_update_kernel(int op, int64_t n, double a, double b, double c, double d,
+                                     double* u0, double* u1, const double* w0,
+                                     const double* w1, const double* w2) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    if (op == 0) {
+      u1[i] = fma(-a, u0[i], u1[i]);
+    } else if (op == 1) {
+      const double wv = (w0[i] - a * w1[i] - b * w2[i]) * c;
+      u0[i] = wv;
+      u1[i] = fma(d, wv, u1[i]);
+    } else if (op == 2) {
+      u0[i] = a * w0[i];
+    } else {
+      u0[i] = w0[i] - w1[i];
+    }
+  }
+}
+
+constexpr int kDotBlocks = 1024;
+
+// up to three dot products at once: per-block partial sums in a fixed order,
+// then one block sums the partials (deterministic for a given n)
+__global__ void __launch_bounds__(256) multidot_kernel(int nd, int64_t n, const double* a0,
+                                                       const double* b0, const double* a1,
+                                                       const double* b1, const double* a2,
+                                                       const double* b2, double* partial) {
+  __shared__ double s_red[3][256];
+  double acc[3] = {0.0, 0.0, 0.0};
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    acc[0] = fma(a0[i], b0[i], acc[0]);
+    if (nd > 1) acc[1] = fma(a1[i], b1[i], acc[1]);
+    if (nd > 2) acc[2] = fma(a2[i], b2[i], acc[2]);
+  }
+#pragma unroll
+  for (int d = 0; d < 3; ++d) s_red[d][threadIdx.x] = acc[d];
+  __syncthreads();
+  for (int h = 128; h >= 1; h >>= 1) {
+    if ((int)threadIdx.x < h)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) s_red[d][threadIdx.x] += s_red[d][threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x < 3) partial[threadIdx.x * kDotBlocks + blockIdx.x] = s_red[threadIdx.x][0];
+}
+
+__global__ void __launch_bounds__(256) dot_finish_kernel(const double* partial, double* out) {
+  __shared__ double s_red[3][256];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    double acc = 0.0;
+    for (int b = threadIdx.x; b < kDotBlocks; b += 256) acc += partial[d * kDotBlocks + b];
+    s_red[d][threadIdx.x] = acc;
+  }
+  __syncthreads();
+  for (int h = 128; h >= 1; h >>= 1) {
+    if ((int)threadIdx.x < h)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) s_red[d][threadIdx.x] += s_red[d][threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x < 3) out[threadIdx.x] = s_red[threadIdx.x][0];
+}
+
+static int vec_grid(int64_t n) {
+  const int64_t b = (n + 255) / 256;
+  return (int)(b < 8192 ? (b > 0 ? b : 1) : 8192);
+}
+
+hipError_t launch_spmv_sym(const int64_t* colptr, const int32_t* rowval, const double* nzval,
+                           const double* x, double* y, int64_t n, double scale, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  const int64_t blocks = (n + 3) / 4;  // four rows (wavefronts) per workgroup
+  ClockScope clk_(st, "skeleton_spmv");
+  hipLaunchKernelGGL(spmv_sym_kernel, dim3((unsigned)(blocks < 65536 ? blocks : 65536)), dim3(256),
+                     0, st, colptr, rowval, nzval, x, y, n, scale);
+  return hipGetLastError();
+}
+
+hipError_t launch_block_inv(const int64_t* colptr, const int32_t* rowval, const double* nzval,
+                            double* binv, int64_t nblk, int bs, hipStream_t st) {
+  if (nblk <= 0) return hipSuccess;
+  if (bs < 1 || bs > kBlkMax) return hipErrorInvalidValue;
+  const int64_t blocks = (nblk + 1) / 2;  // two blocks (wavefronts) per workgroup
+  hipLaunchKernelGGL(block_inv_kernel, dim3((unsigned)(blocks < 65536 ? blocks : 65536)),
+                     dim3(128), 0, st, colptr, rowval, nzval, binv, nblk, bs);
+  return hipGetLastError();
+}
+
+hipError_t launch_block_apply(const double* binv, const double* r, double* z, int64_t n, int bs,
+                              hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(block_apply_kernel, dim3(vec_grid(n)), dim3(256), 0, st, binv, r, z, n, bs);
+  return hipGetLastError();
+}
+
+hipError_t launch_krylov_update(int op, int64_t n, double a, double b, double c, double d,
+                                double* u0, double* u1, const double* w0, const double* w1,
+                                const double* w2, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(krylov_update_kernel, dim3(vec_grid(n)), dim3(256), 0, st, op, n, a, b, c,
+                     d, u0, u1, w0, w1, w2);
+  return hipGetLastError();
+}
+
+int block_jacobi_max() { return kBlkMax; }
+
+size_t multidot_scratch_doubles() { return 3 * (size_t)kDotBlocks + 3; }
+
+hipError_t launch_multidot(int nd, int64_t n, const double* a0, const double* b0,
+                           const double* a1, const double* b1, const double* a2,
+                           const double* b2, double* scratch, hipStream_t st) {
+  hipLaunchKernelGGL(multidot_kernel, dim3(kDotBlocks), dim3(256), 0, st, nd, n, a0, b0,
+                     a1 ? a1 : a0, b1 ? b1 : b0, a2 ? a2 : a0, b2 ? b2 : b0, scratch);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(dot_finish_kernel, dim3(1), dim3(256), 0, st, scratch,
+                     scratch + 3 * kDotBlocks);
   return hipGetLastError();
 }
 

@@ -486,6 +486,21 @@ class HybridAffineFEOperator:
         self.ctx.synchronize()
         return out
 
+    def skeleton_solve(self, rtol=1e-12, maxit=0):
+        """Device-side skeleton solve (hdg_skeleton_solve: MINRES with a facet
+        block-Jacobi preconditioner, on the resident skeleton system) in place of the host
+        ``A \\ b`` of src/HybridAffineFEOperators.jl:74.  Returns the free
+        trace values (device tensor), the iterations and the final relative
+        residual; raises when rtol was not met."""
+        torch = _t()
+        with self.ctx.stream():
+            lam = torch.empty((self.info.nfree,), dtype=torch.float64, device="cuda")
+        it, res = C.c_int(0), C.c_double(0.0)
+        check(lib().hdg_skeleton_solve(self.mesh, float(rtol), int(maxit),
+                                       lam.data_ptr() if lam.numel() else None, C.byref(it),
+                                       C.byref(res)), self.ctx.h, "skeleton_solve")
+        return lam, it.value, res.value
+
     def refinement_stats(self):
         """Refinement sweeps of the last condensation (mixed precision)."""
         st = _lib.RefinementStats()
@@ -599,9 +614,17 @@ class HybridSolution:
         return self.free_values[a: a + self.ncells * self.n_pot].reshape(self.ncells, self.n_pot)
 
 
-def solve(op: HybridAffineFEOperator):
-    """solve(op) (src/HybridAffineFEOperators.jl:67-100): host sparse direct
-    solve of the skeleton system, then GPU recovery."""
+def solve(op: HybridAffineFEOperator, skeleton_solver="host", rtol=1e-12):
+    """solve(op) (src/HybridAffineFEOperators.jl:67-100): the skeleton solve,
+    then GPU recovery.  skeleton_solver="host": the reference's sparse direct
+    solve on the host; "device": hdg_skeleton_solve (iterative, to rtol)."""
+    if skeleton_solver == "device":
+        lam_d, _, _ = op.skeleton_solve(rtol=rtol)
+        out = op.recover(lam_d).cpu().numpy()
+        return HybridSolution(out, op.info.ncells, op.info.n_flux, op.info.n_pot,
+                              lam_d.cpu().numpy())
+    if skeleton_solver != "host":
+        raise HDGError(_lib.HDG_ERR_ARG, f"unknown skeleton solver {skeleton_solver!r}")
     import scipy.sparse.linalg as spla
     K, rhs = op.skeleton_system()
     lam = spla.spsolve(K.tocsc(), rhs) if op.info.nfree else np.zeros(0)

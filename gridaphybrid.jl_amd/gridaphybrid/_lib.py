@@ -40,7 +40,8 @@ EXPORTS = [
     "hdg_backward_static_condensation", "hdg_mesh_create", "hdg_mesh_create_cartesian",
     "hdg_mesh_destroy", "hdg_mesh_get_info", "hdg_mesh_set_dirichlet",
     "hdg_mesh_set_dirichlet_mms", "hdg_mesh_set_source", "hdg_condense_assemble",
-    "hdg_skeleton_system", "hdg_cell_blocks", "hdg_recover", "hdg_cell_systems",
+    "hdg_skeleton_system", "hdg_skeleton_solve", "hdg_cell_blocks", "hdg_recover",
+    "hdg_cell_systems",
     "hdg_condense", "hdg_assemble", "hdg_mesh_singular_count", "hdg_mesh_interpolate_trace",
     "hdg_local_factors", "hdg_batch_create", "hdg_batch_destroy", "hdg_batch_get_classes",
     "hdg_batch_launch", "hdg_batch_condense_assemble", "hdg_batch_recover",
@@ -50,8 +51,8 @@ EXPORTS = [
 ]
 
 BATCH_AUTOTUNE = 1
-FAMILY_NAMES = {0: "wave", 1: "grid8x8", 2: "colwave", 3: "grid32x16", 4: "blocked",
-                5: "panel_mfma", 6: "tile_mfma", 7: "grid16x16", 8: "flux_schur", 9: "flux_fused",
+FAMILY_NAMES = {0: "wave", 1: "grid8x8", 3: "grid32x16", 4: "blocked",
+                5: "panel_mfma", 7: "grid16x16", 8: "flux_schur", 9: "flux_fused",
                 10: "flux_ref"}
 FAMILY_FLUX_SCHUR = 8
 FAMILY_FLUX_FUSED = 9
@@ -172,6 +173,8 @@ def lib():
         L.hdg_mesh_cell_quadrature.argtypes = [vp, vp, vp]
         L.hdg_mesh_set_trace_fields.argtypes = [vp, i32]
         L.hdg_condense_assemble.argtypes = [vp, i32, C.POINTER(i64)]
+        L.hdg_skeleton_solve.argtypes = [vp, C.c_double, i32, vp, C.POINTER(C.c_int),
+                                         C.POINTER(C.c_double)]
         L.hdg_skeleton_system.argtypes = [vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp),
                                           C.POINTER(vp)]
         L.hdg_cell_blocks.argtypes = [vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp)]

@@ -316,6 +316,21 @@ int hdg_mesh_interpolate_trace(hdg_mesh* mesh, double* lam_free);
  * [nnz], nzval double [nnz], rhs double [nfree]. */
 int hdg_skeleton_system(hdg_mesh* mesh, const int64_t** colptr, const int32_t** rowval,
                         const double** nzval, const double** rhs);
+/* Device-side skeleton solve: replaces the host `x_skel = A \ b` of
+ * src/HybridAffineFEOperators.jl:74 (and src/HybridLinearSolvers.jl:45) by
+ * preconditioned MINRES on the resident system of hdg_skeleton_system,
+ * enqueued on the context's stream.  The skeleton matrices of both weak forms
+ * are symmetric (the product reads column i of the CSC as row i) but not all
+ * definite; the preconditioner is the inverse of each facet's diagonal block,
+ * negated where that block's diagonal is negative.  lam_free
+ * (device, nfree) receives the solution from a zero start.  Stops when
+ * |b - K x| <= rtol |b| or after maxit iterations (maxit <= 0: the smaller of
+ * 10 nfree and 50,000).
+ * *iters and *relres (either may be NULL) report the iterations and the final
+ * relative residual, recomputed from K x.  HDG_ERR_STATE when the tolerance
+ * was not met (lam_free holds the last iterate). */
+int hdg_skeleton_solve(hdg_mesh* mesh, double rtol, int maxit, double* lam_free, int* iters,
+                       double* relres);
 /* Device pointers of the per-cell condensed blocks S [ncells][nb][nb], g
  * [ncells][nb] and trace dof ids [ncells][nb]. */
 int hdg_cell_blocks(hdg_mesh* mesh, const double** S, const double** g,
@@ -394,10 +409,9 @@ typedef struct {
   double tuned_ms;     /* measured condensation time of the pick (0 if untuned) */
 } hdg_batch_class;
 /* Kernel families: 0 one wavefront per cell (fixed sizes), 1 8x8 grid with
- * register tiles, 2 column-per-wavefront with look-ahead, 3 32x16 grid,
- * 4 blocked HBM-workspace elimination, 5 blocked elimination with a
- * single-wavefront panel and FP64 MFMA trailing updates, 6 register-tile
- * blocked elimination with MFMA updates, 7 16x16 grid, 8 flux-block Schur
+ * register tiles, 3 32x16 grid, 4 blocked HBM-workspace elimination,
+ * 5 blocked elimination with a single-wavefront panel and FP64 MFMA trailing
+ * updates, 7 16x16 grid (2 and 6 are reserved), 8 flux-block Schur
  * complement: where the flux block of A_ii is C (x) M (per-cell coefficient
  * matrix C, fixed scalar mass matrix M; both weak forms here), the flux
  * unknowns are eliminated with C^-1 (x) M^-1 in two FP64 MFMA products and
