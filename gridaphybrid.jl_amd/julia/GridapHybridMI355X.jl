@@ -442,9 +442,39 @@ src/HybridAffineFEOperators.jl:67-100: the skeleton system is solved on the
 host (`K \\ rhs`), the interior fields are recovered on the GPU, and the free
 values [flux; potential; trace] become an FEFunction of the trial space.
 """
-function Gridap.solve(op::MI355XHybridAffineFEOperator)
+function Gridap.solve(op::MI355XHybridAffineFEOperator; skeleton_solver::Symbol=:host,
+                      rtol::Float64=1e-12)
+  skeleton_solver === :device &&
+    return FEFunction(op.trial, _recover(op, skeleton_solve_device(op; rtol=rtol)))
   K, rhs = skeleton_system(op)
   FEFunction(op.trial, _recover(op, K \ rhs))
+end
+
+"""
+    skeleton_solve_device(op; rtol=1e-12, maxit=0) -> lambda (host vector)
+
+The skeleton solve on the GPU (hdg_skeleton_solve: MINRES with a facet
+block-Jacobi preconditioner on the resident system) instead of the host
+`A \\ b` of src/HybridAffineFEOperators.jl:74; errors when rtol is not met.
+"""
+function skeleton_solve_device(op::MI355XHybridAffineFEOperator; rtol::Float64=1e-12,
+                               maxit::Integer=0)
+  n = op.info.nfree
+  n == 0 && return Float64[]
+  dlam = Ref{Ptr{Cvoid}}()
+  @assert ccall((:hipMalloc, "libamdhip64"), Cint, (Ref{Ptr{Cvoid}}, Csize_t), dlam, 8n) == 0
+  try
+    iters, relres = Ref{Cint}(0), Ref{Float64}(0.0)
+    check(op.ctx, ccall((:hdg_skeleton_solve, libhdg), Cint,
+                        (Ptr{Cvoid}, Float64, Cint, Ptr{Cvoid}, Ref{Cint}, Ref{Float64}),
+                        op.mesh, rtol, maxit, dlam[], iters, relres), "hdg_skeleton_solve")
+    lam = Vector{Float64}(undef, n)
+    check(op.ctx, ccall((:hdg_memcpy_d2h, libhdg), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Int64),
+                        op.ctx.h, lam, dlam[], 8n), "d2h")
+    return lam
+  finally
+    ccall((:hipFree, "libamdhip64"), Cint, (Ptr{Cvoid},), dlam[])
+  end
 end
 
 skeleton_system(op::MI355XHybridAffineFEOperator) =
