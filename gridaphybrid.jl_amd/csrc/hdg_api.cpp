@@ -1433,14 +1433,18 @@ int hdg_skeleton_solve(hdg_mesh* m, double rtol, int maxit, double* lam_free, in
   const int bsf = m->th.ntf / std::max(1, m->trace_fields);
   const int bs = (bsf >= 1 && bsf <= block_jacobi_max() && n % bsf == 0) ? bsf : 1;
   const int64_t nblk = n / bs;
-  // r1, r2, y, v, w (three), t, the block inverses and the dot scratch
+  // r1, r2, y, v, w (three), t, the block inverses, the dot scratch and the
+  // MINRES scalars
   DevBuf ws;
   const size_t nd = multidot_scratch_doubles();
-  HIP_TRY(ctx, ws.alloc((8 * (size_t)n + (size_t)n * bs + nd) * sizeof(double)),
+  using MS = MinresScalar;
+  HIP_TRY(ctx,
+          ws.alloc((8 * (size_t)n + (size_t)n * bs + nd + MS::kCount) * sizeof(double)),
           "alloc solve workspace");
   double* r1 = ws.as<double>();
   double *r2 = r1 + n, *y = r2 + n, *v = y + n, *wa = v + n, *wb = wa + n, *wc = wb + n,
-         *t = wc + n, *binv = t + n, *scr = binv + (size_t)n * bs;
+         *t = wc + n, *binv = t + n, *scr = binv + (size_t)n * bs, *sc = scr + nd;
+  const double* dres = scr + nd - 3;  // the sums of the last dot products
   double* x = lam_free;
   HIP_TRY(ctx, launch_block_inv(colptr, rowval, nzval, binv, nblk, bs, st), "block inverses");
   double h[3];
@@ -1448,7 +1452,7 @@ int hdg_skeleton_solve(hdg_mesh* m, double rtol, int maxit, double* lam_free, in
                   const double* b1) -> hipError_t {
     hipError_t e = launch_multidot(k, n, a0, b0, a1, b1, nullptr, nullptr, scr, st);
     if (e != hipSuccess) return e;
-    e = hipMemcpyAsync(h, scr + nd - 3, 3 * sizeof(double), hipMemcpyDeviceToHost, st);
+    e = hipMemcpyAsync(h, dres, 3 * sizeof(double), hipMemcpyDeviceToHost, st);
     if (e != hipSuccess) return e;
     return hipStreamSynchronize(st);
   };
@@ -1463,10 +1467,14 @@ int hdg_skeleton_solve(hdg_mesh* m, double rtol, int maxit, double* lam_free, in
     return e;
   };
   // preconditioned MINRES (Paige and Saunders): symmetric K of any inertia,
-  // with the SPD block preconditioner above
+  // with the SPD block preconditioner above.  The recurrences run on the
+  // device (minres_scalar_kernel), so an iteration is a chain of launches
+  // with no host round trip; the host reads the state every kCheck iterations.
+  constexpr int kCheck = 50;
   HIP_TRY(ctx, hipMemsetAsync(x, 0, n * sizeof(double), st), "zero x");
   HIP_TRY(ctx, hipMemsetAsync(wa, 0, n * sizeof(double), st), "zero w");
   HIP_TRY(ctx, hipMemsetAsync(wb, 0, n * sizeof(double), st), "zero w");
+  HIP_TRY(ctx, hipMemsetAsync(sc, 0, MS::kCount * sizeof(double), st), "zero scalars");
   HIP_TRY(ctx, hipMemcpyAsync(r1, rhs, n * sizeof(double), hipMemcpyDeviceToDevice, st), "r1");
   HIP_TRY(ctx, hipMemcpyAsync(r2, rhs, n * sizeof(double), hipMemcpyDeviceToDevice, st), "r2");
   HIP_TRY(ctx, launch_block_apply(binv, r1, y, n, bs, st), "precond");
@@ -1475,50 +1483,45 @@ int hdg_skeleton_solve(hdg_mesh* m, double rtol, int maxit, double* lam_free, in
   if (bnorm == 0.0) return HDG_OK;  // x = 0
   if (!(h[0] > 0.0)) return fail(ctx, HDG_ERR_STATE, "skeleton solve: indefinite preconditioner");
   const double beta1 = std::sqrt(h[0]);
-  double oldb = 0.0, beta = beta1, dbar = 0.0, epsln = 0.0, phibar = beta1, cs = -1.0, sn = 0.0;
+  HIP_TRY(ctx, launch_minres_scalar(0, dres, sc, st), "minres init");
   double *w = wa, *w2 = wb, *w1 = wc;
+  double hs[MS::kCount];
   int it = 0;
   bool conv = false;
   while (it < maxit && !conv) {
     ++it;
-    HIP_TRY(ctx, launch_krylov_update(2, n, 1.0 / beta, 0, 0, 0, v, nullptr, y, nullptr, nullptr,
-                                      st), "v");
+    HIP_TRY(ctx, launch_krylov_update_dev(2, n, sc, MS::kInvBeta, -1, -1, -1, v, nullptr, y,
+                                          nullptr, nullptr, st), "v");
     HIP_TRY(ctx, launch_spmv_sym(colptr, rowval, nzval, v, y, n, 1.0, st), "spmv");
     if (it >= 2)
-      HIP_TRY(ctx, launch_krylov_update(0, n, beta / oldb, 0, 0, 0, r1, y, nullptr, nullptr,
-                                        nullptr, st), "y");
-    HIP_TRY(ctx, dots(1, v, y, nullptr, nullptr), "dot");
-    const double alfa = h[0];
-    HIP_TRY(ctx, launch_krylov_update(0, n, alfa / beta, 0, 0, 0, r2, y, nullptr, nullptr,
-                                      nullptr, st), "y");
+      HIP_TRY(ctx, launch_krylov_update_dev(0, n, sc, MS::kBetaOldb, -1, -1, -1, r1, y, nullptr,
+                                            nullptr, nullptr, st), "y");
+    HIP_TRY(ctx, launch_multidot(1, n, v, y, nullptr, nullptr, nullptr, nullptr, scr, st), "dot");
+    HIP_TRY(ctx, launch_minres_scalar(1, dres, sc, st), "alfa");
+    HIP_TRY(ctx, launch_krylov_update_dev(0, n, sc, MS::kAlfaBeta, -1, -1, -1, r2, y, nullptr,
+                                          nullptr, nullptr, st), "y");
     std::swap(r1, r2);  // r1 <- r2, r2 <- y, y <- the old r1
     std::swap(r2, y);
     HIP_TRY(ctx, launch_block_apply(binv, r2, y, n, bs, st), "precond");
-    oldb = beta;
-    HIP_TRY(ctx, dots(1, r2, y, nullptr, nullptr), "dot");
-    if (h[0] < 0.0) return fail(ctx, HDG_ERR_STATE, "skeleton solve: indefinite preconditioner");
-    beta = std::sqrt(h[0]);
-    const double oldeps = epsln;
-    const double delta = cs * dbar + sn * alfa;
-    const double gbar = sn * dbar - cs * alfa;
-    epsln = sn * beta;
-    dbar = -cs * beta;
-    const double gamma = std::max(std::hypot(gbar, beta), 1e-300);
-    cs = gbar / gamma;
-    sn = beta / gamma;
-    const double phi = cs * phibar;
-    phibar = sn * phibar;
+    HIP_TRY(ctx, launch_multidot(1, n, r2, y, nullptr, nullptr, nullptr, nullptr, scr, st), "dot");
+    HIP_TRY(ctx, launch_minres_scalar(2, dres, sc, st), "rotation");
     // w1 <- w2, w2 <- w, w <- (v - oldeps w1 - delta w2) / gamma; x += phi w
     std::swap(w1, w2);  // w1 = old w2, w2 = old w1 (dead)
     std::swap(w2, w);   // w2 = old w, w = dead buffer
-    HIP_TRY(ctx, launch_krylov_update(1, n, oldeps, delta, 1.0 / gamma, phi, w, x, v, w1, w2, st),
-            "w, x");
-    // phibar estimates the preconditioned residual norm; confirm with the
-    // true residual when it meets rtol, and every 100 iterations
-    if (phibar <= rtol * beta1 || it % 100 == 0 || beta == 0.0) {
-      HIP_TRY(ctx, true_residual(bnorm), "residual");
-      conv = res <= rtol;
-      if (beta == 0.0) break;  // the Krylov space is exhausted
+    HIP_TRY(ctx, launch_krylov_update_dev(1, n, sc, MS::kOldEps, MS::kDelta, MS::kInvGamma,
+                                          MS::kPhi, w, x, v, w1, w2, st), "w, x");
+    if (it % kCheck == 0 || it == maxit) {
+      HIP_TRY(ctx, hipMemcpyAsync(hs, sc, sizeof hs, hipMemcpyDeviceToHost, st), "d2h");
+      HIP_TRY(ctx, hipStreamSynchronize(st), "sync");
+      if (hs[MS::kIndefinite] != 0.0)
+        return fail(ctx, HDG_ERR_STATE, "skeleton solve: indefinite preconditioner");
+      // phibar estimates the preconditioned residual norm; the true residual
+      // decides
+      if (hs[MS::kPhibar] <= rtol * beta1 || hs[MS::kFrozen] != 0.0) {
+        HIP_TRY(ctx, true_residual(bnorm), "residual");
+        conv = res <= rtol;
+        if (hs[MS::kFrozen] != 0.0) break;  // the Krylov space is exhausted
+      }
     }
   }
   if (!conv) HIP_TRY(ctx, true_residual(bnorm), "residual");

@@ -332,6 +332,18 @@ hipError_t launch_krylov_update(int op, int64_t n, double a, double b, double c,
                                 double* u0, double* u1, const double* w0, const double* w1,
                                 const double* w2, hipStream_t st);
 int block_jacobi_max();
+// MINRES scalar state kept on the device (indices into a double array)
+struct MinresScalar {
+  enum {
+    kBeta1, kOldb, kBeta, kDbar, kEpsln, kPhibar, kCs, kSn, kAlfa, kInvBeta, kBetaOldb,
+    kAlfaBeta, kOldEps, kDelta, kInvGamma, kPhi, kFrozen, kIndefinite, kCount
+  };
+};
+hipError_t launch_krylov_update_dev(int op, int64_t n, const double* sc, int ia, int ib, int ic,
+                                    int id, double* u0, double* u1, const double* w0,
+                                    const double* w1, const double* w2, hipStream_t st);
+// step 0: init from dot[0] = (r1, M^-1 r1); 1: alfa = dot[0]; 2: beta^2 = dot[0]
+hipError_t launch_minres_scalar(int step, const double* dot, double* sc, hipStream_t st);
 size_t multidot_scratch_doubles();
 // nd (1..3) dot products (a_k, b_k); the sums land in scratch[3 * 1024 + k]
 hipError_t launch_multidot(int nd, int64_t n, const double* a0, const double* b0,

@@ -2682,15 +2682,22 @@ __global__ void block_apply_kernel(const double* __restrict__ binv,
   }
 }
 
-// MINRES vector updates:
+// MINRES vector updates; with sc non-null the coefficients are read from the
+// device scalar state (MinresScalar indices in ia, ib, ic, id; < 0: 0):
 // op 0: y = y - a u0                        (u0 = r1 or r2)
-// op 1: w = (v - a w1 - b w2) / c, x += d w (u0 = w, u1 = x, w0 = v, w1 = w1, w2 = w2;
-//       c, d in the extra arguments)
+// op 1: w = (v - a w1 - b w2) c, x += d w   (u0 = w, u1 = x, w0 = v, w1 = w1, w2 = w2)
 // op 2: u0 = a w0
 // op 3: u0 = w0 - w1                        (the residual b - K x)
 __global__ void krylov_update_kernel(int op, int64_t n, double a, double b, double c, double d,
                                      double* u0, double* u1, const double* w0,
-                                     const double* w1, const double* w2) {
+                                     const double* w1, const double* w2, const double* sc,
+                                     int ia, int ib, int ic, int id) {
+  if (sc) {
+    a = ia >= 0 ? sc[ia] : 0.0;
+    b = ib >= 0 ? sc[ib] : 0.0;
+    c = ic >= 0 ? sc[ic] : 0.0;
+    d = id >= 0 ? sc[id] : 0.0;
+  }
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
     if (op == 0) {
@@ -2708,6 +2715,69 @@ __global__ void krylov_update_kernel(int op, int64_t n, double a, double b, doub
 }
 
 constexpr int kDotBlocks = 1024;
+
+// MINRES recurrences on the device (one thread): step 0 initialises from
+// (r1, y) = beta1^2; step 1 takes alfa = (v, y); step 2 takes beta^2 = (r2, y)
+// and advances the Givens rotation.  A zero beta (the Krylov space is
+// exhausted) or a negative beta^2 (an indefinite preconditioner) freezes the
+// state: every later update is then a no-op, and the host reads the flags.
+__global__ void minres_scalar_kernel(int step, const double* __restrict__ dot,
+                                     double* __restrict__ sc) {
+  using M = MinresScalar;
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const bool frozen = sc[M::kFrozen] != 0.0;
+  if (step == 0) {
+    const double b1 = dot[0] > 0.0 ? sqrt(dot[0]) : 0.0;
+    sc[M::kBeta1] = b1;
+    sc[M::kBeta] = b1;
+    sc[M::kPhibar] = b1;
+    sc[M::kOldb] = 0.0;
+    sc[M::kDbar] = 0.0;
+    sc[M::kEpsln] = 0.0;
+    sc[M::kCs] = -1.0;
+    sc[M::kSn] = 0.0;
+    sc[M::kInvBeta] = b1 > 0.0 ? 1.0 / b1 : 0.0;
+    sc[M::kFrozen] = b1 > 0.0 ? 0.0 : 1.0;
+    sc[M::kIndefinite] = dot[0] < 0.0 ? 1.0 : 0.0;
+  } else if (step == 1) {
+    const double alfa = frozen ? 0.0 : dot[0];
+    sc[M::kAlfa] = alfa;
+    sc[M::kAlfaBeta] = frozen ? 0.0 : alfa / sc[M::kBeta];
+  } else {
+    if (frozen) {
+      sc[M::kPhi] = 0.0;
+      sc[M::kInvGamma] = 0.0;
+      sc[M::kInvBeta] = 0.0;
+      sc[M::kBetaOldb] = 0.0;
+      sc[M::kAlfaBeta] = 0.0;
+      return;
+    }
+    if (dot[0] < 0.0) sc[M::kIndefinite] = 1.0;
+    const double alfa = sc[M::kAlfa], cs = sc[M::kCs], sn = sc[M::kSn], dbar = sc[M::kDbar];
+    const double oldb = sc[M::kBeta];
+    const double beta = dot[0] > 0.0 ? sqrt(dot[0]) : 0.0;
+    const double oldeps = sc[M::kEpsln];
+    const double delta = cs * dbar + sn * alfa;
+    const double gbar = sn * dbar - cs * alfa;
+    const double gamma = fmax(hypot(gbar, beta), 1e-300);
+    const double cs2 = gbar / gamma, sn2 = beta / gamma;
+    const double phibar = sc[M::kPhibar];
+    sc[M::kOldb] = oldb;
+    sc[M::kBeta] = beta;
+    sc[M::kOldEps] = oldeps;
+    sc[M::kDelta] = delta;
+    sc[M::kEpsln] = sn * beta;
+    sc[M::kDbar] = -cs * beta;
+    sc[M::kCs] = cs2;
+    sc[M::kSn] = sn2;
+    sc[M::kPhi] = cs2 * phibar;
+    sc[M::kPhibar] = sn2 * phibar;
+    sc[M::kInvGamma] = 1.0 / gamma;
+    sc[M::kInvBeta] = beta > 0.0 ? 1.0 / beta : 0.0;
+    sc[M::kBetaOldb] = beta / oldb;
+    if (!(beta > 0.0)) sc[M::kFrozen] = 1.0;
+  }
+}
 
 // up to three dot products at once: per-block partial sums in a fixed order,
 // then one block sums the partials (deterministic for a given n)
@@ -2790,7 +2860,21 @@ hipError_t launch_krylov_update(int op, int64_t n, double a, double b, double c,
                                 const double* w2, hipStream_t st) {
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(krylov_update_kernel, dim3(vec_grid(n)), dim3(256), 0, st, op, n, a, b, c,
-                     d, u0, u1, w0, w1, w2);
+                     d, u0, u1, w0, w1, w2, (const double*)nullptr, -1, -1, -1, -1);
+  return hipGetLastError();
+}
+
+hipError_t launch_krylov_update_dev(int op, int64_t n, const double* sc, int ia, int ib, int ic,
+                                    int id, double* u0, double* u1, const double* w0,
+                                    const double* w1, const double* w2, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(krylov_update_kernel, dim3(vec_grid(n)), dim3(256), 0, st, op, n, 0.0, 0.0,
+                     0.0, 0.0, u0, u1, w0, w1, w2, sc, ia, ib, ic, id);
+  return hipGetLastError();
+}
+
+hipError_t launch_minres_scalar(int step, const double* dot, double* sc, hipStream_t st) {
+  hipLaunchKernelGGL(minres_scalar_kernel, dim3(1), dim3(64), 0, st, step, dot, sc);
   return hipGetLastError();
 }
 
