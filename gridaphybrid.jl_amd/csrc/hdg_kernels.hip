@@ -4460,6 +4460,80 @@ __global__ void __launch_bounds__(64 * kInvWaves)
   }
 }
 
+// flux_invert_kernel for 32 < nu <= 64 with the two inverses computed side by
+// side: a workgroup of two wavefronts per cell shares the staged factors,
+// wavefront 0 forming R_uu^-1 and wavefront 1 U^-1.  The factors are staged
+// packed (U by columns as the slot keeps it, the multipliers by steps over
+// the positions after the step), 34 KB per cell, so four cells fit a CU.
+// Each lane does exactly what the single-wavefront kernel does for its column
+// and pass, with the same operands, so the results are the same bit for bit.
+constexpr int kInv2Lds = 64 * 65 / 2 + 64 * 63 / 2 + 2 * 64;  // doubles per cell
+
+__global__ void __launch_bounds__(128)
+    flux_invert2_kernel(double* __restrict__ slot, int64_t usz, int64_t ncells, int nu,
+                        int64_t ri_off) {
+  constexpr int NUM = 64;
+  extern __shared__ double s_inv[];
+  const int lane = threadIdx.x & 63, pass = threadIdx.x >> 6;
+  const int nup = nu * (nu + 1) / 2, nn = nu * nu;
+  double* sU = s_inv;                   // U packed by columns: (k, c), k <= c, at c(c+1)/2 + k
+  double* sL = sU + NUM * (NUM + 1) / 2;  // multipliers: (i, k), k > i, at i(2 NUM - i - 1)/2 + k - i - 1
+  double* sP = sL + NUM * (NUM - 1) / 2;  // row at each position (-1 past nu)
+  double* sD = sP + NUM;                  // 1 / U[c][c]
+  for (int64_t cell = blockIdx.x; cell < ncells; cell += gridDim.x) {
+    double* Uc = slot + cell * usz;
+    const double* Lc = Uc + nup;
+    for (int t = threadIdx.x; t < NUM; t += 128) sP[t] = t < nu ? Lc[nn + t] : -1.0;
+    for (int t = threadIdx.x; t < nup; t += 128) sU[t] = Uc[t];
+    for (int t = threadIdx.x; t < NUM; t += 128) {
+      const double d = t < nu ? Uc[t * (t + 1) / 2 + t] : 0.0;
+      sD[t] = d != 0.0 ? 1.0 / d : 0.0;
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < NUM * (NUM - 1) / 2; t += 128) {
+      // t -> (i, k): row i of the packed strictly-upper layout
+      int i = 0, rem = t;
+      while (rem >= NUM - 1 - i) rem -= NUM - 1 - i, ++i;
+      const int k = i + 1 + rem;
+      sL[t] = (i < nu && k < nu) ? Lc[i * nu + (int)sP[k]] : 0.0;
+    }
+    __syncthreads();
+    double* out = Uc + ri_off;  // R_uu^-1 then U^-1, [col][row]
+    const int j = lane;
+    double y[NUM];
+#pragma unroll
+    for (int k = 0; k < NUM; ++k)
+      y[k] = pass == 0 ? ((int)sP[k] == j ? 1.0 : 0.0) : (k == j ? 1.0 : 0.0);
+    if (pass == 0) {  // y <- L^-1 y (positions)
+#pragma unroll
+      for (int i = 0; i < NUM - 1; ++i) {
+        if (i + 1 < nu) {
+          const double* li = sL + i * (2 * NUM - i - 1) / 2 - (i + 1);
+#pragma unroll
+          for (int k = i + 1; k < NUM; ++k) y[k] = fma(-li[k], y[i], y[k]);
+        }
+      }
+    }
+    // y <- U^-1 y
+#pragma unroll
+    for (int c = NUM - 1; c >= 0; --c) {
+      if (c < nu) {
+        const double* col = sU + c * (c + 1) / 2;
+        y[c] *= sD[c];
+#pragma unroll
+        for (int k = 0; k < c; ++k) y[k] = fma(-col[k], y[c], y[k]);
+      }
+    }
+    if (j < nu) {
+      double* col = out + (int64_t)pass * nn + (int64_t)j * nu;
+#pragma unroll
+      for (int k = 0; k < NUM; ++k)
+        if (k < nu) col[k] = y[k];
+    }
+    __syncthreads();  // the staged factors are reused by the next cell
+  }
+}
+
 static hipError_t launch_flux_invert(double* slot, int64_t usz, int64_t ncells, int na, int nu,
                                      hipStream_t st) {
   if (ncells <= 0) return hipSuccess;
@@ -4473,8 +4547,9 @@ static hipError_t launch_flux_invert(double* slot, int64_t usz, int64_t ncells, 
     hipLaunchKernelGGL((flux_invert_kernel<32>), dim3(grid), dim3(64 * kInvWaves), lds, st,
                        slot, usz, ncells, nu, fs.ri);
   else if (nu <= 64)
-    hipLaunchKernelGGL((flux_invert_kernel<64>), dim3(grid), dim3(64 * kInvWaves), lds, st,
-                       slot, usz, ncells, nu, fs.ri);
+    hipLaunchKernelGGL(flux_invert2_kernel, dim3((int)std::min<int64_t>(ncells, 65536)),
+                       dim3(128), (size_t)kInv2Lds * sizeof(double), st, slot, usz, ncells, nu,
+                       fs.ri);
   else
     return hipErrorInvalidValue;
   return hipGetLastError();
