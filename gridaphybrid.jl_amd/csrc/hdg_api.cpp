@@ -1375,7 +1375,7 @@ int hdg_cell_blocks(hdg_mesh* m, const double** S, const double** g, const int32
 
 int hdg_local_factors(hdg_mesh* m, const double** U, const double** Y) {
   if (!m) return HDG_ERR_ARG;
-  if (U) *U = (m->mixed || m->used_family == kFamilyFluxFused) ? nullptr : m->pU;
+  if (U) *U = m->mixed ? nullptr : m->pU;
   if (Y) *Y = m->pY;
   return m->condensed ? HDG_OK : fail(m->ctx, HDG_ERR_STATE, "no factors before condensation");
 }
@@ -1574,12 +1574,10 @@ int recover_facet_major(hdg_mesh* m, const double* lam_free, double* free_values
             "mixed recover kernel");
     return HDG_OK;
   }
-  if (m->used_family == kFamilyFluxFused || m->used_family == kFamilyFluxSchur ||
-      m->used_family == kFamilyFluxRef) {
+  if (m->used_family == kFamilyFluxSchur || m->used_family == kFamilyFluxRef) {
     const CellLayout L(m->th);
     HIP_TRY(ctx, launch_recover_flux(m->tdev, m->meshdev(), m->pU, m->pY, (int64_t)L.U,
-                                     (int64_t)L.Y, m->used_family == kFamilyFluxFused,
-                                     m->used_family == kFamilyFluxRef,
+                                     (int64_t)L.Y, m->used_family == kFamilyFluxRef,
                                      m->d_cell_dofs.as<int32_t>(), lam_free,
                                      m->d_lam_dir.as<double>(), free_values, m->nfree,
                                      ctx->stream),

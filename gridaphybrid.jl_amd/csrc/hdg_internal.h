@@ -254,7 +254,6 @@ struct SysShape {
   int ni = 0, nb = 0;
   int flux_na = 0;
   bool flux = false;
-  bool fused = false;  // the fused flux-block kernel fits (family 9)
   bool ref = false;    // the reference decomposition applies (family 10)
 };
 SysShape sys_shape(const Tables& T);
@@ -269,11 +268,8 @@ size_t factor_slot(const SysShape& sh);
 // ni - n_flux interior unknowns is condensed by an elimination kernel.  The
 // resident factors are then those of the reduced system.
 constexpr int kFamilyFluxSchur = 8;
-// Family id of the fused flux-block condensation: the remaining interior
-// unknowns are eliminated in the same kernel (Gauss-Jordan of R_uu); the
-// rows of W = A_ii^-1 [A_ib b_i] of those unknowns are kept in Y (natural
-// order) with R_uu^-1 and the blocks A_uu, A_qu, A_uq in the factor slot.
-constexpr int kFamilyFluxFused = 9;
+// (family 9, a fused variant of family 8 that never won a class, is removed;
+// its number stays reserved)
 // Family 10: the flux elimination of family 8 with the reduced system formed
 // from the reference decomposition (TablesHost::fr_*) instead of two MFMA
 // products per cell (Darcy on cells with det J > 0).
@@ -286,15 +282,13 @@ int condense_grid_options(int fam, int* gm, int max);
 size_t condense_workspace_bytes(const SysShape& sh, int fam, int64_t ncells);
 hipError_t launch_condense_flux(const Tables& T, const MeshDev& M, const CondenseOut& out,
                                 hipStream_t st, Workspace& ws, int gridmul, bool ref);
-hipError_t launch_condense_fused_flux(const Tables& T, const MeshDev& M, const CondenseOut& out,
-                                      hipStream_t st, int gridmul);
-// Recovery after a flux-block condensation (families 8 and 9; fused selects
-// family 9's slot layout): x_u = w_u - W_u lambda from the kept rows W_u of
-// A_ii^-1 [A_ib b_i], x_q = (C^-1 (x) M^-1)(b_q - A_qu x_u - A_qb lambda); then
+// Recovery after a flux-block condensation (families 8 and 10): x_u =
+// U^-1 (y - Y lambda) from the reduced factors, x_q = (C^-1 (x) M^-1)(b_q -
+// A_qu x_u - A_qb lambda); then
 // FP64 refinement against the residual of the full interior system (the
 // reduced system is worse conditioned than A_ii).
 hipError_t launch_recover_flux(const Tables& T, const MeshDev& M, const double* U,
-                               const double* Y, int64_t usz, int64_t ysz, bool fused, bool ref,
+                               const double* Y, int64_t usz, int64_t ysz, bool ref,
                                const int32_t* cell_dofs, const double* lam_free,
                                const double* lam_dir, double* out, int64_t nfree,
                                hipStream_t st);

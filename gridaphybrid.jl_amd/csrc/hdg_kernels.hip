@@ -2285,7 +2285,6 @@ enum Family {
   kFamPanel = 5,
   kFamGrid16 = 7,
   kFamFlux = kFamilyFluxSchur,
-  kFamFused = kFamilyFluxFused,
   kFamRef = kFamilyFluxRef,
   kFamCount = 11
 };
@@ -2423,7 +2422,7 @@ static hipError_t launch_condense(const Gen& gen, const CondenseOut& out, int64_
 int condense_variants(const SysShape& sh, int* fams, int max) {
   int k = 0;
   for (int f = 0; f < kFamCount && k < max; ++f)
-    if (family_ok(f, sh.ni, sh.nb) || (f == kFamFlux && sh.flux) || (f == kFamFused && sh.fused) ||
+    if (family_ok(f, sh.ni, sh.nb) || (f == kFamFlux && sh.flux) ||
         (f == kFamRef && sh.ref))
       fams[k++] = f;
   return k;
@@ -2434,7 +2433,7 @@ int condense_grid_options(int fam, int* gm, int max) {
   static const int wave[] = {16, 32, 48, 64}, grid8[] = {20, 40, 80},
                    g32[] = {2, 3, 4}, pan[] = {1, 2},
                    large[] = {4, 8, 16, 108, 116},
-                   g16[] = {2, 4, 6}, fus[] = {1, 2}, fixed[] = {0};
+                   g16[] = {2, 4, 6}, fixed[] = {0};
   const int* v = fixed;
   int n = 1;
   switch (fam) {
@@ -2444,7 +2443,6 @@ int condense_grid_options(int fam, int* gm, int max) {
     case kFamPanel: v = pan, n = 2; break;
     case kFamLarge: v = large, n = 5; break;
     case kFamGrid16: v = g16, n = 3; break;
-    case kFamFused: v = fus, n = 2; break;
     default: break;
   }
   int k = 0;
@@ -2457,8 +2455,6 @@ static size_t flux_workspace_bytes(int nr, int64_t ncells);
 int condense_default_family(const SysShape& sh) {
   // small systems stay with the register-resident kernels (one wavefront or
   // an 8x8 grid per cell), which no two-stage scheme beats
-  // (the fused variant, family 9, is an autotuner candidate: its one-wavefront
-  // Gauss-Jordan steps make it slower than the reduced grid elimination here)
   if ((sh.flux || sh.ref) && !wave_class(sh.ni, sh.nb) && sh.ni + sh.nb > 96)
     return sh.ref ? kFamRef : kFamFlux;
   return default_family(sh.ni, sh.nb);
@@ -2468,7 +2464,6 @@ size_t condense_workspace_bytes(const SysShape& sh, int fam, int64_t ncells) {
   const int ni = sh.ni, nb = sh.nb;
   if (fam < 0) fam = condense_default_family(sh);
   if (fam == kFamFlux || fam == kFamRef) return flux_workspace_bytes(ni - sh.flux_na + nb, ncells);
-  if (fam == kFamFused) return 0;
   if (fam == kFamPanel) {
     const int n = ni + nb;
     return panel_bytes(n, panel_grid(n, INT64_MAX, 0) * 2, nullptr);  // any grid multiplier <= 2x
@@ -2547,7 +2542,6 @@ hipError_t launch_condense_fused(const Tables& T, const MeshDev& M, const Conden
   if (fam < 0) fam = condense_default_family(sys_shape(T));
   if (fam == kFamFlux) return launch_condense_flux(T, M, out, st, ws, gridmul, false);
   if (fam == kFamRef) return launch_condense_flux(T, M, out, st, ws, gridmul, true);
-  if (fam == kFamFused) return launch_condense_fused_flux(T, M, out, st, gridmul);
   if (T.phys == kDarcy) {
     FusedGen<kDarcy> gen{T, M};
     return launch_condense<FusedGen<kDarcy>, false>(gen, out, M.ncells, T.ni, T.nb, st, ws, fam,
@@ -3898,438 +3892,6 @@ static size_t flux_lds(const FluxPlan& fp, int np) {
   return ((size_t)fp.minv_off + (size_t)np * np) * sizeof(double);
 }
 
-// --------------------------------------------------------------------------
-// Fused flux-block condensation (family "flux_fused").  The flux unknowns are
-// eliminated as in flux_schur_kernel, and the remaining interior unknowns in
-// the same kernel, so no reduced system goes through HBM:
-//   Z   = A_qq^-1 [A_qu A_qb b_q]                                (f64 MFMA)
-//   R_u = [A_uu A_ub b_u] - A_uq Z                   (u rows only, into LDS)
-//   [I | W_u | R_uu^-1] = Gauss-Jordan of [R_uu | R_ub r_u | I]
-//                         (one wavefront, partial pivoting, explicit swaps)
-//   R_b   = [A_bu A_bb b_b] - A_bq Z        (boundary rows, 16 at a time)
-//   [S g] = R_bb - R_bu W_u                                      (f64 MFMA)
-// W_u (the rows of W = A_ii^-1 [A_ib b_i] of the remaining interior
-// unknowns) and R_uu^-1 are kept for the recovery, with the blocks A_uu,
-// A_qu, A_uq.  While wavefront 0 runs the Gauss-Jordan steps the others
-// generate the first boundary-row tile.
-// --------------------------------------------------------------------------
-struct FusedPlan {
-  int nu, nc, KS, KU;  // remaining interior unknowns, reduced columns (u, b, rhs), k-steps
-  int ldg, ncp;        // G1 leading dimension (odd), padded reduced columns
-  int ldr;             // row length of the Gauss-Jordan system: nc + nu
-  int tcols;           // columns of a generated row tile: 4 KS + ncp
-  int tile_off, rbu_off, minv_off;  // LDS offsets (doubles); Gauss-Jordan rows start at 0
-  int cdiag;
-  uint32_t zmask[8];
-};
-
-template <int PHYS, int CT, int KSM>
-__global__ void __launch_bounds__(64 * kFluxWaves, 3)  // two workgroups per CU
-    flux_fused_kernel(FusedGen<PHYS> gen, CondenseOut out, FusedPlan fp,
-                      double* __restrict__ slot, int64_t usz, double* __restrict__ W,
-                      int64_t ysz) {
-  constexpr int NW = kFluxWaves, NT = 64 * NW, LT = kFluxTileLd;
-  constexpr int RTZ = (KSM + 3) / 4;
-  const Tables& T = gen.T;
-  const int na = T.na, np = T.np, ncomp = T.fb_nc, ni = T.ni, nb = T.nb;
-  const int nu = fp.nu, nc = fp.nc, KS = fp.KS, KU = fp.KU, ldg = fp.ldg, ldr = fp.ldr;
-  const int tcols = fp.tcols;
-  const int nk = 4 * KS, nuk = 4 * KU;
-  const int64_t ncells = gen.M.ncells;
-  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  const int i16 = lane & 15, q4 = lane >> 4;
-  extern __shared__ double s_dyn[];
-  double* s_ru = s_dyn;                  // [nu][ldr] Gauss-Jordan rows (after G1 is dead)
-  double* s_tile = s_dyn + fp.tile_off;  // [tcols][LT] row tile, or Z_u [nuk][nk]
-  double* s_minv = s_dyn + fp.minv_off;  // [np][np]
-  __shared__ FusedShared s_gen;
-  __shared__ double s_cinv[36];
-  __shared__ uint16_t s_split[16 * RTZ];
-  __shared__ double s_cw[72];
-  __shared__ int s_bad, s_info;
-  double* s_rbu = s_dyn + fp.rbu_off;  // R_bu of one boundary-row tile, [u column][row]
-  for (int t = tid; t < np * np; t += NT) s_minv[t] = T.fb_minv[t];
-  for (int t = tid; t < 16 * RTZ; t += NT)
-    s_split[t] = t < na ? (uint16_t)(((t / np) << 8) | (t % np)) : (uint16_t)0xffff;
-
-  for (int64_t cell = blockIdx.x; cell < ncells; cell += gridDim.x) {
-    gen.begin(cell, s_gen);
-    __syncthreads();
-    if (tid == 0) {
-      s_bad = flux_cinv(ncomp, T.fb_cid, s_gen.alpha, s_cw, s_cinv);
-      s_info = s_bad;
-    }
-    double* Sl = slot + cell * usz;  // [R_uu^-1][A_uu][A_qu][A_uq], column-major
-    double* Wc = W + cell * ysz;     // [nb+1][ni]
-    // ---- G1 = [A_qu A_qb b_q] into LDS --------------------------------------
-    for (int t0 = tid; t0 < nk * ldg; t0 += 4 * NT) {
-      int rows[4], cols[4];
-      bool ok[4];
-      double v[4];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const int t = t0 + u * NT, c = t / nk, k = t - c * nk;
-        rows[u] = k;
-        cols[u] = na + c;
-        ok[u] = t < nk * ldg && k < na && c < nc;
-      }
-      gen.template entries<4>(s_gen, rows, cols, ok, v);
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const int t = t0 + u * NT, c = t / nk, k = t - c * nk;
-        if (t < nk * ldg) s_dyn[k * ldg + c] = v[u];
-      }
-    }
-    __syncthreads();
-    for (int t = tid; t < na * nu; t += NT) {
-      const int c = t / na, k = t - c * na;
-      Sl[2 * nu * nu + t] = s_dyn[k * ldg + c];  // A_qu [u column][flux row]
-    }
-    __syncthreads();
-    const bool bad = s_bad != 0;
-    if (!bad && !fp.cdiag)
-      for (int t = tid; t < np * nc; t += NT) {
-        const int i = t / nc, c = t - i * nc;
-        double v[6];
-#pragma unroll
-        for (int b = 0; b < 6; ++b) v[b] = b < ncomp ? s_dyn[(b * np + i) * ldg + c] : 0.0;
-#pragma unroll
-        for (int a = 0; a < 6; ++a) {
-          if (a >= ncomp) break;
-          double x = 0.0;
-#pragma unroll
-          for (int b = 0; b < 6; ++b) x = fma(s_cinv[6 * a + b], v[b], x);
-          s_dyn[(a * np + i) * ldg + c] = x;
-        }
-      }
-    __syncthreads();
-    // ---- Z = (I (x) M^-1) G1 (C^-1 applied to the rows when diagonal) --------
-    double zf[CT][KSM];
-#pragma unroll
-    for (int rt = 0; rt < RTZ; ++rt) {
-      dbl4 acc[CT];
-#pragma unroll
-      for (int ct = 0; ct < CT; ++ct) acc[ct] = dbl4{0.0, 0.0, 0.0, 0.0};
-      const int si = s_split[16 * rt + i16];
-      const double* mrow = s_minv + (si == 0xffff ? 0 : (si & 255) * np);
-#pragma unroll 2
-      for (int s = 0; s < KS; ++s) {
-        if (!((fp.zmask[rt] >> s) & 1u)) continue;  // wave-uniform
-        const int sk = s_split[4 * s + q4];
-        const double a = (si != 0xffff && (si >> 8) == (sk >> 8)) ? mrow[sk & 255] : 0.0;
-        const double* brow = s_dyn + (4 * s + q4) * ldg + i16;
-#pragma unroll
-        for (int ct = 0; ct < CT; ++ct)
-          if (16 * (w + NW * ct) < nc)
-            acc[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, brow[16 * (w + NW * ct)], acc[ct],
-                                                           0, 0, 0);
-      }
-#pragma unroll
-      for (int v = 0; v < 4; ++v)
-        if (4 * rt + v < KSM) {
-          double sc = 1.0;
-          if (fp.cdiag) {
-            const int row = 16 * rt + q4 + 4 * v;
-            sc = row < na ? s_cinv[7 * (row / np)] : 0.0;
-          }
-#pragma unroll
-          for (int ct = 0; ct < CT; ++ct) zf[ct][4 * rt + v] = acc[ct][v] * sc;
-        }
-    }
-    __syncthreads();  // G1 dead: its region holds the Gauss-Jordan rows and tiles
-    // identity block of the Gauss-Jordan rows; padding rows of R_bu
-    for (int t = tid; t < nu * nu; t += NT) {
-      const int r = t / nu, k = t - r * nu;
-      s_ru[r * ldr + nc + k] = r == k ? 1.0 : 0.0;
-    }
-    for (int t = tid; t < (nuk - nu) * 16; t += NT) s_rbu[nu * 16 + t] = 0.0;
-    // ---- R_u = [A_uu A_ub b_u] - A_uq Z, 16 u rows at a time ------------------
-#pragma unroll 1
-    for (int rr = 0; 16 * rr < nu; ++rr) {
-      for (int t0 = tid; t0 < 16 * (nk + fp.ncp); t0 += 4 * NT) {
-        int rows[4], cols[4];
-        bool ok[4];
-        double v[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const int t = t0 + u * NT, col = t >> 4, row = 16 * rr + (t & 15);
-          rows[u] = na + row;
-          cols[u] = col < nk ? col : na + col - nk;
-          ok[u] = t < 16 * (nk + fp.ncp) && row < nu && (col < na || (col >= nk && col - nk < nc));
-        }
-        gen.template entries<4>(s_gen, rows, cols, ok, v);
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const int t = t0 + u * NT, col = t >> 4;
-          if (t < 16 * (nk + fp.ncp)) s_tile[col * LT + (t & 15)] = col < nk ? -v[u] : v[u];
-        }
-      }
-      __syncthreads();
-      for (int t = tid; t < 16 * (na + nu); t += NT) {  // kept blocks A_uq, A_uu
-        const int col = t >> 4, i = t & 15, row = 16 * rr + i;
-        if (row >= nu) continue;
-        if (col < na)
-          Sl[2 * nu * nu + na * nu + (int64_t)col * nu + row] = -s_tile[col * LT + i];
-        else
-          Sl[nu * nu + (int64_t)(col - na) * nu + row] = s_tile[(nk + col - na) * LT + i];
-      }
-#pragma unroll
-      for (int ct = 0; ct < CT; ++ct) {
-        const int c0 = 16 * (w + NW * ct);
-        if (c0 >= nc) continue;  // wave-uniform
-        const int col = c0 + i16;
-        dbl4 acc;
-#pragma unroll
-        for (int v = 0; v < 4; ++v) acc[v] = s_tile[(nk + col) * LT + q4 + 4 * v];
-#pragma unroll
-        for (int s = 0; s < KSM; ++s)
-          if (s < KS)
-            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(s_tile[(4 * s + q4) * LT + i16], zf[ct][s],
-                                                       acc, 0, 0, 0);
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          const int row = 16 * rr + q4 + 4 * v;
-          if (row < nu && col < nc) s_ru[row * ldr + col] = acc[v];
-        }
-      }
-      __syncthreads();
-    }
-    // ---- Gauss-Jordan on [R_uu | R_ub r_u | I] by wavefront 0; meanwhile the
-    //      other wavefronts generate the first boundary-row tile ----------------
-    auto gen_btile = [&](int rb, int t_begin, int t_step) {
-      // tile columns: flux [0, nk) as -A_bq, then the reduced columns (u, b, rhs)
-      for (int t0 = t_begin; t0 < 16 * (nk + fp.ncp); t0 += 4 * t_step) {
-        int rows[4], cols[4];
-        bool ok[4];
-        double v[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const int t = t0 + u * t_step, col = t >> 4, row = 16 * rb + (t & 15);
-          rows[u] = ni + row;
-          cols[u] = col < nk ? col : na + col - nk;
-          ok[u] = t < 16 * (nk + fp.ncp) && row < nb && (col < na || (col >= nk && col - nk < nc));
-        }
-        gen.template entries<4>(s_gen, rows, cols, ok, v);
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const int t = t0 + u * t_step, col = t >> 4;
-          if (t < 16 * (nk + fp.ncp)) s_tile[col * LT + (t & 15)] = col < nk ? -v[u] : v[u];
-        }
-      }
-    };
-    if (w == 0) {
-      if (!bad) {
-#pragma unroll 1
-        for (int j = 0; j < nu; ++j) {
-          double colj = lane < nu ? s_ru[lane * ldr + j] : 0.0;
-          const int64_t key = (lane >= j && lane < nu) ? dbits(fabs(colj)) : (int64_t)-1;
-          const int64_t m = wave_max_i64(key);
-          const int p = wave_min_i32(key == m ? lane : INT_MAX);  // largest, lowest row on ties
-          const double piv = wave_readlane_f64(colj, p);
-          if (piv == 0.0) {
-            if (lane == 0 && s_info == 0) s_info = na + j + 1;
-            continue;  // wave-uniform
-          }
-          if (p != j) {
-            const double cj = wave_readlane_f64(colj, j);
-            for (int c = lane; c < ldr; c += 64) {
-              const double t = s_ru[p * ldr + c];
-              s_ru[p * ldr + c] = s_ru[j * ldr + c];
-              s_ru[j * ldr + c] = t;
-            }
-            colj = lane == p ? cj : (lane == j ? piv : colj);
-            wave_sync_lds();
-          }
-          const double inv = 1.0 / piv;
-          double pr[2];
-#pragma unroll
-          for (int h = 0; h < 2; ++h) {
-            const int c = lane + 64 * h;
-            pr[h] = c < ldr ? s_ru[j * ldr + c] * inv : 0.0;
-            if (c < ldr) s_ru[j * ldr + c] = pr[h];
-          }
-          // eliminate column j from the other rows, four rows per pass: the
-          // multipliers come from the register column by readlane
-#pragma unroll 1
-          for (int r0 = 0; r0 < nu; r0 += 4) {
-            double f[4], v[4][2];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-              const int r = r0 + u;
-              const double fr = wave_readlane_f64(colj, r < 64 ? r : 63);
-              f[u] = (r < nu && r != j) ? fr : 0.0;
-#pragma unroll
-              for (int h = 0; h < 2; ++h) {
-                const int c = lane + 64 * h;
-                v[u][h] = (r < nu && c < ldr) ? s_ru[r * ldr + c] : 0.0;
-              }
-            }
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-              const int r = r0 + u;
-#pragma unroll
-              for (int h = 0; h < 2; ++h) {
-                const int c = lane + 64 * h;
-                if (r < nu && r != j && c < ldr) s_ru[r * ldr + c] = fma(-f[u], pr[h], v[u][h]);
-              }
-            }
-          }
-          wave_sync_lds();
-        }
-      }
-    } else {
-      gen_btile(0, tid - 64, NT - 64);
-    }
-    __syncthreads();
-    // kept R_uu^-1 [col][row] and W_u (rows n_flux.. of W = A_ii^-1 [A_ib b_i])
-    for (int t = tid; t < nu * nu; t += NT) {
-      const int j = t / nu, kk = t - j * nu;
-      Sl[t] = s_ru[kk * ldr + nc + j];
-    }
-    for (int t = tid; t < nu * (nb + 1); t += NT) {
-      const int j = t / nu, kk = t - j * nu;
-      Wc[(int64_t)j * ni + na + kk] = s_ru[kk * ldr + nu + j];
-    }
-    // ---- [S g] = R_bb - R_bu W_u, R_b = [A_bu A_bb b_b] - A_bq Z, 16 rows a pass -
-#pragma unroll 1
-    for (int rb = 0; 16 * rb < nb; ++rb) {
-      if (rb > 0) {
-        gen_btile(rb, tid, NT);
-        __syncthreads();
-      }
-      dbl4 accs[CT];
-#pragma unroll
-      for (int ct = 0; ct < CT; ++ct) {
-        const int c0 = 16 * (w + NW * ct), col = c0 + i16;
-        accs[ct] = dbl4{0.0, 0.0, 0.0, 0.0};
-        if (c0 >= nc) continue;  // wave-uniform
-        dbl4 acc;
-#pragma unroll
-        for (int v = 0; v < 4; ++v) acc[v] = s_tile[(nk + col) * LT + q4 + 4 * v];
-#pragma unroll
-        for (int s = 0; s < KSM; ++s)
-          if (s < KS)
-            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(s_tile[(4 * s + q4) * LT + i16], zf[ct][s],
-                                                       acc, 0, 0, 0);
-        accs[ct] = acc;
-        // R_bu, the u columns of this row tile, for every wavefront: [k][16]
-        if (c0 < nu)
-#pragma unroll
-          for (int v = 0; v < 4; ++v)
-            if (col < nu) s_rbu[col * 16 + q4 + 4 * v] = acc[v];
-      }
-      __syncthreads();
-#pragma unroll
-      for (int ct = 0; ct < CT; ++ct) {
-        const int c0 = 16 * (w + NW * ct), col = c0 + i16;
-        if (c0 >= nc || c0 + 16 <= nu) continue;  // wave-uniform: no b/rhs column here
-        dbl4 acc = accs[ct];
-        for (int s = 0; s < KU; ++s) {
-          const int kk = 4 * s + q4;
-          const double b = (col >= nu && col < nc && kk < nu) ? s_ru[kk * ldr + col] : 0.0;
-          acc = __builtin_amdgcn_mfma_f64_16x16x4f64(-s_rbu[kk * 16 + i16], b, acc, 0, 0, 0);
-        }
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          const int row = 16 * rb + q4 + 4 * v;
-          if (row >= nb || col < nu || col >= nc) continue;
-          if (col < nc - 1) {
-            if (out.S) out.S[(cell * nb + (col - nu)) * nb + row] = acc[v];
-          } else if (out.g) {
-            out.g[cell * nb + row] = acc[v];
-          }
-        }
-      }
-    }
-    if (tid == 0 && out.info) out.info[cell] = s_info;
-    __syncthreads();  // LDS reuse by the next cell
-  }
-}
-
-// Host plan of the fused kernel (sizes, LDS regions, skipped Z blocks).
-static bool fused_fits(int fb_nc, int na, int np, int ni, int nb, int* ct, int* ksm,
-                       FusedPlan* fp) {
-  int c = 0, k = 0;
-  if (!flux_fits(fb_nc, na, np, ni, nb, &c, &k)) return false;
-  if (k == 21) k = 24;  // the fused kernel's instantiations
-  const int nu = ni - na, nc = nu + nb + 1, KS = (na + 3) / 4, KU = (nu + 3) / 4;
-  const int ncp = 16 * ((nc + 15) / 16), ldg = ncp | 1, ldr = nc + nu;
-  if (nu > 64 || ldr > 128) return false;  // Gauss-Jordan: a lane per row, two columns per lane
-  if (nb > 128) return false;              // boundary values of the recovery in LDS
-  const int nk = 4 * KS, nuk = 4 * KU, tcols = nk + ncp;
-  const size_t tile_off = (size_t)nu * ldr;
-  const size_t rbu_off = tile_off + (size_t)tcols * kFluxTileLd;
-  const size_t minv_off = std::max((size_t)nk * ldg, rbu_off + (size_t)nuk * 16);
-  if ((minv_off + (size_t)np * np) * sizeof(double) > 150 * 1024) return false;
-  if (ct) *ct = c;
-  if (ksm) *ksm = k;
-  if (fp) {
-    fp->nu = nu;
-    fp->nc = nc;
-    fp->KS = KS;
-    fp->KU = KU;
-    fp->ldg = ldg;
-    fp->ncp = ncp;
-    fp->ldr = ldr;
-    fp->tcols = tcols;
-    fp->tile_off = (int)tile_off;
-    fp->rbu_off = (int)rbu_off;
-    fp->minv_off = (int)minv_off;
-    for (int t = 0; t < 8; ++t) {
-      fp->zmask[t] = 0;
-      for (int s = 0; s < KS && s < 32; ++s) {
-        bool nz = false;
-        for (int i = 16 * t; i < std::min(16 * t + 16, na) && !nz; ++i)
-          for (int kk = 4 * s; kk < std::min(4 * s + 4, na) && !nz; ++kk) nz = i / np == kk / np;
-        if (nz) fp->zmask[t] |= 1u << s;
-      }
-    }
-  }
-  return true;
-}
-
-static size_t fused_slot(int na, int nu) {
-  return 2 * (size_t)nu * nu + 2 * (size_t)na * nu;  // [R_uu^-1][A_uu][A_qu][A_uq]
-}
-
-template <int PHYS>
-static hipError_t launch_fused_phys(const Tables& T, const MeshDev& M, const CondenseOut& out,
-                                    hipStream_t st, int gridmul) {
-  FusedPlan fp;
-  int ct = 0, ksm = 0;
-  if (!T.fb_minv || !fused_fits(T.fb_nc, T.na, T.np, T.ni, T.nb, &ct, &ksm, &fp))
-    return hipErrorInvalidValue;
-  if (M.ncells <= 0) return hipSuccess;
-  if (!out.U || !out.Y || out.u_stride < (int64_t)fused_slot(T.na, fp.nu) ||
-      out.y_stride < (int64_t)(T.nb + 1) * T.ni)
-    return hipErrorInvalidValue;
-  fp.cdiag = 1;
-  for (int a = 0; a < T.fb_nc; ++a)
-    for (int b = 0; b < T.fb_nc; ++b)
-      if (a != b && T.fb_cid[6 * a + b] >= 0) fp.cdiag = 0;
-  const size_t lds = ((size_t)fp.minv_off + (size_t)T.np * T.np) * sizeof(double);
-  const int grid = (int)std::min<int64_t>(M.ncells, 256 * (gridmul > 0 ? gridmul : 2));
-  FusedGen<PHYS> gen{T, M};
-  ClockScope clk_(st, "flux_fused");
-#define HDG_FUSED(CT_, KS_)                                                                   \
-  if (ct == CT_ && ksm == KS_)                                                                \
-    hipLaunchKernelGGL((flux_fused_kernel<PHYS, CT_, KS_>), dim3(grid), dim3(64 * kFluxWaves), \
-                       lds, st, gen, out, fp, out.U, out.u_stride, out.Y, out.y_stride);
-  HDG_FUSED(1, 8)
-  HDG_FUSED(1, 16)
-  HDG_FUSED(1, 24)
-  HDG_FUSED(2, 8)
-  HDG_FUSED(2, 16)
-#undef HDG_FUSED
-  return hipGetLastError();
-}
-
-hipError_t launch_condense_fused_flux(const Tables& T, const MeshDev& M, const CondenseOut& out,
-                                      hipStream_t st, int gridmul) {
-  return T.phys == kDarcy ? launch_fused_phys<kDarcy>(T, M, out, st, gridmul)
-                          : launch_fused_phys<kElasticity>(T, M, out, st, gridmul);
-}
-
 // Regions of the flux family's per-cell factor slot (doubles from the slot's
 // start): the reduced factors U (packed by columns), the multipliers L
 // [step][row] and the pivot order P, then the kept interior blocks A_uu
@@ -4673,7 +4235,6 @@ size_t factor_slot(const SysShape& sh) {
   if (sh.flux_na <= 0) return dense;
   const size_t m = (size_t)(sh.ni - sh.flux_na), na = (size_t)sh.flux_na;
   size_t s = std::max(dense, (size_t)flux_slot((int)na, (int)m).size);
-  if (sh.fused) s = std::max(s, fused_slot(sh.flux_na, (int)m));
   return s;
 }
 
@@ -4697,7 +4258,6 @@ SysShape sys_shape(const Tables& T) {
   sh.ref = T.fb_minv && T.fr_P && T.fr_G &&
            ref_fits(T.phys, T.D, T.fb_nc, T.na, T.np, T.ni, T.nb, T.fr_npair);
   sh.flux_na = sh.flux || sh.ref ? T.na : 0;
-  sh.fused = sh.flux && fused_fits(T.fb_nc, T.na, T.np, T.ni, T.nb, nullptr, nullptr, nullptr);
   return sh;
 }
 
@@ -4709,18 +4269,15 @@ SysShape sys_shape(const TablesHost& t) {
   sh.ref = !t.fb_minv.empty() && !t.fr_P.empty() &&
            ref_fits(t.phys, t.D, t.fb_nc, t.na, t.np, t.ni, t.nb, t.fr_npair);
   sh.flux_na = sh.flux || sh.ref ? t.na : 0;
-  sh.fused = sh.flux && fused_fits(t.fb_nc, t.na, t.np, t.ni, t.nb, nullptr, nullptr, nullptr);
   return sh;
 }
 
 // Flux recovery, one wavefront per cell, for both flux families.  The cell's
-// slot holds R_uu^-1, the interior blocks A_uu, A_qu, A_uq and the rows W_u of
-// W = A_ii^-1 [A_ib b_i] of the remaining interior unknowns (RecSlot gives
-// where):
+// slot holds R_uu^-1, the interior blocks A_uu, A_qu, A_uq and U^-1, the
+// inverse of the reduced system's upper factor; Y = L^-1 P [R_ub r_u] are the
+// reduced system's lower-factor rows (RecSlot gives where):
 //   c   = b_i - A_ib lambda                 (boundary columns of the program)
-//   x_u = w_u - W_u lambda                  (family 9 keeps the rows W_u), or
-//   x_u = U^-1 (y - Y lambda)               (family 8: Y = L^-1 P [R_ub r_u] of
-//                                            the reduced system, U^-1 kept)
+//   x_u = U^-1 (y - Y lambda)
 //   x_q = A_qq^-1 (c_q - A_qu x_u)          (A_qq^-1 = C^-1 (x) M^-1)
 // then up to kFluxRefineSteps FP64 refinement steps against the residual of
 // the full interior system, r = c - A_ii x, with the structured solve as the
@@ -4746,8 +4303,8 @@ struct RecSlot {
   const double* slot;  // per-cell slots, stride usz
   int64_t usz;
   int64_t ri, auu, aqu, auq;  // R_uu^-1 [col][row], A_uu, A_qu, A_uq (offsets in the slot)
-  int64_t uinv;               // U^-1 [col][row], or -1: x_u = w - W lambda directly
-  const double* w;     // W_u or Y: w + cell * wsz + column * wld + u row
+  int64_t uinv;               // U^-1 [col][row]
+  const double* w;     // Y: w + cell * wsz + column * wld + u row
   int64_t wsz;
   int wld;
 };
@@ -4903,7 +4460,7 @@ __global__ void __launch_bounds__(256)
     const double* Auq = base + rs.auq;
     const double* Wc = rs.w + cell * rs.wsz;
     const double* bi = gen.M.srcq + cell * srcq_stride(T);
-    // ---- c = b_i - A_ib lambda and x_u = w_u - W_u lambda ------------------------
+    // ---- c = b_i - A_ib lambda and y - Y lambda -----------------------------------
     double cu = 0.0;
 #pragma unroll
     for (int t = 0; t < KQ + 1; ++t) {
@@ -4927,7 +4484,7 @@ __global__ void __launch_bounds__(256)
     }
     wave_sync_lds();
     if (lane < nu) cu = uv[lane];
-    if (rs.uinv >= 0) {  // x_u = U^-1 (y - Y lambda)
+    {  // x_u = U^-1 (y - Y lambda)
       const double* Ui = base + rs.uinv;
       double x = 0.0;
       if (lane < nu)
@@ -5055,38 +4612,27 @@ __global__ void __launch_bounds__(256)
 }
 
 hipError_t launch_recover_flux(const Tables& T, const MeshDev& M, const double* U,
-                               const double* Y, int64_t usz, int64_t ysz, bool fused, bool ref,
+                               const double* Y, int64_t usz, int64_t ysz, bool ref,
                                const int32_t* cell_dofs, const double* lam_free,
                                const double* lam_dir, double* out, int64_t nfree,
                                hipStream_t st) {
   const int na = T.na, nu = T.ni - na, nb = T.nb;
   if (na > (ref ? kRecMaxNa : kMixedMaxNi) || nu > 64 || nb > kFusedMaxNb || T.np > kFluxMaxNp)
     return hipErrorInvalidValue;
-  if (ref && (fused || T.phys != kDarcy || !T.fr_G)) return hipErrorInvalidValue;
+  if (ref && (T.phys != kDarcy || !T.fr_G)) return hipErrorInvalidValue;
   RecSlot rs;
   rs.slot = U;
   rs.usz = usz;
-  if (fused) {  // [R_uu^-1][A_uu][A_qu][A_uq]; W_u rows in the interior rows of Y
-    rs.ri = 0;
-    rs.auu = (int64_t)nu * nu;
-    rs.aqu = 2 * (int64_t)nu * nu;
-    rs.auq = rs.aqu + (int64_t)na * nu;
-    rs.uinv = -1;
-    rs.w = Y + na;
-    rs.wsz = ysz;
-    rs.wld = T.ni;
-  } else {
-    const FluxSlot fs = flux_slot(na, nu);
-    if (usz < fs.size || ysz < (int64_t)(nb + 1) * nu) return hipErrorInvalidValue;
-    rs.ri = fs.ri;
-    rs.auu = fs.auu;
-    rs.aqu = fs.aqu;
-    rs.auq = fs.auq;
-    rs.uinv = fs.uinv;
-    rs.w = Y;
-    rs.wsz = ysz;
-    rs.wld = nu;
-  }
+  const FluxSlot fs = flux_slot(na, nu);
+  if (usz < fs.size || ysz < (int64_t)(nb + 1) * nu) return hipErrorInvalidValue;
+  rs.ri = fs.ri;
+  rs.auu = fs.auu;
+  rs.aqu = fs.aqu;
+  rs.auq = fs.auq;
+  rs.uinv = fs.uinv;
+  rs.w = Y;
+  rs.wsz = ysz;
+  rs.wld = nu;
   if (M.ncells > 0) {
     const int64_t blocks = (M.ncells + kRecWaves - 1) / kRecWaves;
     const int grid = (int)(blocks < 65536 ? blocks : 65536);

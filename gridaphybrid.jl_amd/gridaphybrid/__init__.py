@@ -454,12 +454,10 @@ class HybridAffineFEOperator:
         nc, nb = self.info.ncells, self.info.nb
         lay = self.factor_layout()
         ni = int(lay.rows)
-        if not U.value:   # W = A_ii^-1 [A_ib b_i], natural order (mixed, fused flux block)
+        if not U.value:   # W = A_ii^-1 [A_ib b_i], natural order (mixed precision)
             Yh = np.empty((nc, nb + 1, ni))
             check(lib().hdg_memcpy_d2h(self.ctx.h, Yh.ctypes.data, Y, Yh.nbytes), self.ctx.h,
                   "d2h")
-            if int(lay.family) == _lib.FAMILY_FLUX_FUSED:
-                Yh[:, :, :self.info.n_flux] = 0.0   # only the rows of the non-flux unknowns are kept
             return None, Yh.transpose(0, 2, 1)
         # per-cell slots are sized for the full interior; the factors of a
         # reduced system sit at the start of each slot

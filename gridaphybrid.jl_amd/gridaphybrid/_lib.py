@@ -52,10 +52,9 @@ EXPORTS = [
 
 BATCH_AUTOTUNE = 1
 FAMILY_NAMES = {0: "wave", 1: "grid8x8", 3: "grid32x16", 4: "blocked",
-                5: "panel_mfma", 7: "grid16x16", 8: "flux_schur", 9: "flux_fused",
+                5: "panel_mfma", 7: "grid16x16", 8: "flux_schur",
                 10: "flux_ref"}
 FAMILY_FLUX_SCHUR = 8
-FAMILY_FLUX_FUSED = 9
 FAMILY_FLUX_REF = 10
 
 

@@ -346,10 +346,7 @@ int hdg_cell_blocks(hdg_mesh* mesh, const double** S, const double** g,
  * and Y are the factors of the reduced system of the m = ni - n_flux
  * non-flux interior unknowns, at the start of each cell's slot (the rest of
  * the slot holds the recovery's kept blocks and the inverses of the reduced
- * factors; not part of this interface).  After a
- * fused flux-block condensation (family 9) *U is NULL and Y holds the rows
- * n_flux..ni-1 of W = A_ii^-1 [A_ib b_i] (natural order; its flux rows are
- * not written).  Cell c's slots start at c * u_stride and c * y_stride
+ * factors; not part of this interface).  Cell c's slots start at c * u_stride and c * y_stride
  * (hdg_mesh_factor_layout). */
 int hdg_local_factors(hdg_mesh* mesh, const double** U, const double** Y);
 /* Layout of the resident factors: rows m of the factored system (ni, or
@@ -411,15 +408,15 @@ typedef struct {
 /* Kernel families: 0 one wavefront per cell (fixed sizes), 1 8x8 grid with
  * register tiles, 3 32x16 grid, 4 blocked HBM-workspace elimination,
  * 5 blocked elimination with a single-wavefront panel and FP64 MFMA trailing
- * updates, 7 16x16 grid (2 and 6 are reserved), 8 flux-block Schur
+ * updates, 7 16x16 grid (2, 6 and 9 are reserved), 8 flux-block Schur
  * complement: where the flux block of A_ii is C (x) M (per-cell coefficient
  * matrix C, fixed scalar mass matrix M; both weak forms here), the flux
  * unknowns are eliminated with C^-1 (x) M^-1 in two FP64 MFMA products and
- * the reduced system is condensed by one of the families above, 9 the same
- * elimination with the remaining interior unknowns eliminated in the same
- * kernel (Gauss-Jordan of the reduced interior block), keeping the rows of
- * W = A_ii^-1 [A_ib b_i] of those unknowns (the default where it fits, else 8).  With the environment variable HDG_AUTOTUNE_LOG set,
- * the autotuner prints every candidate's time to stderr. */
+ * the reduced system is condensed by one of the families above, 10 the same
+ * elimination with the Darcy flux products formed from reference-cell
+ * products and each cell's Jacobian cofactors (Darcy with det J > 0; the
+ * default there).  With the environment variable HDG_AUTOTUNE_LOG set, the
+ * autotuner prints every candidate's time to stderr. */
 
 /* flags: HDG_BATCH_AUTOTUNE times every kernel family and grid size able to
  * condense each class on the batch's own cells and keeps the fastest;

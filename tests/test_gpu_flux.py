@@ -16,9 +16,8 @@ pytestmark = pytest.mark.gpu
 
 RTOL = 1e-12
 FLUX = 8
-FUSED = 9
 REF = 10
-FLUX_FAMS = (FLUX, FUSED, REF)
+FLUX_FAMS = (FLUX, REF)
 
 
 def _darcy(ctx, k, n, seed, family=None, kappa=None):
@@ -79,7 +78,7 @@ def test_flux_darcy_vs_oracle_and_dense(ctx, k, n):
     op, kappa = _darcy(ctx, k, n, seed=40 + n)
     rows, fam = op.factor_rows()
     assert fam in FLUX_FAMS
-    assert rows == (op.info.ni if fam == FUSED else op.info.ni - op.info.n_flux)
+    assert rows == op.info.ni - op.info.n_flux
     ref = O.hdg_solve(O.Problem.darcy(3, k, mms=O.MMS_SINPROD), (n,) * 3, coef=kappa)
     S, g, fv = _check_vs_oracle(op, ref, 1e-11)
     A, b = _device_cell_systems(op, 0, min(op.info.ncells, 16))
@@ -141,45 +140,6 @@ def test_flux_singular_coefficient_reported(ctx):
     kappa[5] = np.inf
     with pytest.raises(gh.SingularError):
         _darcy(ctx, 2, 2, seed=0, kappa=kappa)
-
-
-@pytest.mark.parametrize("k,n", [(2, 4)])
-def test_fused_matches_flux_schur(ctx, k, n):
-    """Family 9 (fused) against family 8 (flux_schur + reduced elimination) on
-    the same cells: S, g to 1e-12 and the recovered values to 1e-11."""
-    print(f"fused vs flux_schur darcy 3D k={k} n={n}", flush=True)
-    a, kappa = _darcy(ctx, k, n, seed=70, family=FUSED)
-    b, _ = _darcy(ctx, k, n, seed=70, family=FLUX)
-    assert a.factor_rows()[1] == FUSED and b.factor_rows()[1] == FLUX
-    Sa, ga, _ = a.cell_blocks()
-    Sb, gb, _ = b.cell_blocks()
-    assert rel(Sa, Sb) < RTOL and rel(ga, gb) < RTOL
-    lam = a.interpolate_trace()
-    assert rel(a.recover(lam).cpu().numpy(), b.recover(lam).cpu().numpy()) < 1e-11
-    a.close()
-    b.close()
-
-
-@pytest.mark.parametrize("phys,k,n", [(0, 2, 3), (1, 1, 2)])
-def test_fused_family_vs_oracle(ctx, phys, k, n):
-    """Family 9 (the fused flux-block kernel, an autotuner candidate) against
-    the oracle, with its W rows checked as A_ii^-1 [A_ib b_i]."""
-    print(f"fused family vs oracle phys={phys} k={k} n={n}", flush=True)
-    if phys == 0:
-        op, kappa = _darcy(ctx, k, n, seed=80, family=FUSED)
-        ref = O.hdg_solve(O.Problem.darcy(3, k, mms=O.MMS_SINPROD), (n,) * 3, coef=kappa)
-        _check_vs_oracle(op, ref, 1e-11)
-    else:
-        op, nu = _elasticity(ctx, k, n, seed=81, family=FUSED)
-        prob = O.Problem.elasticity(3, k, tau=float(n), nu=0.3, mms=O.ELAS_MMS_SIN)
-        ref = O.hdg_solve(prob, (n,) * 3, coef=nu)
-        _check_vs_oracle(op, ref, 1e-11, adaptive=True)
-    assert op.factor_rows()[1] == FUSED
-    A, b = _device_cell_systems(op, 0, min(op.info.ncells, 16))
-    U, Y = op.local_factors()
-    assert U is None
-    check_local_factors(op, A, b, U, Y, min(op.info.ncells, 16))
-    op.close()
 
 
 @pytest.mark.parametrize("k,n", [(2, 4)])

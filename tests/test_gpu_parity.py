@@ -114,8 +114,7 @@ def _operator(ctx, D, k, n, kappa=None, source=None, spaces=None, mms=O.MMS_LINE
 
 def check_local_factors(op, A, b, U, Y, ncheck):
     """U and Y = L^-1 P [B c] are partial-pivoting LU factors of the system the
-    recovery back-substitutes (or, with U None, Y is W = A_ii^-1 [A_ib b_i]
-    with a small residual): A_ii itself, or, after the flux-block
+    recovery back-substitutes: A_ii itself, or, after the flux-block
     elimination, R_uu of the reduced augmented system
     [R_uu R_ub r_u] = [A_(u,b)(u,b) b_(u,b)] - A_(u,b)q A_qq^-1 [A_q(u,b) b_q]
     formed here in numpy from the cell systems.  (The pivot order itself may
@@ -123,14 +122,7 @@ def check_local_factors(op, A, b, U, Y, ncheck):
     import scipy.linalg as sl
     ni = op.info.ni
     rows, fam = op.factor_rows()
-    if U is None:   # fused flux-block family: rows n_flux.. of W = A_ii^-1 [A_ib b_i]
-        assert fam == 9 and rows == ni
-        na = op.info.n_flux
-        for c in range(ncheck):
-            B = np.concatenate([A[c, :ni, ni:], b[c, :ni, None]], axis=1)
-            Wt = np.linalg.solve(A[c, :ni, :ni], B)
-            assert rel(Y[c][na:], Wt[na:]) < 1e-8, rel(Y[c][na:], Wt[na:])
-        return
+    assert U is not None
     na = ni - rows
     for c in range(ncheck):
         Ab = np.concatenate([A[c], b[c][:, None]], axis=1)
