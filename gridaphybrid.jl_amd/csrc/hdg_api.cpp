@@ -647,6 +647,10 @@ int hdg_mesh_create(hdg_ctx* ctx, const hdg_problem* prob, const hdg_mesh_desc* 
   std::vector<int64_t> fptr((size_t)nff + 1, 0);
   std::vector<int32_t> gcell((size_t)nff * 2, -1);
   std::vector<uint8_t> glpos((size_t)nff * 2, 0);
+  // cells with a Dirichlet trace dof: only their rhs rows need the lifting
+  std::vector<uint8_t> cdir((size_t)nc, 0);
+  for (int64_t c = 0; c < nc; ++c)
+    for (int j = 0; j < nb && !cdir[(size_t)c]; ++j) cdir[(size_t)c] = cell_dofs[(size_t)c * nb + j] < 0;
   pfacet.reserve((size_t)nff * (2 * nF - 1));
   pcell.reserve((size_t)nff * (2 * nF - 1) * 2);
   plpos.reserve((size_t)nff * (2 * nF - 1) * 2);
@@ -665,7 +669,7 @@ int hdg_mesh_create(hdg_ctx* ctx, const hdg_problem* prob, const hdg_mesh_desc* 
       if (c < 0) continue;
       gcell[2 * G + k] = c;
       const int lg = m->facet_lpos[2 * f + k];
-      glpos[2 * G + k] = (uint8_t)lg;
+      glpos[2 * G + k] = (uint8_t)(lg | (cdir[(size_t)c] ? kGlposDirichlet : 0));
       for (int lf = 0; lf < nF; ++lf) {
         const int32_t F = ffid[desc->cell_facets[(int64_t)c * nF + lf]];
         if (F < 0) continue;
@@ -716,7 +720,7 @@ int hdg_mesh_create(hdg_ctx* ctx, const hdg_problem* prob, const hdg_mesh_desc* 
     }
     gperm.resize(gcell.size());
     for (size_t t = 0; t < gcell.size(); ++t)
-      gperm[t] = gcell[t] < 0 ? 0 : (uint8_t)m->fperm[(int64_t)gcell[t] * nF + glpos[t]];
+      gperm[t] = gcell[t] < 0 ? 0 : (uint8_t)m->fperm[(int64_t)gcell[t] * nF + (glpos[t] & 15)];
     iperm.resize((size_t)ncode * ntf);
     for (int p = 0; p < ncode; ++p)
       for (int i = 0; i < nm; ++i)

@@ -170,6 +170,7 @@ struct DenseIn {
 
 
 // Facet-pair structure of the skeleton matrix (see the assembly kernels).
+constexpr uint8_t kGlposDirichlet = 0x80;
 struct PairTable {
   const int64_t* fptr;     // [nfree_facets+1] first nnz of facet G's columns
   const int32_t* pptr;     // [nfree_facets+1] first pair of facet G
@@ -177,7 +178,8 @@ struct PairTable {
   const int32_t* pcell;    // [npairs][2] contributing cells (-1 if absent)
   const uint8_t* plpos;    // [npairs][2] (lpos_G | lpos_F << 4)
   const int32_t* gcell;    // [nfree_facets][2] cells around G (-1 if absent)
-  const uint8_t* glpos;    // [nfree_facets][2] local position of G in them
+  const uint8_t* glpos;    // [nfree_facets][2] local position of G in them (bits 0-3),
+                           // | kGlposDirichlet when that cell has a Dirichlet trace dof
   // facet orientation (nullptr when every (cell, local facet) has code 0):
   const uint8_t* pperm;    // [npairs][2] codes (of G | of F << 4) in the contributing cell
   const uint8_t* gperm;    // [nfree_facets][2] code of G in the cells around it

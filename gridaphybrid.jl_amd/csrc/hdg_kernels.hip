@@ -1593,7 +1593,9 @@ __global__ void assemble_kernel(PairTable P, const double* __restrict__ S,
 }
 
 // rhs: for free dof (G, m), sum over cells around G (ascending) of
-// g_c[i] - sum_{j Dirichlet} S_c[i, j] lam_dir (Gridap _attach_dirichlet)
+// g_c[i] - sum_{j Dirichlet} S_c[i, j] lam_dir (Gridap _attach_dirichlet).
+// Only cells flagged in glpos have Dirichlet dofs; the others skip the scan
+// of their dof row, which otherwise is most of the kernel's traffic.
 __global__ void rhs_kernel(PairTable P, const double* __restrict__ S, const double* __restrict__ g,
                            const int32_t* __restrict__ cell_dofs, const double* __restrict__ lam_dir,
                            double* __restrict__ rhs) {
@@ -1608,14 +1610,17 @@ __global__ void rhs_kernel(PairTable P, const double* __restrict__ S, const doub
       const int32_t c = P.gcell[2 * G + k];
       if (c < 0) continue;
       const int mi = P.iperm ? P.iperm[P.gperm[2 * G + k] * ntf + m] : m;
-      const int i = P.glpos[2 * G + k] * ntf + mi;
+      const uint8_t gl = P.glpos[2 * G + k];
+      const int i = (gl & 15) * ntf + mi;
       double v = g[(int64_t)c * nb + i];
-      const int32_t* d = cell_dofs + (int64_t)c * nb;
-      for (int j = 0; j < nb; ++j) {
-        const int32_t dj = d[j];
-        if (dj < 0) {
-          const double ld = lam_dir[-(int64_t)dj - 1];
-          if (ld != 0.0) v -= S[((int64_t)c * nb + j) * nb + i] * ld;
+      if (gl & kGlposDirichlet) {
+        const int32_t* d = cell_dofs + (int64_t)c * nb;
+        for (int j = 0; j < nb; ++j) {
+          const int32_t dj = d[j];
+          if (dj < 0) {
+            const double ld = lam_dir[-(int64_t)dj - 1];
+            if (ld != 0.0) v -= S[((int64_t)c * nb + j) * nb + i] * ld;
+          }
         }
       }
       acc += v;
