@@ -3479,7 +3479,10 @@ __global__ void __launch_bounds__(64 * kFluxWaves, 3)  // two workgroups per CU
 // recovery applies A_qu and A_uq from the reference matrices as well).
 // --------------------------------------------------------------------------
 constexpr int kRefThreads = 256;
-constexpr int kRefEntries = 5;  // entries of R per thread
+// entries of R per thread: 3 keeps the kernel at 144 VGPRs, three wavefronts
+// per SIMD (5: 203 VGPRs, two).  Measured at 64^3 k=2: 9.4 ms per step
+// against 11.4 with 5; at 24^3 k=3: 3.5 against 4.0
+constexpr int kRefEntries = 3;
 constexpr int kRefTerms = 3;    // entry-program terms per entry of R0 kept in registers
 constexpr int kRefSplit = 16;   // threads per row of the rhs column's sum
 
