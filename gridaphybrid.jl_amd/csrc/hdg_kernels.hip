@@ -3579,14 +3579,20 @@ __global__ void __launch_bounds__(kRefThreads, 2)
   }
   // H_e = M^-1 G_e on the band's columns (M^-1 is symmetric, so the rhs
   // column's G_e^T M^-1 z_e is H_e^T z_e)
-  for (int t = tid; t < np * np; t += NT) s_minv[t] = T.fb_minv[t];
+  // M^-1 staged transposed (M^-1[j][i] at i * np + j): the threads of a
+  // wavefront take consecutive j, which the row-major copy put in one bank
+  // when np = 64
+  for (int t = tid; t < np * np; t += NT) {
+    const int i = t / np, j = t - i * np;
+    s_minv[t] = T.fb_minv[j * np + i];
+  }
   __syncthreads();
   for (int t = tid; t < nk * BR; t += NT) {
     const int rl = t / nk, k = t - rl * nk, e = k / np, j = k - e * np;
     double h = 0.0;
     if (rl < rows)
       for (int i = 0; i < np; ++i)
-        h = fma(s_minv[j * np + i], T.fr_G[((size_t)e * np + i) * nr + r0 + rl], h);
+        h = fma(s_minv[i * np + j], T.fr_G[((size_t)e * np + i) * nr + r0 + rl], h);
     s_H[rl * hld + k] = h;
   }
   __syncthreads();  // s_minv is reused as s_xr
