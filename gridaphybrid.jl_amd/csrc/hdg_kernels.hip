@@ -3687,11 +3687,11 @@ static hipError_t launch_flux_ref(const Tables& T, const MeshDev& M, int64_t fir
                                   double* R, const RefPlan& rp, double* blocks, int64_t bstride,
                                   hipStream_t st) {
   if (count <= 0) return hipSuccess;
-  // about 2048 workgroups over all bands (eight per CU, two resident); with
+  // about 2048 workgroups over all bands (eight per CU, three resident); with
   // np = 64 (k = 3) the per-workgroup set-up (P values, H_e) is four times
-  // dearer and 1024 do better.  Measured at 64^3 k=2: 11.4 ms per step with
-  // 2048, 12.5 with 1024, 12.1 with 4096; at 24^3 k=3: 4.50 with 2048, 4.00
-  // with 1024
+  // dearer and 1024 do better.  Measured at 64^3 k=2: 9.36 ms per step with
+  // 2048, 9.37 with 2304, 11.5 with 1536, 14.3 with 768; at 24^3 k=3: 3.07
+  // with 1024, 3.16 with 2304, 3.30 with 1536, 3.82 with 768
   const int target = rp.np >= 64 ? 1024 : 2048;
   const int64_t groups = std::max<int64_t>(
       1, std::min<int64_t>(count, (target + rp.nbands - 1) / rp.nbands));
