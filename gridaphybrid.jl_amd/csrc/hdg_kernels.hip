@@ -4135,7 +4135,10 @@ static hipError_t launch_flux_invert(double* slot, int64_t usz, int64_t ncells, 
 // 3D flux classes, else the grid family of the size.
 static hipError_t launch_reduced(const double* R, int nr, const CondenseOut& o, int64_t count,
                                  int nu, int nb, hipStream_t st, int rfam, int gridmul) {
-  const int grid = (int)std::min<int64_t>(count, 256 * (gridmul > 0 ? gridmul : 16));
+  // 32 row-per-lane workgroups per CU (about four times the resident eight):
+  // measured at 64^3 k=2, 12.3 ms per step against 13.2 with 16, 14.3 with 8
+  // and 12.4 with one cell per workgroup
+  const int grid = (int)std::min<int64_t>(count, 256 * (gridmul > 0 ? gridmul : 32));
 #define HDG_ROWLU(NU_, NB_)                                                                  \
   if (nu == NU_ && nb == NB_) {                                                              \
     ClockScope clk_(st, "reduced_rowlu");                                                    \
